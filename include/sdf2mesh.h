@@ -1,0 +1,229 @@
+/*
+ * sdf2mesh.h - C ABI of the MI355X meshing library (libsdf2mesh.so).
+ *
+ * The library turns a signed distance function (SDF) into a triangle mesh with the
+ * reference's mass-point dual contouring, computed by HIP kernels for gfx950.
+ *
+ * Plain C types only (no HIP or torch types), so a Rust `-sys` crate, ctypes or any
+ * other FFI can bind it.  Every entry point is thread-compatible: use one context per
+ * host thread.  Errors: every `int` return is an s2m_status; the message of the last
+ * failure on the calling thread is returned by s2m_last_error().
+ *
+ * Reference interfaces replaced (paths relative to the sdf2mesh repository):
+ *   s2m_create / s2m_destroy       wgpu adapter/device/queue set-up, main.rs:180-196
+ *   s2m_program_from_tape          Sdf3DShader::from_path + create_shader_module
+ *                                  (shader.rs:44-67,220-225) for programs built from
+ *                                  the library primitives and operators
+ *                                  (sdf3d_primitives.wgsl, sdf_op.wgsl)
+ *   s2m_mesh                       the per-slice loop and host scan, main.rs:294-357,
+ *                                  plus TriangleMesh::from(VertexList), mesh.rs:334-341
+ *   s2m_write_stl / s2m_write_ply  TriangleMesh::write_to_file, mesh.rs:182-210
+ *   s2m_params                     AppState + From<&Arguments>, main.rs:28-33,139-175
+ */
+#ifndef SDF2MESH_H
+#define SDF2MESH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------
+ * Status codes.  The first six mirror ShaderProcessingError (shadertoy.rs:70-80):
+ * RequestError, ShaderError, ParseErrors, WgslError, ValidationError, MissingSdf.
+ * ---------------------------------------------------------------------------------- */
+enum s2m_status {
+    S2M_OK = 0,
+    S2M_E_REQUEST = 1,     /* network request (ShaderToy); never available offline  */
+    S2M_E_SHADER = 2,      /* shader text error (e.g. function not found)            */
+    S2M_E_PARSE = 3,       /* parse error in WGSL / GLSL input                       */
+    S2M_E_WGSL = 4,        /* lowering / code generation error                       */
+    S2M_E_VALIDATION = 5,  /* type or program validation error                       */
+    S2M_E_MISSING_SDF = 6, /* no sdf3d (or named) function, shader.rs:96-98          */
+    S2M_E_HIP = 7,         /* HIP runtime or hiprtc failure                          */
+    S2M_E_OOM = 8,         /* device or host allocation failed                       */
+    S2M_E_BUDGET = 9,      /* request exceeds the configured HBM budget (serving)    */
+    S2M_E_ARG = 10,        /* invalid argument                                       */
+    S2M_E_IO = 11,         /* file I/O error                                         */
+    S2M_E_BUSY = 12        /* no result ready yet (batch/server polling)             */
+};
+
+/* ------------------------------------------------------------------------------------
+ * SDF tape: a postfix program over the reference's primitive and operator library.
+ * Each primitive pushes one distance, each operator pops d2 (top) and d1 (below) and
+ * pushes op(d1, d2).  A valid tape leaves exactly one value on the stack.
+ * Semantics follow the WGSL library literally (fp32, operation order as written):
+ *   SPHERE    f[0..2]=c, f[3]=s            sdf3d_sphere(p - c, s)      primitives.wgsl:27-30
+ *   BOX       f[0..2]=c, f[3..5]=b         sdf3d_box(p - c, b)         primitives.wgsl:7-11
+ *   TORUS     f[0..2]=c, f[3..4]=t         sdf3d_torus(p - c, t)       primitives.wgsl:32-36
+ *   CYLINDER  f[0..2]=c, f[3]=h, f[4]=r    sdf3d_cylinder(p - c, h, r) primitives.wgsl:13-17
+ *   CAPSULE   f[0..2]=a, f[3..5]=b, f[6]=r sdf3d_capsule(p, a, b, r)   primitives.wgsl:19-25
+ *   MANDELBULB f[0]=power, f[1]=bailout, f[2]=offset, iarg = iterations | (swap_yz<<16)
+ *             the distance estimator of examples/mandelmesh.frag:3-28
+ *   UNION min(d1,d2)  INTERSECTION max(d1,d2)  SUBTRACTION max(-d1,d2)
+ *   SMOOTH_UNION / SMOOTH_INTERSECTION / SMOOTH_SUBTRACTION  f[0]=k,  sdf_op.wgsl:7-23
+ * `p - c` is a per-component fp32 subtraction, so c = 0 leaves p unchanged.
+ * ---------------------------------------------------------------------------------- */
+enum s2m_opcode {
+    S2M_OP_SPHERE = 1,
+    S2M_OP_BOX = 2,
+    S2M_OP_TORUS = 3,
+    S2M_OP_CYLINDER = 4,
+    S2M_OP_CAPSULE = 5,
+    S2M_OP_MANDELBULB = 6,
+    S2M_OP_UNION = 16,
+    S2M_OP_INTERSECTION = 17,
+    S2M_OP_SUBTRACTION = 18,
+    S2M_OP_SMOOTH_UNION = 19,
+    S2M_OP_SMOOTH_INTERSECTION = 20,
+    S2M_OP_SMOOTH_SUBTRACTION = 21
+};
+
+typedef struct s2m_op {
+    uint32_t code; /* s2m_opcode */
+    uint32_t iarg; /* integer operand (MANDELBULB iterations and flags) */
+    float f[10];   /* float operands */
+} s2m_op;          /* 48 bytes */
+
+#define S2M_MAX_TAPE 4096u  /* longest accepted tape                  */
+#define S2M_MAX_STACK 16u   /* deepest accepted operand stack         */
+#define S2M_MANDEL_SWAP_YZ (1u << 16) /* apply p.xyz = p.xzy first (mandelmesh.frag:4) */
+
+/* ------------------------------------------------------------------------------------
+ * Meshing parameters: the AppState uniform (main.rs:28-33) as plain fields.
+ * ---------------------------------------------------------------------------------- */
+#define S2M_COMPAT_POW2 0x1u            /* round res up as `2 << ilog2(res)` unless a
+                                           power of two (main.rs:141-148)            */
+#define S2M_COMPAT_DROP_LAST_SLICE 0x2u /* drop cells of the top z-slice, as the
+                                           reference's one-slice readback lag does
+                                           (main.rs:298-356: staging buffers are mapped
+                                           before the slice's submit)                  */
+#define S2M_CORNERS_EXACT 0x4u          /* evaluate the SDF at the reference's per-cell
+                                           corner coordinates cmin=min+h*i, cmax=cmin+h
+                                           (dualcontour.wgsl:22-43) instead of a shared
+                                           lattice min+h*i                             */
+#define S2M_NO_NORMALS 0x8u             /* skip the 4-tap normal (STL does not use it) */
+#define S2M_REFERENCE_COMPAT (S2M_COMPAT_POW2 | S2M_COMPAT_DROP_LAST_SLICE | S2M_CORNERS_EXACT)
+
+typedef struct s2m_params {
+    float bb_min[3];  /* AppState.bb_min.xyz                                  */
+    float bb_max[3];  /* AppState.bb_max.xyz                                  */
+    uint32_t res[3];  /* AppState.dims.xyz (cells per axis)                   */
+    float eps;        /* AppState.bb_max.w: normal tap distance (1e-4)        */
+    uint32_t flags;   /* S2M_* flags above                                    */
+    uint32_t slab;    /* cell planes per device slab, 0 = automatic           */
+} s2m_params;
+
+/* Fill p the way From<&Arguments> does (main.rs:139-175): cube of edge `bounds`
+ * centred at the origin (lib.rs:115-118), res on all three axes, eps 1e-4,
+ * flags = S2M_REFERENCE_COMPAT. */
+void s2m_params_reference(s2m_params* p, uint32_t resolution, float bounds);
+
+/* Resolution after the reference's rounding rule (main.rs:141-148). */
+uint32_t s2m_round_resolution(uint32_t res);
+
+/* ------------------------------------------------------------------------------------
+ * Mesh results.  Vertices are in ascending cell order (z, then y, then x), as the
+ * reference's VertexList (mesh.rs:213-248); triangles in VertexList order with the
+ * x-, y- then z-edge quad of each cell (mesh.rs:267-324, lib.rs:188-211).
+ * ---------------------------------------------------------------------------------- */
+typedef struct s2m_mesh_out {
+    float* vtx;             /* 6 floats per vertex: pos.xyz, normal.xyz            */
+    uint64_t nvtx;
+    uint32_t* tri;          /* 3 vertex indices per triangle                       */
+    uint64_t ntri;
+    uint64_t invalid_quads; /* quads skipped for a missing vertex (mesh.rs:270-278) */
+    uint64_t* cell_keys;    /* per vertex x | y<<16 | z<<32 (mesh.rs:224-226)      */
+    uint8_t* signs;         /* per vertex bits [c100>0]|[c010>0]<<1|[c001>0]<<2|
+                               [c000>0]<<3 (dualcontour.wgsl:57-69)                */
+} s2m_mesh_out;
+
+/* Device-resident result of s2m_mesh_device: pointers into context-owned HBM, valid
+ * until the next meshing call on the same context. */
+typedef struct s2m_device_mesh {
+    const float* vtx;        /* 6 floats per vertex (device memory)   */
+    const uint32_t* tri;     /* 3 u32 per triangle (device memory)    */
+    const uint64_t* cell_keys;
+    const uint8_t* signs;
+    uint64_t nvtx;
+    uint64_t ntri;
+    uint64_t invalid_quads;
+} s2m_device_mesh;
+
+/* Per-stage device time of the last meshing call, from HIP events recorded on the
+ * context's stream (milliseconds).  Filled only when profiling is on. */
+typedef struct s2m_timing {
+    double total_ms;    /* first kernel start to last kernel end                   */
+    double eval_ms;     /* lattice evaluation kernels                               */
+    double contour_ms;  /* active-cell count, scan, vertex emission                 */
+    double normal_ms;   /* 4-tap normals                                            */
+    double quad_ms;     /* quad count, scan, triangle emission                      */
+    double d2h_ms;      /* device-to-host copies (s2m_mesh only)                    */
+    uint64_t eval_launches;
+    uint64_t lattice_points; /* SDF evaluations done on the lattice                 */
+    uint64_t slabs;
+} s2m_timing;
+
+typedef struct s2m_ctx s2m_ctx;
+typedef struct s2m_program s2m_program;
+
+/* ---- context -------------------------------------------------------------------- */
+s2m_ctx* s2m_create(int device);
+void s2m_destroy(s2m_ctx* ctx);
+const char* s2m_last_error(void);
+int s2m_set_profiling(s2m_ctx* ctx, int on);
+int s2m_last_timing(const s2m_ctx* ctx, s2m_timing* out);
+
+/* ---- programs ------------------------------------------------------------------- */
+/* Validate a tape (stack discipline, opcodes, sizes) and upload it to the device. */
+int s2m_program_from_tape(s2m_ctx* ctx, const s2m_op* ops, uint32_t n_ops, s2m_program** out);
+void s2m_program_free(s2m_program* prog);
+/* Host-side tape validation only (no device needed): returns S2M_OK or S2M_E_VALIDATION. */
+int s2m_tape_validate(const s2m_op* ops, uint32_t n_ops);
+
+/* ---- meshing -------------------------------------------------------------------- */
+/* Mesh `prog` with `params`; results are copied into library-allocated host memory
+ * that s2m_mesh_free releases.  cell_keys/signs are filled only if want_keys != 0. */
+int s2m_mesh(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params,
+             int want_keys, s2m_mesh_out* out);
+/* Same computation, result left in device memory owned by ctx (no PCIe transfer). */
+int s2m_mesh_device(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params,
+                    s2m_device_mesh* out);
+void s2m_mesh_free(s2m_mesh_out* out);
+
+/* ---- output --------------------------------------------------------------------- */
+/* ASCII STL exactly as STLWriter (mesh.rs:8-48,155-180), floats in Rust `{}` form. */
+int s2m_write_stl(const s2m_mesh_out* mesh, const char* path);
+/* ASCII PLY exactly as PLYWriter (mesh.rs:50-141,198-210). */
+int s2m_write_ply(const s2m_mesh_out* mesh, const char* path);
+/* Dispatch on the extension, case-insensitive (mesh.rs:182-196).  Unknown
+ * extensions write nothing and return S2M_OK, as the reference logs and continues. */
+int s2m_write_to_file(const s2m_mesh_out* mesh, const char* path);
+/* Format one f32 as Rust's `{}` Display does (shortest round-trip digits, never an
+ * exponent).  Returns the string length; buf needs at least 64 bytes. */
+int s2m_format_f32(float x, char* buf);
+
+/* ---- test hooks ----------------------------------------------------------------- */
+/* Per-axis coordinate tables used for corner evaluation.  coord receives the
+ * distinct coordinates (at most 2*res+1), lo/hi (res entries each) the index of each
+ * cell's min and max corner.  Returns the number of coordinates, or 0 on error. */
+uint32_t s2m_axis_table(float bmin, float bmax, uint32_t res, int exact_corners,
+                        float* coord, uint32_t* lo, uint32_t* hi);
+/* Evaluate the program on the corner lattice of params (x fastest), copy it to
+ * `out` (capacity cap floats) and report the lattice extent in dims[3]. */
+int s2m_eval_lattice(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params,
+                     float* out, uint64_t cap, uint32_t dims[3]);
+/* Contour given per-cell corner values (8 per cell in the order 000,100,010,110,001,
+ * 101,011,111; cells in (z,y,x) order) with cell bounds from per-axis tables
+ * (cell_lo/cell_hi hold res_x, then res_y, then res_z floats).  No SDF is involved,
+ * so normals are zero.  flags: S2M_COMPAT_DROP_LAST_SLICE only. */
+int s2m_contour_cells(s2m_ctx* ctx, const float* corners, const uint32_t res[3],
+                      const float* cell_lo, const float* cell_hi, uint32_t flags,
+                      s2m_mesh_out* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SDF2MESH_H */

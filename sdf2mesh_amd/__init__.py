@@ -1,0 +1,347 @@
+"""sdf2mesh_amd - MI355X-native meshing path of sdf2mesh.
+
+Python mirror of the reference's host interface for this path, over the C ABI of
+libsdf2mesh.so (include/sdf2mesh.h):
+
+    ctx  = Context(0)                                 # wgpu device set-up (main.rs:180-196)
+    prog = ctx.program(tape.torus_example())          # Sdf3DShader + shader module
+    mesh = ctx.mesh(prog, Params.reference(128, 2.0)) # slice loop + TriangleMesh::from
+    mesh.write_to_file("torus.stl")                   # TriangleMesh::write_to_file
+
+The library must be built (``python -m sdf2mesh_amd.build``); nothing here falls back
+to a CPU path: every meshing call runs the HIP kernels or raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+from . import tape
+from .tape import TAPE_DTYPE
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libsdf2mesh.so"
+
+# ---- status codes (s2m_status) ---------------------------------------------------------
+OK, E_REQUEST, E_SHADER, E_PARSE, E_WGSL, E_VALIDATION, E_MISSING_SDF = range(7)
+E_HIP, E_OOM, E_BUDGET, E_ARG, E_IO, E_BUSY = range(7, 13)
+STATUS_NAMES = {
+    0: "Ok", 1: "RequestError", 2: "ShaderError", 3: "ParseErrors", 4: "WgslError",
+    5: "ValidationError", 6: "MissingSdf", 7: "HipError", 8: "OutOfMemory",
+    9: "BudgetExceeded", 10: "InvalidArgument", 11: "IoError", 12: "Busy",
+}
+
+# ---- flags -------------------------------------------------------------------------------
+COMPAT_POW2 = 0x1
+COMPAT_DROP_LAST_SLICE = 0x2
+CORNERS_EXACT = 0x4
+NO_NORMALS = 0x8
+REFERENCE_COMPAT = COMPAT_POW2 | COMPAT_DROP_LAST_SLICE | CORNERS_EXACT
+
+
+class S2MError(RuntimeError):
+    """A failed library call; .code is the s2m_status value."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+# ---- C structs -----------------------------------------------------------------------------
+class _Params(C.Structure):
+    _fields_ = [("bb_min", C.c_float * 3), ("bb_max", C.c_float * 3), ("res", C.c_uint32 * 3),
+                ("eps", C.c_float), ("flags", C.c_uint32), ("slab", C.c_uint32)]
+
+
+class _MeshOut(C.Structure):
+    _fields_ = [("vtx", C.POINTER(C.c_float)), ("nvtx", C.c_uint64),
+                ("tri", C.POINTER(C.c_uint32)), ("ntri", C.c_uint64),
+                ("invalid_quads", C.c_uint64), ("cell_keys", C.POINTER(C.c_uint64)),
+                ("signs", C.POINTER(C.c_uint8))]
+
+
+class _DeviceMesh(C.Structure):
+    _fields_ = [("vtx", C.c_void_p), ("tri", C.c_void_p), ("cell_keys", C.c_void_p),
+                ("signs", C.c_void_p), ("nvtx", C.c_uint64), ("ntri", C.c_uint64),
+                ("invalid_quads", C.c_uint64)]
+
+
+class _Timing(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("eval_ms", C.c_double), ("contour_ms", C.c_double),
+                ("normal_ms", C.c_double), ("quad_ms", C.c_double), ("d2h_ms", C.c_double),
+                ("eval_launches", C.c_uint64), ("lattice_points", C.c_uint64),
+                ("slabs", C.c_uint64)]
+
+
+_lib_handle: Optional[C.CDLL] = None
+
+# name -> (restype, argtypes); every entry point of include/sdf2mesh.h
+_SIGNATURES = {
+    "s2m_create": (C.c_void_p, [C.c_int]),
+    "s2m_destroy": (None, [C.c_void_p]),
+    "s2m_last_error": (C.c_char_p, []),
+    "s2m_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "s2m_last_timing": (C.c_int, [C.c_void_p, C.POINTER(_Timing)]),
+    "s2m_program_from_tape": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32,
+                                        C.POINTER(C.c_void_p)]),
+    "s2m_program_free": (None, [C.c_void_p]),
+    "s2m_tape_validate": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "s2m_params_reference": (None, [C.POINTER(_Params), C.c_uint32, C.c_float]),
+    "s2m_round_resolution": (C.c_uint32, [C.c_uint32]),
+    "s2m_mesh": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_Params), C.c_int,
+                           C.POINTER(_MeshOut)]),
+    "s2m_mesh_device": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_Params),
+                                  C.POINTER(_DeviceMesh)]),
+    "s2m_mesh_free": (None, [C.POINTER(_MeshOut)]),
+    "s2m_write_stl": (C.c_int, [C.POINTER(_MeshOut), C.c_char_p]),
+    "s2m_write_ply": (C.c_int, [C.POINTER(_MeshOut), C.c_char_p]),
+    "s2m_write_to_file": (C.c_int, [C.POINTER(_MeshOut), C.c_char_p]),
+    "s2m_format_f32": (C.c_int, [C.c_float, C.c_char_p]),
+    "s2m_axis_table": (C.c_uint32, [C.c_float, C.c_float, C.c_uint32, C.c_int, C.c_void_p,
+                                    C.c_void_p, C.c_void_p]),
+    "s2m_eval_lattice": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_Params), C.c_void_p,
+                                   C.c_uint64, C.POINTER(C.c_uint32)]),
+    "s2m_contour_cells": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32),
+                                    C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(_MeshOut)]),
+}
+
+
+def lib() -> C.CDLL:
+    """Load libsdf2mesh.so (built in-tree); raises if it is missing."""
+    global _lib_handle
+    if _lib_handle is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is not built; run `python -m sdf2mesh_amd.build`")
+        h = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib_handle = h
+    return _lib_handle
+
+
+def _check(rc: int) -> None:
+    if rc != OK:
+        raise S2MError(rc, lib().s2m_last_error().decode(errors="replace"))
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    return C.c_void_p(a.ctypes.data)
+
+
+# ---- params ------------------------------------------------------------------------------
+@dataclass
+class Params:
+    """AppState of main.rs:28-33 as plain fields (see s2m_params)."""
+    bb_min: tuple = (-1.0, -1.0, -1.0)
+    bb_max: tuple = (1.0, 1.0, 1.0)
+    res: tuple = (128, 128, 128)
+    eps: float = 1e-4
+    flags: int = REFERENCE_COMPAT
+    slab: int = 0
+
+    @classmethod
+    def reference(cls, resolution: int = 256, bounds: float = 2.0, flags: int = REFERENCE_COMPAT,
+                  slab: int = 0) -> "Params":
+        """From<&Arguments> (main.rs:139-175): cube of edge `bounds` at the origin."""
+        p = _Params()
+        lib().s2m_params_reference(C.byref(p), resolution, bounds)
+        return cls(tuple(p.bb_min), tuple(p.bb_max), tuple(p.res), p.eps, flags, slab)
+
+    def c(self) -> _Params:
+        p = _Params()
+        p.bb_min[:] = [float(np.float32(v)) for v in self.bb_min]
+        p.bb_max[:] = [float(np.float32(v)) for v in self.bb_max]
+        p.res[:] = [int(v) for v in self.res]
+        p.eps = self.eps
+        p.flags = self.flags
+        p.slab = self.slab
+        return p
+
+
+def round_resolution(res: int) -> int:
+    """main.rs:141-148: `2 << ilog2(res)` unless res is a power of two."""
+    return int(lib().s2m_round_resolution(res))
+
+
+# ---- results -----------------------------------------------------------------------------
+@dataclass
+class TriangleMesh:
+    """TriangleMesh (mesh.rs:143-211) plus the VertexList metadata."""
+    vertices: np.ndarray        # (V, 6) float32: pos.xyz, normal.xyz
+    triangles: np.ndarray       # (T, 3) uint32
+    invalid_quads: int
+    cell_keys: Optional[np.ndarray] = None  # (V,) uint64 x | y<<16 | z<<32
+    signs: Optional[np.ndarray] = None      # (V,) uint8
+
+    def _c(self) -> _MeshOut:
+        m = _MeshOut()
+        self._v = np.ascontiguousarray(self.vertices, np.float32)
+        self._t = np.ascontiguousarray(self.triangles, np.uint32)
+        m.vtx = self._v.ctypes.data_as(C.POINTER(C.c_float))
+        m.nvtx = len(self._v)
+        m.tri = self._t.ctypes.data_as(C.POINTER(C.c_uint32))
+        m.ntri = len(self._t)
+        m.invalid_quads = self.invalid_quads
+        return m
+
+    def write_stl_to_file(self, path) -> None:
+        _check(lib().s2m_write_stl(C.byref(self._c()), str(path).encode()))
+
+    def write_ply_to_file(self, path) -> None:
+        _check(lib().s2m_write_ply(C.byref(self._c()), str(path).encode()))
+
+    def write_to_file(self, path) -> None:
+        _check(lib().s2m_write_to_file(C.byref(self._c()), str(path).encode()))
+
+
+def _take_mesh(out: _MeshOut, want_keys: bool) -> TriangleMesh:
+    nv, nt = int(out.nvtx), int(out.ntri)
+    try:
+        v = np.ctypeslib.as_array(out.vtx, (max(nv, 1) * 6,))[: nv * 6].reshape(nv, 6).copy()
+        t = np.ctypeslib.as_array(out.tri, (max(nt, 1) * 3,))[: nt * 3].reshape(nt, 3).copy()
+        keys = signs = None
+        if want_keys:
+            keys = np.ctypeslib.as_array(out.cell_keys, (max(nv, 1),))[:nv].copy()
+            signs = np.ctypeslib.as_array(out.signs, (max(nv, 1),))[:nv].copy()
+        return TriangleMesh(v, t, int(out.invalid_quads), keys, signs)
+    finally:
+        lib().s2m_mesh_free(C.byref(out))
+
+
+def format_f32(x: float) -> str:
+    """Rust `{}` Display of an f32, as the writers print it."""
+    buf = C.create_string_buffer(64)
+    n = lib().s2m_format_f32(float(np.float32(x)), buf)
+    return buf.raw[:n].decode()
+
+
+def axis_table(bmin: float, bmax: float, res: int, exact: bool):
+    """Per-axis corner coordinates and each cell's (lo, hi) coordinate index."""
+    coord = np.zeros(2 * res + 2, np.float32)
+    lo = np.zeros(max(res, 1), np.uint32)
+    hi = np.zeros(max(res, 1), np.uint32)
+    n = lib().s2m_axis_table(float(np.float32(bmin)), float(np.float32(bmax)), res, int(exact),
+                             _ptr(coord), _ptr(lo), _ptr(hi))
+    return coord[:n].copy(), lo[:res].copy(), hi[:res].copy()
+
+
+def tape_validate(t: np.ndarray) -> bool:
+    t = np.ascontiguousarray(t, TAPE_DTYPE)
+    return lib().s2m_tape_validate(_ptr(t), len(t)) == OK
+
+
+class Program:
+    """A program uploaded to a context's device."""
+
+    def __init__(self, ctx: "Context", handle: int, tape_array: np.ndarray, flops: int):
+        self._ctx = ctx
+        self.handle = handle
+        self.tape = tape_array
+        self.flops_per_point = flops
+
+    def close(self) -> None:
+        if self.handle:
+            lib().s2m_program_free(C.c_void_p(self.handle))
+            self.handle = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device, one HIP stream and its reusable device buffers."""
+
+    def __init__(self, device: int = 0):
+        h = lib().s2m_create(device)
+        if not h:
+            raise S2MError(E_HIP, lib().s2m_last_error().decode(errors="replace"))
+        self.handle = h
+        self.device = device
+
+    def close(self) -> None:
+        if self.handle:
+            lib().s2m_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_profiling(self, on: bool = True) -> None:
+        _check(lib().s2m_set_profiling(C.c_void_p(self.handle), int(on)))
+
+    def last_timing(self) -> dict:
+        t = _Timing()
+        _check(lib().s2m_last_timing(C.c_void_p(self.handle), C.byref(t)))
+        return {k: getattr(t, k) for k, _ in _Timing._fields_}
+
+    def program(self, src) -> Program:
+        """Upload a tape (numpy TAPE_DTYPE array) or a tape.Node expression."""
+        flops = src.flops() if isinstance(src, tape.Node) else 0
+        t = src.tape() if isinstance(src, tape.Node) else np.ascontiguousarray(src, TAPE_DTYPE)
+        out = C.c_void_p()
+        _check(lib().s2m_program_from_tape(C.c_void_p(self.handle), _ptr(t), len(t), C.byref(out)))
+        return Program(self, out.value, t, flops)
+
+    def mesh(self, prog: Program, params: Params, want_keys: bool = True) -> TriangleMesh:
+        """Mesh the program; results copied to host memory."""
+        p = params.c()
+        out = _MeshOut()
+        _check(lib().s2m_mesh(C.c_void_p(self.handle), C.c_void_p(prog.handle), C.byref(p),
+                              int(want_keys), C.byref(out)))
+        return _take_mesh(out, want_keys)
+
+    def mesh_device(self, prog: Program, params: Params) -> dict:
+        """Mesh the program; results stay in device memory owned by the context."""
+        p = params.c()
+        dm = _DeviceMesh()
+        _check(lib().s2m_mesh_device(C.c_void_p(self.handle), C.c_void_p(prog.handle),
+                                     C.byref(p), C.byref(dm)))
+        return {"nvtx": int(dm.nvtx), "ntri": int(dm.ntri), "invalid_quads": int(dm.invalid_quads),
+                "vtx_ptr": dm.vtx, "tri_ptr": dm.tri}
+
+    def eval_lattice(self, prog: Program, params: Params) -> np.ndarray:
+        """SDF values on the corner lattice, shape (nz, ny, nx)."""
+        p = params.c()
+        dims = (C.c_uint32 * 3)()
+        # first call with no buffer reports the extent
+        rc = lib().s2m_eval_lattice(C.c_void_p(self.handle), C.c_void_p(prog.handle), C.byref(p),
+                                    None, 0, dims)
+        n = int(dims[0]) * int(dims[1]) * int(dims[2])
+        if n == 0:
+            return np.zeros((int(dims[2]), int(dims[1]), int(dims[0])), np.float32)
+        out = np.empty(n, np.float32)
+        _check(lib().s2m_eval_lattice(C.c_void_p(self.handle), C.c_void_p(prog.handle), C.byref(p),
+                                      _ptr(out), n, dims))
+        return out.reshape(int(dims[2]), int(dims[1]), int(dims[0]))
+
+    def contour_cells(self, corners: np.ndarray, cell_lo, cell_hi, flags: int = 0) -> TriangleMesh:
+        """Contour per-cell corner values, corners shape (rz, ry, rx, 8); cell_lo/cell_hi
+        are 3 per-axis arrays of cell min/max coordinates."""
+        corners = np.ascontiguousarray(corners, np.float32)
+        rz, ry, rx = corners.shape[:3]
+        res = (C.c_uint32 * 3)(rx, ry, rz)
+        lo = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float32) for a in cell_lo]))
+        hi = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float32) for a in cell_hi]))
+        out = _MeshOut()
+        _check(lib().s2m_contour_cells(C.c_void_p(self.handle), _ptr(corners), res, _ptr(lo),
+                                       _ptr(hi), flags, C.byref(out)))
+        return _take_mesh(out, True)

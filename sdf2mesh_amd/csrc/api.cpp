@@ -1,0 +1,755 @@
+// api.cpp - host runtime behind the C ABI (include/sdf2mesh.h).
+//
+// One context = one device + one non-blocking HIP stream + device buffers that are
+// grown on demand and reused across jobs.  A job is streamed through the device in
+// z-slabs of cells; the only host synchronisation per job is the final read of the
+// device counters (vertex / triangle totals), after all slabs are queued.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+using namespace s2m;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                       \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(S2M_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_),    \
+                        __FILE__, __LINE__);                                                \
+    } while (0)
+
+// Device buffer that only grows.
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need)
+    {
+        if (need <= bytes) return hipSuccess;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            bytes = 0;
+            if (e != hipSuccess) return e;
+        }
+        size_t alloc = std::max<size_t>(need, 256);
+        hipError_t e = hipMalloc(&p, alloc);
+        if (e == hipSuccess) bytes = alloc;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
+// Per-axis corner coordinate table (host side).
+struct AxisTable {
+    std::vector<float> coord;
+    std::vector<uint32_t> lo, hi;
+};
+
+}  // namespace
+
+struct s2m_program {
+    int device = 0;
+    s2m_op* d_ops = nullptr;
+    uint32_t n = 0;
+    std::vector<s2m_op> ops;
+};
+
+struct s2m_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool profiling = false;
+    s2m_timing timing{};
+    DevBuf lattice, vidx, bcount, boffset, counters, tables;
+    DevBuf vtx, keys, signs, tri, corners;
+    uint64_t vcap = 0, tcap = 0;
+    uint64_t last_nvtx = 0, last_ntri = 0, last_invalid = 0;
+    std::vector<hipEvent_t> events;
+    size_t events_used = 0;
+
+    hipEvent_t next_event()
+    {
+        if (events_used == events.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            events.push_back(e);
+        }
+        return events[events_used++];
+    }
+};
+
+// ---------------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------------
+namespace s2m {
+FastDiv make_fastdiv(uint32_t d)
+{
+    FastDiv f{d, 1, 0};
+    if (d <= 1) return f;
+    uint32_t l = 0;
+    while ((uint64_t(1) << l) < d) ++l;
+    // m = floor(2^32 * (2^l - d) / d) + 1
+    const uint64_t num = ((uint64_t(1) << l) - d) << 32;
+    f.m = (uint32_t)(num / d + 1);
+    f.l = l;
+    return f;
+}
+}  // namespace s2m
+
+namespace {
+
+// cell_bounds (dualcontour.wgsl:22-27) along one axis: size = (max-min)/f32(res-1),
+// cmin_i = min + size*f32(i), cmax_i = cmin_i + size.  exact != 0 keeps both values and
+// merges those that coincide; otherwise cell i uses lattice points i and i+1 at
+// min + size*f32(j).  coord lists distinct coordinates in lattice order; lo/hi give
+// each cell's corner indices.
+void build_axis(float bmin, float bmax, uint32_t res, bool exact, AxisTable& t)
+{
+    t.coord.clear();
+    t.lo.assign(res, 0);
+    t.hi.assign(res, 0);
+    if (res == 0) return;
+    const float v = (float)(uint32_t)(res - 1u);
+    const float size = (bmax - bmin) / v;
+    if (!exact) {
+        t.coord.resize((size_t)res + 1);
+        for (uint32_t j = 0; j <= res; ++j) t.coord[j] = bmin + size * (float)j;
+        for (uint32_t i = 0; i < res; ++i) {
+            t.lo[i] = i;
+            t.hi[i] = i + 1;
+        }
+        return;
+    }
+    // lattice position j carries L_j = min + size*j (j < res) and H_j = L_{j-1} + size
+    // (j >= 1); both are kept when they differ (NaN never compares equal, so a NaN
+    // coordinate is kept twice, like two distinct values).
+    t.coord.reserve(2 * (size_t)res + 1);
+    float prev_lo = 0.0f;
+    for (uint32_t j = 0; j <= res; ++j) {
+        const bool has_lo = j < res, has_hi = j >= 1;
+        const float L = has_lo ? bmin + size * (float)j : 0.0f;
+        const float H = has_hi ? prev_lo + size : 0.0f;
+        if (has_hi) {
+            t.hi[j - 1] = (uint32_t)t.coord.size();
+            t.coord.push_back(H);
+        }
+        if (has_lo) {
+            if (has_hi && L == H && !std::signbit(L) == !std::signbit(H)) {
+                t.lo[j] = t.hi[j - 1];
+            } else {
+                t.lo[j] = (uint32_t)t.coord.size();
+                t.coord.push_back(L);
+            }
+            prev_lo = L;
+        }
+    }
+}
+
+uint32_t round_res(uint32_t res)
+{
+    // main.rs:141-148: res = 2 << res.ilog2() when res has more than one bit set
+    if (__builtin_popcount(res) > 1) {
+        uint32_t lg = 31u - (uint32_t)__builtin_clz(res);
+        return lg >= 31 ? 0u : (2u << lg);
+    }
+    return res;
+}
+
+bool tape_ok(const s2m_op* ops, uint32_t n)
+{
+    if (!ops || n == 0 || n > S2M_MAX_TAPE) return false;
+    int sp = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t c = ops[i].code;
+        if (c >= S2M_OP_SPHERE && c <= S2M_OP_MANDELBULB) {
+            if (++sp > (int)S2M_MAX_STACK) return false;
+            if (c == S2M_OP_MANDELBULB && (ops[i].iarg & 0xffffu) > 1024u) return false;
+        } else if (c >= S2M_OP_UNION && c <= S2M_OP_SMOOTH_SUBTRACTION) {
+            if (sp < 2) return false;
+            --sp;
+        } else {
+            return false;
+        }
+    }
+    return sp == 1;
+}
+
+struct Job {
+    uint32_t res[3];
+    uint32_t nz_cells;  // cell planes processed (res_z, minus the top one under the lag)
+    AxisTable ax[3];
+    bool exact;
+};
+
+int prepare_job(const s2m_params* p, Job& j)
+{
+    if (!p) return fail(S2M_E_ARG, "params is NULL");
+    for (int a = 0; a < 3; ++a) {
+        uint32_t r = p->res[a];
+        if (p->flags & S2M_COMPAT_POW2) r = round_res(r);
+        if (r > 65535u)
+            return fail(S2M_E_ARG, "resolution %u exceeds 65535 (u16 cell keys, mesh.rs:214)", r);
+        j.res[a] = r;
+    }
+    j.exact = (p->flags & S2M_CORNERS_EXACT) != 0;
+    for (int a = 0; a < 3; ++a) build_axis(p->bb_min[a], p->bb_max[a], j.res[a], j.exact, j.ax[a]);
+    j.nz_cells = j.res[2];
+    if ((p->flags & S2M_COMPAT_DROP_LAST_SLICE) && j.nz_cells > 0) j.nz_cells -= 1;
+    return S2M_OK;
+}
+
+uint32_t pick_slab(const s2m_params* p, const Job& j)
+{
+    const uint64_t plane = (uint64_t)j.res[0] * j.res[1];
+    if (plane == 0) return 1;
+    uint64_t t = p->slab;
+    if (t == 0) t = std::max<uint64_t>(1, (8ull << 20) / plane);  // ~8M cells per slab
+    // keep the per-slab block count within one scan block
+    const uint64_t max_t = std::max<uint64_t>(1, ((uint64_t)kMaxScanBlocks * kCellsPerBlock) / plane);
+    t = std::min<uint64_t>(t, max_t);
+    t = std::min<uint64_t>(t, std::max<uint32_t>(j.nz_cells, 1));
+    return (uint32_t)t;
+}
+
+// Upload the three axis tables: [cx cy cz | lox hix loy hiy loz hiz] as one buffer.
+int upload_tables(s2m_ctx* ctx, const Job& j, const float* (&c)[3], const uint32_t* (&lo)[3],
+                  const uint32_t* (&hi)[3])
+{
+    size_t nf = 0, nu = 0;
+    for (int a = 0; a < 3; ++a) {
+        nf += j.ax[a].coord.size();
+        nu += 2 * j.ax[a].lo.size();
+    }
+    std::vector<uint8_t> host(nf * 4 + nu * 4 + 64);
+    float* fp = reinterpret_cast<float*>(host.data());
+    uint32_t* up = reinterpret_cast<uint32_t*>(host.data() + nf * 4);
+    size_t fo[3], lo_o[3], hi_o[3], f = 0, u = 0;
+    for (int a = 0; a < 3; ++a) {
+        fo[a] = f;
+        std::copy(j.ax[a].coord.begin(), j.ax[a].coord.end(), fp + f);
+        f += j.ax[a].coord.size();
+        lo_o[a] = u;
+        std::copy(j.ax[a].lo.begin(), j.ax[a].lo.end(), up + u);
+        u += j.ax[a].lo.size();
+        hi_o[a] = u;
+        std::copy(j.ax[a].hi.begin(), j.ax[a].hi.end(), up + u);
+        u += j.ax[a].hi.size();
+    }
+    HIP_TRY(ctx->tables.ensure(host.size()));
+    HIP_TRY(hipMemcpyAsync(ctx->tables.p, host.data(), host.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    // the host vector must outlive the async copy
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const float* dfp = ctx->tables.as<float>();
+    const uint32_t* dup = reinterpret_cast<const uint32_t*>(ctx->tables.as<uint8_t>() + nf * 4);
+    for (int a = 0; a < 3; ++a) {
+        c[a] = dfp + fo[a];
+        lo[a] = dup + lo_o[a];
+        hi[a] = dup + hi_o[a];
+    }
+    return S2M_OK;
+}
+
+// Mesh one job on the device.  corners != NULL: contour the given per-cell corner
+// dump (test hook) instead of evaluating a program.
+int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Job& j,
+            const float* h_corners)
+{
+    int rc = S2M_OK;
+    ctx->timing = s2m_timing{};
+    ctx->events_used = 0;
+    const uint64_t plane = (uint64_t)j.res[0] * j.res[1];
+    const uint64_t total_cells = plane * j.nz_cells;
+    ctx->last_nvtx = ctx->last_ntri = ctx->last_invalid = 0;
+    if (total_cells == 0) return S2M_OK;
+    if (total_cells > 0xffffffffull * 4ull)
+        return fail(S2M_E_ARG, "grid of %llu cells is too large", (unsigned long long)total_cells);
+
+    const float* dc[3];
+    const uint32_t* dlo[3];
+    const uint32_t* dhi[3];
+    rc = upload_tables(ctx, j, dc, dlo, dhi);
+    if (rc) return rc;
+
+    const uint32_t T = pick_slab(p, j);
+    const uint32_t nx = (uint32_t)j.ax[0].coord.size(), ny = (uint32_t)j.ax[1].coord.size();
+    const uint32_t pitch = (nx + 63u) & ~63u;
+    // largest lattice z extent of any slab
+    uint32_t max_nzl = 0;
+    for (uint32_t z0 = 0; z0 < j.nz_cells; z0 += T) {
+        const uint32_t z1 = std::min(z0 + T, j.nz_cells);
+        max_nzl = std::max(max_nzl, j.ax[2].hi[z1 - 1] - j.ax[2].lo[z0] + 1);
+    }
+    const uint64_t slab_cells = plane * T;
+    const uint32_t max_blocks = (uint32_t)((slab_cells + kCellsPerBlock - 1) / kCellsPerBlock);
+
+    if (!h_corners) HIP_TRY(ctx->lattice.ensure((size_t)pitch * ny * max_nzl * sizeof(float)));
+    HIP_TRY(ctx->vidx.ensure((size_t)(T + 1) * plane * sizeof(uint32_t)));
+    HIP_TRY(ctx->bcount.ensure((size_t)max_blocks * sizeof(uint32_t)));
+    HIP_TRY(ctx->boffset.ensure((size_t)max_blocks * sizeof(uint32_t)));
+    HIP_TRY(ctx->counters.ensure(kNumCounters * sizeof(uint64_t)));
+    if (h_corners) {
+        HIP_TRY(ctx->corners.ensure((size_t)total_cells * 8 * sizeof(float)));
+        HIP_TRY(hipMemcpyAsync(ctx->corners.p, h_corners, (size_t)total_cells * 8 * sizeof(float),
+                               hipMemcpyHostToDevice, ctx->stream));
+    }
+
+    // Output capacity: a closed surface crosses O(res^2) cells; start from 12 per
+    // row-plane cell and grow once if the counters report more.
+    uint64_t want_v = std::min<uint64_t>(total_cells, std::max<uint64_t>(65536, 12 * plane));
+    want_v = std::max<uint64_t>(want_v, ctx->vcap);
+    uint64_t want_t = std::max<uint64_t>(3 * want_v, ctx->tcap);
+    const bool normals = prog && !(p->flags & S2M_NO_NORMALS);
+
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        HIP_TRY(ctx->vtx.ensure(want_v * 6 * sizeof(float)));
+        HIP_TRY(ctx->keys.ensure(want_v * sizeof(uint64_t)));
+        HIP_TRY(ctx->signs.ensure(want_v));
+        HIP_TRY(ctx->tri.ensure(want_t * 3 * sizeof(uint32_t)));
+        ctx->vcap = want_v;
+        ctx->tcap = want_t;
+
+        MeshBufs b{};
+        b.vtx = ctx->vtx.as<float>();
+        b.keys = ctx->keys.as<uint64_t>();
+        b.signs = ctx->signs.as<uint8_t>();
+        b.tri = ctx->tri.as<uint32_t>();
+        b.vcap = want_v;
+        b.tcap = want_t;
+        b.counters = ctx->counters.as<uint64_t>();
+        b.bcount = ctx->bcount.as<uint32_t>();
+        b.boffset = ctx->boffset.as<uint32_t>();
+
+        HIP_TRY(hipMemsetAsync(b.counters, 0, kNumCounters * sizeof(uint64_t), ctx->stream));
+        HIP_TRY(hipMemsetAsync(ctx->vidx.p, 0xff, (size_t)plane * sizeof(uint32_t), ctx->stream));
+
+        ctx->events_used = 0;
+        std::vector<hipEvent_t> ev;  // per slab: 5 marks (eval, cells, normals, quads, end)
+        hipEvent_t ev_start = nullptr, ev_end = nullptr;
+        if (ctx->profiling) {
+            ev_start = ctx->next_event();
+            HIP_TRY(hipEventRecord(ev_start, ctx->stream));
+        }
+        uint64_t lattice_points = 0, slabs = 0;
+        for (uint32_t z0 = 0; z0 < j.nz_cells; z0 += T) {
+            const uint32_t z1 = std::min(z0 + T, j.nz_cells);
+            const uint32_t nzc = z1 - z0;
+            const uint32_t lz0 = j.ax[2].lo[z0];
+            const uint32_t nzl = j.ax[2].hi[z1 - 1] - lz0 + 1;
+
+            SlabArgs a{};
+            a.lat = ctx->lattice.as<float>();
+            a.pitch = pitch;
+            a.lat_ny = ny;
+            a.lz0 = lz0;
+            a.cx = dc[0];
+            a.cy = dc[1];
+            a.cz = dc[2];
+            a.lox = dlo[0];
+            a.hix = dhi[0];
+            a.loy = dlo[1];
+            a.hiy = dhi[1];
+            a.loz = dlo[2];
+            a.hiz = dhi[2];
+            a.rx = j.res[0];
+            a.ry = j.res[1];
+            a.z0 = z0;
+            a.ncells = (uint32_t)(plane * nzc);
+            a.div_rx = make_fastdiv(j.res[0]);
+            a.div_rxy = make_fastdiv((uint32_t)plane);
+            a.vidx = ctx->vidx.as<uint32_t>();
+            a.flags = p->flags;
+            const uint32_t nb = (uint32_t)(((uint64_t)a.ncells + kCellsPerBlock - 1) / kCellsPerBlock);
+
+            hipEvent_t m[5] = {};
+            if (ctx->profiling)
+                for (auto& e : m) e = ctx->next_event();
+            if (m[0]) HIP_TRY(hipEventRecord(m[0], ctx->stream));
+            if (!h_corners) {
+                HIP_TRY(launch_eval_lattice(ctx->stream, prog->d_ops, prog->n, dc[0], dc[1], dc[2],
+                                            nx, ny, lz0, nzl, pitch, ctx->lattice.as<float>()));
+                lattice_points += (uint64_t)nx * ny * nzl;
+            }
+            if (m[1]) HIP_TRY(hipEventRecord(m[1], ctx->stream));
+            if (h_corners) {
+                const float* src = ctx->corners.as<float>() + (size_t)z0 * plane * 8;
+                HIP_TRY(launch_cells_from_corners(ctx->stream, a, src, b, 0));
+                HIP_TRY(launch_scan(ctx->stream, b, nb, 0));
+                HIP_TRY(launch_cells_from_corners(ctx->stream, a, src, b, 1));
+            } else {
+                HIP_TRY(launch_cells(ctx->stream, a, b, 0, 0));
+                HIP_TRY(launch_scan(ctx->stream, b, nb, 0));
+                HIP_TRY(launch_cells(ctx->stream, a, b, 1, normals ? 0 : 1));
+            }
+            if (m[2]) HIP_TRY(hipEventRecord(m[2], ctx->stream));
+            if (normals) HIP_TRY(launch_normals(ctx->stream, prog->d_ops, prog->n, p->eps, b));
+            if (m[3]) HIP_TRY(hipEventRecord(m[3], ctx->stream));
+            HIP_TRY(launch_quads(ctx->stream, a, b, 0));
+            HIP_TRY(launch_scan(ctx->stream, b, nb, 1));
+            HIP_TRY(launch_quads(ctx->stream, a, b, 1));
+            if (m[4]) HIP_TRY(hipEventRecord(m[4], ctx->stream));
+            if (z1 < j.nz_cells) {  // last map plane becomes the previous plane
+                uint32_t* vp = ctx->vidx.as<uint32_t>();
+                HIP_TRY(hipMemcpyAsync(vp, vp + (size_t)nzc * plane, plane * sizeof(uint32_t),
+                                       hipMemcpyDeviceToDevice, ctx->stream));
+            }
+            if (ctx->profiling) ev.insert(ev.end(), m, m + 5);
+            ++slabs;
+        }
+        if (ctx->profiling) {
+            ev_end = ctx->next_event();
+            HIP_TRY(hipEventRecord(ev_end, ctx->stream));
+        }
+        uint64_t cnt[kNumCounters];
+        HIP_TRY(hipMemcpyAsync(cnt, b.counters, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (ctx->profiling) {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_end));
+            ctx->timing.total_ms = ms;
+            for (size_t s = 0; s + 4 < ev.size(); s += 5) {
+                float t0, t1, t2, t3;
+                HIP_TRY(hipEventElapsedTime(&t0, ev[s], ev[s + 1]));
+                HIP_TRY(hipEventElapsedTime(&t1, ev[s + 1], ev[s + 2]));
+                HIP_TRY(hipEventElapsedTime(&t2, ev[s + 2], ev[s + 3]));
+                HIP_TRY(hipEventElapsedTime(&t3, ev[s + 3], ev[s + 4]));
+                ctx->timing.eval_ms += t0;
+                ctx->timing.contour_ms += t1;
+                ctx->timing.normal_ms += t2;
+                ctx->timing.quad_ms += t3;
+            }
+        }
+        ctx->timing.eval_launches = h_corners ? 0 : slabs;
+        ctx->timing.lattice_points = lattice_points;
+        ctx->timing.slabs = slabs;
+        if (cnt[kVtxTotal] >= 0xffffffffull)
+            return fail(S2M_E_ARG, "mesh has %llu vertices; indices are u32",
+                        (unsigned long long)cnt[kVtxTotal]);
+        if (cnt[kVtxTotal] <= want_v && cnt[kTriTotal] <= want_t) {
+            ctx->last_nvtx = cnt[kVtxTotal];
+            ctx->last_ntri = cnt[kTriTotal];
+            ctx->last_invalid = cnt[kInvalid];
+            return S2M_OK;
+        }
+        want_v = std::max<uint64_t>(want_v, cnt[kVtxTotal] + cnt[kVtxTotal] / 8 + 1024);
+        want_t = std::max<uint64_t>(want_t, cnt[kTriTotal] + cnt[kTriTotal] / 8 + 2048);
+    }
+    return fail(S2M_E_OOM, "output buffers kept overflowing");
+}
+
+int copy_out(s2m_ctx* ctx, int want_keys, s2m_mesh_out* out, double* d2h_ms)
+{
+    std::memset(out, 0, sizeof *out);
+    const uint64_t nv = ctx->last_nvtx, nt = ctx->last_ntri;
+    out->nvtx = nv;
+    out->ntri = nt;
+    out->invalid_quads = ctx->last_invalid;
+    out->vtx = (float*)std::malloc(std::max<uint64_t>(nv, 1) * 6 * sizeof(float));
+    out->tri = (uint32_t*)std::malloc(std::max<uint64_t>(nt, 1) * 3 * sizeof(uint32_t));
+    if (want_keys) {
+        out->cell_keys = (uint64_t*)std::malloc(std::max<uint64_t>(nv, 1) * sizeof(uint64_t));
+        out->signs = (uint8_t*)std::malloc(std::max<uint64_t>(nv, 1));
+    }
+    if (!out->vtx || !out->tri || (want_keys && (!out->cell_keys || !out->signs))) {
+        s2m_mesh_free(out);
+        return fail(S2M_E_OOM, "host allocation failed");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (ctx->profiling) {
+        e0 = ctx->next_event();
+        e1 = ctx->next_event();
+        HIP_TRY(hipEventRecord(e0, ctx->stream));
+    }
+    if (nv) HIP_TRY(hipMemcpyAsync(out->vtx, ctx->vtx.p, nv * 6 * sizeof(float),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    if (nt) HIP_TRY(hipMemcpyAsync(out->tri, ctx->tri.p, nt * 3 * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    if (want_keys && nv) {
+        HIP_TRY(hipMemcpyAsync(out->cell_keys, ctx->keys.p, nv * sizeof(uint64_t),
+                               hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(out->signs, ctx->signs.p, nv, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    if (e1) HIP_TRY(hipEventRecord(e1, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (e1) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+        *d2h_ms = ms;
+    }
+    return S2M_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------------
+extern "C" {
+
+const char* s2m_last_error(void) { return g_err.c_str(); }
+
+uint32_t s2m_round_resolution(uint32_t res) { return round_res(res); }
+
+void s2m_params_reference(s2m_params* p, uint32_t resolution, float bounds)
+{
+    if (!p) return;
+    std::memset(p, 0, sizeof *p);
+    // Bounds3D::cube(a, zero) (lib.rs:115-118): min = 0 - a*0.5, max = 0 + a*0.5
+    const float half = bounds * 0.5f;
+    for (int a = 0; a < 3; ++a) {
+        p->bb_min[a] = 0.0f - half;
+        p->bb_max[a] = 0.0f + half;
+        p->res[a] = resolution;
+    }
+    p->eps = 0.0001f;
+    p->flags = S2M_REFERENCE_COMPAT;
+}
+
+s2m_ctx* s2m_create(int device)
+{
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    if (e != hipSuccess || n <= 0) {
+        fail(S2M_E_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+        return nullptr;
+    }
+    if (device < 0 || device >= n) {
+        fail(S2M_E_ARG, "device %d out of range (%d devices)", device, n);
+        return nullptr;
+    }
+    e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        fail(S2M_E_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+        return nullptr;
+    }
+    s2m_ctx* ctx = new s2m_ctx();
+    ctx->device = device;
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        fail(S2M_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+        delete ctx;
+        return nullptr;
+    }
+    return ctx;
+}
+
+void s2m_destroy(s2m_ctx* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (DevBuf* b : {&ctx->lattice, &ctx->vidx, &ctx->bcount, &ctx->boffset, &ctx->counters,
+                      &ctx->tables, &ctx->vtx, &ctx->keys, &ctx->signs, &ctx->tri, &ctx->corners})
+        b->release();
+    for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int s2m_set_profiling(s2m_ctx* ctx, int on)
+{
+    if (!ctx) return fail(S2M_E_ARG, "ctx is NULL");
+    ctx->profiling = on != 0;
+    return S2M_OK;
+}
+
+int s2m_last_timing(const s2m_ctx* ctx, s2m_timing* out)
+{
+    if (!ctx || !out) return fail(S2M_E_ARG, "NULL argument");
+    *out = ctx->timing;
+    return S2M_OK;
+}
+
+int s2m_tape_validate(const s2m_op* ops, uint32_t n_ops)
+{
+    return tape_ok(ops, n_ops) ? S2M_OK
+                               : fail(S2M_E_VALIDATION, "invalid tape (opcode, operand count or "
+                                                        "stack depth)");
+}
+
+int s2m_program_from_tape(s2m_ctx* ctx, const s2m_op* ops, uint32_t n_ops, s2m_program** out)
+{
+    if (!ctx || !out) return fail(S2M_E_ARG, "NULL argument");
+    *out = nullptr;
+    int rc = s2m_tape_validate(ops, n_ops);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    s2m_program* p = new s2m_program();
+    p->device = ctx->device;
+    p->n = n_ops;
+    p->ops.assign(ops, ops + n_ops);
+    hipError_t e = hipMalloc(&p->d_ops, n_ops * sizeof(s2m_op));
+    if (e == hipSuccess)
+        e = hipMemcpy(p->d_ops, ops, n_ops * sizeof(s2m_op), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (p->d_ops) (void)hipFree(p->d_ops);
+        delete p;
+        return fail(S2M_E_HIP, "tape upload: %s", hipGetErrorString(e));
+    }
+    *out = p;
+    return S2M_OK;
+}
+
+void s2m_program_free(s2m_program* prog)
+{
+    if (!prog) return;
+    if (prog->d_ops) {
+        (void)hipSetDevice(prog->device);
+        (void)hipFree(prog->d_ops);
+    }
+    delete prog;
+}
+
+int s2m_mesh_device(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params,
+                    s2m_device_mesh* out)
+{
+    if (!ctx || !prog || !params || !out) return fail(S2M_E_ARG, "NULL argument");
+    if (prog->device != ctx->device) return fail(S2M_E_ARG, "program built for another device");
+    HIP_TRY(hipSetDevice(ctx->device));
+    Job j;
+    int rc = prepare_job(params, j);
+    if (rc) return rc;
+    rc = run_job(ctx, prog, params, j, nullptr);
+    if (rc) return rc;
+    out->vtx = ctx->vtx.as<float>();
+    out->tri = ctx->tri.as<uint32_t>();
+    out->cell_keys = ctx->keys.as<uint64_t>();
+    out->signs = ctx->signs.as<uint8_t>();
+    out->nvtx = ctx->last_nvtx;
+    out->ntri = ctx->last_ntri;
+    out->invalid_quads = ctx->last_invalid;
+    return S2M_OK;
+}
+
+int s2m_mesh(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, int want_keys,
+             s2m_mesh_out* out)
+{
+    if (!out) return fail(S2M_E_ARG, "out is NULL");
+    std::memset(out, 0, sizeof *out);
+    s2m_device_mesh dm;
+    int rc = s2m_mesh_device(ctx, prog, params, &dm);
+    if (rc) return rc;
+    double d2h = 0;
+    rc = copy_out(ctx, want_keys, out, &d2h);
+    ctx->timing.d2h_ms = d2h;
+    return rc;
+}
+
+void s2m_mesh_free(s2m_mesh_out* out)
+{
+    if (!out) return;
+    std::free(out->vtx);
+    std::free(out->tri);
+    std::free(out->cell_keys);
+    std::free(out->signs);
+    std::memset(out, 0, sizeof *out);
+}
+
+uint32_t s2m_axis_table(float bmin, float bmax, uint32_t res, int exact_corners, float* coord,
+                        uint32_t* lo, uint32_t* hi)
+{
+    AxisTable t;
+    build_axis(bmin, bmax, res, exact_corners != 0, t);
+    if (coord) std::copy(t.coord.begin(), t.coord.end(), coord);
+    if (lo) std::copy(t.lo.begin(), t.lo.end(), lo);
+    if (hi) std::copy(t.hi.begin(), t.hi.end(), hi);
+    return (uint32_t)t.coord.size();
+}
+
+int s2m_eval_lattice(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, float* out,
+                     uint64_t cap, uint32_t dims[3])
+{
+    if (!ctx || !prog || !params || !dims) return fail(S2M_E_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    Job j;
+    int rc = prepare_job(params, j);
+    if (rc) return rc;
+    for (int a = 0; a < 3; ++a) dims[a] = (uint32_t)j.ax[a].coord.size();
+    const uint64_t n = (uint64_t)dims[0] * dims[1] * dims[2];
+    if (n == 0) return S2M_OK;
+    if (!out || cap < n) return fail(S2M_E_ARG, "output holds %llu floats, lattice has %llu",
+                                     (unsigned long long)cap, (unsigned long long)n);
+    const float* dc[3];
+    const uint32_t* dlo[3];
+    const uint32_t* dhi[3];
+    rc = upload_tables(ctx, j, dc, dlo, dhi);
+    if (rc) return rc;
+    HIP_TRY(ctx->lattice.ensure((size_t)n * sizeof(float)));
+    HIP_TRY(launch_eval_lattice(ctx->stream, prog->d_ops, prog->n, dc[0], dc[1], dc[2], dims[0],
+                                dims[1], 0, dims[2], dims[0], ctx->lattice.as<float>()));
+    HIP_TRY(hipMemcpyAsync(out, ctx->lattice.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return S2M_OK;
+}
+
+int s2m_contour_cells(s2m_ctx* ctx, const float* corners, const uint32_t res[3],
+                      const float* cell_lo, const float* cell_hi, uint32_t flags,
+                      s2m_mesh_out* out)
+{
+    if (!ctx || !corners || !res || !cell_lo || !cell_hi || !out)
+        return fail(S2M_E_ARG, "NULL argument");
+    std::memset(out, 0, sizeof *out);
+    HIP_TRY(hipSetDevice(ctx->device));
+    for (int a = 0; a < 3; ++a)
+        if (res[a] > 65535u) return fail(S2M_E_ARG, "resolution exceeds 65535");
+    // The given cell bounds become coordinate tables coord = [lo..., hi...] with
+    // lo[i] = i and hi[i] = res + i, so the slab loop reads them unchanged.
+    s2m_params p{};
+    p.flags = (flags & S2M_COMPAT_DROP_LAST_SLICE) | S2M_NO_NORMALS;
+    Job j;
+    j.exact = true;
+    size_t off = 0;
+    for (int a = 0; a < 3; ++a) {
+        p.res[a] = res[a];
+        j.res[a] = res[a];
+        AxisTable& t = j.ax[a];
+        t.coord.assign(cell_lo + off, cell_lo + off + res[a]);
+        t.coord.insert(t.coord.end(), cell_hi + off, cell_hi + off + res[a]);
+        t.lo.resize(res[a]);
+        t.hi.resize(res[a]);
+        for (uint32_t i = 0; i < res[a]; ++i) {
+            t.lo[i] = i;
+            t.hi[i] = res[a] + i;
+        }
+        off += res[a];
+    }
+    j.nz_cells = res[2];
+    if ((p.flags & S2M_COMPAT_DROP_LAST_SLICE) && j.nz_cells > 0) j.nz_cells -= 1;
+    int rc = run_job(ctx, nullptr, &p, j, corners);
+    if (rc) return rc;
+    double d2h = 0;
+    return copy_out(ctx, 1, out, &d2h);
+}
+
+}  // extern "C"

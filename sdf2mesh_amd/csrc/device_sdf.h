@@ -1,0 +1,175 @@
+// device_sdf.h - the SDF primitive/operator library and the tape interpreter, as
+// device functions for gfx950.
+//
+// fp32 throughout, each vector expression of the WGSL library expanded per component in
+// the order the WGSL text evaluates it.  The library is compiled with
+// -ffp-contract=off and correctly rounded division and square root, so every value
+// below is the IEEE result of the written operation sequence.
+//   sdf3d_primitives.wgsl:7-36   box, cylinder, capsule, sphere, torus
+//   sdf_op.wgsl:7-23             smooth union / intersection / subtraction
+//   examples/mandelmesh.frag:3-28 the Mandelbulb distance estimator
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/sdf2mesh.h"
+
+namespace s2m {
+
+__device__ __forceinline__ float len2(float x, float y) { return sqrtf(x * x + y * y); }
+__device__ __forceinline__ float len3(float x, float y, float z)
+{
+    return sqrtf((x * x + y * y) + z * z);
+}
+// WGSL clamp(e, low, high) = min(max(e, low), high)
+__device__ __forceinline__ float wclamp(float e, float lo, float hi)
+{
+    return fminf(fmaxf(e, lo), hi);
+}
+// WGSL mix(e1, e2, e3) = e1 * (1 - e3) + e2 * e3
+__device__ __forceinline__ float wmix(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+
+// sdf3d_sphere: length(p) - s
+__device__ __forceinline__ float sd_sphere(float x, float y, float z, float s)
+{
+    return len3(x, y, z) - s;
+}
+
+// sdf3d_box: q = abs(p) - 0.5*b; length(max(q,0)) + min(max(q.x,max(q.y,q.z)),0)
+__device__ __forceinline__ float sd_box(float x, float y, float z, float bx, float by, float bz)
+{
+    const float qx = fabsf(x) - 0.5f * bx;
+    const float qy = fabsf(y) - 0.5f * by;
+    const float qz = fabsf(z) - 0.5f * bz;
+    const float outside = len3(fmaxf(qx, 0.0f), fmaxf(qy, 0.0f), fmaxf(qz, 0.0f));
+    return outside + fminf(fmaxf(qx, fmaxf(qy, qz)), 0.0f);
+}
+
+// sdf3d_cylinder: d = abs(vec2(length(p.xz), p.y)) - vec2(r, h);
+// min(max(d.x,d.y),0) + length(max(d,0))
+__device__ __forceinline__ float sd_cylinder(float x, float y, float z, float h, float r)
+{
+    const float dx = fabsf(len2(x, z)) - r;
+    const float dy = fabsf(y) - h;
+    return fminf(fmaxf(dx, dy), 0.0f) + len2(fmaxf(dx, 0.0f), fmaxf(dy, 0.0f));
+}
+
+// sdf3d_capsule: pa = p - a; ba = b - a; h = clamp(dot(pa,ba)/dot(ba,ba), 0, 1);
+// length(pa - ba*h) - r
+__device__ __forceinline__ float sd_capsule(float x, float y, float z, const float* f)
+{
+    const float pax = x - f[0], pay = y - f[1], paz = z - f[2];
+    const float bax = f[3] - f[0], bay = f[4] - f[1], baz = f[5] - f[2];
+    const float h = wclamp(((pax * bax + pay * bay) + paz * baz) / ((bax * bax + bay * bay) + baz * baz),
+                           0.0f, 1.0f);
+    return len3(pax - bax * h, pay - bay * h, paz - baz * h) - f[6];
+}
+
+// sdf3d_torus: q = vec2(length(p.xz) - t.x, p.y); length(q) - t.y
+__device__ __forceinline__ float sd_torus(float x, float y, float z, float tx, float ty)
+{
+    return len2(len2(x, z) - tx, y) - ty;
+}
+
+// sdf_op_smooth_union: h = clamp(0.5 + 0.5*(d2-d1)/k, 0, 1); mix(d2,d1,h) - k*h*(1-h)
+__device__ __forceinline__ float op_smooth_union(float d1, float d2, float k)
+{
+    const float h = wclamp(0.5f + (0.5f * (d2 - d1)) / k, 0.0f, 1.0f);
+    return wmix(d2, d1, h) - (k * h) * (1.0f - h);
+}
+// sdf_op_smooth_intersection: h = clamp(0.5 - 0.5*(d2-d1)/k, 0, 1); mix(d2,d1,h) + k*h*(1-h)
+__device__ __forceinline__ float op_smooth_intersection(float d1, float d2, float k)
+{
+    const float h = wclamp(0.5f - (0.5f * (d2 - d1)) / k, 0.0f, 1.0f);
+    return wmix(d2, d1, h) + (k * h) * (1.0f - h);
+}
+// sdf_op_smooth_subtraction: h = clamp(0.5 - 0.5*(d2+d1)/k, 0, 1); mix(d2,-d1,h) + k*h*(1-h)
+__device__ __forceinline__ float op_smooth_subtraction(float d1, float d2, float k)
+{
+    const float h = wclamp(0.5f - (0.5f * (d2 + d1)) / k, 0.0f, 1.0f);
+    return wmix(d2, -d1, h) + (k * h) * (1.0f - h);
+}
+
+// examples/mandelmesh.frag:3-28 with power, bailout, offset and the iteration count as
+// operands.  A bailed-out point keeps r = length(z): the loop `continue`s, so r is
+// recomputed from the unchanged z in every remaining iteration.
+__device__ __forceinline__ float sd_mandelbulb(float px, float py, float pz, float power,
+                                               float bailout, float offset, uint32_t iarg)
+{
+    if (iarg & S2M_MANDEL_SWAP_YZ) {  // p.xyz = p.xzy
+        const float t = py;
+        py = pz;
+        pz = t;
+    }
+    const uint32_t iters = iarg & 0xffffu;
+    float zx = px, zy = py, zz = pz;
+    float r = 0.0f, dr = 1.0f;
+    for (uint32_t i = 0; i < iters; ++i) {
+        r = len3(zx, zy, zz);
+        if (r > bailout) continue;
+        float theta = atanf(zy / zx);
+        float phi = asinf(zz / r);
+        dr = (powf(r, power - 1.0f) * dr) * power + 1.0f;
+        r = powf(r, power);
+        theta = theta * power;
+        phi = phi * power;
+        const float ct = cosf(theta), cp = cosf(phi), st = sinf(theta), sp = sinf(phi);
+        zx = r * (ct * cp) + px;
+        zy = r * (st * cp) + py;
+        zz = r * sp + pz;
+    }
+    return ((0.5f * logf(r)) * r) / dr - offset;
+}
+
+// One primitive of the tape at p.
+__device__ __forceinline__ float eval_primitive(uint32_t code, uint32_t iarg, const float* f,
+                                                float px, float py, float pz)
+{
+    switch (code) {
+    case S2M_OP_SPHERE: return sd_sphere(px - f[0], py - f[1], pz - f[2], f[3]);
+    case S2M_OP_BOX: return sd_box(px - f[0], py - f[1], pz - f[2], f[3], f[4], f[5]);
+    case S2M_OP_TORUS: return sd_torus(px - f[0], py - f[1], pz - f[2], f[3], f[4]);
+    case S2M_OP_CYLINDER: return sd_cylinder(px - f[0], py - f[1], pz - f[2], f[3], f[4]);
+    case S2M_OP_CAPSULE: return sd_capsule(px, py, pz, f);
+    default: return sd_mandelbulb(px, py, pz, f[0], f[1], f[2], iarg);
+    }
+}
+
+__device__ __forceinline__ float apply_op(uint32_t code, float d1, float d2, float k)
+{
+    switch (code) {
+    case S2M_OP_UNION: return fminf(d1, d2);
+    case S2M_OP_INTERSECTION: return fmaxf(d1, d2);
+    case S2M_OP_SUBTRACTION: return fmaxf(-d1, d2);
+    case S2M_OP_SMOOTH_UNION: return op_smooth_union(d1, d2, k);
+    case S2M_OP_SMOOTH_INTERSECTION: return op_smooth_intersection(d1, d2, k);
+    default: return op_smooth_subtraction(d1, d2, k);
+    }
+}
+
+// Tape interpreter.  The tape index is wave-uniform, so opcode and operands are read
+// through the scalar cache and the dispatch is a scalar branch.  The top of the operand
+// stack lives in `acc`; deeper entries in a small register array indexed by the
+// (uniform) stack depth.
+__device__ __forceinline__ float tape_eval(const s2m_op* __restrict__ ops, uint32_t n, float px,
+                                           float py, float pz)
+{
+    float st[S2M_MAX_STACK];
+    float acc = 0.0f;
+    uint32_t sp = 0;  // entries below acc
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t code = ops[i].code;
+        const float* f = ops[i].f;
+        if (code < S2M_OP_UNION) {
+            const float d = eval_primitive(code, ops[i].iarg, f, px, py, pz);
+            if (i != 0) st[sp++] = acc;
+            acc = d;
+        } else {
+            const float d1 = st[--sp];
+            acc = apply_op(code, d1, acc, f[0]);
+        }
+    }
+    return acc;
+}
+
+}  // namespace s2m

@@ -1,0 +1,440 @@
+// kernels.hip - HIP kernels of the meshing path for gfx950 (wave64).
+//
+// Per z-slab of cells the host launches, in stream order:
+//   k_eval_lattice     SDF at every corner-lattice point of the slab       (VALU bound)
+//   k_cells<0>         active-cell count per block of 1024 cells           (L2/HBM bound)
+//   k_scan             exclusive scan of block counts + running vertex total
+//   k_cells<1>         vertex position/signs, ordered write, cell->vertex map
+//   k_normals          4-tap normal of the slab's new vertices
+//   k_quads<0>/k_scan/k_quads<1>  the same count/scan/emit scheme for triangles
+// Cells are numbered in (z, y, x) order inside the slab and every block owns 1024
+// consecutive cells, so block offsets from the scan give each vertex and triangle its
+// place in the reference's output order (VertexList order, mesh.rs:280-321).
+#include "device_sdf.h"
+#include "internal.h"
+
+namespace s2m {
+
+__device__ __forceinline__ uint32_t fdiv(uint32_t n, FastDiv f)
+{
+    const uint64_t t = (uint64_t)__umulhi(n, f.m) + n;
+    return (uint32_t)(t >> f.l);
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+// Exclusive prefix of v over the 256 threads of the block (4 waves); *total = sum.
+// Contains two __syncthreads: every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_exclusive_256(uint32_t v, uint32_t* lds, uint32_t* total)
+{
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    uint32_t inc = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t t = __shfl_up(inc, d, 64);
+        if (lane >= (uint32_t)d) inc += t;
+    }
+    if (lane == 63) lds[w] = inc;
+    __syncthreads();
+    uint32_t woff = 0, tot = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kBlock / 64; ++k) {
+        const uint32_t s = lds[k];
+        woff += (k < w) ? s : 0u;
+        tot += s;
+    }
+    __syncthreads();
+    *total = tot;
+    return woff + inc - v;
+}
+
+// ---------------------------------------------------------------------------------
+// Lattice evaluation: one thread per lattice point, 64 consecutive x per wave so each
+// wave stores 256 contiguous bytes.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_eval_lattice(const s2m_op* __restrict__ ops, uint32_t nops,
+                                                      const float* __restrict__ cx,
+                                                      const float* __restrict__ cy,
+                                                      const float* __restrict__ cz, uint32_t nx,
+                                                      uint32_t ny, uint32_t lz0, uint32_t pitch,
+                                                      float* __restrict__ lat)
+{
+    const uint32_t ix = blockIdx.x * 64u + lane_id();
+    const uint32_t iy = blockIdx.y * 4u + (threadIdx.x >> 6);
+    const uint32_t iz = blockIdx.z;
+    if (ix >= nx || iy >= ny) return;
+    const float v = tape_eval(ops, nops, cx[ix], cy[iy], cz[lz0 + iz]);
+    lat[((size_t)iz * ny + iy) * pitch + ix] = v;
+}
+
+// ---------------------------------------------------------------------------------
+// One cell: dualcontour.wgsl:72-131 (crossings, centroid) and :57-69 (sign bits).
+// c[] in corner order 000,100,010,110,001,101,011,111.
+// ---------------------------------------------------------------------------------
+struct Edge {
+    uint8_t a, b;       // corner indices
+    uint8_t axis;       // component that carries t
+    uint8_t fx, fy, fz; // fixed components (0 or 1), the t component entry is unused
+};
+// z-edges 0-3, y-edges 4-7, x-edges 8-11 (dualcontour.wgsl:98-114)
+__constant__ Edge kEdges[12] = {
+    {0, 4, 2, 0, 0, 0}, {2, 6, 2, 0, 1, 0}, {1, 5, 2, 1, 0, 0}, {3, 7, 2, 1, 1, 0},
+    {0, 2, 1, 0, 0, 0}, {4, 6, 1, 0, 0, 1}, {1, 3, 1, 1, 0, 0}, {5, 7, 1, 1, 0, 1},
+    {0, 1, 0, 0, 0, 0}, {4, 5, 0, 0, 0, 1}, {2, 3, 0, 0, 1, 0}, {6, 7, 0, 0, 1, 1},
+};
+
+__device__ __forceinline__ bool contour_cell(const float c[8], const float lo[3], const float hi[3],
+                                             float pos[3])
+{
+    float ax = 0.0f, ay = 0.0f, az = 0.0f, cnt = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 12; ++e) {
+        const float v0 = c[kEdges[e].a], v1 = c[kEdges[e].b];
+        const float t = (0.0f - v0) / (v1 - v0);  // _cell_adapt
+        float x = kEdges[e].fx, y = kEdges[e].fy, z = kEdges[e].fz;
+        if (kEdges[e].axis == 0) x = t;
+        else if (kEdges[e].axis == 1) y = t;
+        else z = t;
+        if ((v0 > 0.0f) == (v1 > 0.0f)) x = y = z = 0.0f;  // _cell_change
+        if (x > 0.0f || y > 0.0f || z > 0.0f) {
+            ax = ax + x;
+            ay = ay + y;
+            az = az + z;
+            cnt = cnt + 1.0f;
+        }
+    }
+    if (cnt <= 1.0f) return false;
+    pos[0] = lo[0] + ((hi[0] - lo[0]) * ax) / cnt;
+    pos[1] = lo[1] + ((hi[1] - lo[1]) * ay) / cnt;
+    pos[2] = lo[2] + ((hi[2] - lo[2]) * az) / cnt;
+    return true;
+}
+
+__device__ __forceinline__ uint32_t sign_bits(const float c[8])
+{
+    return (uint32_t)(c[1] > 0.0f) | ((uint32_t)(c[2] > 0.0f) << 1) |
+           ((uint32_t)(c[4] > 0.0f) << 2) | ((uint32_t)(c[0] > 0.0f) << 3);
+}
+
+// Corners of cell (x, y, zl) from the slab lattice, and the cell's bounds.
+__device__ __forceinline__ void gather_lattice(const SlabArgs& a, uint32_t x, uint32_t y, uint32_t zl,
+                                               float c[8], float lo[3], float hi[3])
+{
+    const uint32_t zg = a.z0 + zl;
+    const uint32_t xi[2] = {a.lox[x], a.hix[x]};
+    const uint32_t yi[2] = {a.loy[y], a.hiy[y]};
+    const uint32_t zi[2] = {a.loz[zg], a.hiz[zg]};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const size_t off = ((size_t)(zi[(k >> 2) & 1] - a.lz0) * a.lat_ny + yi[(k >> 1) & 1]) * a.pitch +
+                           xi[k & 1];
+        c[k] = a.lat[off];
+    }
+    lo[0] = a.cx[xi[0]];
+    hi[0] = a.cx[xi[1]];
+    lo[1] = a.cy[yi[0]];
+    hi[1] = a.cy[yi[1]];
+    lo[2] = a.cz[zi[0]];
+    hi[2] = a.cz[zi[1]];
+}
+
+__device__ __forceinline__ void cell_coords(const SlabArgs& a, uint32_t i, uint32_t& x, uint32_t& y,
+                                            uint32_t& zl)
+{
+    const uint32_t row = fdiv(i, a.div_rx);  // y + ry * zl
+    x = i - row * a.rx;
+    zl = fdiv(i, a.div_rxy);
+    y = row - zl * a.ry;
+}
+
+// ---------------------------------------------------------------------------------
+// Cell pass.  EMIT = 0: block active counts.  EMIT = 1: ordered vertex write and the
+// cell -> vertex index map.  FROM_CORNERS selects the per-cell corner dump source.
+// ---------------------------------------------------------------------------------
+template <int EMIT, int FROM_CORNERS>
+__global__ __launch_bounds__(256) void k_cells(SlabArgs a, MeshBufs b, const float* __restrict__ corners,
+                                               int zero_normals)
+{
+    __shared__ uint32_t lds[kBlock / 64];
+    const uint32_t base = blockIdx.x * kCellsPerBlock;
+    uint64_t running = EMIT ? (uint64_t)b.boffset[blockIdx.x] : 0;
+    uint32_t count = 0;
+    for (uint32_t j = 0; j < kRounds; ++j) {
+        const uint32_t i = base + j * kBlock + threadIdx.x;
+        bool act = false;
+        float pos[3], c[8], lo[3], hi[3];
+        uint32_t x = 0, y = 0, zl = 0;
+        if (i < a.ncells) {
+            cell_coords(a, i, x, y, zl);
+            if (FROM_CORNERS) {
+                const float* src = corners + 8 * (size_t)i;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) c[k] = src[k];
+                const uint32_t zg = a.z0 + zl;
+                lo[0] = a.cx[a.lox[x]];
+                hi[0] = a.cx[a.hix[x]];
+                lo[1] = a.cy[a.loy[y]];
+                hi[1] = a.cy[a.hiy[y]];
+                lo[2] = a.cz[a.loz[zg]];
+                hi[2] = a.cz[a.hiz[zg]];
+            } else {
+                gather_lattice(a, x, y, zl, c, lo, hi);
+            }
+            act = contour_cell(c, lo, hi, pos);
+        }
+        uint32_t total;
+        const uint32_t rank = block_exclusive_256(act ? 1u : 0u, lds, &total);
+        if (EMIT && i < a.ncells) {
+            const uint64_t v = running + rank;
+            const size_t m = ((size_t)(zl + 1) * a.ry + y) * a.rx + x;
+            a.vidx[m] = act ? (uint32_t)v : kNone;
+            if (act && v < b.vcap) {
+                float* o = b.vtx + 6 * v;
+                o[0] = pos[0];
+                o[1] = pos[1];
+                o[2] = pos[2];
+                if (zero_normals) {
+                    o[3] = 0.0f;
+                    o[4] = 0.0f;
+                    o[5] = 0.0f;
+                }
+                b.keys[v] = (uint64_t)x | ((uint64_t)y << 16) | ((uint64_t)(a.z0 + zl) << 32);
+                b.signs[v] = (uint8_t)sign_bits(c);
+            }
+        }
+        running += total;
+        count += total;
+    }
+    if (!EMIT && threadIdx.x == 0) b.bcount[blockIdx.x] = count;
+}
+
+// ---------------------------------------------------------------------------------
+// Single-block exclusive scan of the per-block counts.  which = 0: vertices, 1:
+// triangles.  Adds the running total of earlier slabs and advances it.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ bcount, uint32_t nb,
+                                               uint32_t* __restrict__ boffset,
+                                               uint64_t* __restrict__ counters, uint32_t which)
+{
+    __shared__ uint64_t wsum[kScanThreads / 64];
+    const uint32_t per = (nb + kScanThreads - 1) / kScanThreads;
+    const uint32_t beg = min(threadIdx.x * per, nb), end = min(beg + per, nb);
+    uint64_t s = 0;
+    for (uint32_t i = beg; i < end; ++i) s += bcount[i];
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    uint64_t inc = s;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t t = __shfl_up(inc, d, 64);
+        if (lane >= (uint32_t)d) inc += t;
+    }
+    if (lane == 63) wsum[w] = inc;
+    uint64_t* tot_ptr = counters + (which ? kTriTotal : kVtxTotal);
+    const uint64_t base = *tot_ptr;
+    __syncthreads();
+    uint64_t woff = 0, total = 0;
+    for (uint32_t k = 0; k < kScanThreads / 64; ++k) {
+        woff += (k < w) ? wsum[k] : 0ull;
+        total += wsum[k];
+    }
+    uint64_t run = base + woff + inc - s;
+    for (uint32_t i = beg; i < end; ++i) {
+        boffset[i] = (uint32_t)run;
+        run += bcount[i];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        counters[which ? kTriPrev : kVtxPrev] = base;
+        *tot_ptr = base + total;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Normals of the vertices emitted by the current slab: sdf3d_normal + normalize
+// (sdf3d_normal.wgsl:4-9, dualcontour.wgsl:170-171).  Grid-stride over the range
+// [prev, total) read from the device counters, so no host round trip is needed.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_normals(const s2m_op* __restrict__ ops, uint32_t nops,
+                                                 float eps, float* __restrict__ vtx,
+                                                 const uint64_t* __restrict__ counters, uint64_t vcap)
+{
+    const uint64_t beg = counters[kVtxPrev];
+    const uint64_t end = min(counters[kVtxTotal], vcap);
+    for (uint64_t v = beg + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < end;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        float* o = vtx + 6 * v;
+        const float px = o[0], py = o[1], pz = o[2];
+        // taps v1=(1,-1,-1), v2=(-1,-1,1), v3=(-1,1,-1), v4=(1,1,1); p + v*eps
+        const float d1 = tape_eval(ops, nops, px + eps, py + (-eps), pz + (-eps));
+        const float d2 = tape_eval(ops, nops, px + (-eps), py + (-eps), pz + eps);
+        const float d3 = tape_eval(ops, nops, px + (-eps), py + eps, pz + (-eps));
+        const float d4 = tape_eval(ops, nops, px + eps, py + eps, pz + eps);
+        // v1*d1 + v2*d2 + v3*d3 + v4*d4, summed left to right
+        const float gx = ((d1 + (-d2)) + (-d3)) + d4;
+        const float gy = (((-d1) + (-d2)) + d3) + d4;
+        const float gz = (((-d1) + d2) + (-d3)) + d4;
+        const float len = sqrtf((gx * gx + gy * gy) + gz * gz);
+        o[3] = gx / len;
+        o[4] = gy / len;
+        o[5] = gz / len;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Quad pass (mesh.rs:267-324).  Per cell with a vertex: the x-, y- and z-edge quads,
+// neighbour vertices from the map planes of this and the previous cell plane.  A quad
+// with a missing vertex is skipped and counted (mesh.rs:270-278).  EMIT = 0 counts
+// triangles per block, EMIT = 1 writes them in order.
+// ---------------------------------------------------------------------------------
+template <int EMIT>
+__global__ __launch_bounds__(256) void k_quads(SlabArgs a, MeshBufs b)
+{
+    __shared__ uint32_t lds[kBlock / 64];
+    const uint32_t base = blockIdx.x * kCellsPerBlock;
+    uint64_t running = EMIT ? (uint64_t)b.boffset[blockIdx.x] : 0;
+    uint32_t count = 0, invalid = 0;
+    const size_t plane = (size_t)a.rx * a.ry;
+    const bool lag = (a.flags & S2M_COMPAT_DROP_LAST_SLICE) != 0;
+    for (uint32_t j = 0; j < kRounds; ++j) {
+        const uint32_t i = base + j * kBlock + threadIdx.x;
+        uint32_t q[3][4];
+        uint32_t nq = 0;  // bit e set: quad e is valid
+        if (i < a.ncells) {
+            uint32_t x, y, zl;
+            cell_coords(a, i, x, y, zl);
+            const uint32_t* cur = a.vidx + (size_t)(zl + 1) * plane;
+            const uint32_t* prv = a.vidx + (size_t)zl * plane;
+            const uint32_t v = cur[(size_t)y * a.rx + x];
+            if (v != kNone) {
+                const uint32_t s = (v < b.vcap) ? b.signs[v] : 0u;
+                const bool c0 = s & 1, c1 = (s >> 1) & 1, c2 = (s >> 2) & 1, c3 = (s >> 3) & 1;
+                const uint32_t zg = a.z0 + zl;
+                const size_t r0 = (size_t)y * a.rx, rm = (size_t)(y - 1) * a.rx;
+                // x-edge: Quad(v(x,y-1,z-1), v(x,y,z-1), v(x,y,z), v(x,y-1,z)).swap(c.0)
+                if (c0 != c3 && y > 0) {
+                    if (zg > 0) {
+                        uint32_t t[4] = {prv[rm + x], prv[r0 + x], v, cur[rm + x]};
+                        if (c0) { q[0][0] = t[3]; q[0][1] = t[2]; q[0][2] = t[1]; q[0][3] = t[0]; }
+                        else { q[0][0] = t[0]; q[0][1] = t[1]; q[0][2] = t[2]; q[0][3] = t[3]; }
+                        if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 1;
+                        else ++invalid;
+                    } else if (lag) {
+                        ++invalid;  // the lagged reference looks up label z-1 and misses
+                    }
+                }
+                // y-edge: Quad(v(x-1,y,z-1), v(x,y,z-1), v(x,y,z), v(x-1,y,z)).swap(!c.1)
+                if (c1 != c3 && x > 0) {
+                    if (zg > 0) {
+                        uint32_t t[4] = {prv[r0 + x - 1], prv[r0 + x], v, cur[r0 + x - 1]};
+                        if (!c1) { q[1][0] = t[3]; q[1][1] = t[2]; q[1][2] = t[1]; q[1][3] = t[0]; }
+                        else { q[1][0] = t[0]; q[1][1] = t[1]; q[1][2] = t[2]; q[1][3] = t[3]; }
+                        if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 2;
+                        else ++invalid;
+                    } else if (lag) {
+                        ++invalid;
+                    }
+                }
+                // z-edge: Quad(v(x-1,y-1,z), v(x,y-1,z), v(x,y,z), v(x-1,y,z)).swap(c.2)
+                if (c2 != c3 && x > 0 && y > 0) {
+                    uint32_t t[4] = {cur[rm + x - 1], cur[rm + x], v, cur[r0 + x - 1]};
+                    if (c2) { q[2][0] = t[3]; q[2][1] = t[2]; q[2][2] = t[1]; q[2][3] = t[0]; }
+                    else { q[2][0] = t[0]; q[2][1] = t[1]; q[2][2] = t[2]; q[2][3] = t[3]; }
+                    if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 4;
+                    else ++invalid;
+                }
+            }
+        }
+        const uint32_t ntri = 2u * (uint32_t)__builtin_popcount(nq);
+        uint32_t total;
+        const uint32_t rank = block_exclusive_256(ntri, lds, &total);
+        if (EMIT && nq) {
+            uint64_t t = running + rank;
+#pragma unroll
+            for (int e = 0; e < 3; ++e) {
+                if (!(nq & (1u << e))) continue;
+                // make_triangles: (q2, q1, q0), (q0, q3, q2)  (lib.rs:199-205)
+                if (t + 1 < b.tcap) {
+                    uint32_t* o = b.tri + 3 * t;
+                    o[0] = q[e][2];
+                    o[1] = q[e][1];
+                    o[2] = q[e][0];
+                    o[3] = q[e][0];
+                    o[4] = q[e][3];
+                    o[5] = q[e][2];
+                }
+                t += 2;
+            }
+        }
+        running += total;
+        count += total;
+    }
+    if (!EMIT) {
+        if (invalid) atomicAdd((unsigned long long*)&b.counters[kInvalid], (unsigned long long)invalid);
+        if (threadIdx.x == 0) b.bcount[blockIdx.x] = count;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------------
+hipError_t launch_eval_lattice(hipStream_t s, const s2m_op* ops, uint32_t nops, const float* cx,
+                               const float* cy, const float* cz, uint32_t nx, uint32_t ny,
+                               uint32_t lz0, uint32_t nzl, uint32_t pitch, float* lat)
+{
+    if (nx == 0 || ny == 0 || nzl == 0) return hipSuccess;
+    dim3 grid((nx + 63) / 64, (ny + 3) / 4, nzl);
+    hipLaunchKernelGGL(k_eval_lattice, grid, dim3(256), 0, s, ops, nops, cx, cy, cz, nx, ny, lz0,
+                       pitch, lat);
+    return hipGetLastError();
+}
+
+hipError_t launch_cells(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int emit,
+                        int write_zero_normals)
+{
+    const uint32_t nb = (a.ncells + kCellsPerBlock - 1) / kCellsPerBlock;
+    if (nb == 0) return hipSuccess;
+    if (emit)
+        hipLaunchKernelGGL((k_cells<1, 0>), dim3(nb), dim3(kBlock), 0, s, a, b, nullptr,
+                           write_zero_normals);
+    else
+        hipLaunchKernelGGL((k_cells<0, 0>), dim3(nb), dim3(kBlock), 0, s, a, b, nullptr, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_cells_from_corners(hipStream_t s, const SlabArgs& a, const float* corners,
+                                     const MeshBufs& b, int emit)
+{
+    const uint32_t nb = (a.ncells + kCellsPerBlock - 1) / kCellsPerBlock;
+    if (nb == 0) return hipSuccess;
+    if (emit)
+        hipLaunchKernelGGL((k_cells<1, 1>), dim3(nb), dim3(kBlock), 0, s, a, b, corners, 1);
+    else
+        hipLaunchKernelGGL((k_cells<0, 1>), dim3(nb), dim3(kBlock), 0, s, a, b, corners, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan(hipStream_t s, const MeshBufs& b, uint32_t nblocks, uint32_t which)
+{
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, s, b.bcount, nblocks, b.boffset,
+                       b.counters, which);
+    return hipGetLastError();
+}
+
+hipError_t launch_normals(hipStream_t s, const s2m_op* ops, uint32_t nops, float eps,
+                          const MeshBufs& b)
+{
+    hipLaunchKernelGGL(k_normals, dim3(1024), dim3(256), 0, s, ops, nops, eps, b.vtx, b.counters,
+                       b.vcap);
+    return hipGetLastError();
+}
+
+hipError_t launch_quads(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int emit)
+{
+    const uint32_t nb = (a.ncells + kCellsPerBlock - 1) / kCellsPerBlock;
+    if (nb == 0) return hipSuccess;
+    if (emit) hipLaunchKernelGGL((k_quads<1>), dim3(nb), dim3(kBlock), 0, s, a, b);
+    else hipLaunchKernelGGL((k_quads<0>), dim3(nb), dim3(kBlock), 0, s, a, b);
+    return hipGetLastError();
+}
+
+}  // namespace s2m

@@ -1,0 +1,179 @@
+"""Pins for the oracle's SDF evaluators (sdf3d_primitives.wgsl, sdf_op.wgsl,
+examples/mandelmesh.frag): closed-form distances, textbook reductions, and agreement of
+the fp32 evaluator with the fp64 one."""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from sdf2mesh_amd import tape as T
+
+
+def ev(node, p, f64=False):
+    pts = np.asarray([p], np.float32)
+    return float((oracle.eval_f64 if f64 else oracle.eval_f32)(node, pts)[0])
+
+
+CLOSED = [
+    # sphere at c radius s: distance to centre minus s
+    (T.sphere((0.1, 0.2, 0.3), 0.5), (0.6, 0.2, 0.3), 0.0),
+    (T.sphere((0.1, 0.2, 0.3), 0.5), (0.1, 0.2, 0.3), -0.5),
+    (T.sphere((0.0, 0.0, 0.0), 0.5), (0.0, 3.0, 4.0), 4.5),
+    # box with full edge lengths b (0.5*b is the half extent)
+    (T.box((0, 0, 0), (2, 2, 2)), (2.0, 0.0, 0.0), 1.0),
+    (T.box((0, 0, 0), (2, 2, 2)), (0.0, 0.0, 0.0), -1.0),
+    (T.box((0, 0, 0), (2, 2, 2)), (2.0, 2.0, 0.0), math.sqrt(2.0)),
+    (T.box((0, 0, 0), (2, 4, 6)), (0.5, 0.0, 0.0), -0.5),
+    (T.box((1, 0, 0), (2, 2, 2)), (3.0, 0.0, 0.0), 1.0),
+    # torus (R, r) in the xz plane
+    (T.torus((0, 0, 0), (0.5, 0.2)), (0.7, 0.0, 0.0), 0.0),
+    (T.torus((0, 0, 0), (0.5, 0.2)), (0.5, 0.0, 0.0), -0.2),
+    (T.torus((0, 0, 0), (0.5, 0.2)), (0.0, 0.0, 0.0), 0.3),
+    (T.torus((0, 0, 0), (0.5, 0.2)), (0.0, 0.0, -0.5), -0.2),
+    (T.torus((0, 0, 0), (0.5, 0.2)), (0.0, 1.0, 0.5), 0.8),
+    # cylinder along y: radius r, half height h
+    (T.cylinder((0, 0, 0), 0.5, 0.3), (0.0, 0.0, 0.0), -0.3),
+    (T.cylinder((0, 0, 0), 0.5, 0.3), (1.0, 0.0, 0.0), 0.7),
+    (T.cylinder((0, 0, 0), 0.5, 0.3), (0.0, 2.0, 0.0), 1.5),
+    (T.cylinder((0, 0, 0), 0.5, 0.3), (1.0, 2.0, 0.0), math.hypot(0.7, 1.5)),
+    # capsule segment a-b, radius r
+    (T.capsule((0, 0, 0), (1, 0, 0), 0.1), (0.5, 1.0, 0.0), 0.9),
+    (T.capsule((0, 0, 0), (1, 0, 0), 0.1), (-1.0, 0.0, 0.0), 0.9),
+    (T.capsule((0, 0, 0), (1, 0, 0), 0.1), (2.0, 0.0, 0.0), 0.9),
+    (T.capsule((0, 0, 0), (0, 0, 2), 0.25), (0.0, 0.0, 1.0), -0.25),
+]
+
+
+@pytest.mark.parametrize("node,p,want", CLOSED)
+def test_primitive_closed_forms(node, p, want):
+    assert ev(node, p, f64=True) == pytest.approx(want, abs=1e-7)
+    assert ev(node, p) == pytest.approx(want, abs=2e-6)
+
+
+def test_plain_csg_ops():
+    a, b = T.sphere((0, 0, 0), 0.5), T.sphere((0.6, 0, 0), 0.5)
+    p = (0.1, 0.05, 0.0)
+    da, db = ev(a, p), ev(b, p)
+    assert ev(T.union(a, b), p) == min(da, db)
+    assert ev(T.intersection(a, b), p) == max(da, db)
+    assert ev(T.subtraction(a, b), p) == max(-da, db)
+
+
+@pytest.mark.parametrize("k", [0.05, 0.1, 0.3])
+def test_smooth_ops_equal_inputs(k):
+    # h = 1/2 for d1 == d2:  union -> d - k/4, intersection -> d + k/4 (k is stored as f32)
+    k = float(np.float32(k))
+    s = T.sphere((0.2, 0.1, 0.0), 0.4)
+    p = (0.9, 0.3, -0.2)
+    d = ev(s, p, True)
+    assert ev(T.smooth_union(s, s, k), p, True) == pytest.approx(d - k / 4, abs=1e-12)
+    assert ev(T.smooth_intersection(s, s, k), p, True) == pytest.approx(d + k / 4, abs=1e-12)
+
+
+@pytest.mark.parametrize("k", [0.05, 0.1, 0.3])
+def test_smooth_subtraction_opposite_inputs(k):
+    # at the origin d1 = -0.3 (sphere r 0.3) and d2 = 1 - 0.7 = +0.3 (sphere r 0.7 at x=1):
+    # d1 + d2 = 0 gives h = 1/2 and mix(d2, -d1, 1/2) = d2, so the result is d2 + k/4
+    k = float(np.float32(k))
+    a = T.sphere((0.0, 0.0, 0.0), 0.3)
+    b = T.sphere((1.0, 0.0, 0.0), 0.7)
+    d2 = ev(b, (0, 0, 0), True)
+    assert ev(a, (0, 0, 0), True) == -d2
+    assert ev(T.smooth_subtraction(a, b, k), (0, 0, 0), True) == pytest.approx(d2 + k / 4, abs=1e-12)
+    # far from both (|d1 + d2| > k) it is max(-d1, d2)
+    q = (-0.5, 0.0, 0.0)
+    d1, d2 = ev(a, q), ev(b, q)
+    assert abs(d1 + d2) > k
+    assert ev(T.smooth_subtraction(a, b, k), q) == max(-d1, d2)
+
+
+def test_smooth_union_reduces_to_min_far_apart():
+    a, b = T.sphere((-0.5, 0, 0), 0.2), T.sphere((0.5, 0, 0), 0.2)
+    for p in [(-0.5, 0.1, 0.0), (0.5, 0.0, 0.1), (-0.9, 0.0, 0.0)]:
+        da, db = ev(a, p), ev(b, p)
+        assert abs(da - db) > 0.05
+        assert ev(T.smooth_union(a, b, 0.05), p) == min(da, db)
+        assert ev(T.smooth_intersection(a, b, 0.05), p) == max(da, db)
+
+
+OFF = float(np.float32(0.003))  # tape operands are f32
+
+
+def f32(v):
+    return float(np.float32(v))
+
+
+def _axis_recurrence(x, iters, power=8.0, bailout=2.0):
+    """Mandelbulb on the positive x axis: theta = phi = 0 every step, so the map is the
+    real recurrence z <- |z|^8 + x, dr <- 8 |z|^7 dr + 1 (bailout by `continue`)."""
+    z, dr, r = x, 1.0, 0.0
+    for _ in range(iters):
+        r = abs(z)
+        if r > bailout:
+            continue
+        dr = r ** (power - 1) * dr * power + 1
+        r = r ** power
+        z = r + x
+    return 0.5 * math.log(r) * r / dr - OFF
+
+
+@pytest.mark.parametrize("x", [0.3, 0.6, 0.9, 1.0, 1.05, 1.5, 2.5, 3.0])
+@pytest.mark.parametrize("iters", [1, 2, 5, 8])
+def test_mandelbulb_real_axis(x, iters):
+    node = T.mandelbulb(iters)
+    x = f32(x)
+    want = _axis_recurrence(x, iters)
+    got64 = ev(node, (x, 0.0, 0.0), True)
+    assert got64 == pytest.approx(want, rel=1e-12, abs=1e-12)
+    got32 = ev(node, (x, 0.0, 0.0))
+    assert got32 == pytest.approx(want, rel=1e-4, abs=1e-5)
+
+
+def test_mandelbulb_immediate_bailout_and_one_step():
+    p = (1.2, -2.0, 1.1)  # |p| > 2: every iteration continues, r = |p|, dr = 1
+    r = math.sqrt(sum(f32(v) ** 2 for v in p))
+    assert ev(T.mandelbulb(8), p, True) == pytest.approx(0.5 * math.log(r) * r - OFF, rel=1e-12)
+    q = (0.3, -0.4, 0.2)  # one iteration: r1 = |q|^8, dr1 = 8|q|^7 + 1, whatever the angles
+    r0 = math.sqrt(sum(f32(v) ** 2 for v in q))
+    r1, dr1 = r0 ** 8, 8 * r0 ** 7 + 1
+    assert ev(T.mandelbulb(1), q, True) == pytest.approx(0.5 * math.log(r1) * r1 / dr1 - OFF,
+                                                         rel=1e-12)
+
+
+def test_mandelbulb_swizzle():
+    # p.xyz = p.xzy: the estimator at (x, y, z) without the swap equals the one at
+    # (x, z, y) with it
+    a = T.mandelbulb(4, swap_yz=True)
+    b = T.mandelbulb(4, swap_yz=False)
+    p = (0.31, -0.47, 0.22)
+    assert ev(a, p, True) == ev(b, (p[0], p[2], p[1]), True)
+
+
+def test_fp32_tracks_fp64_on_csg64():
+    node = T.csg64()
+    rng = np.random.default_rng(7)
+    pts = rng.uniform(-1, 1, (20000, 3)).astype(np.float32)
+    d32 = oracle.eval_f32(node, pts).astype(np.float64)
+    d64 = oracle.eval_f64(node, pts)
+    assert np.max(np.abs(d32 - d64)) < 2e-6
+
+
+@pytest.mark.parametrize("seed", range(1, 13))
+def test_fp32_tracks_fp64_random_csg(seed):
+    node = T.random_csg(seed)
+    rng = np.random.default_rng(seed)
+    pts = rng.uniform(-1, 1, (4000, 3)).astype(np.float32)
+    d32 = oracle.eval_f32(node, pts).astype(np.float64)
+    d64 = oracle.eval_f64(node, pts)
+    assert np.max(np.abs(d32 - d64)) < 2e-6
+
+
+def test_tape_check():
+    good = T.csg64().tape()
+    assert oracle.tape_check(good)
+    assert not oracle.tape_check(good[:-1])          # two values left on the stack
+    assert not oracle.tape_check(good[1:])           # operator before operands
+    bad = good.copy()
+    bad[0]["code"] = 99
+    assert not oracle.tape_check(bad)
