@@ -19,6 +19,7 @@
  *                                  plus TriangleMesh::from(VertexList), mesh.rs:334-341
  *   s2m_write_stl / s2m_write_ply  TriangleMesh::write_to_file, mesh.rs:182-210
  *   s2m_params                     AppState + From<&Arguments>, main.rs:28-33,139-175
+ *   s2m_mesh_batch(_device)        no reference counterpart: many jobs per device pass
  */
 #ifndef SDF2MESH_H
 #define SDF2MESH_H
@@ -105,6 +106,11 @@ typedef struct s2m_op {
                                            (dualcontour.wgsl:22-43) instead of a shared
                                            lattice min+h*i                             */
 #define S2M_NO_NORMALS 0x8u             /* skip the 4-tap normal (STL does not use it) */
+#define S2M_DENSE_EVAL 0x10u            /* evaluate every lattice point with the full
+                                           program; by default tiles of 8^3 cells whose
+                                           SDF provably keeps one sign are skipped and
+                                           each tile runs a pruned program that gives
+                                           identical values on the tile                 */
 #define S2M_REFERENCE_COMPAT (S2M_COMPAT_POW2 | S2M_COMPAT_DROP_LAST_SLICE | S2M_CORNERS_EXACT)
 
 typedef struct s2m_params {
@@ -164,6 +170,9 @@ typedef struct s2m_timing {
     uint64_t eval_launches;
     uint64_t lattice_points; /* SDF evaluations done on the lattice                 */
     uint64_t slabs;
+    uint64_t tiles_total;    /* tiles of 8^3 cells (pruned path; 0 when dense)       */
+    uint64_t tiles_active;   /* tiles whose lattice points were evaluated            */
+    uint64_t eval_flops;     /* algorithmic flops executed on the lattice            */
 } s2m_timing;
 
 typedef struct s2m_ctx s2m_ctx;
@@ -182,6 +191,10 @@ int s2m_program_from_tape(s2m_ctx* ctx, const s2m_op* ops, uint32_t n_ops, s2m_p
 void s2m_program_free(s2m_program* prog);
 /* Host-side tape validation only (no device needed): returns S2M_OK or S2M_E_VALIDATION. */
 int s2m_tape_validate(const s2m_op* ops, uint32_t n_ops);
+/* Algorithmic fp32 operation count of one evaluation of the tape: + - * / min max abs
+ * sqrt and each transcendental count 1, clamp 2, mix 3, length(vec3) 6, length(vec2) 4,
+ * dot(vec3) 5, constant subexpressions folded (the static count of the WGSL text). */
+uint64_t s2m_tape_flops(const s2m_op* ops, uint32_t n_ops);
 
 /* ---- meshing -------------------------------------------------------------------- */
 /* Mesh `prog` with `params`; results are copied into library-allocated host memory
@@ -192,6 +205,20 @@ int s2m_mesh(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params,
 int s2m_mesh_device(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params,
                     s2m_device_mesh* out);
 void s2m_mesh_free(s2m_mesh_out* out);
+
+/* ---- batching ------------------------------------------------------------------- */
+/* Mesh n independent jobs together: the tiles of all jobs go through one pass of the
+ * tile-sparse kernels (one set of launches for the batch), so small jobs share the
+ * device instead of each paying its own launch and synchronisation latency.  Each
+ * job's result is identical to meshing it alone with s2m_mesh.  outs[i] receives job
+ * i's mesh (free each with s2m_mesh_free).  Batches too large for one pass are split
+ * into consecutive passes. */
+int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
+                   const s2m_params* params, int want_keys, s2m_mesh_out* outs);
+/* One pass only, results left in the context's device buffers (valid until its next
+ * meshing call): outs[i] points at job i's vertices and triangles. */
+int s2m_mesh_batch_device(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
+                          const s2m_params* params, s2m_device_mesh* outs);
 
 /* ---- output --------------------------------------------------------------------- */
 /* ASCII STL exactly as STLWriter (mesh.rs:8-48,155-180), floats in Rust `{}` form. */

@@ -41,6 +41,7 @@ COMPAT_POW2 = 0x1
 COMPAT_DROP_LAST_SLICE = 0x2
 CORNERS_EXACT = 0x4
 NO_NORMALS = 0x8
+DENSE_EVAL = 0x10
 REFERENCE_COMPAT = COMPAT_POW2 | COMPAT_DROP_LAST_SLICE | CORNERS_EXACT
 
 
@@ -75,7 +76,8 @@ class _Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("eval_ms", C.c_double), ("contour_ms", C.c_double),
                 ("normal_ms", C.c_double), ("quad_ms", C.c_double), ("d2h_ms", C.c_double),
                 ("eval_launches", C.c_uint64), ("lattice_points", C.c_uint64),
-                ("slabs", C.c_uint64)]
+                ("slabs", C.c_uint64), ("tiles_total", C.c_uint64), ("tiles_active", C.c_uint64),
+                ("eval_flops", C.c_uint64)]
 
 
 _lib_handle: Optional[C.CDLL] = None
@@ -91,6 +93,7 @@ _SIGNATURES = {
                                         C.POINTER(C.c_void_p)]),
     "s2m_program_free": (None, [C.c_void_p]),
     "s2m_tape_validate": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "s2m_tape_flops": (C.c_uint64, [C.c_void_p, C.c_uint32]),
     "s2m_params_reference": (None, [C.POINTER(_Params), C.c_uint32, C.c_float]),
     "s2m_round_resolution": (C.c_uint32, [C.c_uint32]),
     "s2m_mesh": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_Params), C.c_int,
@@ -98,6 +101,10 @@ _SIGNATURES = {
     "s2m_mesh_device": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_Params),
                                   C.POINTER(_DeviceMesh)]),
     "s2m_mesh_free": (None, [C.POINTER(_MeshOut)]),
+    "s2m_mesh_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(_Params),
+                                 C.c_int, C.POINTER(_MeshOut)]),
+    "s2m_mesh_batch_device": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p),
+                                        C.POINTER(_Params), C.POINTER(_DeviceMesh)]),
     "s2m_write_stl": (C.c_int, [C.POINTER(_MeshOut), C.c_char_p]),
     "s2m_write_ply": (C.c_int, [C.POINTER(_MeshOut), C.c_char_p]),
     "s2m_write_to_file": (C.c_int, [C.POINTER(_MeshOut), C.c_char_p]),
@@ -232,6 +239,12 @@ def axis_table(bmin: float, bmax: float, res: int, exact: bool):
     return coord[:n].copy(), lo[:res].copy(), hi[:res].copy()
 
 
+def tape_flops(t) -> int:
+    """Algorithmic fp32 operations of one evaluation of the program (s2m_tape_flops)."""
+    t = t.tape() if isinstance(t, tape.Node) else np.ascontiguousarray(t, TAPE_DTYPE)
+    return int(lib().s2m_tape_flops(_ptr(t), len(t)))
+
+
 def tape_validate(t: np.ndarray) -> bool:
     t = np.ascontiguousarray(t, TAPE_DTYPE)
     return lib().s2m_tape_validate(_ptr(t), len(t)) == OK
@@ -295,8 +308,8 @@ class Context:
 
     def program(self, src) -> Program:
         """Upload a tape (numpy TAPE_DTYPE array) or a tape.Node expression."""
-        flops = src.flops() if isinstance(src, tape.Node) else 0
         t = src.tape() if isinstance(src, tape.Node) else np.ascontiguousarray(src, TAPE_DTYPE)
+        flops = tape_flops(t)
         out = C.c_void_p()
         _check(lib().s2m_program_from_tape(C.c_void_p(self.handle), _ptr(t), len(t), C.byref(out)))
         return Program(self, out.value, t, flops)
@@ -308,6 +321,29 @@ class Context:
         _check(lib().s2m_mesh(C.c_void_p(self.handle), C.c_void_p(prog.handle), C.byref(p),
                               int(want_keys), C.byref(out)))
         return _take_mesh(out, want_keys)
+
+    def mesh_batch(self, progs, params, want_keys: bool = True) -> list:
+        """Mesh many jobs in shared device passes; one TriangleMesh per job."""
+        n = len(progs)
+        hp = (C.c_void_p * n)(*[p.handle for p in progs])
+        pa = (_Params * n)(*[q.c() for q in params])
+        outs = (_MeshOut * n)()
+        rc = lib().s2m_mesh_batch(C.c_void_p(self.handle), n, hp, pa, int(want_keys), outs)
+        if rc != OK:
+            for o in outs:
+                lib().s2m_mesh_free(C.byref(o))
+            _check(rc)
+        return [_take_mesh(outs[i], want_keys) for i in range(n)]
+
+    def mesh_batch_device(self, progs, params) -> list:
+        """One shared pass; results stay in device memory owned by the context."""
+        n = len(progs)
+        hp = (C.c_void_p * n)(*[p.handle for p in progs])
+        pa = (_Params * n)(*[q.c() for q in params])
+        outs = (_DeviceMesh * n)()
+        _check(lib().s2m_mesh_batch_device(C.c_void_p(self.handle), n, hp, pa, outs))
+        return [{"nvtx": int(o.nvtx), "ntri": int(o.ntri), "invalid_quads": int(o.invalid_quads)}
+                for o in outs]
 
     def mesh_device(self, prog: Program, params: Params) -> dict:
         """Mesh the program; results stay in device memory owned by the context."""

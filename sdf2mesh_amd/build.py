@@ -33,7 +33,7 @@ DEVICE = [
     "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-SOURCES = ["kernels.hip", "api.cpp", "writer.cpp"]
+SOURCES = ["kernels.hip", "sparse.hip", "api.cpp", "writer.cpp"]
 
 
 def _compile(src: str) -> Path:

@@ -92,6 +92,12 @@ struct s2m_ctx {
     s2m_timing timing{};
     DevBuf lattice, vidx, bcount, boffset, counters, tables;
     DevBuf vtx, keys, signs, tri, corners;
+    DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
+    DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_boxes, sp_vbox, sp_seg, sp_boundary;
+    DevBuf sp_job_out;
+    uint64_t slot_cap = 0;
+    int cus = 256;
+    std::vector<uint8_t> host_img;  // staging of the last pass's tables (kept until its sync)
     uint64_t vcap = 0, tcap = 0;
     uint64_t last_nvtx = 0, last_ntri = 0, last_invalid = 0;
     std::vector<hipEvent_t> events;
@@ -204,11 +210,38 @@ bool tape_ok(const s2m_op* ops, uint32_t n)
     return sp == 1;
 }
 
+// Algorithmic flops of one op (the static count of the WGSL library text; a translation
+// p - c counts 3 unless c = 0, where the WGSL form is the untranslated call).
+uint32_t op_flops(const s2m_op& o)
+{
+    const bool tr = !(o.f[0] == 0.0f && o.f[1] == 0.0f && o.f[2] == 0.0f && !std::signbit(o.f[0]) &&
+                      !std::signbit(o.f[1]) && !std::signbit(o.f[2]));
+    switch (o.code) {
+    case S2M_OP_SPHERE: return 7 + (tr ? 3 : 0);     // length3, -s
+    case S2M_OP_BOX: return 19 + (tr ? 3 : 0);       // abs3, -3, max3, length3, max2, min, +
+    case S2M_OP_TORUS: return 10 + (tr ? 3 : 0);     // length2, -, length2, -
+    case S2M_OP_CYLINDER: return 17 + (tr ? 3 : 0);  // length2, abs2, -2, max, min, +, max2, length2
+    case S2M_OP_CAPSULE: return 24;                  // pa, dot, /, clamp, ba*h, -, length3, -r
+    case S2M_OP_MANDELBULB: return (o.iarg & 0xffffu) * 30u + 5u;
+    case S2M_OP_UNION: case S2M_OP_INTERSECTION: return 1;
+    case S2M_OP_SUBTRACTION: return 2;
+    default: return 13;                              // smooth operators
+    }
+}
+
+bool tape_prunable(const std::vector<s2m_op>& ops)
+{
+    for (const s2m_op& o : ops)
+        if (o.code == S2M_OP_MANDELBULB) return false;
+    return true;
+}
+
 struct Job {
     uint32_t res[3];
     uint32_t nz_cells;  // cell planes processed (res_z, minus the top one under the lag)
     AxisTable ax[3];
     bool exact;
+    bool tiled = false;
 };
 
 int prepare_job(const s2m_params* p, Job& j)
@@ -278,6 +311,344 @@ int upload_tables(s2m_ctx* ctx, const Job& j, const float* (&c)[3], const uint32
         hi[a] = dup + hi_o[a];
     }
     return S2M_OK;
+}
+
+// ---------------------------------------------------------------------------------
+// Tile-sparse pipeline (sparse.hip): host side.
+// ---------------------------------------------------------------------------------
+// Device tables of one job, packed into one host byte image: axis coordinate tables,
+// cell -> coordinate index tables, tile extents and index boxes, per-op flops.
+struct SpTables {
+    std::vector<uint8_t> bytes;
+    size_t off_c[3], off_lo[3], off_hi[3], off_ext, off_ibox, off_flops;
+    uint32_t nt[3];
+    uint32_t bmax[3];
+    double pad;
+};
+
+void build_sp_tables(const Job& j, const s2m_program* prog, float eps, SpTables& T)
+{
+    std::vector<uint8_t>& B = T.bytes;
+    B.clear();
+    auto put = [&](const void* src, size_t n) {
+        const size_t off = (B.size() + 255) & ~size_t(255);
+        B.resize(off + n);
+        if (n) std::memcpy(B.data() + off, src, n);
+        return off;
+    };
+    for (int a = 0; a < 3; ++a) {
+        T.off_c[a] = put(j.ax[a].coord.data(), j.ax[a].coord.size() * 4);
+        T.off_lo[a] = put(j.ax[a].lo.data(), j.ax[a].lo.size() * 4);
+        T.off_hi[a] = put(j.ax[a].hi.data(), j.ax[a].hi.size() * 4);
+    }
+    const uint32_t ncell[3] = {j.res[0], j.res[1], j.nz_cells};
+    for (int a = 0; a < 3; ++a) {
+        T.nt[a] = (ncell[a] + kTile - 1) / kTile;
+        T.bmax[a] = 1;
+    }
+    const size_t next = 2 * ((size_t)T.nt[0] + T.nt[1] + T.nt[2]);
+    std::vector<float> ext(next);
+    std::vector<uint32_t> ibox(next);
+    double maxabs = 0;
+    size_t off = 0;
+    for (int a = 0; a < 3; ++a) {
+        const AxisTable& ax = j.ax[a];
+        for (uint32_t ti = 0; ti < T.nt[a]; ++ti) {
+            const uint32_t c0 = ti * kTile, c1 = std::min(c0 + kTile, ncell[a]) - 1;
+            const uint32_t i0 = ax.lo[c0], i1 = ax.hi[c1];
+            float mn = ax.coord[i0], mx = ax.coord[i0];
+            for (uint32_t i = i0; i <= i1; ++i) {
+                mn = std::fmin(mn, ax.coord[i]);
+                mx = std::fmax(mx, ax.coord[i]);
+                maxabs = std::fmax(maxabs, std::fabs((double)ax.coord[i]));
+            }
+            ext[off + ti] = mn;
+            ext[off + T.nt[a] + ti] = mx;
+            ibox[off + ti] = i0;
+            ibox[off + T.nt[a] + ti] = i1;
+            T.bmax[a] = std::max(T.bmax[a], i1 - i0 + 1);
+        }
+        off += 2 * (size_t)T.nt[a];
+    }
+    T.off_ext = put(ext.data(), next * 4);
+    T.off_ibox = put(ibox.data(), next * 4);
+    std::vector<uint32_t> fl(prog->n);
+    for (uint32_t i = 0; i < prog->n; ++i) fl[i] = op_flops(prog->ops[i]);
+    T.off_flops = put(fl.data(), fl.size() * 4);
+    // tile boxes must also hold the normal taps p +- eps of vertices in their cells and
+    // absorb the fp32 rounding of p + eps and of the vertex position
+    T.pad = 2.0 * std::fabs((double)eps) + 1e-6 * (1.0 + maxabs);
+}
+
+// Result range of one job of a sparse run, in the context's output buffers.
+struct SpRange {
+    uint64_t v0, nv, t0, nt, invalid;
+};
+
+// Mesh `n` jobs with the sparse pipeline.  n == 1 may be cut into z-slabs; a batch runs
+// as one pass with every job whole.  Capacities (slots, vertices, triangles) start from
+// estimates and grow if the device counters report more.
+int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const s2m_params* const* ps,
+               const Job* jobs, std::vector<SpRange>& out)
+{
+    out.assign(n, SpRange{});
+    std::vector<SpTables> tabs(n);
+    std::vector<size_t> tab_off(n);
+    size_t tab_bytes = 0;
+    uint32_t list_stride = 1, box_stride = 1;
+    uint64_t ntiles_all = 0, nseg_all = 0, plane_max = 0, cells_all = 0, planes_sum = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        build_sp_tables(jobs[i], progs[i], ps[i]->eps, tabs[i]);
+        tab_off[i] = tab_bytes;
+        tab_bytes += (tabs[i].bytes.size() + 255) & ~size_t(255);
+        list_stride = std::max(list_stride, progs[i]->n);
+        box_stride = std::max(box_stride, tabs[i].bmax[0] * tabs[i].bmax[1] * tabs[i].bmax[2]);
+        ntiles_all += (uint64_t)tabs[i].nt[0] * tabs[i].nt[1] * tabs[i].nt[2];
+        nseg_all += (uint64_t)jobs[i].nz_cells * jobs[i].res[1] * tabs[i].nt[0];
+        plane_max = std::max<uint64_t>(plane_max, (uint64_t)jobs[i].res[0] * jobs[i].res[1]);
+        cells_all += (uint64_t)jobs[i].res[0] * jobs[i].res[1] * jobs[i].nz_cells;
+        planes_sum += (uint64_t)jobs[i].res[0] * jobs[i].res[1];
+    }
+    if (box_stride * sizeof(float) > 40u * 1024u)
+        return fail(S2M_E_ARG, "tile box of %u points exceeds the LDS staging limit", box_stride);
+    // slabs (single job only): whole tile layers, per-pass buffers bounded to ~1 GiB
+    uint32_t layers = tabs[0].nt[2];
+    if (n == 1) {
+        if (ps[0]->slab) layers = std::max<uint32_t>(1, (ps[0]->slab + kTile - 1) / kTile);
+        layers = std::min(layers, std::max<uint32_t>(tabs[0].nt[2], 1));
+        auto pass_bytes = [&](uint64_t L) {
+            const uint64_t t = (uint64_t)tabs[0].nt[0] * tabs[0].nt[1] * L;
+            return t * (list_stride * 2 + 12) + 16ull * L * kTile * jobs[0].res[1] * tabs[0].nt[0];
+        };
+        while (layers > 1 && pass_bytes(layers) > (1ull << 30)) layers = (layers + 1) / 2;
+    }
+    const uint32_t npass = n == 1 ? (tabs[0].nt[2] + layers - 1) / std::max(layers, 1u) : 1;
+    const uint64_t tiles_pass = n == 1 ? (uint64_t)tabs[0].nt[0] * tabs[0].nt[1] * layers : ntiles_all;
+    const uint64_t seg_pass =
+        n == 1 ? (uint64_t)layers * kTile * jobs[0].res[1] * tabs[0].nt[0] : nseg_all;
+    if (tiles_pass > 0x7fffffffull || seg_pass > 0xffffffffull ||
+        (seg_pass + 4095) / 4096 > kMaxScanBlocks)
+        return fail(S2M_E_ARG, "grid too large for one pass; mesh the jobs separately");
+
+    // device tables + job descriptors (one per pass for a single job's slabs, else per job)
+    std::vector<uint8_t>& img = ctx->host_img;
+    const size_t njob_desc = n == 1 ? npass : n;
+    const size_t desc_off = (tab_bytes + 255) & ~size_t(255);
+    img.assign(desc_off + njob_desc * sizeof(SpJob), 0);
+    for (uint32_t i = 0; i < n; ++i)
+        std::memcpy(img.data() + tab_off[i], tabs[i].bytes.data(), tabs[i].bytes.size());
+    HIP_TRY(ctx->tile_tab.ensure(img.size()));
+    uint8_t* dimg = ctx->tile_tab.as<uint8_t>();
+    std::vector<SpJob> desc(njob_desc);
+    uint64_t tile_base = 0, seg_base = 0;
+    for (size_t d = 0; d < njob_desc; ++d) {
+        const uint32_t i = n == 1 ? 0 : (uint32_t)d;
+        const Job& j = jobs[i];
+        const SpTables& T = tabs[i];
+        uint8_t* base = dimg + tab_off[i];
+        SpJob J{};
+        J.rx = j.res[0];
+        J.ry = j.res[1];
+        if (n == 1) {
+            J.z0 = (uint32_t)d * layers * kTile;
+            J.nz = std::min(j.nz_cells - std::min(j.nz_cells, J.z0), layers * kTile);
+            J.tz0 = (uint32_t)d * layers;
+            J.tile_base = 0;
+            J.seg_base = 0;
+        } else {
+            J.z0 = 0;
+            J.nz = j.nz_cells;
+            J.tz0 = 0;
+            J.tile_base = tile_base;
+            J.seg_base = seg_base;
+            tile_base += (uint64_t)T.nt[0] * T.nt[1] * T.nt[2];
+            seg_base += (uint64_t)j.nz_cells * j.res[1] * T.nt[0];
+        }
+        J.ntx = T.nt[0];
+        J.nty = T.nt[1];
+        J.ntz = T.nt[2];
+        J.nops = progs[i]->n;
+        J.flags = ps[i]->flags;
+        J.bx = T.bmax[0];
+        J.by = T.bmax[1];
+        J.out_index = i;
+        J.eps = ps[i]->eps;
+        J.pad = T.pad;
+        J.ops = progs[i]->d_ops;
+        J.cx = reinterpret_cast<const float*>(base + T.off_c[0]);
+        J.cy = reinterpret_cast<const float*>(base + T.off_c[1]);
+        J.cz = reinterpret_cast<const float*>(base + T.off_c[2]);
+        J.lox = reinterpret_cast<const uint32_t*>(base + T.off_lo[0]);
+        J.hix = reinterpret_cast<const uint32_t*>(base + T.off_hi[0]);
+        J.loy = reinterpret_cast<const uint32_t*>(base + T.off_lo[1]);
+        J.hiy = reinterpret_cast<const uint32_t*>(base + T.off_hi[1]);
+        J.loz = reinterpret_cast<const uint32_t*>(base + T.off_lo[2]);
+        J.hiz = reinterpret_cast<const uint32_t*>(base + T.off_hi[2]);
+        J.ext = reinterpret_cast<const float*>(base + T.off_ext);
+        J.ibox = reinterpret_cast<const uint32_t*>(base + T.off_ibox);
+        J.op_flops = reinterpret_cast<const uint32_t*>(base + T.off_flops);
+        J.boundary = ctx->sp_boundary.as<uint32_t>();
+        desc[d] = J;
+    }
+    HIP_TRY(ctx->sp_boundary.ensure(std::max<uint64_t>(plane_max, 1) * sizeof(uint32_t)));
+    for (auto& J : desc) J.boundary = ctx->sp_boundary.as<uint32_t>();
+    std::memcpy(img.data() + desc_off, desc.data(), desc.size() * sizeof(SpJob));
+    HIP_TRY(hipMemcpyAsync(dimg, img.data(), img.size(), hipMemcpyHostToDevice, ctx->stream));
+    const SpJob* ddesc = reinterpret_cast<const SpJob*>(dimg + desc_off);
+
+    HIP_TRY(ctx->tile_lists.ensure(tiles_pass * list_stride * sizeof(uint16_t)));
+    HIP_TRY(ctx->tile_lens.ensure(tiles_pass * sizeof(uint32_t)));
+    HIP_TRY(ctx->tile_flops.ensure(tiles_pass * sizeof(uint32_t)));
+    HIP_TRY(ctx->sp_slot_of_tile.ensure(tiles_pass * sizeof(int32_t)));
+    HIP_TRY(ctx->sp_seg.ensure(std::max<uint64_t>(seg_pass, 1) * 4 * sizeof(uint32_t)));
+    const uint32_t nsb_max = (uint32_t)((seg_pass + 4095) / 4096) + 1;
+    HIP_TRY(ctx->bcount.ensure((size_t)nsb_max * sizeof(uint32_t)));
+    HIP_TRY(ctx->boffset.ensure((size_t)nsb_max * sizeof(uint32_t)));
+    HIP_TRY(ctx->counters.ensure(kNumCounters * sizeof(uint64_t)));
+    HIP_TRY(ctx->sp_job_out.ensure((size_t)n * kJobOut * sizeof(uint64_t)));
+
+    // a closed surface crosses O(res^2) cells: start from 6 per row-plane cell
+    uint64_t want_v = std::min<uint64_t>(cells_all, std::max<uint64_t>(65536, 6 * planes_sum));
+    want_v = std::max<uint64_t>(want_v, ctx->vcap);
+    uint64_t want_t = std::max<uint64_t>(3 * want_v, ctx->tcap);
+    uint64_t want_s = std::min<uint64_t>(tiles_pass, std::max<uint64_t>({4096, tiles_pass / 4, ctx->slot_cap}));
+    const int pblocks = std::max(64, ctx->cus * 4);
+
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        HIP_TRY(ctx->vtx.ensure(want_v * 6 * sizeof(float)));
+        HIP_TRY(ctx->keys.ensure(want_v * sizeof(uint64_t)));
+        HIP_TRY(ctx->signs.ensure(want_v));
+        HIP_TRY(ctx->tri.ensure(want_t * 3 * sizeof(uint32_t)));
+        HIP_TRY(ctx->sp_tile_of_slot.ensure(want_s * sizeof(uint32_t)));
+        HIP_TRY(ctx->sp_job_of_slot.ensure(want_s * sizeof(uint32_t)));
+        HIP_TRY(ctx->sp_boxes.ensure(want_s * box_stride * sizeof(float)));
+        HIP_TRY(ctx->sp_vbox.ensure(want_s * 512 * sizeof(uint32_t)));
+        ctx->vcap = std::max(ctx->vcap, want_v);
+        ctx->tcap = std::max(ctx->tcap, want_t);
+        ctx->slot_cap = std::max(ctx->slot_cap, want_s);
+
+        MeshBufs b{};
+        b.vtx = ctx->vtx.as<float>();
+        b.keys = ctx->keys.as<uint64_t>();
+        b.signs = ctx->signs.as<uint8_t>();
+        b.tri = ctx->tri.as<uint32_t>();
+        b.vcap = want_v;
+        b.tcap = want_t;
+        b.counters = ctx->counters.as<uint64_t>();
+        b.bcount = ctx->bcount.as<uint32_t>();
+        b.boffset = ctx->boffset.as<uint32_t>();
+
+        HIP_TRY(hipMemsetAsync(b.counters, 0, kNumCounters * sizeof(uint64_t), ctx->stream));
+        HIP_TRY(hipMemsetAsync(ctx->sp_job_out.p, 0, (size_t)n * kJobOut * sizeof(uint64_t), ctx->stream));
+        if (n == 1 && npass > 1)
+            HIP_TRY(hipMemsetAsync(ctx->sp_boundary.p, 0xff, plane_max * sizeof(uint32_t), ctx->stream));
+
+        ctx->events_used = 0;
+        std::vector<hipEvent_t> ev;
+        hipEvent_t ev_start = nullptr, ev_end = nullptr;
+        if (ctx->profiling) {
+            ev_start = ctx->next_event();
+            HIP_TRY(hipEventRecord(ev_start, ctx->stream));
+        }
+        uint64_t tiles_total = 0;
+        for (uint32_t pass = 0; pass < npass; ++pass) {
+            SpPass P{};
+            P.jobs = n == 1 ? ddesc + pass : ddesc;
+            P.njobs = n == 1 ? 1 : n;
+            P.list_stride = list_stride;
+            if (n == 1) {
+                const SpJob& J = desc[pass];
+                P.ntiles = (uint64_t)J.ntx * J.nty * ((J.nz + kTile - 1) / kTile);
+                P.nseg = (uint64_t)J.nz * J.ry * J.ntx;
+                P.boundary_cells = (uint64_t)J.rx * J.ry;
+            } else {
+                P.ntiles = ntiles_all;
+                P.nseg = nseg_all;
+            }
+            P.lists = ctx->tile_lists.as<uint16_t>();
+            P.lens = ctx->tile_lens.as<uint32_t>();
+            P.flops = ctx->tile_flops.as<uint32_t>();
+            P.slot_of_tile = ctx->sp_slot_of_tile.as<int32_t>();
+            P.tile_of_slot = ctx->sp_tile_of_slot.as<uint32_t>();
+            P.job_of_slot = ctx->sp_job_of_slot.as<uint32_t>();
+            P.slot_cap = (uint32_t)want_s;
+            P.box_stride = box_stride;
+            P.boxes = ctx->sp_boxes.as<float>();
+            P.vbox = ctx->sp_vbox.as<uint32_t>();
+            P.seg_v = ctx->sp_seg.as<uint32_t>();
+            P.seg_t = P.seg_v + P.nseg;
+            P.seg_off_v = P.seg_t + P.nseg;
+            P.seg_off_t = P.seg_off_v + P.nseg;
+            P.counters = b.counters;
+            P.job_out = ctx->sp_job_out.as<uint64_t>();
+            hipEvent_t m[5] = {};
+            if (ctx->profiling)
+                for (auto& e : m) e = ctx->next_event();
+            HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters));
+            const int boundary = (n == 1 && pass + 1 < npass) ? 1 : 0;
+            HIP_TRY(launch_sparse_pass(ctx->stream, P, b, pblocks, boundary, ctx->profiling ? m : nullptr));
+            if (ctx->profiling) ev.insert(ev.end(), m, m + 5);
+            tiles_total += P.ntiles;
+        }
+        HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters));  // fold the last pass
+        if (ctx->profiling) {
+            ev_end = ctx->next_event();
+            HIP_TRY(hipEventRecord(ev_end, ctx->stream));
+        }
+        uint64_t cnt[kNumCounters];
+        std::vector<uint64_t> jo((size_t)n * kJobOut);
+        HIP_TRY(hipMemcpyAsync(cnt, b.counters, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(jo.data(), ctx->sp_job_out.p, jo.size() * 8, hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        if (ctx->profiling) {
+            float ms = 0;
+            HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_end));
+            ctx->timing.total_ms = ms;
+            for (size_t q = 0; q + 4 < ev.size(); q += 5) {
+                float t0, t1, t2, t3;
+                HIP_TRY(hipEventElapsedTime(&t0, ev[q], ev[q + 1]));
+                HIP_TRY(hipEventElapsedTime(&t1, ev[q + 1], ev[q + 2]));
+                HIP_TRY(hipEventElapsedTime(&t2, ev[q + 2], ev[q + 3]));
+                HIP_TRY(hipEventElapsedTime(&t3, ev[q + 3], ev[q + 4]));
+                ctx->timing.eval_ms += t0;
+                ctx->timing.contour_ms += t1;
+                ctx->timing.normal_ms += t2;  // vertex emission + normals
+                ctx->timing.quad_ms += t3;
+            }
+        }
+        ctx->timing.eval_launches = npass;
+        ctx->timing.slabs = npass;
+        ctx->timing.lattice_points = cnt[kEvalPoints];
+        ctx->timing.tiles_total = tiles_total;
+        ctx->timing.tiles_active = cnt[kTilesActive];
+        ctx->timing.eval_flops = cnt[kEvalFlops];
+        if (cnt[kVtxTotal] >= 0xffffffffull || cnt[kTriTotal] >= 0xffffffffull)
+            return fail(S2M_E_ARG, "output of %llu vertices / %llu triangles exceeds u32 indices",
+                        (unsigned long long)cnt[kVtxTotal], (unsigned long long)cnt[kTriTotal]);
+        const bool ok_v = cnt[kVtxTotal] <= want_v, ok_t = cnt[kTriTotal] <= want_t;
+        const bool ok_s = cnt[kSlotsMax] <= want_s;
+        if (ok_v && ok_t && ok_s) {
+            for (uint32_t i = 0; i < n; ++i) {
+                SpRange& r = out[i];
+                r.v0 = jo[(size_t)i * kJobOut + 0];
+                r.t0 = jo[(size_t)i * kJobOut + 1];
+                r.invalid = jo[(size_t)i * kJobOut + 2];
+                const uint64_t v1 = (i + 1 < n) ? jo[(size_t)(i + 1) * kJobOut + 0] : cnt[kVtxTotal];
+                const uint64_t t1 = (i + 1 < n) ? jo[(size_t)(i + 1) * kJobOut + 1] : cnt[kTriTotal];
+                r.nv = v1 - r.v0;
+                r.nt = t1 - r.t0;
+            }
+            ctx->last_nvtx = cnt[kVtxTotal];
+            ctx->last_ntri = cnt[kTriTotal];
+            ctx->last_invalid = cnt[kInvalid];
+            return S2M_OK;
+        }
+        if (!ok_s) want_s = std::min<uint64_t>(tiles_pass, cnt[kSlotsMax] + cnt[kSlotsMax] / 8 + 64);
+        if (!ok_v) want_v = cnt[kVtxTotal] + cnt[kVtxTotal] / 8 + 1024;
+        if (!ok_t) want_t = cnt[kTriTotal] + cnt[kTriTotal] / 8 + 2048;
+    }
+    return fail(S2M_E_OOM, "output buffers kept overflowing");
 }
 
 // Mesh one job on the device.  corners != NULL: contour the given per-cell corner
@@ -450,8 +821,13 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             }
         }
         ctx->timing.eval_launches = h_corners ? 0 : slabs;
-        ctx->timing.lattice_points = lattice_points;
         ctx->timing.slabs = slabs;
+        ctx->timing.lattice_points = lattice_points;
+        if (prog) {
+            uint64_t f = 0;
+            for (const s2m_op& o : prog->ops) f += op_flops(o);
+            ctx->timing.eval_flops = f * lattice_points;
+        }
         if (cnt[kVtxTotal] >= 0xffffffffull)
             return fail(S2M_E_ARG, "mesh has %llu vertices; indices are u32",
                         (unsigned long long)cnt[kVtxTotal]);
@@ -554,6 +930,9 @@ s2m_ctx* s2m_create(int device)
     }
     s2m_ctx* ctx = new s2m_ctx();
     ctx->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
+        ctx->cus = prop.multiProcessorCount;
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         fail(S2M_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
@@ -569,7 +948,11 @@ void s2m_destroy(s2m_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     for (DevBuf* b : {&ctx->lattice, &ctx->vidx, &ctx->bcount, &ctx->boffset, &ctx->counters,
-                      &ctx->tables, &ctx->vtx, &ctx->keys, &ctx->signs, &ctx->tri, &ctx->corners})
+                      &ctx->tables, &ctx->vtx, &ctx->keys, &ctx->signs, &ctx->tri, &ctx->corners,
+                      &ctx->tile_tab, &ctx->tile_lists, &ctx->tile_lens,
+                      &ctx->tile_flops, &ctx->sp_slot_of_tile, &ctx->sp_tile_of_slot,
+                      &ctx->sp_job_of_slot, &ctx->sp_boxes, &ctx->sp_vbox, &ctx->sp_seg,
+                      &ctx->sp_boundary, &ctx->sp_job_out})
         b->release();
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
@@ -595,6 +978,14 @@ int s2m_tape_validate(const s2m_op* ops, uint32_t n_ops)
     return tape_ok(ops, n_ops) ? S2M_OK
                                : fail(S2M_E_VALIDATION, "invalid tape (opcode, operand count or "
                                                         "stack depth)");
+}
+
+uint64_t s2m_tape_flops(const s2m_op* ops, uint32_t n_ops)
+{
+    if (!tape_ok(ops, n_ops)) return 0;
+    uint64_t f = 0;
+    for (uint32_t i = 0; i < n_ops; ++i) f += op_flops(ops[i]);
+    return f;
 }
 
 int s2m_program_from_tape(s2m_ctx* ctx, const s2m_op* ops, uint32_t n_ops, s2m_program** out)
@@ -639,7 +1030,16 @@ int s2m_mesh_device(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* par
     Job j;
     int rc = prepare_job(params, j);
     if (rc) return rc;
-    rc = run_job(ctx, prog, params, j, nullptr);
+    j.tiled = !(params->flags & S2M_DENSE_EVAL) && tape_prunable(prog->ops);
+    ctx->timing = s2m_timing{};
+    ctx->last_nvtx = ctx->last_ntri = ctx->last_invalid = 0;
+    const uint64_t cells = (uint64_t)j.res[0] * j.res[1] * j.nz_cells;
+    if (j.tiled && cells > 0) {
+        std::vector<SpRange> r;
+        rc = run_sparse(ctx, 1, &prog, &params, &j, r);
+    } else {
+        rc = run_job(ctx, prog, params, j, nullptr);
+    }
     if (rc) return rc;
     out->vtx = ctx->vtx.as<float>();
     out->tri = ctx->tri.as<uint32_t>();
@@ -663,6 +1063,167 @@ int s2m_mesh(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, in
     rc = copy_out(ctx, want_keys, out, &d2h);
     ctx->timing.d2h_ms = d2h;
     return rc;
+}
+
+namespace {
+
+// Copy job ranges of the context's output buffers to fresh host meshes.
+int copy_ranges(s2m_ctx* ctx, const SpRange* r, uint32_t n, int want_keys, s2m_mesh_out* outs)
+{
+    for (uint32_t i = 0; i < n; ++i) {
+        s2m_mesh_out* o = &outs[i];
+        std::memset(o, 0, sizeof *o);
+        o->nvtx = r[i].nv;
+        o->ntri = r[i].nt;
+        o->invalid_quads = r[i].invalid;
+        o->vtx = (float*)std::malloc(std::max<uint64_t>(r[i].nv, 1) * 6 * sizeof(float));
+        o->tri = (uint32_t*)std::malloc(std::max<uint64_t>(r[i].nt, 1) * 3 * sizeof(uint32_t));
+        if (want_keys) {
+            o->cell_keys = (uint64_t*)std::malloc(std::max<uint64_t>(r[i].nv, 1) * sizeof(uint64_t));
+            o->signs = (uint8_t*)std::malloc(std::max<uint64_t>(r[i].nv, 1));
+        }
+        if (!o->vtx || !o->tri || (want_keys && (!o->cell_keys || !o->signs)))
+            return fail(S2M_E_OOM, "host allocation failed");
+        if (r[i].nv) {
+            HIP_TRY(hipMemcpyAsync(o->vtx, ctx->vtx.as<float>() + 6 * r[i].v0, r[i].nv * 24,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            if (want_keys) {
+                HIP_TRY(hipMemcpyAsync(o->cell_keys, ctx->keys.as<uint64_t>() + r[i].v0, r[i].nv * 8,
+                                       hipMemcpyDeviceToHost, ctx->stream));
+                HIP_TRY(hipMemcpyAsync(o->signs, ctx->signs.as<uint8_t>() + r[i].v0, r[i].nv,
+                                       hipMemcpyDeviceToHost, ctx->stream));
+            }
+        }
+        if (r[i].nt)
+            HIP_TRY(hipMemcpyAsync(o->tri, ctx->tri.as<uint32_t>() + 3 * r[i].t0, r[i].nt * 12,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return S2M_OK;
+}
+
+int prepare_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const s2m_params* params,
+                  std::vector<Job>& jobs)
+{
+    if (!ctx || (n && (!progs || !params))) return fail(S2M_E_ARG, "NULL argument");
+    jobs.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!progs[i]) return fail(S2M_E_ARG, "program %u is NULL", i);
+        if (progs[i]->device != ctx->device) return fail(S2M_E_ARG, "program built for another device");
+        int rc = prepare_job(&params[i], jobs[i]);
+        if (rc) return rc;
+    }
+    return S2M_OK;
+}
+
+}  // namespace
+
+int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
+                   const s2m_params* params, int want_keys, s2m_mesh_out* outs)
+{
+    if (!outs && n) return fail(S2M_E_ARG, "outs is NULL");
+    for (uint32_t i = 0; i < n; ++i) std::memset(&outs[i], 0, sizeof outs[i]);
+    std::vector<Job> jobs;
+    int rc = prepare_batch(ctx, n, progs, params, jobs);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    ctx->timing = s2m_timing{};
+    // consecutive passes of at most ~64M cells and 2M tiles (empty grids cost nothing)
+    uint32_t i = 0;
+    s2m_timing sum{};
+    while (i < n) {
+        uint32_t k = i;
+        uint64_t cells = 0, tiles = 0;
+        std::vector<const s2m_program*> pp;
+        std::vector<const s2m_params*> qq;
+        std::vector<Job> jj;
+        std::vector<uint32_t> idx;
+        while (k < n) {
+            const Job& j = jobs[k];
+            const uint64_t c = (uint64_t)j.res[0] * j.res[1] * j.nz_cells;
+            const uint64_t t = ((j.res[0] + 7) / 8ull) * ((j.res[1] + 7) / 8ull) * ((j.nz_cells + 7) / 8ull);
+            if (!idx.empty() && (cells + c > (64ull << 20) || tiles + t > (2ull << 20))) break;
+            if (c) {
+                pp.push_back(progs[k]);
+                qq.push_back(&params[k]);
+                jj.push_back(j);
+                idx.push_back(k);
+            }
+            cells += c;
+            tiles += t;
+            ++k;
+        }
+        if (!idx.empty()) {
+            std::vector<SpRange> r;
+            rc = run_sparse(ctx, (uint32_t)idx.size(), pp.data(), qq.data(), jj.data(), r);
+            if (rc) return rc;
+            std::vector<s2m_mesh_out> tmp(idx.size());
+            rc = copy_ranges(ctx, r.data(), (uint32_t)idx.size(), want_keys, tmp.data());
+            for (size_t q = 0; q < idx.size(); ++q) outs[idx[q]] = tmp[q];
+            if (rc) return rc;
+            sum.total_ms += ctx->timing.total_ms;
+            sum.eval_ms += ctx->timing.eval_ms;
+            sum.contour_ms += ctx->timing.contour_ms;
+            sum.normal_ms += ctx->timing.normal_ms;
+            sum.quad_ms += ctx->timing.quad_ms;
+            sum.eval_launches += ctx->timing.eval_launches;
+            sum.lattice_points += ctx->timing.lattice_points;
+            sum.slabs += ctx->timing.slabs;
+            sum.tiles_total += ctx->timing.tiles_total;
+            sum.tiles_active += ctx->timing.tiles_active;
+            sum.eval_flops += ctx->timing.eval_flops;
+        }
+        for (uint32_t q = i; q < k; ++q)  // empty grids: empty meshes
+            if (!outs[q].vtx) {
+                outs[q].vtx = (float*)std::malloc(6 * sizeof(float));
+                outs[q].tri = (uint32_t*)std::malloc(3 * sizeof(uint32_t));
+                if (want_keys) {
+                    outs[q].cell_keys = (uint64_t*)std::malloc(sizeof(uint64_t));
+                    outs[q].signs = (uint8_t*)std::malloc(1);
+                }
+            }
+        i = k;
+    }
+    ctx->timing = sum;
+    return S2M_OK;
+}
+
+int s2m_mesh_batch_device(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
+                          const s2m_params* params, s2m_device_mesh* outs)
+{
+    if (!outs && n) return fail(S2M_E_ARG, "outs is NULL");
+    std::vector<Job> jobs;
+    int rc = prepare_batch(ctx, n, progs, params, jobs);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(ctx->device));
+    ctx->timing = s2m_timing{};
+    std::vector<const s2m_program*> pp;
+    std::vector<const s2m_params*> qq;
+    std::vector<Job> jj;
+    std::vector<uint32_t> idx;
+    for (uint32_t i = 0; i < n; ++i) {
+        std::memset(&outs[i], 0, sizeof outs[i]);
+        if ((uint64_t)jobs[i].res[0] * jobs[i].res[1] * jobs[i].nz_cells == 0) continue;
+        pp.push_back(progs[i]);
+        qq.push_back(&params[i]);
+        jj.push_back(jobs[i]);
+        idx.push_back(i);
+    }
+    if (idx.empty()) return S2M_OK;
+    std::vector<SpRange> r;
+    rc = run_sparse(ctx, (uint32_t)idx.size(), pp.data(), qq.data(), jj.data(), r);
+    if (rc) return rc;
+    for (size_t q = 0; q < idx.size(); ++q) {
+        s2m_device_mesh& o = outs[idx[q]];
+        o.vtx = ctx->vtx.as<float>() + 6 * r[q].v0;
+        o.tri = ctx->tri.as<uint32_t>() + 3 * r[q].t0;
+        o.cell_keys = ctx->keys.as<uint64_t>() + r[q].v0;
+        o.signs = ctx->signs.as<uint8_t>() + r[q].v0;
+        o.nvtx = r[q].nv;
+        o.ntri = r[q].nt;
+        o.invalid_quads = r[q].invalid;
+    }
+    return S2M_OK;
 }
 
 void s2m_mesh_free(s2m_mesh_out* out)

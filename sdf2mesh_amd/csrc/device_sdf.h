@@ -172,4 +172,28 @@ __device__ __forceinline__ float tape_eval(const s2m_op* __restrict__ ops, uint3
     return acc;
 }
 
+// The same interpreter over a pruned tape: `list` holds the indices of the ops to run,
+// in order (a valid postfix program whose value equals the full tape's on its box).
+__device__ __forceinline__ float tape_eval_list(const s2m_op* __restrict__ ops,
+                                                const uint16_t* __restrict__ list, uint32_t n,
+                                                float px, float py, float pz)
+{
+    float st[S2M_MAX_STACK];
+    float acc = 0.0f;
+    uint32_t sp = 0;
+    for (uint32_t k = 0; k < n; ++k) {
+        const s2m_op* o = ops + list[k];
+        const uint32_t code = o->code;
+        if (code < S2M_OP_UNION) {
+            const float d = eval_primitive(code, o->iarg, o->f, px, py, pz);
+            if (k != 0) st[sp++] = acc;
+            acc = d;
+        } else {
+            const float d1 = st[--sp];
+            acc = apply_op(code, d1, acc, o->f[0]);
+        }
+    }
+    return acc;
+}
+
 }  // namespace s2m

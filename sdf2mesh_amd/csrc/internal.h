@@ -29,7 +29,12 @@ enum Counter : uint32_t {
     kTriTotal = 2,   // triangles emitted so far
     kTriPrev = 3,
     kInvalid = 4,    // quads skipped for a missing vertex
-    kNumCounters = 8
+    kTilesActive = 5,  // tiles whose cells were evaluated (tiled path)
+    kEvalPoints = 6,   // lattice points evaluated (tiled path)
+    kEvalFlops = 7,    // algorithmic flops executed by the pruned tapes (tiled path)
+    kSlotsUsed = 8,    // active tiles of the current slab (sparse path)
+    kSlotsMax = 9,     // largest kSlotsUsed of the job so far
+    kNumCounters = 16
 };
 
 // Everything the cell kernels need to address one slab of cells [z0, z0+nz).
@@ -56,6 +61,65 @@ struct SlabArgs {
     uint32_t flags;        // S2M_* flags
 };
 
+constexpr uint32_t kTile = 8;  // cells per tile edge (sparse pipeline)
+
+// Tile-sparse pipeline (sparse.hip).  One job of a pass: a slab of a single job, or a
+// whole job of a batch.  Pointers are device addresses.
+constexpr uint32_t kJobOut = 4;  // per job: first vertex, first triangle, invalid quads, spare
+
+struct SpJob {
+    uint32_t rx, ry;           // cells per row / rows per plane
+    uint32_t nz, z0;           // cell planes of this pass
+    uint32_t ntx, nty, ntz;    // tiles of the whole job per axis
+    uint32_t tz0;              // tile layer of plane z0
+    uint32_t nops, flags;
+    uint32_t bx, by;           // lattice box dims of the job's tiles (slot box indexing)
+    uint32_t out_index;        // row of the pass's job_out table
+    float eps;
+    double pad;                // tile box padding (normal taps, rounding)
+    uint64_t tile_base;        // first tile of the job in the pass's numbering
+    uint64_t seg_base;         // first row segment of the job
+    const s2m_op* ops;
+    const float* cx;
+    const float* cy;
+    const float* cz;
+    const uint32_t* lox;
+    const uint32_t* hix;
+    const uint32_t* loy;
+    const uint32_t* hiy;
+    const uint32_t* loz;
+    const uint32_t* hiz;
+    const float* ext;          // tile coordinate extents: xmin[ntx] xmax[ntx] ymin.. zmax[ntz]
+    const uint32_t* ibox;      // tile lattice index boxes, same layout
+    const uint32_t* op_flops;
+    uint32_t* boundary;        // [ry][rx] vertex of cell plane z0 - 1 (multi-slab job)
+};
+
+struct SpPass {
+    const SpJob* jobs;
+    uint32_t njobs;
+    uint32_t list_stride;      // >= nops of every job
+    uint64_t ntiles;           // tiles of the pass
+    uint64_t nseg;             // row segments of the pass
+    uint64_t boundary_cells;   // rx * ry of jobs[0] when a boundary plane is written
+    uint16_t* lists;           // [tile][list_stride] pruned op indices
+    uint32_t* lens;            // [tile]
+    uint32_t* flops;           // [tile] flops per point of the pruned tape
+    int32_t* slot_of_tile;     // [tile], -1: no active cell
+    uint32_t* tile_of_slot;    // [slot]
+    uint32_t* job_of_slot;     // [slot]
+    uint32_t slot_cap;
+    uint32_t box_stride;       // floats per slot box
+    float* boxes;              // [slot][box_stride]
+    uint32_t* vbox;            // [slot][512] vertex of each cell of the tile, kNone if none
+    uint32_t* seg_v;           // [nseg] vertices per row segment
+    uint32_t* seg_t;           // triangles per row segment
+    uint32_t* seg_off_v;       // exclusive offsets: global vertex index
+    uint32_t* seg_off_t;       // exclusive offsets: global triangle index
+    uint64_t* counters;
+    uint64_t* job_out;         // [job][kJobOut]
+};
+
 struct MeshBufs {
     float* vtx;            // 6 floats per vertex
     uint64_t* keys;
@@ -77,6 +141,9 @@ hipError_t launch_scan(hipStream_t s, const MeshBufs& b, uint32_t nblocks, uint3
 hipError_t launch_normals(hipStream_t s, const s2m_op* ops, uint32_t nops, float eps,
                           const MeshBufs& b);
 hipError_t launch_quads(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int emit);
+hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters);
+hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b, int persistent_blocks,
+                              int boundary_plane, hipEvent_t* marks);
 // Contour corner values given per cell (test hook): cells of the slab read their 8
 // corners from `corners` (8 floats per cell, cells in (z,y,x) order from plane z0).
 hipError_t launch_cells_from_corners(hipStream_t s, const SlabArgs& a, const float* corners,

@@ -86,6 +86,11 @@ __constant__ Edge kEdges[12] = {
 __device__ __forceinline__ bool contour_cell(const float c[8], const float lo[3], const float hi[3],
                                              float pos[3])
 {
+    // no edge changes sign when all eight corners fall on one side of `> 0`
+    uint32_t pos_mask = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) pos_mask |= (uint32_t)(c[k] > 0.0f) << k;
+    if (pos_mask == 0u || pos_mask == 0xffu) return false;
     float ax = 0.0f, ay = 0.0f, az = 0.0f, cnt = 0.0f;
 #pragma unroll
     for (int e = 0; e < 12; ++e) {
@@ -254,6 +259,18 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ bcount, ui
 // (sdf3d_normal.wgsl:4-9, dualcontour.wgsl:170-171).  Grid-stride over the range
 // [prev, total) read from the device counters, so no host round trip is needed.
 // ---------------------------------------------------------------------------------
+__device__ __forceinline__ void write_normal(float* o, float d1, float d2, float d3, float d4)
+{
+    // v1*d1 + v2*d2 + v3*d3 + v4*d4, summed left to right
+    const float gx = ((d1 + (-d2)) + (-d3)) + d4;
+    const float gy = (((-d1) + (-d2)) + d3) + d4;
+    const float gz = (((-d1) + d2) + (-d3)) + d4;
+    const float len = sqrtf((gx * gx + gy * gy) + gz * gz);
+    o[3] = gx / len;
+    o[4] = gy / len;
+    o[5] = gz / len;
+}
+
 __global__ __launch_bounds__(256) void k_normals(const s2m_op* __restrict__ ops, uint32_t nops,
                                                  float eps, float* __restrict__ vtx,
                                                  const uint64_t* __restrict__ counters, uint64_t vcap)
@@ -269,14 +286,7 @@ __global__ __launch_bounds__(256) void k_normals(const s2m_op* __restrict__ ops,
         const float d2 = tape_eval(ops, nops, px + (-eps), py + (-eps), pz + eps);
         const float d3 = tape_eval(ops, nops, px + (-eps), py + eps, pz + (-eps));
         const float d4 = tape_eval(ops, nops, px + eps, py + eps, pz + eps);
-        // v1*d1 + v2*d2 + v3*d3 + v4*d4, summed left to right
-        const float gx = ((d1 + (-d2)) + (-d3)) + d4;
-        const float gy = (((-d1) + (-d2)) + d3) + d4;
-        const float gz = (((-d1) + d2) + (-d3)) + d4;
-        const float len = sqrtf((gx * gx + gy * gy) + gz * gz);
-        o[3] = gx / len;
-        o[4] = gy / len;
-        o[5] = gz / len;
+        write_normal(o, d1, d2, d3, d4);
     }
 }
 

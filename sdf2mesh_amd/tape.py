@@ -85,11 +85,24 @@ class Node:
         return t
 
     def sdf3d_source(self) -> str:
-        """The program as an .sdf3d file for the reference CLI."""
-        return ("use sdf3d::*;\nuse sdf::op;\n\n"
-                "fn sdf3d(p: vec3f) -> f32 {\n"
-                f"    return {self.wgsl()};\n"
-                "}\n")
+        """The program as an .sdf3d file for the reference CLI.  The left spine of
+        operators (a left fold such as CSG-64) becomes a `var d` update chain, which
+        evaluates the same operations in the same order as the nested expression."""
+        spine = []
+        n = self
+        while isinstance(n, Op):
+            spine.append(n)
+            n = n.a
+        lines = ["use sdf3d::*;", "use sdf::op;", "", "fn sdf3d(p: vec3f) -> f32 {"]
+        if len(spine) < 2:
+            lines.append(f"    return {self.wgsl()};")
+        else:
+            lines.append(f"    var d = {n.wgsl()};")
+            for op in reversed(spine):
+                lines.append(f"    d = {op.wgsl_with_left('d')};")
+            lines.append("    return d;")
+        lines.append("}")
+        return "\n".join(lines) + "\n"
 
 
 @dataclass
@@ -162,7 +175,10 @@ class Op(Node):
         out.append((self.code, 0, [self.k]))
 
     def wgsl(self):
-        A, B = self.a.wgsl(), self.b.wgsl()
+        return self.wgsl_with_left(self.a.wgsl())
+
+    def wgsl_with_left(self, A: str) -> str:
+        B = self.b.wgsl()
         if self.code == UNION:
             return f"min({A}, {B})"
         if self.code == INTERSECTION:

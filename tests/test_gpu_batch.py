@@ -1,0 +1,61 @@
+"""Batched meshing (s2m_mesh_batch): every job of a batch must equal the same job meshed
+alone (which the parity tests tie to the oracle), bit for bit."""
+import numpy as np
+import pytest
+
+import sdf2mesh_amd as s2m
+from sdf2mesh_amd import tape
+from sdf2mesh_amd.workloads import batch_jobs
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a, b):
+    assert len(a.vertices) == len(b.vertices)
+    assert np.array_equal(a.cell_keys, b.cell_keys)
+    assert np.array_equal(a.signs, b.signs)
+    assert np.array_equal(a.vertices.view(np.uint32), b.vertices.view(np.uint32))
+    assert np.array_equal(a.triangles, b.triangles)
+    assert a.invalid_quads == b.invalid_quads
+
+
+def test_batch_mixed_equals_solo(ctx):
+    nodes = [tape.random_csg(s) for s in range(1, 13)]
+    nodes += [tape.torus_example(), tape.csg64(), tape.mandelbulb(5)]
+    res = [32, 40, 64, 24, 96, 17, 33, 128, 48, 64, 8, 1, 64, 80, 40]
+    flags = [s2m.REFERENCE_COMPAT, 0, s2m.COMPAT_DROP_LAST_SLICE, s2m.CORNERS_EXACT] * 4
+    progs = [ctx.program(n) for n in nodes]
+    params = [s2m.Params.reference(r, 2.5 if n.tape()[0]["code"] == tape.MANDELBULB else 2.0,
+                                   flags=f) for n, r, f in zip(nodes, res, flags)]
+    params.append(s2m.Params.reference(0, 2.0))  # empty grid
+    progs.append(progs[0])
+    print(f"batch of {len(progs)}", flush=True)
+    batch = ctx.mesh_batch(progs, params)
+    for i, (p, q) in enumerate(zip(progs, params)):
+        solo = ctx.mesh(p, q)
+        print(f"job {i}: {len(solo.vertices)} vertices", flush=True)
+        same(batch[i], solo)
+
+
+def test_batch_config4_equals_solo(ctx):
+    # the batching workload: 256 seeded random CSG trees, resolutions from
+    # {32,64,96,128,192,256} with weights {4,4,3,3,2,1}, rounded as the reference does
+    nodes, params = batch_jobs(256)
+    progs = [ctx.program(n) for n in nodes]
+    batch = ctx.mesh_batch(progs, params)
+    dev = ctx.mesh_batch_device(progs, params)
+    for i in range(0, 256, 5):
+        solo = ctx.mesh(progs[i], params[i])
+        same(batch[i], solo)
+        assert dev[i]["nvtx"] == len(solo.vertices) and dev[i]["ntri"] == len(solo.triangles)
+    print("config-4 batch: %d vertices, %d triangles" % (
+        sum(len(m.vertices) for m in batch), sum(len(m.triangles) for m in batch)), flush=True)
+
+
+def test_batch_repeatable(ctx):
+    nodes, params = batch_jobs(32, seed=7)
+    progs = [ctx.program(n) for n in nodes]
+    a = ctx.mesh_batch(progs, params)
+    b = ctx.mesh_batch(progs[::-1], params[::-1])[::-1]  # order of jobs does not matter
+    for x, y in zip(a, b):
+        same(x, y)
