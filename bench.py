@@ -133,6 +133,91 @@ def run_batch(args) -> dict:
     }
 
 
+def run_serve(args) -> dict:
+    """Serving workload: Poisson request streams (rates 10-2000 jobs/s) of the seeded
+    request mix (resolution log-uniform in [32, 512], 80 % random CSG, 20 % Mandelbulb
+    with 4-8 iterations) through the continuous-batching server.  The comparison is the
+    plain GPU path serving the same stream one request at a time (FIFO): its latencies
+    are replayed from each request's measured service time on one context."""
+    import sdf2mesh_amd as s2m
+    from sdf2mesh_amd.workloads import poisson_arrivals, serve_jobs
+
+    n = args.serve_requests
+    nodes, params = serve_jobs(n)
+    tapes = [x.tape() for x in nodes]
+    # one at a time: measured service time of every request (mesh to host memory)
+    ctx = s2m.Context(0)
+    progs = [ctx.program(t) for t in tapes]
+    for q, p in zip(progs[:8], params[:8]):
+        ctx.mesh(q, p)
+    service = []
+    for q, p in zip(progs, params):
+        t0 = time.perf_counter()
+        ctx.mesh(q, p)
+        service.append(time.perf_counter() - t0)
+    for q in progs:
+        q.close()
+    ctx.close()
+    service = np.asarray(service)
+    solo_jobs_s = n / service.sum()
+
+    rates = [float(r) for r in args.serve_rates.split(",")]
+    per_rate = []
+    with s2m.Server(0, workers=args.serve_workers, max_batch=args.serve_batch) as srv:
+        for t, p in zip(tapes[:8], params[:8]):  # warm every worker's buffers
+            srv.take(srv.submit(t, p))
+        for rate in rates:
+            arr = np.asarray(poisson_arrivals(n, rate, seed=int(rate)))
+            tickets = []
+            t0 = time.perf_counter()
+            for a, t, p in zip(arr, tapes, params):
+                now = time.perf_counter() - t0
+                if a > now:
+                    time.sleep(a - now)
+                tickets.append(srv.submit(t, p))
+            res = [srv.take(k) for k in tickets]
+            elapsed = time.perf_counter() - t0
+            lat = np.asarray([st["latency_ms"] for _, st in res])
+            bs = np.asarray([st["batch_size"] for _, st in res])
+            # FIFO replay of the one-at-a-time server on the same arrivals
+            done, solo_lat = 0.0, []
+            for a, sv in zip(arr, service):
+                done = max(done, a) + sv
+                solo_lat.append((done - a) * 1e3)
+            solo_lat = np.asarray(solo_lat)
+            per_rate.append({
+                "rate": rate, "jobs_per_s": round(n / elapsed, 1),
+                "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                "mean_batch": round(float(bs.mean()), 2),
+                "one_at_a_time_jobs_per_s": round(n / max(done, arr[-1]), 1),
+                "one_at_a_time_p50_ms": round(float(np.percentile(solo_lat, 50)), 2),
+                "one_at_a_time_p99_ms": round(float(np.percentile(solo_lat, 99)), 2),
+            })
+    top = per_rate[-1]
+    return {
+        "metric": "served jobs/s at the highest offered rate, with p50/p99 latency",
+        "value": top["jobs_per_s"],
+        "unit": "jobs/s",
+        "n_gpus": 1,
+        "steps": n,
+        "warmup": 8,
+        "ms_per_step": round(1e3 / top["jobs_per_s"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded request stream)",
+        "config": {"workload": "serve", "requests_per_rate": n, "workers": args.serve_workers,
+                   "max_batch": args.serve_batch},
+        "p50_ms": top["p50_ms"],
+        "p99_ms": top["p99_ms"],
+        "one_at_a_time_jobs_per_s": round(solo_jobs_s, 1),
+        "speedup_vs_one_at_a_time": round(top["jobs_per_s"] / solo_jobs_s, 2),
+        "per_rate": per_rate,
+    }
+
+
 def run(args) -> dict:
     import sdf2mesh_amd as s2m
 
@@ -218,6 +303,10 @@ def main() -> int:
     ap.add_argument("--workload", default="csg64-512")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rates", default="10,30,100,300,1000,2000")
+    ap.add_argument("--serve-workers", type=int, default=2)
+    ap.add_argument("--serve-batch", type=int, default=64)
     ap.add_argument("--dense", action="store_true",
                     help="evaluate every lattice point with the full program (no tile pruning)")
     args = ap.parse_args()
@@ -225,7 +314,13 @@ def main() -> int:
         print(json.dumps({"error": "this build runs on one GPU; jobs shard by job or by z-slab "
                                    "across GPUs only by design, --gpus must be 1"}))
         return 2
-    print(json.dumps(run_batch(args) if args.workload == "batch256" else run(args)), flush=True)
+    if args.workload == "batch256":
+        out = run_batch(args)
+    elif args.workload == "serve":
+        out = run_serve(args)
+    else:
+        out = run(args)
+    print(json.dumps(out), flush=True)
     return 0
 
 

@@ -15,17 +15,24 @@
  *                                  (shader.rs:44-67,220-225) for programs built from
  *                                  the library primitives and operators
  *                                  (sdf3d_primitives.wgsl, sdf_op.wgsl)
+ *   s2m_program_from_source        Sdf3DShader::from_path / from_glsl_fragment_shader
+ *                                  + create_shader_module (shader.rs:44-104,220-225)
+ *   s2m_translate                  Sdf3DShader::write_to_file for --debug-wgsl
+ *                                  (shader.rs:206-209, main.rs:223-225)
  *   s2m_mesh                       the per-slice loop and host scan, main.rs:294-357,
  *                                  plus TriangleMesh::from(VertexList), mesh.rs:334-341
  *   s2m_write_stl / s2m_write_ply  TriangleMesh::write_to_file, mesh.rs:182-210
  *   s2m_params                     AppState + From<&Arguments>, main.rs:28-33,139-175
  *   s2m_mesh_batch(_device)        no reference counterpart: many jobs per device pass
+ *   s2m_server_*                   no reference counterpart: continuous-batching request loop
  */
 #ifndef SDF2MESH_H
 #define SDF2MESH_H
 
+#ifndef __HIPCC_RTC__
 #include <stddef.h>
 #include <stdint.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -189,6 +196,53 @@ int s2m_last_timing(const s2m_ctx* ctx, s2m_timing* out);
 /* Validate a tape (stack discipline, opcodes, sizes) and upload it to the device. */
 int s2m_program_from_tape(s2m_ctx* ctx, const s2m_op* ops, uint32_t n_ops, s2m_program** out);
 void s2m_program_free(s2m_program* prog);
+
+/* ------------------------------------------------------------------------------------
+ * Programs from SDF source (Sdf3DShader::from_path, shader.rs:44-67, and
+ * Sdf3DShader::from_glsl_fragment_shader, shader.rs:73-104; main.rs:198-227).
+ *   S2M_LANG_SDF3D  WGSL text whose `use <module>;` lines import the library modules
+ *                   (sdf3d::*, sdf3d::primitives, sdf3d::normal, sdf::op, sdf::*) and whose
+ *                   `include "file";` lines splice files in (shader.rs:159-203); it defines
+ *                   `fn sdf3d(p: vec3f) -> f32`.
+ *   S2M_LANG_GLSL   a GLSL fragment shader with `void main()` and `float <sdf_name>(vec3)`
+ *                   (sdf_name NULL = "sdf", the --glsl-sdf default, main.rs:218); a
+ *                   function named sdf3d, if present, is the SDF (shader.rs:88-95).
+ * A program built from library primitives at p - c and library operators with constant
+ * operands becomes a tape (pruned sparse path, batchable); any other program is compiled
+ * for gfx950 at run time with hiprtc (dense path).  S2M_FORCE_COMPILED compiles every
+ * program.  Errors: S2M_E_IO (unreadable file), S2M_E_PARSE, S2M_E_VALIDATION (names,
+ * types, recursion, sdf3d_normal not visible), S2M_E_MISSING_SDF (GLSL sdf function
+ * absent), S2M_E_SHADER (GLSL without main), S2M_E_WGSL (code generation), S2M_E_HIP.
+ * ---------------------------------------------------------------------------------- */
+#define S2M_LANG_SDF3D 0u
+#define S2M_LANG_GLSL 1u
+#define S2M_SOURCE_IS_PATH 0x100u /* `source` is a file path, not the text            */
+#define S2M_FORCE_COMPILED 0x200u /* compile even a program that lowers to a tape      */
+
+int s2m_program_from_source(s2m_ctx* ctx, const char* source, uint32_t lang_flags,
+                            const char* sdf_name, s2m_program** out);
+
+#define S2M_PROGRAM_TAPE 0
+#define S2M_PROGRAM_COMPILED 1
+/* S2M_PROGRAM_TAPE or S2M_PROGRAM_COMPILED; -1 for NULL. */
+int s2m_program_kind(const s2m_program* prog);
+/* Length of a tape program's tape (0 for a compiled program); copies min(length, cap)
+ * ops to `ops` when it is not NULL. */
+uint32_t s2m_program_tape(const s2m_program* prog, s2m_op* ops, uint32_t cap);
+
+/* Front-end stages without a device: writes min(size, cap) bytes of the artifact to buf
+ * and returns its size, or 0 with *status (may be NULL) and s2m_last_error() set.
+ *   S2M_STAGE_SOURCE  the source after `use` / `include` expansion, before validation
+ *                     (the --debug-wgsl output, main.rs:223-225)
+ *   S2M_STAGE_TAPE    the tape (s2m_op records); S2M_E_WGSL if the program is not one
+ *   S2M_STAGE_HIP     the generated HIP C++
+ *   S2M_STAGE_CODE    the gfx950 code object hiprtc builds from it */
+#define S2M_STAGE_SOURCE 0u
+#define S2M_STAGE_TAPE 1u
+#define S2M_STAGE_HIP 2u
+#define S2M_STAGE_CODE 3u
+uint64_t s2m_translate(const char* source, uint32_t lang_flags, const char* sdf_name,
+                       uint32_t stage, void* buf, uint64_t cap, int* status);
 /* Host-side tape validation only (no device needed): returns S2M_OK or S2M_E_VALIDATION. */
 int s2m_tape_validate(const s2m_op* ops, uint32_t n_ops);
 /* Algorithmic fp32 operation count of one evaluation of the tape: + - * / min max abs
@@ -212,13 +266,54 @@ void s2m_mesh_free(s2m_mesh_out* out);
  * device instead of each paying its own launch and synchronisation latency.  Each
  * job's result is identical to meshing it alone with s2m_mesh.  outs[i] receives job
  * i's mesh (free each with s2m_mesh_free).  Batches too large for one pass are split
- * into consecutive passes. */
+ * into consecutive passes.  Compiled programs (no tape) are meshed one at a time. */
 int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
                    const s2m_params* params, int want_keys, s2m_mesh_out* outs);
 /* One pass only, results left in the context's device buffers (valid until its next
- * meshing call): outs[i] points at job i's vertices and triangles. */
+ * meshing call): outs[i] points at job i's vertices and triangles.  Tape programs only
+ * (S2M_E_ARG for a compiled one). */
 int s2m_mesh_batch_device(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
                           const s2m_params* params, s2m_device_mesh* outs);
+
+/* ---- serving ------------------------------------------------------------------- */
+/* A request loop with continuous batching.  `workers` threads each own a context (HIP
+ * stream + resident device buffers) on `device`.  Submitted requests wait in a FIFO
+ * queue; an idle worker admits requests from the head of the queue while their
+ * estimated device memory fits in what is left of `hbm_budget` (and at most
+ * `max_batch` of them), meshes them in one batched pass, then returns their memory to
+ * the budget.  Requests arriving while others are on the device are admitted as soon as
+ * a worker and memory free up, and passes of different workers overlap on the GPU.
+ * Every result is identical to meshing the request alone with s2m_mesh. */
+typedef struct s2m_server s2m_server;
+
+typedef struct s2m_server_config {
+    int device;
+    uint64_t hbm_budget; /* bytes of device memory admitted requests may use (0: 16 GiB) */
+    uint32_t max_batch;  /* requests per pass (0: 64)                                  */
+    uint32_t workers;    /* concurrent passes, one HIP stream each (0: 2)              */
+    int want_keys;       /* also return cell keys and sign bits                         */
+} s2m_server_config;
+
+typedef struct s2m_request_stats {
+    double queue_ms;     /* submit -> admitted to a pass                                */
+    double latency_ms;   /* submit -> result ready                                       */
+    uint32_t batch_size; /* requests in the pass that meshed it                          */
+    uint32_t worker;
+} s2m_request_stats;
+
+s2m_server* s2m_server_create(const s2m_server_config* cfg);
+/* Queue a request (the tape is copied).  Fails with S2M_E_BUDGET when its estimated
+ * memory exceeds the whole budget, S2M_E_VALIDATION for an invalid tape. */
+int s2m_server_submit(s2m_server* srv, const s2m_op* ops, uint32_t n_ops, const s2m_params* params,
+                      uint64_t* ticket);
+/* Take a finished request's mesh: S2M_E_BUSY while it is queued or running; with
+ * wait != 0 blocks until it is done.  Each ticket can be taken once. */
+int s2m_server_take(s2m_server* srv, uint64_t ticket, int wait, s2m_mesh_out* out,
+                    s2m_request_stats* stats);
+/* Estimated device bytes of one request (the admission measure). */
+uint64_t s2m_server_request_bytes(const s2m_op* ops, uint32_t n_ops, const s2m_params* params);
+/* Finish queued work, stop the workers and free everything. */
+void s2m_server_destroy(s2m_server* srv);
 
 /* ---- output --------------------------------------------------------------------- */
 /* ASCII STL exactly as STLWriter (mesh.rs:8-48,155-180), floats in Rust `{}` form. */

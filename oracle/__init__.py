@@ -20,6 +20,20 @@ LIB_PATH = HERE / "librefcpu.so"
 # s2m_op layout (include/sdf2mesh.h)
 TAPE_DTYPE = np.dtype([("code", "<u4"), ("iarg", "<u4"), ("f", "<f4", (10,))])
 
+# Oracle-only primitive codes for whole example programs (examples.inc): a one-op tape
+# evaluates examples/martin_cube.sdf3d, examples/p_key.sdf3d (the reference's examples)
+# or tests/golden/engraved_cube.sdf3d (this repository's compiled-path test program).
+ORA_OP_MARTIN_CUBE = 100
+ORA_OP_P_KEY = 101
+ORA_OP_ENGRAVED_CUBE = 102
+
+
+def example_tape(code: int) -> np.ndarray:
+    t = np.zeros(1, TAPE_DTYPE)
+    t[0]["code"] = code
+    return t
+
+
 _h: Optional[C.CDLL] = None
 
 

@@ -59,6 +59,12 @@
 
 #define ORA_API __attribute__((visibility("default")))
 
+/* Oracle-only primitive codes: the reference's example programs (examples.inc), the
+ * checker for programs the front end compiles to native code. */
+#define ORA_OP_MARTIN_CUBE 100u
+#define ORA_OP_P_KEY 101u
+#define ORA_OP_ENGRAVED_CUBE 102u
+
 /* ---- SDF evaluators ------------------------------------------------------------- */
 #define T float
 #define SFX(n) n##_f32
@@ -107,7 +113,8 @@ ORA_API int ora_tape_check(const s2m_op* ops, uint32_t n)
     for (uint32_t i = 0; i < n; ++i) {
         switch (ops[i].code) {
         case S2M_OP_SPHERE: case S2M_OP_BOX: case S2M_OP_TORUS: case S2M_OP_CYLINDER:
-        case S2M_OP_CAPSULE: case S2M_OP_MANDELBULB:
+        case S2M_OP_CAPSULE: case S2M_OP_MANDELBULB: case ORA_OP_MARTIN_CUBE: case ORA_OP_P_KEY:
+        case ORA_OP_ENGRAVED_CUBE:
             if (++sp > (int)S2M_MAX_STACK) return 0;
             break;
         case S2M_OP_UNION: case S2M_OP_INTERSECTION: case S2M_OP_SUBTRACTION:

@@ -44,6 +44,18 @@ NO_NORMALS = 0x8
 DENSE_EVAL = 0x10
 REFERENCE_COMPAT = COMPAT_POW2 | COMPAT_DROP_LAST_SLICE | CORNERS_EXACT
 
+# source programs (s2m_program_from_source / s2m_translate)
+LANG_SDF3D = 0
+LANG_GLSL = 1
+SOURCE_IS_PATH = 0x100
+FORCE_COMPILED = 0x200
+PROGRAM_TAPE = 0
+PROGRAM_COMPILED = 1
+STAGE_SOURCE = 0
+STAGE_TAPE = 1
+STAGE_HIP = 2
+STAGE_CODE = 3
+
 
 class S2MError(RuntimeError):
     """A failed library call; .code is the s2m_status value."""
@@ -72,6 +84,16 @@ class _DeviceMesh(C.Structure):
                 ("invalid_quads", C.c_uint64)]
 
 
+class _ServerConfig(C.Structure):
+    _fields_ = [("device", C.c_int), ("hbm_budget", C.c_uint64), ("max_batch", C.c_uint32),
+                ("workers", C.c_uint32), ("want_keys", C.c_int)]
+
+
+class _RequestStats(C.Structure):
+    _fields_ = [("queue_ms", C.c_double), ("latency_ms", C.c_double), ("batch_size", C.c_uint32),
+                ("worker", C.c_uint32)]
+
+
 class _Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("eval_ms", C.c_double), ("contour_ms", C.c_double),
                 ("normal_ms", C.c_double), ("quad_ms", C.c_double), ("d2h_ms", C.c_double),
@@ -92,6 +114,12 @@ _SIGNATURES = {
     "s2m_program_from_tape": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32,
                                         C.POINTER(C.c_void_p)]),
     "s2m_program_free": (None, [C.c_void_p]),
+    "s2m_program_from_source": (C.c_int, [C.c_void_p, C.c_char_p, C.c_uint32, C.c_char_p,
+                                          C.POINTER(C.c_void_p)]),
+    "s2m_program_kind": (C.c_int, [C.c_void_p]),
+    "s2m_program_tape": (C.c_uint32, [C.c_void_p, C.c_void_p, C.c_uint32]),
+    "s2m_translate": (C.c_uint64, [C.c_char_p, C.c_uint32, C.c_char_p, C.c_uint32, C.c_void_p,
+                                   C.c_uint64, C.POINTER(C.c_int)]),
     "s2m_tape_validate": (C.c_int, [C.c_void_p, C.c_uint32]),
     "s2m_tape_flops": (C.c_uint64, [C.c_void_p, C.c_uint32]),
     "s2m_params_reference": (None, [C.POINTER(_Params), C.c_uint32, C.c_float]),
@@ -105,6 +133,13 @@ _SIGNATURES = {
                                  C.c_int, C.POINTER(_MeshOut)]),
     "s2m_mesh_batch_device": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p),
                                         C.POINTER(_Params), C.POINTER(_DeviceMesh)]),
+    "s2m_server_create": (C.c_void_p, [C.POINTER(_ServerConfig)]),
+    "s2m_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(_Params),
+                                    C.POINTER(C.c_uint64)]),
+    "s2m_server_take": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(_MeshOut),
+                                  C.POINTER(_RequestStats)]),
+    "s2m_server_request_bytes": (C.c_uint64, [C.c_void_p, C.c_uint32, C.POINTER(_Params)]),
+    "s2m_server_destroy": (None, [C.c_void_p]),
     "s2m_write_stl": (C.c_int, [C.POINTER(_MeshOut), C.c_char_p]),
     "s2m_write_ply": (C.c_int, [C.POINTER(_MeshOut), C.c_char_p]),
     "s2m_write_to_file": (C.c_int, [C.POINTER(_MeshOut), C.c_char_p]),
@@ -245,6 +280,39 @@ def tape_flops(t) -> int:
     return int(lib().s2m_tape_flops(_ptr(t), len(t)))
 
 
+def _lang_flags(lang: str, path: bool, force_compiled: bool) -> int:
+    if lang not in ("sdf3d", "glsl"):
+        raise ValueError(f"unknown source language {lang!r}")
+    return ((LANG_GLSL if lang == "glsl" else LANG_SDF3D) | (SOURCE_IS_PATH if path else 0)
+            | (FORCE_COMPILED if force_compiled else 0))
+
+
+def translate(source, stage: int, lang: str = "sdf3d", sdf_name: Optional[str] = None,
+              path: bool = False) -> bytes:
+    """Run the front end without a device: STAGE_SOURCE (text after use/include
+    expansion), STAGE_TAPE (tape bytes), STAGE_HIP (generated HIP C++) or STAGE_CODE
+    (gfx950 code object).  `source` is the text, or a file path with path=True."""
+    src = str(source).encode()
+    flags = _lang_flags(lang, path, False)
+    name = sdf_name.encode() if sdf_name else None
+    st = C.c_int(0)
+    n = lib().s2m_translate(src, flags, name, stage, None, 0, C.byref(st))
+    if n == 0:
+        if st.value != OK:
+            _check(st.value)
+        return b""
+    buf = C.create_string_buffer(n)
+    lib().s2m_translate(src, flags, name, stage, buf, n, C.byref(st))
+    _check(st.value)
+    return buf.raw
+
+
+def translate_tape(source, lang: str = "sdf3d", sdf_name: Optional[str] = None,
+                   path: bool = False) -> np.ndarray:
+    """The tape a source program lowers to (S2MError E_WGSL if it does not)."""
+    return np.frombuffer(translate(source, STAGE_TAPE, lang, sdf_name, path), TAPE_DTYPE).copy()
+
+
 def tape_validate(t: np.ndarray) -> bool:
     t = np.ascontiguousarray(t, TAPE_DTYPE)
     return lib().s2m_tape_validate(_ptr(t), len(t)) == OK
@@ -253,11 +321,16 @@ def tape_validate(t: np.ndarray) -> bool:
 class Program:
     """A program uploaded to a context's device."""
 
-    def __init__(self, ctx: "Context", handle: int, tape_array: np.ndarray, flops: int):
+    def __init__(self, ctx: "Context", handle: int, tape_array: Optional[np.ndarray], flops: int):
         self._ctx = ctx
         self.handle = handle
-        self.tape = tape_array
+        self.tape = tape_array  # None for a compiled program
         self.flops_per_point = flops
+
+    @property
+    def kind(self) -> int:
+        """PROGRAM_TAPE or PROGRAM_COMPILED."""
+        return lib().s2m_program_kind(C.c_void_p(self.handle))
 
     def close(self) -> None:
         if self.handle:
@@ -313,6 +386,22 @@ class Context:
         out = C.c_void_p()
         _check(lib().s2m_program_from_tape(C.c_void_p(self.handle), _ptr(t), len(t), C.byref(out)))
         return Program(self, out.value, t, flops)
+
+    def program_from_source(self, source, lang: str = "sdf3d", sdf_name: Optional[str] = None,
+                            path: bool = False, force_compiled: bool = False) -> Program:
+        """A program from .sdf3d WGSL or GLSL text (or a file with path=True): a tape when
+        it lowers to library primitives and operators, else code compiled by hiprtc."""
+        out = C.c_void_p()
+        _check(lib().s2m_program_from_source(
+            C.c_void_p(self.handle), str(source).encode(), _lang_flags(lang, path, force_compiled),
+            sdf_name.encode() if sdf_name else None, C.byref(out)))
+        h = out.value
+        if lib().s2m_program_kind(C.c_void_p(h)) == PROGRAM_TAPE:
+            n = lib().s2m_program_tape(C.c_void_p(h), None, 0)
+            t = np.zeros(n, TAPE_DTYPE)
+            lib().s2m_program_tape(C.c_void_p(h), _ptr(t), n)
+            return Program(self, h, t, tape_flops(t))
+        return Program(self, h, None, 0)
 
     def mesh(self, prog: Program, params: Params, want_keys: bool = True) -> TriangleMesh:
         """Mesh the program; results copied to host memory."""
@@ -381,3 +470,61 @@ class Context:
         _check(lib().s2m_contour_cells(C.c_void_p(self.handle), _ptr(corners), res, _ptr(lo),
                                        _ptr(hi), flags, C.byref(out)))
         return _take_mesh(out, True)
+
+
+class Server:
+    """Continuous-batching request loop (s2m_server_*): submit tapes, take meshes."""
+
+    def __init__(self, device: int = 0, hbm_budget: int = 0, max_batch: int = 0, workers: int = 0,
+                 want_keys: bool = True):
+        cfg = _ServerConfig(device, hbm_budget, max_batch, workers, int(want_keys))
+        h = lib().s2m_server_create(C.byref(cfg))
+        if not h:
+            raise S2MError(E_HIP, lib().s2m_last_error().decode(errors="replace"))
+        self.handle = h
+        self.want_keys = want_keys
+
+    def submit(self, program, params: Params) -> int:
+        t = program.tape() if isinstance(program, tape.Node) else np.ascontiguousarray(program, TAPE_DTYPE)
+        p = params.c()
+        ticket = C.c_uint64()
+        _check(lib().s2m_server_submit(C.c_void_p(self.handle), _ptr(t), len(t), C.byref(p),
+                                       C.byref(ticket)))
+        return int(ticket.value)
+
+    def take(self, ticket: int, wait: bool = True):
+        """(TriangleMesh, stats dict), or None if not finished and wait is False."""
+        out = _MeshOut()
+        st = _RequestStats()
+        rc = lib().s2m_server_take(C.c_void_p(self.handle), ticket, int(wait), C.byref(out),
+                                   C.byref(st))
+        if rc == E_BUSY:
+            return None
+        if rc != OK:
+            lib().s2m_mesh_free(C.byref(out))
+            _check(rc)
+        stats = {k: getattr(st, k) for k, _ in _RequestStats._fields_}
+        return _take_mesh(out, self.want_keys), stats
+
+    @staticmethod
+    def request_bytes(program, params: Params) -> int:
+        t = program.tape() if isinstance(program, tape.Node) else np.ascontiguousarray(program, TAPE_DTYPE)
+        p = params.c()
+        return int(lib().s2m_server_request_bytes(_ptr(t), len(t), C.byref(p)))
+
+    def close(self) -> None:
+        if self.handle:
+            lib().s2m_server_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

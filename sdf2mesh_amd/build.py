@@ -33,7 +33,37 @@ DEVICE = [
     "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-SOURCES = ["kernels.hip", "sparse.hip", "api.cpp", "writer.cpp"]
+SOURCES = ["kernels.hip", "sparse.hip", "api.cpp", "server.cpp", "writer.cpp", "compile.cpp",
+           "frontend_parse.cpp", "frontend_check.cpp", "frontend_emit.cpp"]
+
+# Device headers a program compiled at run time includes, in order; written into one
+# string (rtc_source.inc) that hiprtc receives ahead of the generated code.
+RTC_HEADERS = [PKG.parent / "include" / "sdf2mesh.h", CSRC / "device_sdf.h", CSRC / "device_dense.h",
+               CSRC / "rtc_prelude.h"]
+RTC_TYPES = """typedef __hip_internal::uint8_t uint8_t;
+typedef __hip_internal::uint16_t uint16_t;
+typedef __hip_internal::uint32_t uint32_t;
+typedef __hip_internal::uint64_t uint64_t;
+typedef __hip_internal::int32_t int32_t;
+"""
+
+
+def _write_rtc_source() -> None:
+    """Concatenate the device headers (without their local #include lines and include
+    guards' #pragma once) into a C++ raw string literal."""
+    parts = [RTC_TYPES]
+    for h in RTC_HEADERS:
+        for line in h.read_text().splitlines():
+            t = line.strip()
+            if t == "#pragma once" or (t.startswith("#include \"") and t.endswith(".h\"")):
+                continue
+            parts.append(line + "\n")
+    text = "".join(parts)
+    assert ")s2mrtc\"" not in text
+    out = BUILD / "rtc_source.inc"
+    new = 'R"s2mrtc(' + text + ')s2mrtc"\n'
+    if not out.exists() or out.read_text() != new:
+        out.write_text(new)
 
 
 def _compile(src: str) -> Path:
@@ -42,7 +72,7 @@ def _compile(src: str) -> Path:
     if src.endswith(".hip"):
         cmd += DEVICE
     else:
-        cmd += ["-x", "c++", f"-I{ROCM / 'include'}", "-D__HIP_PLATFORM_AMD__"]
+        cmd += ["-x", "c++", f"-I{ROCM / 'include'}", f"-I{BUILD}", "-D__HIP_PLATFORM_AMD__"]
     cmd += ["-c", str(CSRC / src), "-o", str(obj)]
     subprocess.run(cmd, check=True)
     return obj
@@ -51,11 +81,13 @@ def _compile(src: str) -> Path:
 def build(verbose: bool = False) -> Path:
     """Compile every source and link the shared library; always rebuilds."""
     BUILD.mkdir(exist_ok=True)
+    _write_rtc_source()
     with cf.ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp),
-           *map(str, objs), "-lpthread"]
+           *map(str, objs), f"-L{ROCM / 'lib'}", "-lhiprtc", "-lpthread",
+           f"-Wl,-rpath,{ROCM / 'lib'}"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
     if verbose:

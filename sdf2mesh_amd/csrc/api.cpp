@@ -83,6 +83,10 @@ struct s2m_program {
     s2m_op* d_ops = nullptr;
     uint32_t n = 0;
     std::vector<s2m_op> ops;
+    bool owns_device_copy = true;  // false: d_ops points into a caller-owned arena
+    // compiled programs (hiprtc code object): dense-path kernels instead of a tape
+    hipModule_t module = nullptr;
+    hipFunction_t k_eval = nullptr, k_normals = nullptr;
 };
 
 struct s2m_ctx {
@@ -651,6 +655,30 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     return fail(S2M_E_OOM, "output buffers kept overflowing");
 }
 
+// Lattice evaluation and vertex normals of a program: the tape interpreter's kernels,
+// or the kernels of a compiled program (same bodies, device_dense.h).
+hipError_t eval_lattice_prog(hipStream_t s, const s2m_program* prog, const float* cx, const float* cy,
+                             const float* cz, uint32_t nx, uint32_t ny, uint32_t lz0, uint32_t nzl,
+                             uint32_t pitch, float* lat)
+{
+    if (!prog->module)
+        return launch_eval_lattice(s, prog->d_ops, prog->n, cx, cy, cz, nx, ny, lz0, nzl, pitch, lat);
+    if (nx == 0 || ny == 0 || nzl == 0) return hipSuccess;
+    void* args[] = {&cx, &cy, &cz, &nx, &ny, &lz0, &pitch, &lat};
+    return hipModuleLaunchKernel(prog->k_eval, (nx + 63) / 64, (ny + 3) / 4, nzl, 256, 1, 1, 0, s, args,
+                                 nullptr);
+}
+
+hipError_t normals_prog(hipStream_t s, const s2m_program* prog, float eps, const MeshBufs& b)
+{
+    if (!prog->module) return launch_normals(s, prog->d_ops, prog->n, eps, b);
+    float* vtx = b.vtx;
+    const uint64_t* counters = b.counters;
+    uint64_t vcap = b.vcap;
+    void* args[] = {&eps, &vtx, &counters, &vcap};
+    return hipModuleLaunchKernel(prog->k_normals, 1024, 1, 1, 256, 1, 1, 0, s, args, nullptr);
+}
+
 // Mesh one job on the device.  corners != NULL: contour the given per-cell corner
 // dump (test hook) instead of evaluating a program.
 int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Job& j,
@@ -767,8 +795,8 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
                 for (auto& e : m) e = ctx->next_event();
             if (m[0]) HIP_TRY(hipEventRecord(m[0], ctx->stream));
             if (!h_corners) {
-                HIP_TRY(launch_eval_lattice(ctx->stream, prog->d_ops, prog->n, dc[0], dc[1], dc[2],
-                                            nx, ny, lz0, nzl, pitch, ctx->lattice.as<float>()));
+                HIP_TRY(eval_lattice_prog(ctx->stream, prog, dc[0], dc[1], dc[2], nx, ny, lz0, nzl, pitch,
+                                          ctx->lattice.as<float>()));
                 lattice_points += (uint64_t)nx * ny * nzl;
             }
             if (m[1]) HIP_TRY(hipEventRecord(m[1], ctx->stream));
@@ -783,7 +811,7 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
                 HIP_TRY(launch_cells(ctx->stream, a, b, 1, normals ? 0 : 1));
             }
             if (m[2]) HIP_TRY(hipEventRecord(m[2], ctx->stream));
-            if (normals) HIP_TRY(launch_normals(ctx->stream, prog->d_ops, prog->n, p->eps, b));
+            if (normals) HIP_TRY(normals_prog(ctx->stream, prog, p->eps, b));
             if (m[3]) HIP_TRY(hipEventRecord(m[3], ctx->stream));
             HIP_TRY(launch_quads(ctx->stream, a, b, 0));
             HIP_TRY(launch_scan(ctx->stream, b, nb, 1));
@@ -1014,11 +1042,31 @@ int s2m_program_from_tape(s2m_ctx* ctx, const s2m_op* ops, uint32_t n_ops, s2m_p
 void s2m_program_free(s2m_program* prog)
 {
     if (!prog) return;
-    if (prog->d_ops) {
+    if (prog->d_ops && prog->owns_device_copy) {
         (void)hipSetDevice(prog->device);
         (void)hipFree(prog->d_ops);
     }
+    if (prog->module) {
+        (void)hipSetDevice(prog->device);
+        (void)hipModuleUnload(prog->module);
+    }
     delete prog;
+}
+
+int s2m_program_kind(const s2m_program* prog)
+{
+    if (!prog) {
+        fail(S2M_E_ARG, "NULL argument");
+        return -1;
+    }
+    return prog->module ? S2M_PROGRAM_COMPILED : S2M_PROGRAM_TAPE;
+}
+
+uint32_t s2m_program_tape(const s2m_program* prog, s2m_op* ops, uint32_t cap)
+{
+    if (!prog || prog->module) return 0;
+    if (ops) std::copy(prog->ops.begin(), prog->ops.begin() + std::min(cap, prog->n), ops);
+    return prog->n;
 }
 
 int s2m_mesh_device(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params,
@@ -1030,7 +1078,7 @@ int s2m_mesh_device(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* par
     Job j;
     int rc = prepare_job(params, j);
     if (rc) return rc;
-    j.tiled = !(params->flags & S2M_DENSE_EVAL) && tape_prunable(prog->ops);
+    j.tiled = !prog->module && !(params->flags & S2M_DENSE_EVAL) && tape_prunable(prog->ops);
     ctx->timing = s2m_timing{};
     ctx->last_nvtx = ctx->last_ntri = ctx->last_invalid = 0;
     const uint64_t cells = (uint64_t)j.res[0] * j.res[1] * j.nz_cells;
@@ -1128,9 +1176,30 @@ int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
     ctx->timing = s2m_timing{};
-    // consecutive passes of at most ~64M cells and 2M tiles (empty grids cost nothing)
+    // consecutive passes of at most ~64M cells and 2M tiles (empty grids cost nothing);
+    // compiled programs have no tape for the sparse pass and are meshed one at a time
     uint32_t i = 0;
     s2m_timing sum{};
+    auto add_timing = [&sum](const s2m_timing& t) {
+        sum.total_ms += t.total_ms;
+        sum.eval_ms += t.eval_ms;
+        sum.contour_ms += t.contour_ms;
+        sum.normal_ms += t.normal_ms;
+        sum.quad_ms += t.quad_ms;
+        sum.d2h_ms += t.d2h_ms;
+        sum.eval_launches += t.eval_launches;
+        sum.lattice_points += t.lattice_points;
+        sum.slabs += t.slabs;
+        sum.tiles_total += t.tiles_total;
+        sum.tiles_active += t.tiles_active;
+        sum.eval_flops += t.eval_flops;
+    };
+    for (uint32_t q = 0; q < n; ++q)
+        if (progs[q]->module) {
+            rc = s2m_mesh(ctx, progs[q], &params[q], want_keys, &outs[q]);
+            if (rc) return rc;
+            add_timing(ctx->timing);
+        }
     while (i < n) {
         uint32_t k = i;
         uint64_t cells = 0, tiles = 0;
@@ -1143,7 +1212,7 @@ int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
             const uint64_t c = (uint64_t)j.res[0] * j.res[1] * j.nz_cells;
             const uint64_t t = ((j.res[0] + 7) / 8ull) * ((j.res[1] + 7) / 8ull) * ((j.nz_cells + 7) / 8ull);
             if (!idx.empty() && (cells + c > (64ull << 20) || tiles + t > (2ull << 20))) break;
-            if (c) {
+            if (c && !progs[k]->module) {
                 pp.push_back(progs[k]);
                 qq.push_back(&params[k]);
                 jj.push_back(j);
@@ -1161,17 +1230,7 @@ int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
             rc = copy_ranges(ctx, r.data(), (uint32_t)idx.size(), want_keys, tmp.data());
             for (size_t q = 0; q < idx.size(); ++q) outs[idx[q]] = tmp[q];
             if (rc) return rc;
-            sum.total_ms += ctx->timing.total_ms;
-            sum.eval_ms += ctx->timing.eval_ms;
-            sum.contour_ms += ctx->timing.contour_ms;
-            sum.normal_ms += ctx->timing.normal_ms;
-            sum.quad_ms += ctx->timing.quad_ms;
-            sum.eval_launches += ctx->timing.eval_launches;
-            sum.lattice_points += ctx->timing.lattice_points;
-            sum.slabs += ctx->timing.slabs;
-            sum.tiles_total += ctx->timing.tiles_total;
-            sum.tiles_active += ctx->timing.tiles_active;
-            sum.eval_flops += ctx->timing.eval_flops;
+            add_timing(ctx->timing);
         }
         for (uint32_t q = i; q < k; ++q)  // empty grids: empty meshes
             if (!outs[q].vtx) {
@@ -1201,6 +1260,9 @@ int s2m_mesh_batch_device(s2m_ctx* ctx, uint32_t n, const s2m_program* const* pr
     std::vector<const s2m_params*> qq;
     std::vector<Job> jj;
     std::vector<uint32_t> idx;
+    for (uint32_t i = 0; i < n; ++i)
+        if (progs[i]->module)
+            return fail(S2M_E_ARG, "program %u is compiled; device batches take tape programs (use s2m_mesh_device)", i);
     for (uint32_t i = 0; i < n; ++i) {
         std::memset(&outs[i], 0, sizeof outs[i]);
         if ((uint64_t)jobs[i].res[0] * jobs[i].res[1] * jobs[i].nz_cells == 0) continue;
@@ -1266,8 +1328,8 @@ int s2m_eval_lattice(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* pa
     rc = upload_tables(ctx, j, dc, dlo, dhi);
     if (rc) return rc;
     HIP_TRY(ctx->lattice.ensure((size_t)n * sizeof(float)));
-    HIP_TRY(launch_eval_lattice(ctx->stream, prog->d_ops, prog->n, dc[0], dc[1], dc[2], dims[0],
-                                dims[1], 0, dims[2], dims[0], ctx->lattice.as<float>()));
+    HIP_TRY(eval_lattice_prog(ctx->stream, prog, dc[0], dc[1], dc[2], dims[0], dims[1], 0, dims[2], dims[0],
+                              ctx->lattice.as<float>()));
     HIP_TRY(hipMemcpyAsync(out, ctx->lattice.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost,
                            ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1314,3 +1376,52 @@ int s2m_contour_cells(s2m_ctx* ctx, const float* corners, const uint32_t res[3],
 }
 
 }  // extern "C"
+
+namespace s2m {
+
+int program_from_code_object(s2m_ctx* ctx, const void* code, size_t size, s2m_program** out)
+{
+    (void)size;
+    HIP_TRY(hipSetDevice(ctx->device));
+    s2m_program* p = new s2m_program();
+    p->device = ctx->device;
+    hipError_t e = hipModuleLoadData(&p->module, code);
+    if (e == hipSuccess) e = hipModuleGetFunction(&p->k_eval, p->module, "s2m_c_eval_lattice");
+    if (e == hipSuccess) e = hipModuleGetFunction(&p->k_normals, p->module, "s2m_c_normals");
+    if (e != hipSuccess) {
+        if (p->module) (void)hipModuleUnload(p->module);
+        delete p;
+        return fail(S2M_E_HIP, "loading the compiled program: %s", hipGetErrorString(e));
+    }
+    *out = p;
+    return S2M_OK;
+}
+
+s2m_program* program_view(int device, const s2m_op* host_ops, uint32_t n, const s2m_op* d_ops)
+{
+    s2m_program* p = new s2m_program();
+    p->device = device;
+    p->n = n;
+    p->ops.assign(host_ops, host_ops + n);
+    p->d_ops = const_cast<s2m_op*>(d_ops);
+    p->owns_device_copy = false;
+    return p;
+}
+
+hipStream_t context_stream(s2m_ctx* ctx) { return ctx->stream; }
+
+bool tape_valid(const s2m_op* ops, uint32_t n) { return tape_ok(ops, n); }
+
+int job_geometry(const s2m_params* p, uint32_t res[3], uint32_t* nz_cells)
+{
+    Job j;
+    int rc = prepare_job(p, j);
+    if (rc) return rc;
+    for (int a = 0; a < 3; ++a) res[a] = j.res[a];
+    *nz_cells = j.nz_cells;
+    return S2M_OK;
+}
+
+int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
+}  // namespace s2m

@@ -9,8 +9,10 @@
 //   sdf_op.wgsl:7-23             smooth union / intersection / subtraction
 //   examples/mandelmesh.frag:3-28 the Mandelbulb distance estimator
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#endif
 
 #include "../../include/sdf2mesh.h"
 
@@ -56,13 +58,14 @@ __device__ __forceinline__ float sd_cylinder(float x, float y, float z, float h,
 
 // sdf3d_capsule: pa = p - a; ba = b - a; h = clamp(dot(pa,ba)/dot(ba,ba), 0, 1);
 // length(pa - ba*h) - r
-__device__ __forceinline__ float sd_capsule(float x, float y, float z, const float* f)
+__device__ __forceinline__ float sd_capsule(float x, float y, float z, float ax, float ay, float az,
+                                            float bx, float by, float bz, float r)
 {
-    const float pax = x - f[0], pay = y - f[1], paz = z - f[2];
-    const float bax = f[3] - f[0], bay = f[4] - f[1], baz = f[5] - f[2];
+    const float pax = x - ax, pay = y - ay, paz = z - az;
+    const float bax = bx - ax, bay = by - ay, baz = bz - az;
     const float h = wclamp(((pax * bax + pay * bay) + paz * baz) / ((bax * bax + bay * bay) + baz * baz),
                            0.0f, 1.0f);
-    return len3(pax - bax * h, pay - bay * h, paz - baz * h) - f[6];
+    return len3(pax - bax * h, pay - bay * h, paz - baz * h) - r;
 }
 
 // sdf3d_torus: q = vec2(length(p.xz) - t.x, p.y); length(q) - t.y
@@ -130,7 +133,7 @@ __device__ __forceinline__ float eval_primitive(uint32_t code, uint32_t iarg, co
     case S2M_OP_BOX: return sd_box(px - f[0], py - f[1], pz - f[2], f[3], f[4], f[5]);
     case S2M_OP_TORUS: return sd_torus(px - f[0], py - f[1], pz - f[2], f[3], f[4]);
     case S2M_OP_CYLINDER: return sd_cylinder(px - f[0], py - f[1], pz - f[2], f[3], f[4]);
-    case S2M_OP_CAPSULE: return sd_capsule(px, py, pz, f);
+    case S2M_OP_CAPSULE: return sd_capsule(px, py, pz, f[0], f[1], f[2], f[3], f[4], f[5], f[6]);
     default: return sd_mandelbulb(px, py, pz, f[0], f[1], f[2], iarg);
     }
 }

@@ -149,4 +149,13 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
 hipError_t launch_cells_from_corners(hipStream_t s, const SlabArgs& a, const float* corners,
                                      const MeshBufs& b, int emit);
 
+// ---- host helpers (api.cpp) for the server ----
+// A program from a gfx950 code object holding s2m_c_eval_lattice and s2m_c_normals.
+int program_from_code_object(s2m_ctx* ctx, const void* code, size_t size, s2m_program** out);
+s2m_program* program_view(int device, const s2m_op* host_ops, uint32_t n, const s2m_op* d_ops);
+hipStream_t context_stream(s2m_ctx* ctx);
+bool tape_valid(const s2m_op* ops, uint32_t n);
+int job_geometry(const s2m_params* p, uint32_t res[3], uint32_t* nz_cells);
+int set_error(int code, const char* msg);
+
 }  // namespace s2m

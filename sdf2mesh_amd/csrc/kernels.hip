@@ -10,6 +10,7 @@
 // Cells are numbered in (z, y, x) order inside the slab and every block owns 1024
 // consecutive cells, so block offsets from the scan give each vertex and triangle its
 // place in the reference's output order (VertexList order, mesh.rs:280-321).
+#include "device_dense.h"
 #include "device_sdf.h"
 #include "internal.h"
 
@@ -59,12 +60,8 @@ __global__ __launch_bounds__(256) void k_eval_lattice(const s2m_op* __restrict__
                                                       uint32_t ny, uint32_t lz0, uint32_t pitch,
                                                       float* __restrict__ lat)
 {
-    const uint32_t ix = blockIdx.x * 64u + lane_id();
-    const uint32_t iy = blockIdx.y * 4u + (threadIdx.x >> 6);
-    const uint32_t iz = blockIdx.z;
-    if (ix >= nx || iy >= ny) return;
-    const float v = tape_eval(ops, nops, cx[ix], cy[iy], cz[lz0 + iz]);
-    lat[((size_t)iz * ny + iy) * pitch + ix] = v;
+    eval_lattice_point([=](float x, float y, float z) { return tape_eval(ops, nops, x, y, z); }, cx,
+                       cy, cz, nx, ny, lz0, pitch, lat);
 }
 
 // ---------------------------------------------------------------------------------
@@ -259,35 +256,15 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ bcount, ui
 // (sdf3d_normal.wgsl:4-9, dualcontour.wgsl:170-171).  Grid-stride over the range
 // [prev, total) read from the device counters, so no host round trip is needed.
 // ---------------------------------------------------------------------------------
-__device__ __forceinline__ void write_normal(float* o, float d1, float d2, float d3, float d4)
-{
-    // v1*d1 + v2*d2 + v3*d3 + v4*d4, summed left to right
-    const float gx = ((d1 + (-d2)) + (-d3)) + d4;
-    const float gy = (((-d1) + (-d2)) + d3) + d4;
-    const float gz = (((-d1) + d2) + (-d3)) + d4;
-    const float len = sqrtf((gx * gx + gy * gy) + gz * gz);
-    o[3] = gx / len;
-    o[4] = gy / len;
-    o[5] = gz / len;
-}
+static_assert(kDenseVtxTotal == kVtxTotal && kDenseVtxPrev == kVtxPrev, "counter layout");
 
 __global__ __launch_bounds__(256) void k_normals(const s2m_op* __restrict__ ops, uint32_t nops,
                                                  float eps, float* __restrict__ vtx,
                                                  const uint64_t* __restrict__ counters, uint64_t vcap)
 {
-    const uint64_t beg = counters[kVtxPrev];
-    const uint64_t end = min(counters[kVtxTotal], vcap);
-    for (uint64_t v = beg + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < end;
-         v += (uint64_t)gridDim.x * blockDim.x) {
-        float* o = vtx + 6 * v;
-        const float px = o[0], py = o[1], pz = o[2];
-        // taps v1=(1,-1,-1), v2=(-1,-1,1), v3=(-1,1,-1), v4=(1,1,1); p + v*eps
-        const float d1 = tape_eval(ops, nops, px + eps, py + (-eps), pz + (-eps));
-        const float d2 = tape_eval(ops, nops, px + (-eps), py + (-eps), pz + eps);
-        const float d3 = tape_eval(ops, nops, px + (-eps), py + eps, pz + (-eps));
-        const float d4 = tape_eval(ops, nops, px + eps, py + eps, pz + eps);
-        write_normal(o, d1, d2, d3, d4);
-    }
+    const auto sdf = [=](float x, float y, float z) { return tape_eval(ops, nops, x, y, z); };
+    normals_range([=](float px, float py, float pz, float e, float* o) { library_normal(sdf, px, py, pz, e, o); },
+                  eps, vtx, counters, vcap);
 }
 
 // ---------------------------------------------------------------------------------

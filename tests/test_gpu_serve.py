@@ -1,0 +1,62 @@
+"""Serving (s2m_server_*): requests admitted under an HBM budget and batched
+continuously; each result must be identical to meshing the request alone."""
+import time
+
+import numpy as np
+import pytest
+
+import sdf2mesh_amd as s2m
+from sdf2mesh_amd import tape
+from sdf2mesh_amd.workloads import serve_jobs
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a, b):
+    assert len(a.vertices) == len(b.vertices)
+    assert np.array_equal(a.cell_keys, b.cell_keys)
+    assert np.array_equal(a.signs, b.signs)
+    assert np.array_equal(a.vertices.view(np.uint32), b.vertices.view(np.uint32))
+    assert np.array_equal(a.triangles, b.triangles)
+    assert a.invalid_quads == b.invalid_quads
+
+
+def test_served_equals_alone(ctx):
+    nodes, params = serve_jobs(48, seed=3)
+    # keep the test short: cap the resolution at 128
+    params = [s2m.Params.reference(min(int(p.res[0]), 128), p.bb_max[0] * 2.0) for p in params]
+    with s2m.Server(0, workers=2, max_batch=8) as srv:
+        tickets = []
+        for i, (n, p) in enumerate(zip(nodes, params)):
+            tickets.append(srv.submit(n, p))
+            if i % 7 == 0:
+                time.sleep(0.002)  # arrivals while earlier passes are running
+        results = [srv.take(t) for t in tickets]
+    batches = [st["batch_size"] for _, st in results]
+    print("served 48 requests, batch sizes", sorted(set(batches)), flush=True)
+    for (m, st), n, p in zip(results, nodes, params):
+        same(m, ctx.mesh(ctx.program(n), p))
+        assert st["latency_ms"] >= st["queue_ms"] >= 0.0
+
+
+def test_budget_rejects_oversized_request():
+    with s2m.Server(0, hbm_budget=1 << 20, workers=1) as srv:
+        small = s2m.Params.reference(16, 2.0)
+        assert s2m.Server.request_bytes(tape.torus_example(), small) < (1 << 20)
+        t = srv.submit(tape.torus_example(), small)
+        m, _ = srv.take(t)
+        assert len(m.vertices) > 0
+        with pytest.raises(s2m.S2MError) as e:
+            srv.submit(tape.torus_example(), s2m.Params.reference(256, 2.0))
+        assert e.value.code == s2m.E_BUDGET
+
+
+def test_take_busy_then_done():
+    with s2m.Server(0, workers=1) as srv:
+        t = srv.submit(tape.csg64(), s2m.Params.reference(128, 2.0))
+        r = srv.take(t, wait=False)
+        if r is None:  # still queued or running
+            r = srv.take(t, wait=True)
+        assert len(r[0].vertices) > 0
+        with pytest.raises(s2m.S2MError):
+            srv.take(t)  # a ticket can be taken once
