@@ -7,9 +7,12 @@ A step meshes one job (SDF lattice evaluation, contouring, normals, quads) with 
 program and parameters already on the device; the mesh stays in HBM.  Mvoxels/s =
 res^3 / 1e6 / seconds per step.  Prints one JSON line with:
 
-  roofline      eval kernel (k_eval_lattice): algorithmic fp32 flops per launch
-                (static op count of the SDF x lattice points) / mean launch time from
-                HIP events on the library's stream, against the fp32 vector peak
+  roofline      the SDF evaluation kernel (k_sp_eval on the tile-pruned path, one
+                launch per pass; k_eval_lattice on the dense path, one per slab):
+                algorithmic fp32 flops executed per launch (static op count of the
+                program each tile runs x its lattice points) / mean launch time from
+                HIP events recorded around that kernel alone on the library's stream,
+                against the fp32 peak (vector ALU: the path has no matrix work)
   cpu_baseline  the oracle (CPU restatement of the reference algorithm: 8 corner
                 evaluations per cell, OpenMP) on a bounded sample of z-slices of the
                 same job
@@ -231,7 +234,7 @@ def run(args) -> dict:
     for _ in range(args.warmup):
         ctx.mesh_device(prog, params)
     ctx.set_profiling(True)
-    agg = {"eval_ms": 0.0, "contour_ms": 0.0, "normal_ms": 0.0, "quad_ms": 0.0,
+    agg = {"prune_ms": 0.0, "eval_ms": 0.0, "contour_ms": 0.0, "normal_ms": 0.0, "quad_ms": 0.0,
            "total_ms": 0.0, "eval_launches": 0, "lattice_points": 0, "slabs": 0,
            "tiles_total": 0, "tiles_active": 0, "eval_flops": 0}
     t0 = time.perf_counter()
@@ -273,19 +276,20 @@ def run(args) -> dict:
                    "slabs": agg["slabs"] // args.steps},
         "roofline": {
             "bound": "valu",
-            "kernel": "k_eval_tiles" if tiled else "k_eval_lattice",
+            "kernel": "k_sp_eval" if tiled else "k_eval_lattice",
             "achieved": round(achieved, 2),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args.workload + ("-dense" if args.dense else ""),
+                                        "k_sp_eval" if tiled else "k_eval_lattice"),
             "launch_ms": round(eval_avg_ms, 4),
             "flops_per_launch": flops_per_launch,
             "points_per_launch": pts_per_launch,
         },
         "stages_ms_per_step": {k: round(agg[k] / args.steps, 3)
-                               for k in ("eval_ms", "contour_ms", "normal_ms", "quad_ms",
-                                         "total_ms")},
+                               for k in ("prune_ms", "eval_ms", "contour_ms", "normal_ms",
+                                         "quad_ms", "total_ms")},
     }
     ctx.close()
     if not args.no_cpu_baseline:
@@ -293,6 +297,34 @@ def run(args) -> dict:
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu_baseline"] = round(value / cb["value"], 1)
     return out
+
+
+def run_copy_calibration(args) -> dict:
+    """A plain device copy of known size (1 GiB read + 1 GiB written per launch), whose
+    FETCH_SIZE / WRITE_SIZE counters scale the counters of the meshing kernels to bytes."""
+    import torch
+
+    n = 1 << 28
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"metric": "copy calibration", "launches": args.steps, "bytes_read_per_launch": 4 * n,
+            "bytes_written_per_launch": 4 * n}
+
+
+def measured_traffic(workload: str, kernel: str):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    summarised in profiles/hbm_traffic.json (scaled by the copy calibration), or None."""
+    f = ROOT / "profiles" / "hbm_traffic.json"
+    if not f.exists():
+        return None
+    entry = json.loads(f.read_text()).get(workload, {})
+    if entry.get("kernel") != kernel:
+        return None
+    return entry.get("bytes_per_launch")
 
 
 def main() -> int:
@@ -314,7 +346,9 @@ def main() -> int:
         print(json.dumps({"error": "this build runs on one GPU; jobs shard by job or by z-slab "
                                    "across GPUs only by design, --gpus must be 1"}))
         return 2
-    if args.workload == "batch256":
+    if args.workload == "copy-calibration":
+        out = run_copy_calibration(args)
+    elif args.workload == "batch256":
         out = run_batch(args)
     elif args.workload == "serve":
         out = run_serve(args)

@@ -170,8 +170,8 @@ typedef struct s2m_device_mesh {
 typedef struct s2m_timing {
     double total_ms;    /* first kernel start to last kernel end                   */
     double eval_ms;     /* lattice evaluation kernels                               */
-    double contour_ms;  /* active-cell count, scan, vertex emission                 */
-    double normal_ms;   /* 4-tap normals                                            */
+    double contour_ms;  /* active-cell count, scan (and vertex emission when dense)  */
+    double normal_ms;   /* 4-tap normals (pruned path: with vertex emission)        */
     double quad_ms;     /* quad count, scan, triangle emission                      */
     double d2h_ms;      /* device-to-host copies (s2m_mesh only)                    */
     uint64_t eval_launches;
@@ -180,6 +180,7 @@ typedef struct s2m_timing {
     uint64_t tiles_total;    /* tiles of 8^3 cells (pruned path; 0 when dense)       */
     uint64_t tiles_active;   /* tiles whose lattice points were evaluated            */
     uint64_t eval_flops;     /* algorithmic flops executed on the lattice            */
+    double prune_ms;         /* interval pruning of tiles (pruned path)              */
 } s2m_timing;
 
 typedef struct s2m_ctx s2m_ctx;

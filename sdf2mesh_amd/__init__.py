@@ -99,7 +99,7 @@ class _Timing(C.Structure):
                 ("normal_ms", C.c_double), ("quad_ms", C.c_double), ("d2h_ms", C.c_double),
                 ("eval_launches", C.c_uint64), ("lattice_points", C.c_uint64),
                 ("slabs", C.c_uint64), ("tiles_total", C.c_uint64), ("tiles_active", C.c_uint64),
-                ("eval_flops", C.c_uint64)]
+                ("eval_flops", C.c_uint64), ("prune_ms", C.c_double)]
 
 
 _lib_handle: Optional[C.CDLL] = None

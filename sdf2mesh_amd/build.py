@@ -18,6 +18,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libsdf2mesh.so"
+CLI = PKG / "sdf2mesh"         # command line tool (cli.cpp) over the C ABI
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 HIPCC = str(ROCM / "bin" / "hipcc")
 ARCH = os.environ.get("S2M_OFFLOAD_ARCH", "gfx950")
@@ -90,8 +91,12 @@ def build(verbose: bool = False) -> Path:
            f"-Wl,-rpath,{ROCM / 'lib'}"]
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    tmp_cli = CLI.with_suffix(".tmp")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-o", str(tmp_cli), str(CSRC / "cli.cpp"),
+                    f"-L{PKG}", "-lsdf2mesh", "-Wl,-rpath,$ORIGIN"], check=True)
+    os.replace(tmp_cli, CLI)
     if verbose:
-        print(f"built {LIB}")
+        print(f"built {LIB} and {CLI}")
     return LIB
 
 

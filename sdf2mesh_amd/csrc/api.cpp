@@ -585,13 +585,13 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.seg_off_t = P.seg_off_v + P.nseg;
             P.counters = b.counters;
             P.job_out = ctx->sp_job_out.as<uint64_t>();
-            hipEvent_t m[5] = {};
+            hipEvent_t m[kSparseMarks] = {};
             if (ctx->profiling)
                 for (auto& e : m) e = ctx->next_event();
             HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters));
             const int boundary = (n == 1 && pass + 1 < npass) ? 1 : 0;
             HIP_TRY(launch_sparse_pass(ctx->stream, P, b, pblocks, boundary, ctx->profiling ? m : nullptr));
-            if (ctx->profiling) ev.insert(ev.end(), m, m + 5);
+            if (ctx->profiling) ev.insert(ev.end(), m, m + kSparseMarks);
             tiles_total += P.ntiles;
         }
         HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters));  // fold the last pass
@@ -609,16 +609,15 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             float ms = 0;
             HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_end));
             ctx->timing.total_ms = ms;
-            for (size_t q = 0; q + 4 < ev.size(); q += 5) {
-                float t0, t1, t2, t3;
-                HIP_TRY(hipEventElapsedTime(&t0, ev[q], ev[q + 1]));
-                HIP_TRY(hipEventElapsedTime(&t1, ev[q + 1], ev[q + 2]));
-                HIP_TRY(hipEventElapsedTime(&t2, ev[q + 2], ev[q + 3]));
-                HIP_TRY(hipEventElapsedTime(&t3, ev[q + 3], ev[q + 4]));
-                ctx->timing.eval_ms += t0;
-                ctx->timing.contour_ms += t1;
-                ctx->timing.normal_ms += t2;  // vertex emission + normals
-                ctx->timing.quad_ms += t3;
+            for (size_t q = 0; q + kSparseMarks - 1 < ev.size(); q += kSparseMarks) {
+                float t[kSparseMarks - 1];
+                for (int k = 0; k + 1 < kSparseMarks; ++k)
+                    HIP_TRY(hipEventElapsedTime(&t[k], ev[q + k], ev[q + k + 1]));
+                ctx->timing.prune_ms += t[0];
+                ctx->timing.eval_ms += t[1];
+                ctx->timing.contour_ms += t[2];
+                ctx->timing.normal_ms += t[3];  // vertex emission + normals
+                ctx->timing.quad_ms += t[4];
             }
         }
         ctx->timing.eval_launches = npass;
@@ -1183,6 +1182,7 @@ int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
     auto add_timing = [&sum](const s2m_timing& t) {
         sum.total_ms += t.total_ms;
         sum.eval_ms += t.eval_ms;
+        sum.prune_ms += t.prune_ms;
         sum.contour_ms += t.contour_ms;
         sum.normal_ms += t.normal_ms;
         sum.quad_ms += t.quad_ms;

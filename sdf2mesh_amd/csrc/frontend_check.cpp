@@ -18,7 +18,7 @@ namespace s2m::fe {
 
 namespace {
 
-const Type kVoid{Base::Void, 1}, kBool{Base::Bool, 1}, kF32{Base::F32, 1}, kI32{Base::I32, 1},
+const Type kBool{Base::Bool, 1}, kF32{Base::F32, 1}, kI32{Base::I32, 1},
     kU32{Base::U32, 1}, kAF{Base::AbsFloat, 1}, kAI{Base::AbsInt, 1};
 
 Type vec(Base b, int n) { return {b, (uint8_t)n}; }

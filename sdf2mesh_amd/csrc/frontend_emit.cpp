@@ -787,7 +787,7 @@ bool emit_hip(const Module& m, Lang lang, const std::string& entry, std::string&
         E.cur_ret = f.ret;
         o << E.signature(f) << "\n{\n";
         E.body(f.body, "    ");
-        if (f.ret.base != Base::Void) o << "    return " << scalar_literal(0, concrete(f.ret).base) << ";  // unreachable\n";
+        if (f.ret.base != Base::Void) o << "    return " << ctype(f.ret) << "{};  // not reached\n";
         o << "}\n";
         if (E.err.code) break;
     }

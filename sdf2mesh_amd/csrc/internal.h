@@ -62,6 +62,8 @@ struct SlabArgs {
 };
 
 constexpr uint32_t kTile = 8;  // cells per tile edge (sparse pipeline)
+// Profiling marks of one sparse pass: prune | eval | count+scan | emit+normals | quads
+constexpr int kSparseMarks = 6;
 
 // Tile-sparse pipeline (sparse.hip).  One job of a pass: a slab of a single job, or a
 // whole job of a batch.  Pointers are device addresses.

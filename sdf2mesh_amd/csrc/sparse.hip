@@ -540,8 +540,9 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
     };
     mark(0);
     hipLaunchKernelGGL(k_sp_prune, dim3((uint32_t)((P.ntiles + 63) / 64)), dim3(64), 0, s, P);
-    hipLaunchKernelGGL(k_sp_eval, dim3(pb), dim3(256), 0, s, P);
     mark(1);
+    hipLaunchKernelGGL(k_sp_eval, dim3(pb), dim3(256), 0, s, P);
+    mark(2);
     hipLaunchKernelGGL((k_sp_cells<0>), dim3(pb), dim3(256), smem, s, P, b);
     if (nsb) {
         hipLaunchKernelGGL(k_sp_reduce, dim3(nsb), dim3(256), 0, s, P.seg_v, P.nseg, b.bcount);
@@ -550,9 +551,9 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
                            P.seg_off_v);
     }
     hipLaunchKernelGGL((k_sp_ranges<0>), dim3(jb), dim3(64), 0, s, P);
-    mark(2);
-    hipLaunchKernelGGL((k_sp_cells<1>), dim3(pb), dim3(256), smem, s, P, b);
     mark(3);
+    hipLaunchKernelGGL((k_sp_cells<1>), dim3(pb), dim3(256), smem, s, P, b);
+    mark(4);
     hipLaunchKernelGGL((k_sp_quads<0>), dim3(pb), dim3(256), 0, s, P, b);
     if (nsb) {
         hipLaunchKernelGGL(k_sp_reduce, dim3(nsb), dim3(256), 0, s, P.seg_t, P.nseg, b.bcount);
@@ -566,7 +567,7 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
         hipLaunchKernelGGL(k_sp_boundary, dim3((uint32_t)((P.boundary_cells + 255) / 256)), dim3(256), 0,
                            s, P);
     }
-    mark(4);
+    mark(5);
     return hipGetLastError();
 }
 

@@ -340,3 +340,25 @@ def test_glsl_named_sdf_and_sdf3d_wrapper():
 def test_hiprtc_builds_code_objects(path, lang):
     code = s2m.translate(path, s2m.STAGE_CODE, lang=lang, path=True)
     assert code[:4] == b"\x7fELF" and len(code) > 1000
+
+
+@pytest.mark.parametrize("src,lang", [
+    # vector-valued user functions, a user normal, integer vectors, every builtin family
+    ("use sdf3d::primitives;\nfn sdf3d(p: vec3f) -> f32 { return sdf3d_sphere(p, 0.6); }\n"
+     "fn sdf3d_normal(p: vec3f, eps: f32) -> vec3f { return vec3(p.x, 2.0 * p.y, 0.0); }\n", "sdf3d"),
+    ("use sdf3d::*;\nfn f(q: vec2f) -> vec2f { return abs(q) - vec2(0.1); }\n"
+     "fn sdf3d(p: vec3f) -> f32 { let i = vec3<i32>(1, 2, 3); var u = 7u; u = u / 2u + u % 3u;\n"
+     "  let a = f(p.xy); let s = sin(p.x) + cos(p.y) + tan(0.1 * p.z) + exp(-p.x) + log(2.0 + p.y)\n"
+     "  + pow(abs(p.z), 1.5) + sqrt(abs(p.x)) + inverseSqrt(1.0 + p.y * p.y) + floor(p.x) + ceil(p.y)\n"
+     "  + fract(p.z) + sign(p.x) + step(0.0, p.y) + smoothstep(0.0, 1.0, p.z) + atan2(p.y, p.x)\n"
+     "  + trunc(p.x) + round(p.y) + exp2(p.z) + log2(2.0 + p.x) + asin(0.5) + acos(0.5);\n"
+     "  return length(a) + dot(cross(p, vec3(0.0, 1.0, 0.0)), normalize(vec3(1.0))) + distance(p, vec3(0.5))\n"
+     "  + f32(i.y) * 0.01 + f32(u) * 0.001 + s * 1e-6 + select(0.0, 1.0, p.x > 0.0) - 0.5; }\n", "sdf3d"),
+    ("#version 450\nvec2 g(vec2 q) { return q * 2.0; }\nfloat sdf(vec3 p) { ivec3 k = ivec3(1, 2, 3);\n"
+     "  float m = mod(p.x, 0.3) + (p.y > 0.0 ? 1.0 : 0.0) + atan(p.y, p.x) + inversesqrt(2.0);\n"
+     "  int n = 0; while (n < 3) { n++; } return length(g(p.xy)) - 0.5 + m * 0.01 + float(k.x + n) * 0.0; }\n"
+     "void main() {}\n", "glsl"),
+])
+def test_hiprtc_builds_varied_programs(src, lang):
+    code = s2m.translate(src, s2m.STAGE_CODE, lang=lang)
+    assert code[:4] == b"\x7fELF"
