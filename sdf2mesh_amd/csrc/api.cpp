@@ -98,7 +98,8 @@ struct s2m_ctx {
     DevBuf vtx, keys, signs, tri, corners;
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_boxes, sp_vbox, sp_seg, sp_boundary;
-    DevBuf sp_job_out;
+    DevBuf sp_stage, sp_slot_stage;
+    DevBuf sp_job_out, sp_super_lists, sp_super_lens;
     uint64_t slot_cap = 0;
     int cus = 256;
     std::vector<uint8_t> host_img;  // staging of the last pass's tables (kept until its sync)
@@ -444,7 +445,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     HIP_TRY(ctx->tile_tab.ensure(img.size()));
     uint8_t* dimg = ctx->tile_tab.as<uint8_t>();
     std::vector<SpJob> desc(njob_desc);
-    uint64_t tile_base = 0, seg_base = 0;
+    uint64_t tile_base = 0, seg_base = 0, super_base = 0, nsuper_pass = 0;
     for (size_t d = 0; d < njob_desc; ++d) {
         const uint32_t i = n == 1 ? 0 : (uint32_t)d;
         const Job& j = jobs[i];
@@ -471,6 +472,14 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         J.ntx = T.nt[0];
         J.nty = T.nt[1];
         J.ntz = T.nt[2];
+        // super-tiles: kSuper^3 tiles, over the tile layers this pass holds
+        J.nsx = (J.ntx + kSuper - 1) / kSuper;
+        J.nsy = (J.nty + kSuper - 1) / kSuper;
+        J.nsz = ((J.nz + kTile - 1) / kTile + kSuper - 1) / kSuper;
+        const uint64_t nsup = (uint64_t)J.nsx * J.nsy * J.nsz;
+        J.super_base = n == 1 ? 0 : super_base;
+        super_base += nsup;
+        nsuper_pass = n == 1 ? std::max(nsuper_pass, nsup) : super_base;
         J.nops = progs[i]->n;
         J.flags = ps[i]->flags;
         J.bx = T.bmax[0];
@@ -501,6 +510,8 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     const SpJob* ddesc = reinterpret_cast<const SpJob*>(dimg + desc_off);
 
     HIP_TRY(ctx->tile_lists.ensure(tiles_pass * list_stride * sizeof(uint16_t)));
+    HIP_TRY(ctx->sp_super_lists.ensure(std::max<uint64_t>(nsuper_pass, 1) * list_stride * sizeof(uint16_t)));
+    HIP_TRY(ctx->sp_super_lens.ensure(std::max<uint64_t>(nsuper_pass, 1) * sizeof(uint32_t)));
     HIP_TRY(ctx->tile_lens.ensure(tiles_pass * sizeof(uint32_t)));
     HIP_TRY(ctx->tile_flops.ensure(tiles_pass * sizeof(uint32_t)));
     HIP_TRY(ctx->sp_slot_of_tile.ensure(tiles_pass * sizeof(int32_t)));
@@ -527,6 +538,8 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         HIP_TRY(ctx->sp_job_of_slot.ensure(want_s * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_boxes.ensure(want_s * box_stride * sizeof(float)));
         HIP_TRY(ctx->sp_vbox.ensure(want_s * 512 * sizeof(uint32_t)));
+        HIP_TRY(ctx->sp_stage.ensure(want_v * sizeof(float4)));
+        HIP_TRY(ctx->sp_slot_stage.ensure(want_s * 2 * sizeof(uint32_t)));
         ctx->vcap = std::max(ctx->vcap, want_v);
         ctx->tcap = std::max(ctx->tcap, want_t);
         ctx->slot_cap = std::max(ctx->slot_cap, want_s);
@@ -569,6 +582,14 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
                 P.ntiles = ntiles_all;
                 P.nseg = nseg_all;
             }
+            if (n == 1) {
+                const SpJob& J = desc[pass];
+                P.nsuper = (uint64_t)J.nsx * J.nsy * J.nsz;
+            } else {
+                P.nsuper = nsuper_pass;
+            }
+            P.super_lists = ctx->sp_super_lists.as<uint16_t>();
+            P.super_lens = ctx->sp_super_lens.as<uint32_t>();
             P.lists = ctx->tile_lists.as<uint16_t>();
             P.lens = ctx->tile_lens.as<uint32_t>();
             P.flops = ctx->tile_flops.as<uint32_t>();
@@ -579,6 +600,9 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.box_stride = box_stride;
             P.boxes = ctx->sp_boxes.as<float>();
             P.vbox = ctx->sp_vbox.as<uint32_t>();
+            P.stage = ctx->sp_stage.as<float4>();
+            P.stage_cap = want_v;
+            P.slot_stage = ctx->sp_slot_stage.as<uint32_t>();
             P.seg_v = ctx->sp_seg.as<uint32_t>();
             P.seg_t = P.seg_v + P.nseg;
             P.seg_off_v = P.seg_t + P.nseg;
@@ -979,7 +1003,8 @@ void s2m_destroy(s2m_ctx* ctx)
                       &ctx->tile_tab, &ctx->tile_lists, &ctx->tile_lens,
                       &ctx->tile_flops, &ctx->sp_slot_of_tile, &ctx->sp_tile_of_slot,
                       &ctx->sp_job_of_slot, &ctx->sp_boxes, &ctx->sp_vbox, &ctx->sp_seg,
-                      &ctx->sp_boundary, &ctx->sp_job_out})
+                      &ctx->sp_boundary, &ctx->sp_job_out, &ctx->sp_super_lists, &ctx->sp_super_lens,
+                      &ctx->sp_stage, &ctx->sp_slot_stage})
         b->release();
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);

@@ -1,0 +1,175 @@
+// device_staged.h - a tile's pruned program staged in LDS and run with wave-uniform
+// dispatch.
+//
+// Every lane of a block evaluates the same pruned program (one tile per block), so the
+// opcode is read once per op for the whole wave (readfirstlane) and the dispatch is a
+// scalar branch; operands are LDS broadcasts.  Programs whose postfix form is a left fold
+// of library primitives (P0 P1 op1 P2 op2 ...: CSG built by accumulation, and every
+// pruned sub-program of one) run without an operand stack, several points per lane
+// sharing each op's decode; other programs run on a uniform-depth stack.  Values are
+// bit-identical to tape_eval (device_sdf.h): the same functions in the same order.
+#pragma once
+#include "device_sdf.h"
+
+namespace s2m {
+
+constexpr uint32_t kStageOps = 128;            // longest program staged in LDS
+constexpr uint32_t kLinearFlag = 0x80000000u;  // SpPass::lens: the program is a left fold
+
+struct StagedTape {
+    uint32_t code[kStageOps];
+    uint32_t iarg[kStageOps];
+    float f[kStageOps][8];  // primitives use f[0..6], operators f[0]
+};
+
+// Copy ops[L[k]], k < n <= kStageOps, into T (all threads of the block take part; the
+// caller synchronises before and after).
+__device__ __forceinline__ void stage_tape(StagedTape& T, const s2m_op* __restrict__ ops,
+                                           const uint16_t* __restrict__ L, uint32_t n)
+{
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        const s2m_op& o = ops[L[k]];
+        T.code[k] = o.code;
+        T.iarg[k] = o.iarg;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) T.f[k][j] = o.f[j];
+    }
+}
+
+__device__ __forceinline__ uint32_t uniform_code(const StagedTape& T, uint32_t k)
+{
+    return __builtin_amdgcn_readfirstlane(T.code[k]);
+}
+
+// Primitive k of T at NP points (code is wave-uniform).  MANDEL = false leaves the
+// Mandelbulb out of the code (left folds never contain one).
+template <int NP, bool MANDEL>
+__device__ __forceinline__ void staged_prim(const StagedTape& T, uint32_t k, uint32_t code,
+                                            const float (&x)[NP], const float (&y)[NP],
+                                            const float (&z)[NP], float (&d)[NP])
+{
+    const float* f = T.f[k];
+    switch (code) {
+    case S2M_OP_SPHERE: {
+        const float c0 = f[0], c1 = f[1], c2 = f[2], s = f[3];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) d[p] = sd_sphere(x[p] - c0, y[p] - c1, z[p] - c2, s);
+        break;
+    }
+    case S2M_OP_BOX: {
+        const float c0 = f[0], c1 = f[1], c2 = f[2], b0 = f[3], b1 = f[4], b2 = f[5];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) d[p] = sd_box(x[p] - c0, y[p] - c1, z[p] - c2, b0, b1, b2);
+        break;
+    }
+    case S2M_OP_TORUS: {
+        const float c0 = f[0], c1 = f[1], c2 = f[2], t0 = f[3], t1 = f[4];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) d[p] = sd_torus(x[p] - c0, y[p] - c1, z[p] - c2, t0, t1);
+        break;
+    }
+    case S2M_OP_CYLINDER: {
+        const float c0 = f[0], c1 = f[1], c2 = f[2], h = f[3], r = f[4];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) d[p] = sd_cylinder(x[p] - c0, y[p] - c1, z[p] - c2, h, r);
+        break;
+    }
+    case S2M_OP_CAPSULE: {
+        const float a0 = f[0], a1 = f[1], a2 = f[2], b0 = f[3], b1 = f[4], b2 = f[5], r = f[6];
+#pragma unroll
+        for (int p = 0; p < NP; ++p) d[p] = sd_capsule(x[p], y[p], z[p], a0, a1, a2, b0, b1, b2, r);
+        break;
+    }
+    default:
+        if (MANDEL) {
+            const float pw = f[0], bo = f[1], off = f[2];
+            const uint32_t ia = __builtin_amdgcn_readfirstlane(T.iarg[k]);
+#pragma unroll
+            for (int p = 0; p < NP; ++p) d[p] = sd_mandelbulb(x[p], y[p], z[p], pw, bo, off, ia);
+        }
+        break;
+    }
+}
+
+// acc = op(acc, d) at NP points (code is wave-uniform)
+template <int NP>
+__device__ __forceinline__ void staged_op(uint32_t code, float k, float (&acc)[NP], const float (&d)[NP])
+{
+    switch (code) {
+    case S2M_OP_UNION:
+#pragma unroll
+        for (int p = 0; p < NP; ++p) acc[p] = fminf(acc[p], d[p]);
+        break;
+    case S2M_OP_INTERSECTION:
+#pragma unroll
+        for (int p = 0; p < NP; ++p) acc[p] = fmaxf(acc[p], d[p]);
+        break;
+    case S2M_OP_SUBTRACTION:
+#pragma unroll
+        for (int p = 0; p < NP; ++p) acc[p] = fmaxf(-acc[p], d[p]);
+        break;
+    case S2M_OP_SMOOTH_UNION:
+#pragma unroll
+        for (int p = 0; p < NP; ++p) acc[p] = op_smooth_union(acc[p], d[p], k);
+        break;
+    case S2M_OP_SMOOTH_INTERSECTION:
+#pragma unroll
+        for (int p = 0; p < NP; ++p) acc[p] = op_smooth_intersection(acc[p], d[p], k);
+        break;
+    default:
+#pragma unroll
+        for (int p = 0; p < NP; ++p) acc[p] = op_smooth_subtraction(acc[p], d[p], k);
+        break;
+    }
+}
+
+// A left-fold program (kLinearFlag): acc = P0; acc = op_i(acc, P_i) for i = 1, 2, ...
+template <int NP>
+__device__ __forceinline__ void run_linear(const StagedTape& T, uint32_t n, const float (&x)[NP],
+                                           const float (&y)[NP], const float (&z)[NP], float (&acc)[NP])
+{
+    staged_prim<NP, false>(T, 0, uniform_code(T, 0), x, y, z, acc);
+    for (uint32_t k = 1; k + 1 < n; k += 2) {
+        float d[NP];
+        staged_prim<NP, false>(T, k, uniform_code(T, k), x, y, z, d);
+        staged_op<NP>(uniform_code(T, k + 1), T.f[k + 1][0], acc, d);
+    }
+}
+
+// Any valid program: the postfix stack machine of tape_eval with a wave-uniform depth.
+__device__ __forceinline__ float run_general(const StagedTape& T, uint32_t n, float px, float py, float pz)
+{
+    float st[S2M_MAX_STACK];
+    float acc = 0.0f;
+    uint32_t sp = 0;
+    const float x[1] = {px}, y[1] = {py}, z[1] = {pz};
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t code = uniform_code(T, k);
+        if (code < S2M_OP_UNION) {
+            float d[1];
+            staged_prim<1, true>(T, k, code, x, y, z, d);
+            if (k != 0) st[sp++] = acc;
+            acc = d[0];
+        } else {
+            float a[1] = {st[--sp]};
+            const float d[1] = {acc};
+            staged_op<1>(code, T.f[k][0], a, d);
+            acc = a[0];
+        }
+    }
+    return acc;
+}
+
+// Is the postfix program of these codes a left fold of library primitives?
+__device__ __forceinline__ bool is_left_fold(const s2m_op* ops, const uint16_t* L, uint32_t n)
+{
+    if (n == 0 || n > kStageOps || (n & 1u) == 0) return false;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t c = ops[L[k]].code;
+        const bool prim = c < S2M_OP_UNION;
+        if (prim != (k == 0 || (k & 1u)) || c == S2M_OP_MANDELBULB) return false;
+    }
+    return true;
+}
+
+}  // namespace s2m

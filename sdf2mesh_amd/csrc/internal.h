@@ -34,6 +34,7 @@ enum Counter : uint32_t {
     kEvalFlops = 7,    // algorithmic flops executed by the pruned tapes (tiled path)
     kSlotsUsed = 8,    // active tiles of the current slab (sparse path)
     kSlotsMax = 9,     // largest kSlotsUsed of the job so far
+    kStageUsed = 10,   // staged active cells of the current pass (sparse path)
     kNumCounters = 16
 };
 
@@ -62,6 +63,8 @@ struct SlabArgs {
 };
 
 constexpr uint32_t kTile = 8;  // cells per tile edge (sparse pipeline)
+constexpr uint32_t kSuper = 4;  // tiles per super-tile edge (first pruning level)
+constexpr uint32_t kSingleSign = 0xffffffffu;  // super_lens: the SDF keeps one sign there
 // Profiling marks of one sparse pass: prune | eval | count+scan | emit+normals | quads
 constexpr int kSparseMarks = 6;
 
@@ -81,6 +84,8 @@ struct SpJob {
     double pad;                // tile box padding (normal taps, rounding)
     uint64_t tile_base;        // first tile of the job in the pass's numbering
     uint64_t seg_base;         // first row segment of the job
+    uint64_t super_base;       // first super-tile of the job
+    uint32_t nsx, nsy, nsz;    // super-tiles of the job in this pass per axis
     const s2m_op* ops;
     const float* cx;
     const float* cy;
@@ -104,6 +109,9 @@ struct SpPass {
     uint64_t ntiles;           // tiles of the pass
     uint64_t nseg;             // row segments of the pass
     uint64_t boundary_cells;   // rx * ry of jobs[0] when a boundary plane is written
+    uint64_t nsuper;           // super-tiles of the pass
+    uint16_t* super_lists;     // [super-tile][list_stride] pruned op indices
+    uint32_t* super_lens;      // [super-tile] list length or kSingleSign
     uint16_t* lists;           // [tile][list_stride] pruned op indices
     uint32_t* lens;            // [tile]
     uint32_t* flops;           // [tile] flops per point of the pruned tape
@@ -114,6 +122,9 @@ struct SpPass {
     uint32_t box_stride;       // floats per slot box
     float* boxes;              // [slot][box_stride]
     uint32_t* vbox;            // [slot][512] vertex of each cell of the tile, kNone if none
+    float4* stage;             // [stage_cap] active cells: position, cell | rank << 9 | signs << 12
+    uint64_t stage_cap;
+    uint32_t* slot_stage;      // [slot][2] first staged cell, staged cells
     uint32_t* seg_v;           // [nseg] vertices per row segment
     uint32_t* seg_t;           // triangles per row segment
     uint32_t* seg_off_v;       // exclusive offsets: global vertex index

@@ -1,21 +1,27 @@
 // sparse.hip - the tile-sparse meshing pipeline, for one job or a batch of jobs.
 //
 // Programs that interval pruning applies to (every library primitive and operator except
-// the Mandelbulb) are meshed tile by tile: cells are grouped in tiles of 8x8x8 and only
-// tiles whose SDF can change sign are evaluated and contoured.  A pass covers the tiles
-// of one slab of one job, or of every job of a batch (each job whole); the tiles of the
-// pass are numbered job after job.  In stream order:
-//   k_sp_prune      one thread per tile: interval bounds of the job's tape on the tile's
-//                   padded box; single-signed tiles are dropped, the others get a slot in
-//                   a pool shared by all jobs and an exactly pruned op list
-//   k_sp_eval       per slot (grid-stride, persistent): the pruned tape at every lattice
-//                   point of the tile's closed box, stored in the slot's compact box
-//   k_sp_cells<0>   per slot: active cells per row segment (z, y, tile x)
+// the Mandelbulb, which is kept whole) are meshed tile by tile: cells are grouped in
+// tiles of 8x8x8 and super-tiles of 4x4x4 tiles, and only tiles whose SDF can change sign
+// are evaluated and contoured.  A pass covers the tiles of one slab of one job, or of
+// every job of a batch (each job whole); tiles, super-tiles and row segments of the pass
+// are numbered job after job.  In stream order:
+//   k_sp_prune_super  one thread per super-tile: interval bounds of the job's program on
+//                   its padded box; single-signed super-tiles are marked, the others get
+//                   an exactly pruned op list
+//   k_sp_prune      one thread per tile: the super-tile's list pruned on the tile's padded
+//                   box; single-signed tiles are dropped, the others get a slot in a pool
+//                   shared by all jobs (and a flag when their list is a left fold)
+//   k_sp_eval<1/0>  per slot (grid-stride, persistent): the pruned program, staged in LDS,
+//                   at every lattice point of the tile's closed box, stored in the slot's
+//                   compact box (left folds / other programs)
+//   k_sp_cells      per slot: vertex and sign bits of each active cell, staged compactly,
+//                   and the active cells of each row segment (z, y, tile x)
 //   reduce / k_scan / downsweep   exclusive scan of the row segments, job after job and
 //                   (z, y, tile x) inside a job: the reference's VertexList order
 //   k_sp_ranges<0>  each job's first vertex index
-//   k_sp_cells<1>   vertices at their ordered index, the slot's cell -> vertex box, and
-//                   the 4-tap normals of the tile's vertices with its pruned tape
+//   k_sp_emit       the staged vertices at their ordered index, the slot's cell -> vertex
+//                   box, and the 4-tap normals with the tile's pruned program
 //   k_sp_quads<0/1> the same count / scan / emit scheme for triangles (vertex order), with
 //                   vertex indices relative to the job's first vertex
 //   k_sp_ranges<1>  each job's first triangle index
@@ -23,6 +29,7 @@
 // Loops over slots read their bound on the device: no host round trip inside a pass.
 #include "device_interval.h"
 #include "device_sdf.h"
+#include "device_staged.h"
 #include "internal.h"
 
 namespace s2m {
@@ -106,21 +113,131 @@ __device__ __forceinline__ bool contour_cell_sp(const float c[8], const float lo
 
 }  // namespace
 
+namespace {
+
+// Exact pruning of a postfix program on the box [bl, bh] (device_interval.h): `in` lists
+// the program's op indices (nullptr: every op of `ops` in order).  The kept op indices
+// are written to `out` in order; returns their count, and the interval of the result in
+// `root`.  The top of the interval stack stays in registers; deeper entries live in the
+// block's LDS (PruneStack, one column per lane: a thread's depth differs from its
+// neighbours' when they prune different jobs).
+struct PruneStack {
+    double lo[S2M_MAX_STACK][64];
+    double hi[S2M_MAX_STACK][64];
+    uint16_t start[S2M_MAX_STACK][64];
+};
+
+__device__ uint32_t prune_program(const s2m_op* __restrict__ ops, const uint16_t* __restrict__ in,
+                                  uint32_t n, const double bl[3], const double bh[3],
+                                  uint16_t* __restrict__ out, Iv& root, PruneStack& S)
+{
+    const uint32_t ln = threadIdx.x & 63u;
+    Iv top = iv_unknown();
+    uint32_t top_start = 0, depth = 0, len = 0;  // depth: entries below `top`
+    for (uint32_t q = 0; q < n; ++q) {
+        const uint32_t i = in ? in[q] : q;
+        const uint32_t code = ops[i].code;
+        if (code < S2M_OP_UNION) {
+            if (q != 0) {  // push the previous top
+                S.lo[depth][ln] = top.lo;
+                S.hi[depth][ln] = top.hi;
+                S.start[depth][ln] = (uint16_t)top_start;
+                ++depth;
+            }
+            top = iv_primitive(code, ops[i].f, bl, bh);
+            top_start = len;
+            out[len++] = (uint16_t)i;
+        } else {
+            const Iv B = top;
+            const uint32_t sB = top_start;
+            --depth;
+            const Iv A{S.lo[depth][ln], S.hi[depth][ln], S.lo[depth][ln] > -kHuge};
+            const uint32_t sA = S.start[depth][ln];
+            int keep;
+            const Iv r = iv_op(code, A, B, ops[i].f[0], keep);
+            if (keep == kKeepBoth) {
+                out[len++] = (uint16_t)i;
+                top = r;
+            } else if (keep == kKeepA) {
+                len = sB;  // drop B's subtree and the operator
+                top = A;
+            } else {       // drop A's subtree and the operator
+                const uint32_t nb = len - sB;
+                for (uint32_t j = 0; j < nb; ++j) out[sA + j] = out[sB + j];
+                len = sA + nb;
+                top = B;
+            }
+            top_start = sA;
+        }
+    }
+    root = top;
+    return len;
+}
+
+__device__ __forceinline__ uint32_t job_of(const SpPass& P, uint64_t index, bool super)
+{
+    uint32_t lo = 0, hi = P.njobs - 1;  // last job whose first tile (super-tile) <= index
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if ((super ? P.jobs[mid].super_base : P.jobs[mid].tile_base) <= index) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+// One thread per super-tile (kSuper^3 tiles): the job's program pruned on the super-tile's
+// padded box.  A super-tile whose SDF keeps one sign gets kSingleSign and no list.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_sp_prune_super(SpPass P)
+{
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P.nsuper) return;
+    const SpJob& J = P.jobs[job_of(P, s, true)];
+    const uint32_t t = (uint32_t)(s - J.super_base);
+    const uint32_t sx = t % J.nsx, sy = (t / J.nsx) % J.nsy, szl = t / (J.nsx * J.nsy);
+    const uint32_t tx0 = sx * kSuper, tx1 = min(tx0 + kSuper, J.ntx) - 1;
+    const uint32_t ty0 = sy * kSuper, ty1 = min(ty0 + kSuper, J.nty) - 1;
+    const uint32_t layers = (J.nz + kTile - 1) / kTile;  // tile layers of this pass
+    const uint32_t tz0 = J.tz0 + szl * kSuper, tz1 = J.tz0 + min(szl * kSuper + kSuper, layers) - 1;
+    const float* ex = J.ext;
+    const float* ey = ex + 2 * J.ntx;
+    const float* ez = ey + 2 * J.nty;
+    const double bl[3] = {(double)ex[tx0] - J.pad, (double)ey[ty0] - J.pad, (double)ez[tz0] - J.pad};
+    const double bh[3] = {(double)ex[J.ntx + tx1] + J.pad, (double)ey[J.nty + ty1] + J.pad,
+                          (double)ez[J.ntz + tz1] + J.pad};
+    __shared__ PruneStack S;
+    Iv root;
+    const uint32_t len = prune_program(J.ops, nullptr, J.nops, bl, bh,
+                                       P.super_lists + (size_t)s * P.list_stride, root, S);
+    const bool one_sign = root.known && (root.lo > 0.0 || root.hi < 0.0);
+    P.super_lens[s] = one_sign ? kSingleSign : len;
+}
+
+// ---------------------------------------------------------------------------------
+// One thread per tile: its super-tile's program pruned on the tile's padded box; tiles
+// that keep one sign are dropped, the others get a slot in the pool shared by all jobs.
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
 {
     const uint64_t tile = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (tile >= P.ntiles) return;
-    // job of the tile: jobs are sorted by tile_base
-    uint32_t lo = 0, hi = P.njobs - 1;
-    while (lo < hi) {
-        const uint32_t mid = (lo + hi + 1) / 2;
-        if (P.jobs[mid].tile_base <= tile) lo = mid;
-        else hi = mid - 1;
-    }
-    const SpJob& J = P.jobs[lo];
+    const uint32_t jid = job_of(P, tile, false);
+    const SpJob& J = P.jobs[jid];
     uint32_t tx, ty, tzl;
     tile_xyz(J, tile, tx, ty, tzl);
+    const uint64_t s =
+        J.super_base + ((uint64_t)(tzl / kSuper) * J.nsy + ty / kSuper) * J.nsx + tx / kSuper;
+    const uint32_t sl = P.super_lens[s];
+    int32_t slot = -1;
+    if (sl == kSingleSign) {  // no surface anywhere in the super-tile
+        P.lens[tile] = 0;
+        P.flops[tile] = 0;
+        P.slot_of_tile[tile] = slot;
+        return;
+    }
     const uint32_t tz = J.tz0 + tzl;
     const float* ex = J.ext;
     const float* ey = ex + 2 * J.ntx;
@@ -130,89 +247,107 @@ __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
                           (double)ez[J.ntz + tz] + J.pad};
     const s2m_op* ops = J.ops;
     uint16_t* L = P.lists + (size_t)tile * P.list_stride;
-    Iv st[S2M_MAX_STACK];
-    uint32_t start[S2M_MAX_STACK];
-    uint32_t sp = 0, len = 0;
-    for (uint32_t i = 0; i < J.nops; ++i) {
-        const uint32_t code = ops[i].code;
-        if (code < S2M_OP_UNION) {
-            st[sp] = iv_primitive(code, ops[i].f, bl, bh);
-            start[sp] = len;
-            ++sp;
-            L[len++] = (uint16_t)i;
-        } else {
-            const Iv B = st[--sp];
-            const uint32_t sB = start[sp];
-            const Iv A = st[--sp];
-            const uint32_t sA = start[sp];
-            int keep;
-            const Iv r = iv_op(code, A, B, ops[i].f[0], keep);
-            if (keep == kKeepBoth) {
-                L[len++] = (uint16_t)i;
-                st[sp] = r;
-            } else if (keep == kKeepA) {
-                len = sB;  // drop B's subtree and the operator
-                st[sp] = A;
-            } else {       // drop A's subtree and the operator
-                const uint32_t nb = len - sB;
-                for (uint32_t j = 0; j < nb; ++j) L[sA + j] = L[sB + j];
-                len = sA + nb;
-                st[sp] = B;
-            }
-            start[sp] = sA;
-            ++sp;
-        }
-    }
-    const Iv root = st[0];
+    __shared__ PruneStack S;
+    Iv root;
+    const uint32_t len =
+        prune_program(ops, P.super_lists + (size_t)s * P.list_stride, sl, bl, bh, L, root, S);
     const bool one_sign = root.known && (root.lo > 0.0 || root.hi < 0.0);
     uint32_t fl = 0;
     for (uint32_t j = 0; j < len; ++j) fl += J.op_flops[L[j]];
-    P.lens[tile] = len;
+    P.lens[tile] = len | (!one_sign && is_left_fold(ops, L, len) ? kLinearFlag : 0u);
     P.flops[tile] = fl;
-    int32_t slot = -1;
     if (!one_sign) {
-        const uint64_t s = atomicAdd((unsigned long long*)&P.counters[kSlotsUsed], 1ull);
-        if (s < P.slot_cap) {
-            slot = (int32_t)s;
-            P.tile_of_slot[s] = (uint32_t)tile;
-            P.job_of_slot[s] = lo;
+        const uint64_t sidx = atomicAdd((unsigned long long*)&P.counters[kSlotsUsed], 1ull);
+        if (sidx < P.slot_cap) {
+            slot = (int32_t)sidx;
+            P.tile_of_slot[sidx] = (uint32_t)tile;
+            P.job_of_slot[sidx] = jid;
         }
     }
     P.slot_of_tile[tile] = slot;
-    if (slot < 0) {  // no vertex in this tile: zero its row segments for both scans
-        const uint32_t z1 = min(tzl * 8u + 8u, J.nz), y1 = min(ty * 8u + 8u, J.ry);
-        for (uint32_t zl = tzl * 8u; zl < z1; ++zl)
-            for (uint32_t y = ty * 8u; y < y1; ++y) {
-                P.seg_v[seg_index(J, zl, y, tx)] = 0;
-                P.seg_t[seg_index(J, zl, y, tx)] = 0;
-            }
-    }
 }
 
 // ---------------------------------------------------------------------------------
+// n / d for n, d < 2^16 with m = ceil(2^32 / d): the error term n * (m*d - 2^32) / 2^32 / d
+// stays below 1/d, so it never reaches the next integer.
+__device__ __forceinline__ uint32_t magic16(uint32_t d)
+{
+    return d <= 1u ? 0u : (uint32_t)((0x100000000ull + d - 1u) / d);
+}
+__device__ __forceinline__ uint32_t div16(uint32_t n, uint32_t d, uint32_t m)
+{
+    return d <= 1u ? n : __umulhi(n, m);
+}
+
+// Per slot (persistent, grid-stride): the tile's pruned program at every lattice point
+// of its closed box, staged in LDS.  LINEAR = 1 runs the slots whose program is a left
+// fold (two points per lane), LINEAR = 0 the others; both launches walk all slots and
+// skip the other kind.
+constexpr int kLinearPoints = 2;
+
+template <int LINEAR>
 __global__ __launch_bounds__(256) void k_sp_eval(SpPass P)
 {
+    __shared__ StagedTape T;
     const uint32_t nslots = active_slots(P);
     for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
-        const uint64_t tile = P.tile_of_slot[slot];
-        const SpJob& J = P.jobs[P.job_of_slot[slot]];
+        const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
+        const uint32_t lens = __builtin_amdgcn_readfirstlane(P.lens[tile]);
+        if (((lens & kLinearFlag) != 0) != (LINEAR != 0)) continue;  // same for the whole block
+        const uint32_t n = lens & ~kLinearFlag;
+        const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(P.job_of_slot[slot])];
         uint32_t tx, ty, tzl;
         tile_xyz(J, tile, tx, ty, tzl);
         const uint32_t tz = J.tz0 + tzl;
         const uint32_t* bxr = J.ibox;
         const uint32_t* byr = bxr + 2 * J.ntx;
         const uint32_t* bzr = byr + 2 * J.nty;
-        const uint32_t x0 = bxr[tx], nx = bxr[J.ntx + tx] - x0 + 1;
-        const uint32_t y0 = byr[ty], ny = byr[J.nty + ty] - y0 + 1;
-        const uint32_t z0 = bzr[tz], nz = bzr[J.ntz + tz] - z0 + 1;
-        const uint32_t npts = nx * ny * nz;
+        const uint32_t x0 = __builtin_amdgcn_readfirstlane(bxr[tx]);
+        const uint32_t y0 = __builtin_amdgcn_readfirstlane(byr[ty]);
+        const uint32_t z0 = __builtin_amdgcn_readfirstlane(bzr[tz]);
+        const uint32_t nx = __builtin_amdgcn_readfirstlane(bxr[J.ntx + tx]) - x0 + 1;
+        const uint32_t ny = __builtin_amdgcn_readfirstlane(byr[J.nty + ty]) - y0 + 1;
+        const uint32_t nz = __builtin_amdgcn_readfirstlane(bzr[J.ntz + tz]) - z0 + 1;
+        const uint32_t nxy = nx * ny, npts = nxy * nz;
+        const uint32_t mx = magic16(nx), mxy = magic16(nxy);
         const uint16_t* L = P.lists + (size_t)tile * P.list_stride;
-        const uint32_t n = P.lens[tile];
+        const uint32_t bx = J.bx, by = J.by;
+        const float* cx = J.cx + x0;
+        const float* cy = J.cy + y0;
+        const float* cz = J.cz + z0;
         float* box = P.boxes + (size_t)slot * P.box_stride;
-        for (uint32_t q = threadIdx.x; q < npts; q += blockDim.x) {
-            const uint32_t ix = q % nx, iy = (q / nx) % ny, iz = q / (nx * ny);
-            box[(iz * J.by + iy) * J.bx + ix] =
-                tape_eval_list(J.ops, L, n, J.cx[x0 + ix], J.cy[y0 + iy], J.cz[z0 + iz]);
+        const bool staged = n <= kStageOps;
+        __syncthreads();  // the previous slot's readers of T are done
+        if (staged) stage_tape(T, J.ops, L, n);
+        __syncthreads();
+        auto point = [&](uint32_t q, float& x, float& y, float& z) {
+            const uint32_t iz = div16(q, nxy, mxy), r = q - iz * nxy;
+            const uint32_t iy = div16(r, nx, mx), ix = r - iy * nx;
+            x = cx[ix];
+            y = cy[iy];
+            z = cz[iz];
+            return (iz * by + iy) * bx + ix;
+        };
+        if (LINEAR) {
+            for (uint32_t q0 = threadIdx.x; q0 < npts; q0 += kLinearPoints * blockDim.x) {
+                float x[kLinearPoints], y[kLinearPoints], z[kLinearPoints], v[kLinearPoints];
+                uint32_t at[kLinearPoints];
+#pragma unroll
+                for (int p = 0; p < kLinearPoints; ++p) {
+                    const uint32_t q = q0 + p * blockDim.x;
+                    at[p] = point(q < npts ? q : q0, x[p], y[p], z[p]);
+                }
+                run_linear<kLinearPoints>(T, n, x, y, z, v);
+#pragma unroll
+                for (int p = 0; p < kLinearPoints; ++p)
+                    if (q0 + p * blockDim.x < npts) box[at[p]] = v[p];
+            }
+        } else {
+            for (uint32_t q = threadIdx.x; q < npts; q += blockDim.x) {
+                float x, y, z;
+                const uint32_t at = point(q, x, y, z);
+                box[at] = staged ? run_general(T, n, x, y, z) : tape_eval_list(J.ops, L, n, x, y, z);
+            }
         }
         if (threadIdx.x == 0) {
             atomicAdd((unsigned long long*)&P.counters[kTilesActive], 1ull);
@@ -224,130 +359,244 @@ __global__ __launch_bounds__(256) void k_sp_eval(SpPass P)
 }
 
 // ---------------------------------------------------------------------------------
-// Cells of one slot per block iteration; thread t owns cells t and t + 256 of the tile
-// (x fastest), so the 8 cells of a row segment are 8 consecutive lanes.
+// k_sp_cells: the cells of one slot per block iteration; thread t owns cells t and
+// t + 256 of the tile (x fastest), so the 8 cells of a row segment are 8 consecutive
+// lanes.  Per active cell: the vertex (dualcontour.wgsl:72-131) and sign bits (:57-69),
+// staged compactly for k_sp_emit with the cell's rank in its row segment; per row
+// segment: its vertex count.  Every cell's entry of the slot's cell -> vertex box is
+// reset to kNone.
 // ---------------------------------------------------------------------------------
-template <int EMIT>
+struct TileAxes {  // per axis: box-relative lattice index and coordinate of lo / hi corners
+    uint32_t lo[3][8], hi[3][8];
+    float clo[3][8], chi[3][8];
+};
+
 __global__ __launch_bounds__(256) void k_sp_cells(SpPass P, MeshBufs b)
 {
     extern __shared__ float box[];  // P.box_stride floats
-    __shared__ float vpos[512 * 3];
-    __shared__ uint32_t vidx_l[512];
-    __shared__ uint32_t nact;
+    __shared__ TileAxes A;
+    __shared__ uint32_t nact, stage_base;
     const uint32_t nslots = active_slots(P);
     for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
-        const uint64_t tile = P.tile_of_slot[slot];
-        const SpJob& J = P.jobs[P.job_of_slot[slot]];
+        const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
+        const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(P.job_of_slot[slot])];
         uint32_t tx, ty, tzl;
         tile_xyz(J, tile, tx, ty, tzl);
         const uint32_t tz = J.tz0 + tzl;
-        const uint32_t* bxr = J.ibox;
-        const uint32_t* byr = bxr + 2 * J.ntx;
-        const uint32_t* bzr = byr + 2 * J.nty;
-        const uint32_t x0 = bxr[tx], y0 = byr[ty], z0l = bzr[tz];
         const float* gbox = P.boxes + (size_t)slot * P.box_stride;
-        __syncthreads();  // the previous slot's readers of box / vpos are done
+        __syncthreads();  // the previous slot's readers of box / A are done
         for (uint32_t q = threadIdx.x; q < P.box_stride; q += blockDim.x) box[q] = gbox[q];
+        if (threadIdx.x < 24) {  // per-axis corner tables of the tile's 8 cells
+            const uint32_t ax = threadIdx.x >> 3, c = threadIdx.x & 7u;
+            const uint32_t t = ax == 0 ? tx : ax == 1 ? ty : tz;
+            const uint32_t r = ax == 0 ? J.rx : ax == 1 ? J.ry : J.z0 + J.nz;  // cells on the axis
+            const uint32_t* lo = ax == 0 ? J.lox : ax == 1 ? J.loy : J.loz;
+            const uint32_t* hi = ax == 0 ? J.hix : ax == 1 ? J.hiy : J.hiz;
+            const float* co = ax == 0 ? J.cx : ax == 1 ? J.cy : J.cz;
+            const uint32_t org = J.ibox[2 * (ax == 0 ? 0 : ax == 1 ? J.ntx : J.ntx + J.nty) + t];
+            const uint32_t g = t * 8u + c;   // cell index along the axis (z: of the job)
+            const uint32_t gi = g < r ? g : r - 1;  // cells past the end are skipped below
+            A.lo[ax][c] = lo[gi] - org;
+            A.hi[ax][c] = hi[gi] - org;
+            A.clo[ax][c] = co[lo[gi]];
+            A.chi[ax][c] = co[hi[gi]];
+        }
         if (threadIdx.x == 0) nact = 0;
         __syncthreads();
+        bool act[2];
+        float pos[2][3];
+        uint32_t meta[2];
+#pragma unroll
         for (uint32_t r = 0; r < 2; ++r) {
             const uint32_t c = r * 256u + threadIdx.x;
             const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
             const uint32_t x = tx * 8u + cxl, y = ty * 8u + cyl, zl = tzl * 8u + czl;
-            const uint32_t z = J.z0 + zl;
             const bool inside = x < J.rx && y < J.ry && zl < J.nz;
-            bool act = false;
-            float pos[3] = {0.0f, 0.0f, 0.0f}, cv[8];
+            act[r] = false;
             uint32_t sg = 0;
             if (inside) {
-                const uint32_t xi[2] = {J.lox[x] - x0, J.hix[x] - x0};
-                const uint32_t yi[2] = {J.loy[y] - y0, J.hiy[y] - y0};
-                const uint32_t zi[2] = {J.loz[z] - z0l, J.hiz[z] - z0l};
+                float cv[8];
+                const uint32_t xi[2] = {A.lo[0][cxl], A.hi[0][cxl]};
+                const uint32_t yi[2] = {A.lo[1][cyl], A.hi[1][cyl]};
+                const uint32_t zi[2] = {A.lo[2][czl], A.hi[2][czl]};
 #pragma unroll
                 for (int k = 0; k < 8; ++k)
                     cv[k] = box[(zi[(k >> 2) & 1] * J.by + yi[(k >> 1) & 1]) * J.bx + xi[k & 1]];
-                const float lo[3] = {J.cx[J.lox[x]], J.cy[J.loy[y]], J.cz[J.loz[z]]};
-                const float hi[3] = {J.cx[J.hix[x]], J.cy[J.hiy[y]], J.cz[J.hiz[z]]};
-                act = contour_cell_sp(cv, lo, hi, pos);
+                const float lo[3] = {A.clo[0][cxl], A.clo[1][cyl], A.clo[2][czl]};
+                const float hi[3] = {A.chi[0][cxl], A.chi[1][cyl], A.chi[2][czl]};
+                act[r] = contour_cell_sp(cv, lo, hi, pos[r]);
                 // cell_sign_changes (dualcontour.wgsl:57-69)
                 sg = (uint32_t)(cv[1] > 0.0f) | ((uint32_t)(cv[2] > 0.0f) << 1) |
                      ((uint32_t)(cv[4] > 0.0f) << 2) | ((uint32_t)(cv[0] > 0.0f) << 3);
             }
-            const uint64_t m = __ballot(act);
+            const uint64_t m = __ballot(act[r]);
             const uint32_t g0 = lane() & ~7u;
             const uint32_t gmask = (uint32_t)((m >> g0) & 0xffull);
             const uint32_t below = (uint32_t)__builtin_popcount(gmask & ((1u << (lane() - g0)) - 1u));
-            const bool row_ok = y < J.ry && zl < J.nz;
-            if (!EMIT) {
-                if (cxl == 0 && row_ok) P.seg_v[seg_index(J, zl, y, tx)] = (uint32_t)__builtin_popcount(gmask);
-                continue;
-            }
-            uint32_t v = kNone;
-            if (act) {
-                const uint64_t vi = (uint64_t)P.seg_off_v[seg_index(J, zl, y, tx)] + below;
-                v = (uint32_t)vi;
-                if (vi < b.vcap) {
-                    float* o = b.vtx + 6 * vi;
-                    o[0] = pos[0];
-                    o[1] = pos[1];
-                    o[2] = pos[2];
-                    o[3] = 0.0f;
-                    o[4] = 0.0f;
-                    o[5] = 0.0f;
-                    b.keys[vi] = (uint64_t)x | ((uint64_t)y << 16) | ((uint64_t)z << 32);
-                    b.signs[vi] = (uint8_t)sg;
-                }
-                const uint32_t k = atomicAdd(&nact, 1u);
-                vpos[3 * k] = pos[0];
-                vpos[3 * k + 1] = pos[1];
-                vpos[3 * k + 2] = pos[2];
-                vidx_l[k] = v;
-            }
-            P.vbox[(size_t)slot * 512u + c] = v;
+            if (cxl == 0 && y < J.ry && zl < J.nz)
+                P.seg_v[seg_index(J, zl, y, tx)] = (uint32_t)__builtin_popcount(gmask);
+            meta[r] = c | (below << 9) | (sg << 12);
+            P.vbox[(size_t)slot * 512u + c] = kNone;
         }
-        if (EMIT && !(J.flags & S2M_NO_NORMALS)) {
-            __syncthreads();
+        // compact staging of the slot's active cells
+        uint32_t k0 = 0, k1 = 0;
+        if (act[0]) k0 = atomicAdd(&nact, 1u);
+        if (act[1]) k1 = atomicAdd(&nact, 1u);
+        __syncthreads();
+        if (threadIdx.x == 0) {
             const uint32_t n = nact;
-            const uint16_t* L = P.lists + (size_t)tile * P.list_stride;
-            const uint32_t len = P.lens[tile];
-            const float eps = J.eps;
-            for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
-                const uint32_t v = vidx_l[k];
-                if (v >= b.vcap) continue;
-                const float px = vpos[3 * k], py = vpos[3 * k + 1], pz = vpos[3 * k + 2];
-                // sdf3d_normal (sdf3d_normal.wgsl:4-9): taps p + v_i * eps, then normalize
-                const float d1 = tape_eval_list(J.ops, L, len, px + eps, py + (-eps), pz + (-eps));
-                const float d2 = tape_eval_list(J.ops, L, len, px + (-eps), py + (-eps), pz + eps);
-                const float d3 = tape_eval_list(J.ops, L, len, px + (-eps), py + eps, pz + (-eps));
-                const float d4 = tape_eval_list(J.ops, L, len, px + eps, py + eps, pz + eps);
-                const float gx = ((d1 + (-d2)) + (-d3)) + d4;
-                const float gy = (((-d1) + (-d2)) + d3) + d4;
-                const float gz = (((-d1) + d2) + (-d3)) + d4;
-                const float ln = sqrtf((gx * gx + gy * gy) + gz * gz);
-                float* o = b.vtx + 6 * (uint64_t)v;
-                o[3] = gx / ln;
-                o[4] = gy / ln;
-                o[5] = gz / ln;
-            }
+            stage_base = n ? (uint32_t)atomicAdd((unsigned long long*)&P.counters[kStageUsed], (unsigned long long)n) : 0u;
+            P.slot_stage[2 * slot] = stage_base;
+            P.slot_stage[2 * slot + 1] = n;
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t r = 0; r < 2; ++r) {
+            if (!act[r]) continue;
+            const uint64_t at = (uint64_t)stage_base + (r ? k1 : k0);
+            if (at < P.stage_cap)
+                P.stage[at] = make_float4(pos[r][0], pos[r][1], pos[r][2], __uint_as_float(meta[r]));
         }
     }
 }
 
 // ---------------------------------------------------------------------------------
-// Multi-block exclusive scan of a u32 array: per-block sums (kScanChunk entries per
-// block), k_scan over the sums (adds and advances the running total), block-local scan
-// plus the block offset.
+// k_sp_emit: the staged vertices of one slot per block iteration at their ordered index
+// (row-segment offset + rank), the cell -> vertex box, and the 4-tap normals of the
+// tile's vertices with its pruned program (sdf3d_normal.wgsl:4-9, dualcontour.wgsl:171).
 // ---------------------------------------------------------------------------------
-constexpr uint32_t kScanChunk = 4096;  // 256 threads x 16
+__global__ __launch_bounds__(256) void k_sp_emit(SpPass P, MeshBufs b)
+{
+    __shared__ StagedTape T;
+    const uint32_t nslots = active_slots(P);
+    for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
+        const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
+        const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(P.job_of_slot[slot])];
+        const uint32_t base = __builtin_amdgcn_readfirstlane(P.slot_stage[2 * slot]);
+        const uint32_t n = __builtin_amdgcn_readfirstlane(P.slot_stage[2 * slot + 1]);
+        uint32_t tx, ty, tzl;
+        tile_xyz(J, tile, tx, ty, tzl);
+        const bool normals = !(J.flags & S2M_NO_NORMALS);
+        const uint16_t* L = P.lists + (size_t)tile * P.list_stride;
+        const uint32_t lens = __builtin_amdgcn_readfirstlane(P.lens[tile]);
+        const uint32_t len = lens & ~kLinearFlag;
+        const bool linear = (lens & kLinearFlag) != 0, staged = len <= kStageOps;
+        __syncthreads();  // the previous slot's readers of T are done
+        if (normals && staged) stage_tape(T, J.ops, L, len);
+        __syncthreads();
+        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+            if ((uint64_t)base + k >= P.stage_cap) break;
+            const float4 e = P.stage[(uint64_t)base + k];
+            const uint32_t meta = __float_as_uint(e.w);
+            const uint32_t c = meta & 511u, rank = (meta >> 9) & 7u, sg = meta >> 12;
+            const uint32_t x = tx * 8u + (c & 7u), y = ty * 8u + ((c >> 3) & 7u), zl = tzl * 8u + (c >> 6);
+            const uint32_t z = J.z0 + zl;
+            const uint64_t vi = (uint64_t)P.seg_off_v[seg_index(J, zl, y, tx)] + rank;
+            P.vbox[(size_t)slot * 512u + c] = (uint32_t)vi;
+            if (vi >= b.vcap) continue;
+            float* o = b.vtx + 6 * vi;
+            o[0] = e.x;
+            o[1] = e.y;
+            o[2] = e.z;
+            b.keys[vi] = (uint64_t)x | ((uint64_t)y << 16) | ((uint64_t)z << 32);
+            b.signs[vi] = (uint8_t)sg;
+            if (!normals) {
+                o[3] = o[4] = o[5] = 0.0f;
+                continue;
+            }
+            const float px = e.x, py = e.y, pz = e.z, eps = J.eps;
+            // sdf3d_normal (sdf3d_normal.wgsl:4-9): taps p + v_i * eps, then normalize
+            const float tx4[4] = {px + eps, px + (-eps), px + (-eps), px + eps};
+            const float ty4[4] = {py + (-eps), py + (-eps), py + eps, py + eps};
+            const float tz4[4] = {pz + (-eps), pz + eps, pz + (-eps), pz + eps};
+            float d[4];
+            if (linear) {
+                run_linear<4>(T, len, tx4, ty4, tz4, d);
+            } else {
+#pragma unroll
+                for (int t = 0; t < 4; ++t)
+                    d[t] = staged ? run_general(T, len, tx4[t], ty4[t], tz4[t])
+                                  : tape_eval_list(J.ops, L, len, tx4[t], ty4[t], tz4[t]);
+            }
+            const float d1 = d[0], d2 = d[1], d3 = d[2], d4 = d[3];
+            const float gx = ((d1 + (-d2)) + (-d3)) + d4;
+            const float gy = (((-d1) + (-d2)) + d3) + d4;
+            const float gz = (((-d1) + d2) + (-d3)) + d4;
+            const float ln = sqrtf((gx * gx + gy * gy) + gz * gz);
+            o[3] = gx / ln;
+            o[4] = gy / ln;
+            o[5] = gz / ln;
+        }
+    }
+}
 
-__global__ __launch_bounds__(256) void k_sp_reduce(const uint32_t* __restrict__ in, uint64_t n,
-                                                   uint32_t* __restrict__ bsum)
+// ---------------------------------------------------------------------------------
+// Exclusive scan of the pass's row-segment counts (vertices or triangles) in segment
+// order: per-block sums of kScanChunk segments, k_scan over the sums (adds and advances
+// the running total), block-local scan plus the block offset.  Only tiles with a slot
+// write counts: a segment of a tile without one counts 0 and is never read, and only
+// segments of tiles with a slot (plus each job's first segment, its start) receive an
+// offset.
+// ---------------------------------------------------------------------------------
+constexpr uint32_t kScanPer = 16;                  // consecutive segments per thread
+constexpr uint32_t kScanChunk = 256 * kScanPer;
+
+// Walks consecutive pass segments, tracking the job and the tile of each.
+struct SegCursor {
+    uint32_t j, tx, y, zl;
+    __device__ void seek(const SpPass& P, uint64_t i)
+    {
+        uint32_t lo = 0, hi = P.njobs - 1;  // last job with seg_base <= i
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi + 1) / 2;
+            if (P.jobs[mid].seg_base <= i) lo = mid;
+            else hi = mid - 1;
+        }
+        j = lo;
+        const SpJob& J = P.jobs[j];
+        tx = y = zl = 0;
+        if (i >= P.nseg || J.ntx == 0 || J.ry == 0) return;  // past the end: never used
+        const uint32_t li = (uint32_t)(i - J.seg_base);
+        tx = li % J.ntx;
+        const uint32_t row = li / J.ntx;
+        y = row % J.ry;
+        zl = row / J.ry;
+    }
+    // does the tile of the cursor's segment have a slot (and so a written count)?
+    __device__ bool active(const SpPass& P) const
+    {
+        const SpJob& J = P.jobs[j];
+        const uint64_t tile = J.tile_base + ((uint64_t)(zl >> 3) * J.nty + (y >> 3)) * J.ntx + tx;
+        return P.slot_of_tile[tile] >= 0;
+    }
+    __device__ bool first(const SpPass& P) const { return tx == 0 && y == 0 && zl == 0; }
+    __device__ void next(const SpPass& P, uint64_t i_next)
+    {
+        const SpJob& J = P.jobs[j];
+        if (++tx < J.ntx) return;
+        tx = 0;
+        if (++y < J.ry) return;
+        y = 0;
+        if (++zl < J.nz) return;
+        if (i_next < P.nseg) seek(P, i_next);  // next job
+    }
+};
+
+template <int WHICH>
+__global__ __launch_bounds__(256) void k_sp_reduce(SpPass P, uint32_t* __restrict__ bsum)
 {
     __shared__ uint32_t lds[4];
-    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + threadIdx.x * 16u;
+    const uint32_t* in = WHICH ? P.seg_t : P.seg_v;
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
     uint32_t s = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < 16; ++k) s += (base + k < n) ? in[base + k] : 0u;
+    if (base < P.nseg) {
+        SegCursor c;
+        c.seek(P, base);
+        for (uint32_t k = 0; k < kScanPer && base + k < P.nseg; ++k) {
+            if (c.active(P)) s += in[base + k];
+            c.next(P, base + k + 1);
+        }
+    }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
     if (lane() == 0) lds[threadIdx.x >> 6] = s;
@@ -355,17 +604,28 @@ __global__ __launch_bounds__(256) void k_sp_reduce(const uint32_t* __restrict__ 
     if (threadIdx.x == 0) bsum[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
 }
 
-__global__ __launch_bounds__(256) void k_sp_downsweep(const uint32_t* __restrict__ in, uint64_t n,
-                                                      const uint32_t* __restrict__ boff,
-                                                      uint32_t* __restrict__ out)
+template <int WHICH>
+__global__ __launch_bounds__(256) void k_sp_downsweep(SpPass P, const uint32_t* __restrict__ boff)
 {
     __shared__ uint32_t lds[4];
-    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + threadIdx.x * 16u;
-    uint32_t v[16], s = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < 16; ++k) {
-        v[k] = (base + k < n) ? in[base + k] : 0u;
-        s += v[k];
+    const uint32_t* in = WHICH ? P.seg_t : P.seg_v;
+    uint32_t* out = WHICH ? P.seg_off_t : P.seg_off_v;
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
+    uint32_t v[kScanPer] = {}, s = 0, act = 0, first = 0;
+    if (base < P.nseg) {
+        SegCursor c;
+        c.seek(P, base);
+        for (uint32_t k = 0; k < kScanPer; ++k) {
+            v[k] = 0;
+            if (base + k >= P.nseg) continue;
+            if (c.active(P)) {
+                v[k] = in[base + k];
+                act |= 1u << k;
+            }
+            if (c.first(P)) first |= 1u << k;
+            s += v[k];
+            c.next(P, base + k + 1);
+        }
     }
     uint32_t inc = s;
 #pragma unroll
@@ -377,9 +637,8 @@ __global__ __launch_bounds__(256) void k_sp_downsweep(const uint32_t* __restrict
     __syncthreads();
     uint32_t run = boff[blockIdx.x] + inc - s;
     for (uint32_t w = 0; w < (threadIdx.x >> 6); ++w) run += lds[w];
-#pragma unroll
-    for (uint32_t k = 0; k < 16; ++k) {
-        if (base + k < n) out[base + k] = run;
+    for (uint32_t k = 0; k < kScanPer; ++k) {
+        if ((act | first) & (1u << k)) out[base + k] = run;
         run += v[k];
     }
 }
@@ -408,32 +667,58 @@ __global__ void k_sp_ranges(SpPass P)
 template <int EMIT>
 __global__ __launch_bounds__(256) void k_sp_quads(SpPass P, MeshBufs b)
 {
+    // vertex of cell (x0 - 1 + i, y0 - 1 + j, z0 - 1 + k) at nb[(k * 9 + j) * 9 + i]: the
+    // tile and the look-back layer the quads reach, gathered once per tile
+    __shared__ uint32_t nb[9 * 9 * 9];
+    __shared__ uint8_t sgn[512];  // sign bits of the tile's cells with a vertex
     const uint32_t nslots = active_slots(P);
     for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
-        const uint64_t tile = P.tile_of_slot[slot];
-        const uint32_t jid = P.job_of_slot[slot];
+        const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
+        const uint32_t jid = __builtin_amdgcn_readfirstlane(P.job_of_slot[slot]);
         const SpJob& J = P.jobs[jid];
         const bool lag = (J.flags & S2M_COMPAT_DROP_LAST_SLICE) != 0;
         const uint32_t vbase = EMIT ? (uint32_t)P.job_out[(size_t)J.out_index * kJobOut + 0] : 0u;
         uint32_t tx, ty, tzl;
         tile_xyz(J, tile, tx, ty, tzl);
+        const uint32_t* own = P.vbox + (size_t)slot * 512u;
+        __syncthreads();  // the previous slot's readers of nb / sgn are done
+        for (uint32_t q = threadIdx.x; q < 729u; q += blockDim.x) {
+            const uint32_t i = q % 9u, j = (q / 9u) % 9u, k = q / 81u;
+            uint32_t v;
+            if (i && j && k) {
+                v = own[((k - 1) * 8u + (j - 1)) * 8u + (i - 1)];
+            } else {
+                const int x = (int)(tx * 8u + i) - 1, y = (int)(ty * 8u + j) - 1;
+                const int z = (int)(J.z0 + tzl * 8u + k) - 1;
+                // cells outside the grid have no vertex (and no entry in the boundary map)
+                v = (x < 0 || y < 0 || z < 0 || (uint32_t)x >= J.rx || (uint32_t)y >= J.ry)
+                        ? kNone
+                        : vertex_at(P, J, (uint32_t)x, (uint32_t)y, (uint32_t)z);
+            }
+            nb[q] = v;
+        }
+        for (uint32_t c = threadIdx.x; c < 512u; c += blockDim.x) {
+            const uint32_t v = own[c];
+            sgn[c] = (v != kNone && v < b.vcap) ? b.signs[v] : (uint8_t)0;
+        }
+        __syncthreads();
         uint32_t invalid = 0;
         for (uint32_t r = 0; r < 2; ++r) {
             const uint32_t c = r * 256u + threadIdx.x;
             const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
             const uint32_t x = tx * 8u + cxl, y = ty * 8u + cyl, zl = tzl * 8u + czl;
             const uint32_t z = J.z0 + zl;
-            const uint32_t v = P.vbox[(size_t)slot * 512u + c];
+            const uint32_t at = ((czl + 1) * 9u + (cyl + 1)) * 9u + (cxl + 1);
+            auto nbv = [&](int dx, int dy, int dz) { return nb[(int)at + (dz * 9 + dy) * 9 + dx]; };
+            const uint32_t v = nb[at];
             uint32_t q[3][4];
             uint32_t nq = 0;
             if (v != kNone) {  // implies the cell is inside the grid
-                const uint32_t s = (v < b.vcap) ? b.signs[v] : 0u;
+                const uint32_t s = sgn[c];
                 const bool c0 = s & 1, c1 = (s >> 1) & 1, c2 = (s >> 2) & 1, c3 = (s >> 3) & 1;
                 if (c0 != c3 && y > 0) {  // x-edge: Quad(v(x,y-1,z-1), v(x,y,z-1), v, v(x,y-1,z)).swap(c.0)
                     if (z > 0) {
-                        const uint32_t t[4] = {vertex_at(P, J, x, y - 1, z - 1),
-                                               vertex_at(P, J, x, y, z - 1), v,
-                                               vertex_at(P, J, x, y - 1, z)};
+                        const uint32_t t[4] = {nbv(0, -1, -1), nbv(0, 0, -1), v, nbv(0, -1, 0)};
                         for (int k = 0; k < 4; ++k) q[0][k] = c0 ? t[3 - k] : t[k];
                         if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 1;
                         else ++invalid;
@@ -443,9 +728,7 @@ __global__ __launch_bounds__(256) void k_sp_quads(SpPass P, MeshBufs b)
                 }
                 if (c1 != c3 && x > 0) {  // y-edge: Quad(v(x-1,y,z-1), v(x,y,z-1), v, v(x-1,y,z)).swap(!c.1)
                     if (z > 0) {
-                        const uint32_t t[4] = {vertex_at(P, J, x - 1, y, z - 1),
-                                               vertex_at(P, J, x, y, z - 1), v,
-                                               vertex_at(P, J, x - 1, y, z)};
+                        const uint32_t t[4] = {nbv(-1, 0, -1), nbv(0, 0, -1), v, nbv(-1, 0, 0)};
                         for (int k = 0; k < 4; ++k) q[1][k] = !c1 ? t[3 - k] : t[k];
                         if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 2;
                         else ++invalid;
@@ -454,9 +737,7 @@ __global__ __launch_bounds__(256) void k_sp_quads(SpPass P, MeshBufs b)
                     }
                 }
                 if (c2 != c3 && x > 0 && y > 0) {  // z-edge: Quad(v(x-1,y-1,z), v(x,y-1,z), v, v(x-1,y,z)).swap(c.2)
-                    const uint32_t t[4] = {vertex_at(P, J, x - 1, y - 1, z),
-                                           vertex_at(P, J, x, y - 1, z), v,
-                                           vertex_at(P, J, x - 1, y, z)};
+                    const uint32_t t[4] = {nbv(-1, -1, 0), nbv(0, -1, 0), v, nbv(-1, 0, 0)};
                     for (int k = 0; k < 4; ++k) q[2][k] = c2 ? t[3 - k] : t[k];
                     if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 4;
                     else ++invalid;
@@ -515,6 +796,7 @@ __global__ void k_sp_pass_begin(uint64_t* counters)
         const uint64_t used = counters[kSlotsUsed];
         if (used > counters[kSlotsMax]) counters[kSlotsMax] = used;
         counters[kSlotsUsed] = 0;
+        counters[kStageUsed] = 0;
     }
 }
 
@@ -539,27 +821,28 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
         if (marks && marks[i]) (void)hipEventRecord(marks[i], s);
     };
     mark(0);
+    if (P.nsuper)
+        hipLaunchKernelGGL(k_sp_prune_super, dim3((uint32_t)((P.nsuper + 63) / 64)), dim3(64), 0, s, P);
     hipLaunchKernelGGL(k_sp_prune, dim3((uint32_t)((P.ntiles + 63) / 64)), dim3(64), 0, s, P);
     mark(1);
-    hipLaunchKernelGGL(k_sp_eval, dim3(pb), dim3(256), 0, s, P);
+    hipLaunchKernelGGL((k_sp_eval<1>), dim3(pb), dim3(256), 0, s, P);
+    hipLaunchKernelGGL((k_sp_eval<0>), dim3(pb), dim3(256), 0, s, P);
     mark(2);
-    hipLaunchKernelGGL((k_sp_cells<0>), dim3(pb), dim3(256), smem, s, P, b);
+    hipLaunchKernelGGL(k_sp_cells, dim3(pb), dim3(256), smem, s, P, b);
     if (nsb) {
-        hipLaunchKernelGGL(k_sp_reduce, dim3(nsb), dim3(256), 0, s, P.seg_v, P.nseg, b.bcount);
+        hipLaunchKernelGGL((k_sp_reduce<0>), dim3(nsb), dim3(256), 0, s, P, b.bcount);
         (void)launch_scan(s, b, nsb, 0);
-        hipLaunchKernelGGL(k_sp_downsweep, dim3(nsb), dim3(256), 0, s, P.seg_v, P.nseg, b.boffset,
-                           P.seg_off_v);
+        hipLaunchKernelGGL((k_sp_downsweep<0>), dim3(nsb), dim3(256), 0, s, P, b.boffset);
     }
     hipLaunchKernelGGL((k_sp_ranges<0>), dim3(jb), dim3(64), 0, s, P);
     mark(3);
-    hipLaunchKernelGGL((k_sp_cells<1>), dim3(pb), dim3(256), smem, s, P, b);
+    hipLaunchKernelGGL(k_sp_emit, dim3(pb), dim3(256), 0, s, P, b);
     mark(4);
     hipLaunchKernelGGL((k_sp_quads<0>), dim3(pb), dim3(256), 0, s, P, b);
     if (nsb) {
-        hipLaunchKernelGGL(k_sp_reduce, dim3(nsb), dim3(256), 0, s, P.seg_t, P.nseg, b.bcount);
+        hipLaunchKernelGGL((k_sp_reduce<1>), dim3(nsb), dim3(256), 0, s, P, b.bcount);
         (void)launch_scan(s, b, nsb, 1);
-        hipLaunchKernelGGL(k_sp_downsweep, dim3(nsb), dim3(256), 0, s, P.seg_t, P.nseg, b.boffset,
-                           P.seg_off_t);
+        hipLaunchKernelGGL((k_sp_downsweep<1>), dim3(nsb), dim3(256), 0, s, P, b.boffset);
     }
     hipLaunchKernelGGL((k_sp_ranges<1>), dim3(jb), dim3(64), 0, s, P);
     hipLaunchKernelGGL((k_sp_quads<1>), dim3(pb), dim3(256), 0, s, P, b);
