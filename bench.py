@@ -276,13 +276,13 @@ def run(args) -> dict:
                    "slabs": agg["slabs"] // args.steps},
         "roofline": {
             "bound": "valu",
-            "kernel": "k_sp_eval" if tiled else "k_eval_lattice",
+            "kernel": "k_sp_tiles" if tiled else "k_eval_lattice",
             "achieved": round(achieved, 2),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
             "traffic": measured_traffic(args.workload + ("-dense" if args.dense else ""),
-                                        "k_sp_eval" if tiled else "k_eval_lattice"),
+                                        "k_sp_tiles" if tiled else "k_eval_lattice"),
             "launch_ms": round(eval_avg_ms, 4),
             "flops_per_launch": flops_per_launch,
             "points_per_launch": pts_per_launch,

@@ -97,8 +97,8 @@ struct s2m_ctx {
     DevBuf lattice, vidx, bcount, boffset, counters, tables;
     DevBuf vtx, keys, signs, tri, corners;
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
-    DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_boxes, sp_vbox, sp_seg, sp_boundary;
-    DevBuf sp_stage, sp_slot_stage;
+    DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_vbox, sp_seg, sp_boundary;
+    DevBuf sp_stage, sp_slot_stage, sp_slot_live;
     DevBuf sp_job_out, sp_super_lists, sp_super_lens;
     uint64_t slot_cap = 0;
     int cus = 256;
@@ -416,6 +416,10 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     }
     if (box_stride * sizeof(float) > 40u * 1024u)
         return fail(S2M_E_ARG, "tile box of %u points exceeds the LDS staging limit", box_stride);
+    for (uint32_t i = 0; i < n; ++i)  // a tile spans at most 9 lattice positions, 2 coordinates each
+        if (tabs[i].bmax[0] > 32 || tabs[i].bmax[1] > 32 || tabs[i].bmax[2] > 32)
+            return fail(S2M_E_ARG, "tile box of %u x %u x %u lattice points", tabs[i].bmax[0], tabs[i].bmax[1],
+                        tabs[i].bmax[2]);
     // slabs (single job only): whole tile layers, per-pass buffers bounded to ~1 GiB
     uint32_t layers = tabs[0].nt[2];
     if (n == 1) {
@@ -527,7 +531,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     want_v = std::max<uint64_t>(want_v, ctx->vcap);
     uint64_t want_t = std::max<uint64_t>(3 * want_v, ctx->tcap);
     uint64_t want_s = std::min<uint64_t>(tiles_pass, std::max<uint64_t>({4096, tiles_pass / 4, ctx->slot_cap}));
-    const int pblocks = std::max(64, ctx->cus * 4);
+    const int pblocks = std::max(64, ctx->cus * 8);
 
     for (int attempt = 0; attempt < 4; ++attempt) {
         HIP_TRY(ctx->vtx.ensure(want_v * 6 * sizeof(float)));
@@ -536,10 +540,13 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         HIP_TRY(ctx->tri.ensure(want_t * 3 * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_tile_of_slot.ensure(want_s * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_job_of_slot.ensure(want_s * sizeof(uint32_t)));
-        HIP_TRY(ctx->sp_boxes.ensure(want_s * box_stride * sizeof(float)));
         HIP_TRY(ctx->sp_vbox.ensure(want_s * 512 * sizeof(uint32_t)));
-        HIP_TRY(ctx->sp_stage.ensure(want_v * sizeof(float4)));
+        // staged cells are allocated in per-block chunks: at most 2 x vertices + blocks x chunk
+        // (two launches of pblocks blocks each)
+        const uint64_t stage_cap = 2 * want_v + 2ull * pblocks * kStageChunkHost;
+        HIP_TRY(ctx->sp_stage.ensure(stage_cap * sizeof(float4)));
         HIP_TRY(ctx->sp_slot_stage.ensure(want_s * 2 * sizeof(uint32_t)));
+        HIP_TRY(ctx->sp_slot_live.ensure(want_s * sizeof(uint64_t)));
         ctx->vcap = std::max(ctx->vcap, want_v);
         ctx->tcap = std::max(ctx->tcap, want_t);
         ctx->slot_cap = std::max(ctx->slot_cap, want_s);
@@ -598,11 +605,11 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.job_of_slot = ctx->sp_job_of_slot.as<uint32_t>();
             P.slot_cap = (uint32_t)want_s;
             P.box_stride = box_stride;
-            P.boxes = ctx->sp_boxes.as<float>();
             P.vbox = ctx->sp_vbox.as<uint32_t>();
             P.stage = ctx->sp_stage.as<float4>();
-            P.stage_cap = want_v;
+            P.stage_cap = 2 * want_v + 2ull * pblocks * kStageChunkHost;
             P.slot_stage = ctx->sp_slot_stage.as<uint32_t>();
+            P.slot_live = ctx->sp_slot_live.as<uint64_t>();
             P.seg_v = ctx->sp_seg.as<uint32_t>();
             P.seg_t = P.seg_v + P.nseg;
             P.seg_off_v = P.seg_t + P.nseg;
@@ -1002,9 +1009,9 @@ void s2m_destroy(s2m_ctx* ctx)
                       &ctx->tables, &ctx->vtx, &ctx->keys, &ctx->signs, &ctx->tri, &ctx->corners,
                       &ctx->tile_tab, &ctx->tile_lists, &ctx->tile_lens,
                       &ctx->tile_flops, &ctx->sp_slot_of_tile, &ctx->sp_tile_of_slot,
-                      &ctx->sp_job_of_slot, &ctx->sp_boxes, &ctx->sp_vbox, &ctx->sp_seg,
+                      &ctx->sp_job_of_slot, &ctx->sp_vbox, &ctx->sp_seg,
                       &ctx->sp_boundary, &ctx->sp_job_out, &ctx->sp_super_lists, &ctx->sp_super_lens,
-                      &ctx->sp_stage, &ctx->sp_slot_stage})
+                      &ctx->sp_stage, &ctx->sp_slot_stage, &ctx->sp_slot_live})
         b->release();
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);

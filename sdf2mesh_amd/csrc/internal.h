@@ -65,6 +65,7 @@ struct SlabArgs {
 constexpr uint32_t kTile = 8;  // cells per tile edge (sparse pipeline)
 constexpr uint32_t kSuper = 4;  // tiles per super-tile edge (first pruning level)
 constexpr uint32_t kSingleSign = 0xffffffffu;  // super_lens: the SDF keeps one sign there
+constexpr uint32_t kStageChunkHost = 1024;  // staged-cell allocation chunk (sparse.hip)
 // Profiling marks of one sparse pass: prune | eval | count+scan | emit+normals | quads
 constexpr int kSparseMarks = 6;
 
@@ -119,12 +120,12 @@ struct SpPass {
     uint32_t* tile_of_slot;    // [slot]
     uint32_t* job_of_slot;     // [slot]
     uint32_t slot_cap;
-    uint32_t box_stride;       // floats per slot box
-    float* boxes;              // [slot][box_stride]
+    uint32_t box_stride;       // floats of a tile's lattice box (LDS)
     uint32_t* vbox;            // [slot][512] vertex of each cell of the tile, kNone if none
     float4* stage;             // [stage_cap] active cells: position, cell | rank << 9 | signs << 12
     uint64_t stage_cap;
     uint32_t* slot_stage;      // [slot][2] first staged cell, staged cells
+    uint64_t* slot_live;       // [slot] bricks (2x2x2 cells) that may hold an active cell
     uint32_t* seg_v;           // [nseg] vertices per row segment
     uint32_t* seg_t;           // triangles per row segment
     uint32_t* seg_off_v;       // exclusive offsets: global vertex index

@@ -256,8 +256,14 @@ __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
     for (uint32_t j = 0; j < len; ++j) fl += J.op_flops[L[j]];
     P.lens[tile] = len | (!one_sign && is_left_fold(ops, L, len) ? kLinearFlag : 0u);
     P.flops[tile] = fl;
+    // slots for the wave's surviving tiles: one counter update per wave
+    const uint64_t w = __ballot(!one_sign);
+    uint64_t wbase = 0;
+    if (lane() == __builtin_ctzll(w | (1ull << 63)) && w)
+        wbase = atomicAdd((unsigned long long*)&P.counters[kSlotsUsed], (unsigned long long)__builtin_popcountll(w));
+    wbase = __shfl(wbase, __builtin_ctzll(w | (1ull << 63)), 64);
     if (!one_sign) {
-        const uint64_t sidx = atomicAdd((unsigned long long*)&P.counters[kSlotsUsed], 1ull);
+        const uint64_t sidx = wbase + (uint64_t)__builtin_popcountll(w & ((1ull << lane()) - 1ull));
         if (sidx < P.slot_cap) {
             slot = (int32_t)sidx;
             P.tile_of_slot[sidx] = (uint32_t)tile;
@@ -265,6 +271,33 @@ __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
         }
     }
     P.slot_of_tile[tile] = slot;
+}
+
+// Per-axis corner tables of a tile's 8 cells: box-relative lattice index and coordinate
+// of each cell's lo / hi corner.
+struct TileAxes {
+    uint32_t lo[3][8], hi[3][8];
+    float clo[3][8], chi[3][8];
+};
+
+// Entry threadIdx.x (< 24) of A: axis threadIdx.x / 8, cell threadIdx.x % 8.  Cells past
+// the end of the grid repeat the last cell's entries (their cells are never contoured).
+__device__ __forceinline__ void load_tile_axes(TileAxes& A, const SpJob& J, uint32_t tx, uint32_t ty,
+                                               uint32_t tz)
+{
+    const uint32_t ax = threadIdx.x >> 3, c = threadIdx.x & 7u;
+    const uint32_t t = ax == 0 ? tx : ax == 1 ? ty : tz;
+    const uint32_t r = ax == 0 ? J.rx : ax == 1 ? J.ry : J.z0 + J.nz;  // cells on the axis
+    const uint32_t* lo = ax == 0 ? J.lox : ax == 1 ? J.loy : J.loz;
+    const uint32_t* hi = ax == 0 ? J.hix : ax == 1 ? J.hiy : J.hiz;
+    const float* co = ax == 0 ? J.cx : ax == 1 ? J.cy : J.cz;
+    const uint32_t org = J.ibox[2 * (ax == 0 ? 0 : ax == 1 ? J.ntx : J.ntx + J.nty) + t];
+    const uint32_t g = t * 8u + c;   // cell index along the axis (z: of the job)
+    const uint32_t gi = g < r ? g : r - 1;
+    A.lo[ax][c] = lo[gi] - org;
+    A.hi[ax][c] = hi[gi] - org;
+    A.clo[ax][c] = co[lo[gi]];
+    A.chi[ax][c] = co[hi[gi]];
 }
 
 // ---------------------------------------------------------------------------------
@@ -285,10 +318,163 @@ __device__ __forceinline__ uint32_t div16(uint32_t n, uint32_t d, uint32_t m)
 // skip the other kind.
 constexpr int kLinearPoints = 2;
 
-template <int LINEAR>
-__global__ __launch_bounds__(256) void k_sp_eval(SpPass P)
+// ---------------------------------------------------------------------------------
+// Bricks: the 4x4x4 blocks of 2x2x2 cells of a tile, brick (bx, by, bz) = bit
+// (bz * 4 + by) * 4 + bx of the slot's `slot_live`.  Per slot (one 64-lane block
+// iteration, lane = brick): interval bounds of the tile's program on the brick's lattice
+// points; a brick whose values provably keep one sign holds no active cell, so its cells
+// are not contoured and its points are not evaluated.  Left-fold programs only (other
+// programs keep every brick).
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_sp_bricks(SpPass P)
 {
+    const uint32_t nslots = active_slots(P);
+    const uint32_t b = threadIdx.x;
+    for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
+        const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
+        const uint32_t lens = __builtin_amdgcn_readfirstlane(P.lens[tile]);
+        if (!(lens & kLinearFlag)) {
+            if (b == 0) P.slot_live[slot] = ~0ull;
+            continue;
+        }
+        const uint32_t n = lens & ~kLinearFlag;
+        const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(P.job_of_slot[slot])];
+        uint32_t tx, ty, tzl;
+        tile_xyz(J, tile, tx, ty, tzl);
+        const uint32_t t[3] = {tx, ty, J.tz0 + tzl};
+        const uint32_t r[3] = {J.rx, J.ry, J.z0 + J.nz};  // cells on each axis (z: of the job)
+        const uint32_t* lo[3] = {J.lox, J.loy, J.loz};
+        const uint32_t* hi[3] = {J.hix, J.hiy, J.hiz};
+        const float* co[3] = {J.cx, J.cy, J.cz};
+        const uint32_t c[3] = {(b & 3u) * 2u, ((b >> 2) & 3u) * 2u, (b >> 4) * 2u};
+        double bl[3], bh[3];
+        for (int a = 0; a < 3; ++a) {  // first cell's lo corner, second cell's hi corner
+            const uint32_t g0 = min(t[a] * 8u + c[a], r[a] - 1), g1 = min(t[a] * 8u + c[a] + 1, r[a] - 1);
+            bl[a] = (double)co[a][lo[a][g0]];
+            bh[a] = (double)co[a][hi[a][g1]];
+        }
+        const uint16_t* L = P.lists + (size_t)tile * P.list_stride;
+        const s2m_op* ops = J.ops;
+        Iv acc = iv_primitive(ops[L[0]].code, ops[L[0]].f, bl, bh);
+        for (uint32_t k = 1; k + 1 < n; k += 2) {
+            const s2m_op& pr = ops[L[k]];
+            const s2m_op& op = ops[L[k + 1]];
+            const Iv d = iv_primitive(pr.code, pr.f, bl, bh);
+            int keep;
+            acc = iv_op(op.code, acc, d, op.f[0], keep);
+        }
+        const bool may_cross = !(acc.known && (acc.lo > 0.0 || acc.hi < 0.0));
+        const uint64_t m = __ballot(may_cross);
+        if (b == 0) P.slot_live[slot] = m;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// The cells of one tile (all threads of the block; the tile's box values in LDS): thread
+// t owns cells t and t + 256 (x fastest), so the 8 cells of a row segment are 8
+// consecutive lanes.  Per active cell: the vertex (dualcontour.wgsl:72-131) and sign
+// bits (:57-69), staged compactly for k_sp_emit with the cell's rank in its row segment;
+// per row segment: its vertex count.  Every cell's entry of the slot's cell -> vertex box
+// is reset to kNone.  Cells of bricks outside `live` are not contoured (their corner
+// values keep one sign and were not evaluated).  Contains barriers.
+// ---------------------------------------------------------------------------------
+// Staged cells are allocated in chunks of kStageChunk per block (one counter update per
+// chunk); a slot's cells are contiguous, so a chunk with fewer than the slot's cells left
+// is abandoned.  Hence at most 2 x (active cells) + blocks x kStageChunk entries are used.
+constexpr uint32_t kStageChunk = kStageChunkHost;
+
+struct StageChunk {  // block-shared allocation state
+    uint32_t base, left, nact, slot_base;
+};
+
+__device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, uint32_t slot, uint32_t tx,
+                                             uint32_t ty, uint32_t tzl, const float* box, const TileAxes& A,
+                                             uint64_t live, StageChunk& C)
+{
+    bool act[2];
+    float pos[2][3];
+    uint32_t meta[2];
+#pragma unroll
+    for (uint32_t r = 0; r < 2; ++r) {
+        const uint32_t c = r * 256u + threadIdx.x;
+        const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
+        const uint32_t x = tx * 8u + cxl, y = ty * 8u + cyl, zl = tzl * 8u + czl;
+        const uint32_t brick = ((czl >> 1) * 4u + (cyl >> 1)) * 4u + (cxl >> 1);
+        const bool inside = x < J.rx && y < J.ry && zl < J.nz;
+        act[r] = false;
+        uint32_t sg = 0;
+        if (inside && ((live >> brick) & 1ull)) {  // a dead brick has no active cell
+            float cv[8];
+            const uint32_t xi[2] = {A.lo[0][cxl], A.hi[0][cxl]};
+            const uint32_t yi[2] = {A.lo[1][cyl], A.hi[1][cyl]};
+            const uint32_t zi[2] = {A.lo[2][czl], A.hi[2][czl]};
+#pragma unroll
+            for (int k = 0; k < 8; ++k)
+                cv[k] = box[(zi[(k >> 2) & 1] * J.by + yi[(k >> 1) & 1]) * J.bx + xi[k & 1]];
+            const float lo[3] = {A.clo[0][cxl], A.clo[1][cyl], A.clo[2][czl]};
+            const float hi[3] = {A.chi[0][cxl], A.chi[1][cyl], A.chi[2][czl]};
+            act[r] = contour_cell_sp(cv, lo, hi, pos[r]);
+            // cell_sign_changes (dualcontour.wgsl:57-69)
+            sg = (uint32_t)(cv[1] > 0.0f) | ((uint32_t)(cv[2] > 0.0f) << 1) |
+                 ((uint32_t)(cv[4] > 0.0f) << 2) | ((uint32_t)(cv[0] > 0.0f) << 3);
+        }
+        const uint64_t m = __ballot(act[r]);
+        const uint32_t g0 = lane() & ~7u;
+        const uint32_t gmask = (uint32_t)((m >> g0) & 0xffull);
+        const uint32_t below = (uint32_t)__builtin_popcount(gmask & ((1u << (lane() - g0)) - 1u));
+        if (cxl == 0 && y < J.ry && zl < J.nz)
+            P.seg_v[seg_index(J, zl, y, tx)] = (uint32_t)__builtin_popcount(gmask);
+        meta[r] = c | (below << 9) | (sg << 12);
+        P.vbox[(size_t)slot * 512u + c] = kNone;
+    }
+    // compact staging of the tile's active cells
+    uint32_t k0 = 0, k1 = 0;
+    if (act[0]) k0 = atomicAdd(&C.nact, 1u);
+    if (act[1]) k1 = atomicAdd(&C.nact, 1u);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t n = C.nact;
+        if (C.left < n) {
+            C.base = (uint32_t)atomicAdd((unsigned long long*)&P.counters[kStageUsed], (unsigned long long)kStageChunk);
+            C.left = kStageChunk;
+        }
+        C.slot_base = C.base + (kStageChunk - C.left);
+        C.left -= n;
+        C.nact = 0;  // for the next slot (read again only after the next barrier)
+        P.slot_stage[2 * slot] = C.slot_base;
+        P.slot_stage[2 * slot + 1] = n;
+    }
+    __syncthreads();
+    const uint32_t stage_base = C.slot_base;
+#pragma unroll
+    for (uint32_t r = 0; r < 2; ++r) {
+        if (!act[r]) continue;
+        const uint64_t at = (uint64_t)stage_base + (r ? k1 : k0);
+        if (at < P.stage_cap)
+            P.stage[at] = make_float4(pos[r][0], pos[r][1], pos[r][2], __uint_as_float(meta[r]));
+    }
+}
+
+constexpr uint32_t kChunk = 2048;  // box points examined per compaction round
+
+// k_sp_tiles: per slot (persistent, grid-stride), the tile's pruned program, staged in
+// LDS, at the lattice points of its closed box that the tile's cells need (all points for
+// programs that are not left folds), into an LDS box; then the tile's cells are contoured
+// from it (contour_tile).  LINEAR = 1 runs the slots whose program is a left fold (two
+// points per lane), LINEAR = 0 the others; both launches walk all slots and skip the
+// other kind.
+template <int LINEAR>
+__global__ __launch_bounds__(256) void k_sp_tiles(SpPass P)
+{
+    extern __shared__ float box[];  // P.box_stride floats
     __shared__ StagedTape T;
+    __shared__ TileAxes A;
+    __shared__ uint32_t bm[3][32];  // per axis and box index: the bricks (along it) using it
+    __shared__ uint16_t list[kChunk];
+    __shared__ uint32_t nlist;
+    __shared__ StageChunk C;
+    if (threadIdx.x == 0) C = StageChunk{0u, 0u, 0u, 0u};  // published by the first slot's barriers
+    uint64_t tiles_done = 0, points_done = 0, flops_done = 0;  // thread 0: this block's share
     const uint32_t nslots = active_slots(P);
     for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
         const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
@@ -315,11 +501,20 @@ __global__ __launch_bounds__(256) void k_sp_eval(SpPass P)
         const float* cx = J.cx + x0;
         const float* cy = J.cy + y0;
         const float* cz = J.cz + z0;
-        float* box = P.boxes + (size_t)slot * P.box_stride;
         const bool staged = n <= kStageOps;
-        __syncthreads();  // the previous slot's readers of T are done
+        __syncthreads();  // the previous slot's readers of T, A, bm, list, box are done
         if (staged) stage_tape(T, J.ops, L, n);
+        if (threadIdx.x < 24) load_tile_axes(A, J, tx, ty, tz);
+        if (LINEAR && threadIdx.x < 96) bm[threadIdx.x >> 5][threadIdx.x & 31u] = 0;
         __syncthreads();
+        if (LINEAR) {
+            if (threadIdx.x < 24) {
+                const uint32_t ax = threadIdx.x >> 3, c = threadIdx.x & 7u;
+                atomicOr(&bm[ax][A.lo[ax][c]], 1u << (c >> 1));
+                atomicOr(&bm[ax][A.hi[ax][c]], 1u << (c >> 1));
+            }
+            __syncthreads();
+        }
         auto point = [&](uint32_t q, float& x, float& y, float& z) {
             const uint32_t iz = div16(q, nxy, mxy), r = q - iz * nxy;
             const uint32_t iy = div16(r, nx, mx), ix = r - iy * nx;
@@ -328,19 +523,52 @@ __global__ __launch_bounds__(256) void k_sp_eval(SpPass P)
             z = cz[iz];
             return (iz * by + iy) * bx + ix;
         };
+        uint32_t evaluated = npts;
         if (LINEAR) {
-            for (uint32_t q0 = threadIdx.x; q0 < npts; q0 += kLinearPoints * blockDim.x) {
-                float x[kLinearPoints], y[kLinearPoints], z[kLinearPoints], v[kLinearPoints];
-                uint32_t at[kLinearPoints];
-#pragma unroll
-                for (int p = 0; p < kLinearPoints; ++p) {
-                    const uint32_t q = q0 + p * blockDim.x;
-                    at[p] = point(q < npts ? q : q0, x[p], y[p], z[p]);
+            const uint64_t lv = P.slot_live[slot];
+            evaluated = 0;
+            for (uint32_t c0 = 0; c0 < npts; c0 += kChunk) {
+                if (threadIdx.x == 0) nlist = 0;
+                __syncthreads();
+                // the chunk's points that some live brick uses, compacted into `list`
+                for (uint32_t k = 0; k < kChunk / 256u; ++k) {
+                    const uint32_t q = c0 + k * 256u + threadIdx.x;
+                    bool need = false;
+                    if (q < npts) {
+                        const uint32_t iz = div16(q, nxy, mxy), r = q - iz * nxy;
+                        const uint32_t iy = div16(r, nx, mx), ix = r - iy * nx;
+                        const uint32_t mbx = bm[0][ix];
+                        for (uint32_t mz = bm[2][iz]; mz && !need; mz &= mz - 1u)
+                            for (uint32_t my = bm[1][iy]; my && !need; my &= my - 1u) {
+                                const uint32_t row = (uint32_t)(__builtin_ctz(mz) * 4 + __builtin_ctz(my)) * 4u;
+                                need = ((lv >> row) & mbx) != 0;
+                            }
+                    }
+                    const uint64_t w = __ballot(need);
+                    uint32_t wbase = 0;
+                    if (lane() == 0 && w) wbase = atomicAdd(&nlist, (uint32_t)__builtin_popcountll(w));
+                    wbase = __shfl(wbase, 0, 64);
+                    if (need)
+                        list[wbase + (uint32_t)__builtin_popcountll(w & ((1ull << lane()) - 1ull))] =
+                            (uint16_t)(q - c0);
                 }
-                run_linear<kLinearPoints>(T, n, x, y, z, v);
+                __syncthreads();
+                const uint32_t m = nlist;
+                evaluated += m;
+                for (uint32_t i0 = threadIdx.x; i0 < m; i0 += kLinearPoints * blockDim.x) {
+                    float x[kLinearPoints], y[kLinearPoints], z[kLinearPoints], v[kLinearPoints];
+                    uint32_t at[kLinearPoints];
 #pragma unroll
-                for (int p = 0; p < kLinearPoints; ++p)
-                    if (q0 + p * blockDim.x < npts) box[at[p]] = v[p];
+                    for (int p = 0; p < kLinearPoints; ++p) {
+                        const uint32_t i = i0 + p * blockDim.x;
+                        at[p] = point(c0 + list[i < m ? i : i0], x[p], y[p], z[p]);
+                    }
+                    run_linear<kLinearPoints>(T, n, x, y, z, v);
+#pragma unroll
+                    for (int p = 0; p < kLinearPoints; ++p)
+                        if (i0 + p * blockDim.x < m) box[at[p]] = v[p];
+                }
+                __syncthreads();  // list and nlist are reused by the next chunk
             }
         } else {
             for (uint32_t q = threadIdx.x; q < npts; q += blockDim.x) {
@@ -349,114 +577,18 @@ __global__ __launch_bounds__(256) void k_sp_eval(SpPass P)
                 box[at] = staged ? run_general(T, n, x, y, z) : tape_eval_list(J.ops, L, n, x, y, z);
             }
         }
+        __syncthreads();  // the box is complete
+        contour_tile(P, J, slot, tx, ty, tzl, box, A, LINEAR ? P.slot_live[slot] : ~0ull, C);
         if (threadIdx.x == 0) {
-            atomicAdd((unsigned long long*)&P.counters[kTilesActive], 1ull);
-            atomicAdd((unsigned long long*)&P.counters[kEvalPoints], (unsigned long long)npts);
-            atomicAdd((unsigned long long*)&P.counters[kEvalFlops],
-                      (unsigned long long)npts * P.flops[tile]);
+            tiles_done += 1;
+            points_done += evaluated;
+            flops_done += (uint64_t)evaluated * P.flops[tile];
         }
     }
-}
-
-// ---------------------------------------------------------------------------------
-// k_sp_cells: the cells of one slot per block iteration; thread t owns cells t and
-// t + 256 of the tile (x fastest), so the 8 cells of a row segment are 8 consecutive
-// lanes.  Per active cell: the vertex (dualcontour.wgsl:72-131) and sign bits (:57-69),
-// staged compactly for k_sp_emit with the cell's rank in its row segment; per row
-// segment: its vertex count.  Every cell's entry of the slot's cell -> vertex box is
-// reset to kNone.
-// ---------------------------------------------------------------------------------
-struct TileAxes {  // per axis: box-relative lattice index and coordinate of lo / hi corners
-    uint32_t lo[3][8], hi[3][8];
-    float clo[3][8], chi[3][8];
-};
-
-__global__ __launch_bounds__(256) void k_sp_cells(SpPass P, MeshBufs b)
-{
-    extern __shared__ float box[];  // P.box_stride floats
-    __shared__ TileAxes A;
-    __shared__ uint32_t nact, stage_base;
-    const uint32_t nslots = active_slots(P);
-    for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
-        const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
-        const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(P.job_of_slot[slot])];
-        uint32_t tx, ty, tzl;
-        tile_xyz(J, tile, tx, ty, tzl);
-        const uint32_t tz = J.tz0 + tzl;
-        const float* gbox = P.boxes + (size_t)slot * P.box_stride;
-        __syncthreads();  // the previous slot's readers of box / A are done
-        for (uint32_t q = threadIdx.x; q < P.box_stride; q += blockDim.x) box[q] = gbox[q];
-        if (threadIdx.x < 24) {  // per-axis corner tables of the tile's 8 cells
-            const uint32_t ax = threadIdx.x >> 3, c = threadIdx.x & 7u;
-            const uint32_t t = ax == 0 ? tx : ax == 1 ? ty : tz;
-            const uint32_t r = ax == 0 ? J.rx : ax == 1 ? J.ry : J.z0 + J.nz;  // cells on the axis
-            const uint32_t* lo = ax == 0 ? J.lox : ax == 1 ? J.loy : J.loz;
-            const uint32_t* hi = ax == 0 ? J.hix : ax == 1 ? J.hiy : J.hiz;
-            const float* co = ax == 0 ? J.cx : ax == 1 ? J.cy : J.cz;
-            const uint32_t org = J.ibox[2 * (ax == 0 ? 0 : ax == 1 ? J.ntx : J.ntx + J.nty) + t];
-            const uint32_t g = t * 8u + c;   // cell index along the axis (z: of the job)
-            const uint32_t gi = g < r ? g : r - 1;  // cells past the end are skipped below
-            A.lo[ax][c] = lo[gi] - org;
-            A.hi[ax][c] = hi[gi] - org;
-            A.clo[ax][c] = co[lo[gi]];
-            A.chi[ax][c] = co[hi[gi]];
-        }
-        if (threadIdx.x == 0) nact = 0;
-        __syncthreads();
-        bool act[2];
-        float pos[2][3];
-        uint32_t meta[2];
-#pragma unroll
-        for (uint32_t r = 0; r < 2; ++r) {
-            const uint32_t c = r * 256u + threadIdx.x;
-            const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
-            const uint32_t x = tx * 8u + cxl, y = ty * 8u + cyl, zl = tzl * 8u + czl;
-            const bool inside = x < J.rx && y < J.ry && zl < J.nz;
-            act[r] = false;
-            uint32_t sg = 0;
-            if (inside) {
-                float cv[8];
-                const uint32_t xi[2] = {A.lo[0][cxl], A.hi[0][cxl]};
-                const uint32_t yi[2] = {A.lo[1][cyl], A.hi[1][cyl]};
-                const uint32_t zi[2] = {A.lo[2][czl], A.hi[2][czl]};
-#pragma unroll
-                for (int k = 0; k < 8; ++k)
-                    cv[k] = box[(zi[(k >> 2) & 1] * J.by + yi[(k >> 1) & 1]) * J.bx + xi[k & 1]];
-                const float lo[3] = {A.clo[0][cxl], A.clo[1][cyl], A.clo[2][czl]};
-                const float hi[3] = {A.chi[0][cxl], A.chi[1][cyl], A.chi[2][czl]};
-                act[r] = contour_cell_sp(cv, lo, hi, pos[r]);
-                // cell_sign_changes (dualcontour.wgsl:57-69)
-                sg = (uint32_t)(cv[1] > 0.0f) | ((uint32_t)(cv[2] > 0.0f) << 1) |
-                     ((uint32_t)(cv[4] > 0.0f) << 2) | ((uint32_t)(cv[0] > 0.0f) << 3);
-            }
-            const uint64_t m = __ballot(act[r]);
-            const uint32_t g0 = lane() & ~7u;
-            const uint32_t gmask = (uint32_t)((m >> g0) & 0xffull);
-            const uint32_t below = (uint32_t)__builtin_popcount(gmask & ((1u << (lane() - g0)) - 1u));
-            if (cxl == 0 && y < J.ry && zl < J.nz)
-                P.seg_v[seg_index(J, zl, y, tx)] = (uint32_t)__builtin_popcount(gmask);
-            meta[r] = c | (below << 9) | (sg << 12);
-            P.vbox[(size_t)slot * 512u + c] = kNone;
-        }
-        // compact staging of the slot's active cells
-        uint32_t k0 = 0, k1 = 0;
-        if (act[0]) k0 = atomicAdd(&nact, 1u);
-        if (act[1]) k1 = atomicAdd(&nact, 1u);
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            const uint32_t n = nact;
-            stage_base = n ? (uint32_t)atomicAdd((unsigned long long*)&P.counters[kStageUsed], (unsigned long long)n) : 0u;
-            P.slot_stage[2 * slot] = stage_base;
-            P.slot_stage[2 * slot + 1] = n;
-        }
-        __syncthreads();
-#pragma unroll
-        for (uint32_t r = 0; r < 2; ++r) {
-            if (!act[r]) continue;
-            const uint64_t at = (uint64_t)stage_base + (r ? k1 : k0);
-            if (at < P.stage_cap)
-                P.stage[at] = make_float4(pos[r][0], pos[r][1], pos[r][2], __uint_as_float(meta[r]));
-        }
+    if (threadIdx.x == 0 && tiles_done) {  // one update per block, not per tile
+        atomicAdd((unsigned long long*)&P.counters[kTilesActive], (unsigned long long)tiles_done);
+        atomicAdd((unsigned long long*)&P.counters[kEvalPoints], (unsigned long long)points_done);
+        atomicAdd((unsigned long long*)&P.counters[kEvalFlops], (unsigned long long)flops_done);
     }
 }
 
@@ -541,10 +673,15 @@ __global__ __launch_bounds__(256) void k_sp_emit(SpPass P, MeshBufs b)
 constexpr uint32_t kScanPer = 16;                  // consecutive segments per thread
 constexpr uint32_t kScanChunk = 256 * kScanPer;
 
-// Walks consecutive pass segments, tracking the job and the tile of each.
+// Walks consecutive pass segments, tracking the job and the tile of each (the job's
+// geometry is kept in registers: the walk reloads it only when it enters the next job).
 struct SegCursor {
-    uint32_t j, tx, y, zl;
-    __device__ void seek(const SpPass& P, uint64_t i)
+    uint32_t tx, y, zl;                  // segment (zl, y, tx) of the current job
+    uint32_t ntx, nty, ry, nz;           // the current job's geometry
+    uint64_t tile_row;                   // tile of (zl >> 3, y >> 3, 0)
+    uint64_t tile_base;
+
+    __device__ void load(const SpPass& P, uint64_t i)
     {
         uint32_t lo = 0, hi = P.njobs - 1;  // last job with seg_base <= i
         while (lo < hi) {
@@ -552,33 +689,39 @@ struct SegCursor {
             if (P.jobs[mid].seg_base <= i) lo = mid;
             else hi = mid - 1;
         }
-        j = lo;
-        const SpJob& J = P.jobs[j];
+        const SpJob& J = P.jobs[lo];
+        ntx = J.ntx;
+        nty = J.nty;
+        ry = J.ry;
+        nz = J.nz;
+        tile_base = J.tile_base;
         tx = y = zl = 0;
-        if (i >= P.nseg || J.ntx == 0 || J.ry == 0) return;  // past the end: never used
-        const uint32_t li = (uint32_t)(i - J.seg_base);
-        tx = li % J.ntx;
-        const uint32_t row = li / J.ntx;
-        y = row % J.ry;
-        zl = row / J.ry;
+        if (i < P.nseg && ntx != 0 && ry != 0) {  // else: past the end, never used
+            const uint32_t li = (uint32_t)(i - J.seg_base);
+            tx = li % ntx;
+            const uint32_t row = li / ntx;
+            y = row % ry;
+            zl = row / ry;
+        }
+        tile_row = tile_base + ((uint64_t)(zl >> 3) * nty + (y >> 3)) * ntx;
     }
     // does the tile of the cursor's segment have a slot (and so a written count)?
-    __device__ bool active(const SpPass& P) const
-    {
-        const SpJob& J = P.jobs[j];
-        const uint64_t tile = J.tile_base + ((uint64_t)(zl >> 3) * J.nty + (y >> 3)) * J.ntx + tx;
-        return P.slot_of_tile[tile] >= 0;
-    }
-    __device__ bool first(const SpPass& P) const { return tx == 0 && y == 0 && zl == 0; }
+    __device__ bool active(const SpPass& P) const { return P.slot_of_tile[tile_row + tx] >= 0; }
+    __device__ bool first() const { return tx == 0 && y == 0 && zl == 0; }
     __device__ void next(const SpPass& P, uint64_t i_next)
     {
-        const SpJob& J = P.jobs[j];
-        if (++tx < J.ntx) return;
+        if (++tx < ntx) return;
         tx = 0;
-        if (++y < J.ry) return;
+        if (++y < ry) {
+            if ((y & 7u) == 0) tile_row += ntx;
+            return;
+        }
         y = 0;
-        if (++zl < J.nz) return;
-        if (i_next < P.nseg) seek(P, i_next);  // next job
+        if (++zl < nz) {
+            tile_row = tile_base + ((uint64_t)(zl >> 3) * nty) * ntx;
+            return;
+        }
+        if (i_next < P.nseg) load(P, i_next);  // next job
     }
 };
 
@@ -591,7 +734,7 @@ __global__ __launch_bounds__(256) void k_sp_reduce(SpPass P, uint32_t* __restric
     uint32_t s = 0;
     if (base < P.nseg) {
         SegCursor c;
-        c.seek(P, base);
+        c.load(P, base);
         for (uint32_t k = 0; k < kScanPer && base + k < P.nseg; ++k) {
             if (c.active(P)) s += in[base + k];
             c.next(P, base + k + 1);
@@ -614,7 +757,7 @@ __global__ __launch_bounds__(256) void k_sp_downsweep(SpPass P, const uint32_t* 
     uint32_t v[kScanPer] = {}, s = 0, act = 0, first = 0;
     if (base < P.nseg) {
         SegCursor c;
-        c.seek(P, base);
+        c.load(P, base);
         for (uint32_t k = 0; k < kScanPer; ++k) {
             v[k] = 0;
             if (base + k >= P.nseg) continue;
@@ -622,7 +765,7 @@ __global__ __launch_bounds__(256) void k_sp_downsweep(SpPass P, const uint32_t* 
                 v[k] = in[base + k];
                 act |= 1u << k;
             }
-            if (c.first(P)) first |= 1u << k;
+            if (c.first()) first |= 1u << k;
             s += v[k];
             c.next(P, base + k + 1);
         }
@@ -825,10 +968,10 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
         hipLaunchKernelGGL(k_sp_prune_super, dim3((uint32_t)((P.nsuper + 63) / 64)), dim3(64), 0, s, P);
     hipLaunchKernelGGL(k_sp_prune, dim3((uint32_t)((P.ntiles + 63) / 64)), dim3(64), 0, s, P);
     mark(1);
-    hipLaunchKernelGGL((k_sp_eval<1>), dim3(pb), dim3(256), 0, s, P);
-    hipLaunchKernelGGL((k_sp_eval<0>), dim3(pb), dim3(256), 0, s, P);
+    hipLaunchKernelGGL(k_sp_bricks, dim3(pb), dim3(64), 0, s, P);
+    hipLaunchKernelGGL((k_sp_tiles<1>), dim3(pb), dim3(256), smem, s, P);
+    hipLaunchKernelGGL((k_sp_tiles<0>), dim3(pb), dim3(256), smem, s, P);
     mark(2);
-    hipLaunchKernelGGL(k_sp_cells, dim3(pb), dim3(256), smem, s, P, b);
     if (nsb) {
         hipLaunchKernelGGL((k_sp_reduce<0>), dim3(nsb), dim3(256), 0, s, P, b.bcount);
         (void)launch_scan(s, b, nsb, 0);

@@ -204,3 +204,30 @@ def test_mandelbulb_lattice_vs_fp64(ctx):
     assert np.mean(err_g <= bound) >= 0.999
     assert err_g.max() <= 4 * err_c.max()
     assert np.quantile(err_g, 0.99) <= 4 * np.quantile(err_c, 0.99) + 1e-12
+
+
+@pytest.mark.parametrize("res,flags", [(37, 0), (61, s2m.CORNERS_EXACT), (99, s2m.CORNERS_EXACT),
+                                       (45, s2m.COMPAT_DROP_LAST_SLICE)])
+@pytest.mark.parametrize("seed", [4, 11])
+def test_partial_tiles(ctx, res, flags, seed):
+    # resolutions that are not multiples of the 8-cell tile: partial tiles, partial
+    # bricks and partial super-tiles at the grid's far faces
+    print(f"partial tiles res={res} flags={flags} seed={seed}", flush=True)
+    node = tape.random_csg(seed)
+    p = s2m.Params.reference(res, 2.0, flags=flags)
+    assert_same_mesh(ctx.mesh(ctx.program(node), p), reference_mesh(node, p))
+
+
+@pytest.mark.parametrize("res", [24, 40])
+def test_csg64_linear_vs_dense_small(ctx, res):
+    # the left-fold path (brick culling, two points per lane) against the dense path on
+    # grids whose bricks straddle the surface everywhere
+    print(f"csg64 linear vs dense res={res}", flush=True)
+    prog = ctx.program(tape.csg64())
+    for flags in (0, s2m.REFERENCE_COMPAT):
+        a = ctx.mesh(prog, s2m.Params.reference(res, 1.3, flags=flags))
+        b = ctx.mesh(prog, s2m.Params.reference(res, 1.3, flags=flags | s2m.DENSE_EVAL))
+        assert np.array_equal(a.vertices[:, :3].view(np.uint32), b.vertices[:, :3].view(np.uint32))
+        assert np.array_equal(a.triangles, b.triangles)
+        assert a.invalid_quads == b.invalid_quads
+        np.testing.assert_allclose(a.vertices[:, 3:], b.vertices[:, 3:], atol=1e-6, rtol=0)
