@@ -170,9 +170,12 @@ typedef struct s2m_device_mesh {
 typedef struct s2m_timing {
     double total_ms;    /* first kernel start to last kernel end                   */
     double eval_ms;     /* lattice evaluation kernels                               */
-    double contour_ms;  /* active-cell count, scan (and vertex emission when dense)  */
-    double normal_ms;   /* 4-tap normals (pruned path: with vertex emission)        */
-    double quad_ms;     /* quad count, scan, triangle emission                      */
+    double contour_ms;  /* active-cell count, scan (and vertex emission when dense);
+                           pruned path: the vertex and triangle offset scans        */
+    double normal_ms;   /* 4-tap normals (pruned path: vertex, normal and triangle
+                           emission)                                                */
+    double quad_ms;     /* quad count, scan, triangle emission (pruned path: the
+                           quad count)                                              */
     double d2h_ms;      /* device-to-host copies (s2m_mesh only)                    */
     uint64_t eval_launches;
     uint64_t lattice_points; /* SDF evaluations done on the lattice                 */

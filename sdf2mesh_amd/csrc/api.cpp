@@ -97,7 +97,7 @@ struct s2m_ctx {
     DevBuf lattice, vidx, bcount, boffset, counters, tables;
     DevBuf vtx, keys, signs, tri, corners;
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
-    DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_vbox, sp_seg, sp_boundary;
+    DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_seg, sp_boundary;
     DevBuf sp_stage, sp_slot_stage, sp_slot_live;
     DevBuf sp_job_out, sp_super_lists, sp_super_lens;
     uint64_t slot_cap = 0;
@@ -400,13 +400,18 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     std::vector<SpTables> tabs(n);
     std::vector<size_t> tab_off(n);
     size_t tab_bytes = 0;
-    uint32_t list_stride = 1, box_stride = 1;
+    uint32_t list_stride = 1, box_stride = 1, prune_depth = 1;
     uint64_t ntiles_all = 0, nseg_all = 0, plane_max = 0, cells_all = 0, planes_sum = 0;
     for (uint32_t i = 0; i < n; ++i) {
         build_sp_tables(jobs[i], progs[i], ps[i]->eps, tabs[i]);
         tab_off[i] = tab_bytes;
         tab_bytes += (tabs[i].bytes.size() + 255) & ~size_t(255);
         list_stride = std::max(list_stride, progs[i]->n);
+        uint32_t depth = 0;  // operand stack entries below the top (k_sp_prune's LDS stack)
+        for (const s2m_op& o : progs[i]->ops) {
+            if (o.code < S2M_OP_UNION) prune_depth = std::max(prune_depth, depth++);
+            else --depth;
+        }
         box_stride = std::max(box_stride, tabs[i].bmax[0] * tabs[i].bmax[1] * tabs[i].bmax[2]);
         ntiles_all += (uint64_t)tabs[i].nt[0] * tabs[i].nt[1] * tabs[i].nt[2];
         nseg_all += (uint64_t)jobs[i].nz_cells * jobs[i].res[1] * tabs[i].nt[0];
@@ -520,9 +525,9 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     HIP_TRY(ctx->tile_flops.ensure(tiles_pass * sizeof(uint32_t)));
     HIP_TRY(ctx->sp_slot_of_tile.ensure(tiles_pass * sizeof(int32_t)));
     HIP_TRY(ctx->sp_seg.ensure(std::max<uint64_t>(seg_pass, 1) * 4 * sizeof(uint32_t)));
-    const uint32_t nsb_max = (uint32_t)((seg_pass + 4095) / 4096) + 1;
-    HIP_TRY(ctx->bcount.ensure((size_t)nsb_max * sizeof(uint32_t)));
-    HIP_TRY(ctx->boffset.ensure((size_t)nsb_max * sizeof(uint32_t)));
+    const uint32_t nsb_max = (uint32_t)((seg_pass + 4095) / 4096) + 1;  // vertex and triangle sums
+    HIP_TRY(ctx->bcount.ensure((size_t)2 * nsb_max * sizeof(uint32_t)));
+    HIP_TRY(ctx->boffset.ensure((size_t)2 * nsb_max * sizeof(uint32_t)));
     HIP_TRY(ctx->counters.ensure(kNumCounters * sizeof(uint64_t)));
     HIP_TRY(ctx->sp_job_out.ensure((size_t)n * kJobOut * sizeof(uint64_t)));
 
@@ -540,7 +545,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         HIP_TRY(ctx->tri.ensure(want_t * 3 * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_tile_of_slot.ensure(want_s * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_job_of_slot.ensure(want_s * sizeof(uint32_t)));
-        HIP_TRY(ctx->sp_vbox.ensure(want_s * 512 * sizeof(uint32_t)));
+        HIP_TRY(ctx->sp_cellbits.ensure(want_s * 16 * sizeof(uint32_t)));
         // staged cells are allocated in per-block chunks: at most 2 x vertices + blocks x chunk
         // (two launches of pblocks blocks each)
         const uint64_t stage_cap = 2 * want_v + 2ull * pblocks * kStageChunkHost;
@@ -580,6 +585,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.jobs = n == 1 ? ddesc + pass : ddesc;
             P.njobs = n == 1 ? 1 : n;
             P.list_stride = list_stride;
+            P.prune_depth = prune_depth;
             if (n == 1) {
                 const SpJob& J = desc[pass];
                 P.ntiles = (uint64_t)J.ntx * J.nty * ((J.nz + kTile - 1) / kTile);
@@ -605,7 +611,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.job_of_slot = ctx->sp_job_of_slot.as<uint32_t>();
             P.slot_cap = (uint32_t)want_s;
             P.box_stride = box_stride;
-            P.vbox = ctx->sp_vbox.as<uint32_t>();
+            P.cellbits = ctx->sp_cellbits.as<uint32_t>();
             P.stage = ctx->sp_stage.as<float4>();
             P.stage_cap = 2 * want_v + 2ull * pblocks * kStageChunkHost;
             P.slot_stage = ctx->sp_slot_stage.as<uint32_t>();
@@ -621,7 +627,8 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
                 for (auto& e : m) e = ctx->next_event();
             HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters));
             const int boundary = (n == 1 && pass + 1 < npass) ? 1 : 0;
-            HIP_TRY(launch_sparse_pass(ctx->stream, P, b, pblocks, boundary, ctx->profiling ? m : nullptr));
+            HIP_TRY(launch_sparse_pass(ctx->stream, P, b, ctx->cus, (uint32_t)pblocks, boundary,
+                                         ctx->profiling ? m : nullptr));
             if (ctx->profiling) ev.insert(ev.end(), m, m + kSparseMarks);
             tiles_total += P.ntiles;
         }
@@ -645,10 +652,10 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
                 for (int k = 0; k + 1 < kSparseMarks; ++k)
                     HIP_TRY(hipEventElapsedTime(&t[k], ev[q + k], ev[q + k + 1]));
                 ctx->timing.prune_ms += t[0];
-                ctx->timing.eval_ms += t[1];
-                ctx->timing.contour_ms += t[2];
-                ctx->timing.normal_ms += t[3];  // vertex emission + normals
-                ctx->timing.quad_ms += t[4];
+                ctx->timing.eval_ms += t[1];     // evaluation + contouring
+                ctx->timing.quad_ms += t[2];     // quad count
+                ctx->timing.contour_ms += t[3];  // vertex and triangle offset scans
+                ctx->timing.normal_ms += t[4];   // vertex, normal and triangle emission
             }
         }
         ctx->timing.eval_launches = npass;
@@ -1009,7 +1016,7 @@ void s2m_destroy(s2m_ctx* ctx)
                       &ctx->tables, &ctx->vtx, &ctx->keys, &ctx->signs, &ctx->tri, &ctx->corners,
                       &ctx->tile_tab, &ctx->tile_lists, &ctx->tile_lens,
                       &ctx->tile_flops, &ctx->sp_slot_of_tile, &ctx->sp_tile_of_slot,
-                      &ctx->sp_job_of_slot, &ctx->sp_vbox, &ctx->sp_seg,
+                      &ctx->sp_job_of_slot, &ctx->sp_cellbits, &ctx->sp_seg,
                       &ctx->sp_boundary, &ctx->sp_job_out, &ctx->sp_super_lists, &ctx->sp_super_lens,
                       &ctx->sp_stage, &ctx->sp_slot_stage, &ctx->sp_slot_live})
         b->release();

@@ -66,7 +66,7 @@ constexpr uint32_t kTile = 8;  // cells per tile edge (sparse pipeline)
 constexpr uint32_t kSuper = 4;  // tiles per super-tile edge (first pruning level)
 constexpr uint32_t kSingleSign = 0xffffffffu;  // super_lens: the SDF keeps one sign there
 constexpr uint32_t kStageChunkHost = 1024;  // staged-cell allocation chunk (sparse.hip)
-// Profiling marks of one sparse pass: prune | eval | count+scan | emit+normals | quads
+// Profiling marks of one sparse pass: prune | eval+contour | quad count | scan | emission
 constexpr int kSparseMarks = 6;
 
 // Tile-sparse pipeline (sparse.hip).  One job of a pass: a slab of a single job, or a
@@ -107,6 +107,7 @@ struct SpPass {
     const SpJob* jobs;
     uint32_t njobs;
     uint32_t list_stride;      // >= nops of every job
+    uint32_t prune_depth;      // >= 1, >= the deepest operand stack of any job's program - 1
     uint64_t ntiles;           // tiles of the pass
     uint64_t nseg;             // row segments of the pass
     uint64_t boundary_cells;   // rx * ry of jobs[0] when a boundary plane is written
@@ -121,7 +122,8 @@ struct SpPass {
     uint32_t* job_of_slot;     // [slot]
     uint32_t slot_cap;
     uint32_t box_stride;       // floats of a tile's lattice box (LDS)
-    uint32_t* vbox;            // [slot][512] vertex of each cell of the tile, kNone if none
+    uint32_t* cellbits;        // [slot][16] bit c: cell c of the tile has a vertex (row r = c / 8
+                               // in byte r % 4 of word r / 4)
     float4* stage;             // [stage_cap] active cells: position, cell | rank << 9 | signs << 12
     uint64_t stage_cap;
     uint32_t* slot_stage;      // [slot][2] first staged cell, staged cells
@@ -156,8 +158,10 @@ hipError_t launch_normals(hipStream_t s, const s2m_op* ops, uint32_t nops, float
                           const MeshBufs& b);
 hipError_t launch_quads(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int emit);
 hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters);
-hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b, int persistent_blocks,
-                              int boundary_plane, hipEvent_t* marks);
+// cus: compute units of the device; tile_blocks: most blocks of a k_sp_tiles launch (the
+// staging bound).
+hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b, int cus,
+                              uint32_t tile_blocks, int boundary_plane, hipEvent_t* marks);
 // Contour corner values given per cell (test hook): cells of the slab read their 8
 // corners from `corners` (8 floats per cell, cells in (z,y,x) order from plane z0).
 hipError_t launch_cells_from_corners(hipStream_t s, const SlabArgs& a, const float* corners,

@@ -12,21 +12,28 @@
 //   k_sp_prune      one thread per tile: the super-tile's list pruned on the tile's padded
 //                   box; single-signed tiles are dropped, the others get a slot in a pool
 //                   shared by all jobs (and a flag when their list is a left fold)
-//   k_sp_eval<1/0>  per slot (grid-stride, persistent): the pruned program, staged in LDS,
-//                   at every lattice point of the tile's closed box, stored in the slot's
-//                   compact box (left folds / other programs)
-//   k_sp_cells      per slot: vertex and sign bits of each active cell, staged compactly,
-//                   and the active cells of each row segment (z, y, tile x)
-//   reduce / k_scan / downsweep   exclusive scan of the row segments, job after job and
-//                   (z, y, tile x) inside a job: the reference's VertexList order
-//   k_sp_ranges<0>  each job's first vertex index
-//   k_sp_emit       the staged vertices at their ordered index, the slot's cell -> vertex
-//                   box, and the 4-tap normals with the tile's pruned program
-//   k_sp_quads<0/1> the same count / scan / emit scheme for triangles (vertex order), with
-//                   vertex indices relative to the job's first vertex
-//   k_sp_ranges<1>  each job's first triangle index
+//   k_sp_bricks     per slot, one lane per brick of 2x2x2 cells: interval bounds of a
+//                   left fold on the brick; bricks that keep one sign are not evaluated
+//   k_sp_tiles<1/0> per slot (persistent): the pruned program, staged in LDS, at the
+//                   lattice points the tile's live bricks use (left folds) or at all
+//                   points of its box (other programs), into an LDS box; then the
+//                   tile's cells: vertex and sign bits of each active cell, staged
+//                   compactly, the slot's cell bits and each row segment's vertex count
+//   k_sp_quadcount  per slot: valid quads of the staged cells (neighbour cells' bits),
+//                   two triangles each, per row segment; invalid quads counted
+//   reduce / k_scan / downsweep   exclusive scans of the row segments' vertex and
+//                   triangle counts, job after job and (z, y, tile x) inside a job: the
+//                   reference's VertexList order
+//   k_sp_ranges     each job's first vertex and triangle index
+//   k_sp_emit       per slot: the staged vertices at their ordered index with their
+//                   4-tap normals (the tile's pruned program), then the triangles of
+//                   their quads, vertex indices relative to the job's first vertex
 //   k_sp_boundary   (multi-slab single job) cell -> vertex map of the slab's last plane
+// A cell's vertex index is never stored: it is the first index of its row segment plus
+// the number of cells with a vertex before it in the segment (the slot's cell bits).
 // Loops over slots read their bound on the device: no host round trip inside a pass.
+#include <algorithm>
+
 #include "device_interval.h"
 #include "device_sdf.h"
 #include "device_staged.h"
@@ -34,9 +41,19 @@
 
 namespace s2m {
 
+// threadIdx.x through an opaque copy: values derived from it are recomputed where they
+// are used instead of being hoisted out of the kernels' slot loops and held in registers
+// for the whole kernel (which costs occupancy).
+__device__ __forceinline__ uint32_t thread_id()
+{
+    uint32_t t;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"((uint32_t)threadIdx.x));
+    return t;
+}
+
 namespace {
 
-__device__ __forceinline__ uint32_t lane() { return threadIdx.x & 63u; }
+__device__ __forceinline__ uint32_t lane() { return thread_id() & 63u; }
 
 __device__ __forceinline__ uint32_t active_slots(const SpPass& P)
 {
@@ -60,6 +77,59 @@ __device__ __forceinline__ void tile_xyz(const SpJob& J, uint64_t tile, uint32_t
     tzl = t / (J.ntx * J.nty);
 }
 
+// Existence bits of row r (= z % 8 * 8 + y % 8) of a tile with a slot: bit x % 8.
+__device__ __forceinline__ uint32_t slot_row_bits(const SpPass& P, int32_t slot, uint32_t r)
+{
+    return (P.cellbits[(size_t)slot * 16u + (r >> 2)] >> ((r & 3u) * 8u)) & 0xffu;
+}
+
+// Cells (8 * tx - 1 + i, y, z), i = 0..8, of job J (z is the job's plane index, y and z
+// may be -1; cells outside the grid have no vertex): bit i of the result is set iff the
+// cell has a vertex.  INDEX: idx[i] = its pass-level vertex index, kNone if none (the
+// first vertex of the row segment plus the cell's rank in it).  Plane z0 - 1 comes from
+// the previous slab's boundary map.
+template <bool INDEX>
+__device__ uint32_t nb_row(const SpPass& P, const SpJob& J, uint32_t tx, int y, int z, uint32_t* idx)
+{
+    uint32_t bits = 0;
+    if (INDEX)
+        for (int i = 0; i < 9; ++i) idx[i] = kNone;
+    if (y < 0 || z < 0 || (uint32_t)y >= J.ry || (uint32_t)z >= J.z0 + J.nz) return 0;
+    if ((uint32_t)z < J.z0) {
+        for (int i = 0; i < 9; ++i) {
+            const int x = (int)(tx * 8u) + i - 1;
+            if (x < 0 || (uint32_t)x >= J.rx) continue;
+            const uint32_t v = J.boundary[(size_t)y * J.rx + (uint32_t)x];
+            if (v != kNone) bits |= 1u << i;
+            if (INDEX) idx[i] = v;
+        }
+        return bits;
+    }
+    const uint32_t zl = (uint32_t)z - J.z0;
+    const uint64_t trow =
+        J.tile_base + ((uint64_t)(((uint32_t)z >> 3) - J.tz0) * J.nty + ((uint32_t)y >> 3)) * J.ntx;
+    const uint32_t r = ((uint32_t)z & 7u) * 8u + ((uint32_t)y & 7u);
+    for (int c = 0; c < 2; ++c) {  // c = 0: the tile column's own cells, 1: the cell before it
+        if (c == 1 && tx == 0) break;
+        const uint32_t t = c ? tx - 1 : tx;
+        const int32_t slot = P.slot_of_tile[trow + t];
+        if (slot < 0) continue;
+        const uint32_t b8 = slot_row_bits(P, slot, r);
+        if (c ? !(b8 & 0x80u) : !b8) continue;
+        bits |= c ? 1u : b8 << 1;
+        if (INDEX) {
+            const uint32_t first = P.seg_off_v[seg_index(J, zl, (uint32_t)y, t)];
+            if (c) {
+                idx[0] = first + (uint32_t)__builtin_popcount(b8 & 0x7fu);
+            } else {
+                for (uint32_t i = 0; i < 8; ++i)
+                    if ((b8 >> i) & 1u) idx[i + 1] = first + (uint32_t)__builtin_popcount(b8 & ((1u << i) - 1u));
+            }
+        }
+    }
+    return bits;
+}
+
 // Pass-level vertex index of cell (x, y, z) of job J (z is the job's plane index), kNone
 // if it has no vertex.  Plane z0 - 1 comes from the previous slab's boundary map.
 __device__ __forceinline__ uint32_t vertex_at(const SpPass& P, const SpJob& J, uint32_t x, uint32_t y,
@@ -70,7 +140,9 @@ __device__ __forceinline__ uint32_t vertex_at(const SpPass& P, const SpJob& J, u
         J.tile_base + ((uint64_t)((z >> 3) - J.tz0) * J.nty + (y >> 3)) * J.ntx + (x >> 3);
     const int32_t slot = P.slot_of_tile[tile];
     if (slot < 0) return kNone;
-    return P.vbox[(size_t)slot * 512u + (((z & 7u) * 8u + (y & 7u)) * 8u + (x & 7u))];
+    const uint32_t b8 = slot_row_bits(P, slot, (z & 7u) * 8u + (y & 7u)), i = x & 7u;
+    if (!((b8 >> i) & 1u)) return kNone;
+    return P.seg_off_v[seg_index(J, z - J.z0, y, x >> 3)] + (uint32_t)__builtin_popcount(b8 & ((1u << i) - 1u));
 }
 
 // dualcontour.wgsl:72-131: crossings on the 12 edges in the reference's order, counted
@@ -119,19 +191,26 @@ namespace {
 // the program's op indices (nullptr: every op of `ops` in order).  The kept op indices
 // are written to `out` in order; returns their count, and the interval of the result in
 // `root`.  The top of the interval stack stays in registers; deeper entries live in the
-// block's LDS (PruneStack, one column per lane: a thread's depth differs from its
-// neighbours' when they prune different jobs).
+// block's dynamic LDS, P.prune_depth entries per lane (one column per lane: a thread's
+// depth differs from its neighbours' when they prune different jobs).  Blocks of 64.
 struct PruneStack {
-    double lo[S2M_MAX_STACK][64];
-    double hi[S2M_MAX_STACK][64];
-    uint16_t start[S2M_MAX_STACK][64];
+    double* lo;       // [depth][64]
+    double* hi;       // [depth][64]
+    uint16_t* start;  // [depth][64]
 };
+
+__device__ __forceinline__ PruneStack prune_stack(const SpPass& P)
+{
+    extern __shared__ double prune_lds[];
+    const uint32_t d = P.prune_depth;
+    return PruneStack{prune_lds, prune_lds + 64u * d, reinterpret_cast<uint16_t*>(prune_lds + 128u * d)};
+}
 
 __device__ uint32_t prune_program(const s2m_op* __restrict__ ops, const uint16_t* __restrict__ in,
                                   uint32_t n, const double bl[3], const double bh[3],
-                                  uint16_t* __restrict__ out, Iv& root, PruneStack& S)
+                                  uint16_t* __restrict__ out, Iv& root, const PruneStack& S)
 {
-    const uint32_t ln = threadIdx.x & 63u;
+    const uint32_t ln = thread_id() & 63u;
     Iv top = iv_unknown();
     uint32_t top_start = 0, depth = 0, len = 0;  // depth: entries below `top`
     for (uint32_t q = 0; q < n; ++q) {
@@ -139,9 +218,9 @@ __device__ uint32_t prune_program(const s2m_op* __restrict__ ops, const uint16_t
         const uint32_t code = ops[i].code;
         if (code < S2M_OP_UNION) {
             if (q != 0) {  // push the previous top
-                S.lo[depth][ln] = top.lo;
-                S.hi[depth][ln] = top.hi;
-                S.start[depth][ln] = (uint16_t)top_start;
+                S.lo[depth * 64u + ln] = top.lo;
+                S.hi[depth * 64u + ln] = top.hi;
+                S.start[depth * 64u + ln] = (uint16_t)top_start;
                 ++depth;
             }
             top = iv_primitive(code, ops[i].f, bl, bh);
@@ -151,8 +230,9 @@ __device__ uint32_t prune_program(const s2m_op* __restrict__ ops, const uint16_t
             const Iv B = top;
             const uint32_t sB = top_start;
             --depth;
-            const Iv A{S.lo[depth][ln], S.hi[depth][ln], S.lo[depth][ln] > -kHuge};
-            const uint32_t sA = S.start[depth][ln];
+            const double alo = S.lo[depth * 64u + ln], ahi = S.hi[depth * 64u + ln];
+            const Iv A{alo, ahi, alo > -kHuge};
+            const uint32_t sA = S.start[depth * 64u + ln];
             int keep;
             const Iv r = iv_op(code, A, B, ops[i].f[0], keep);
             if (keep == kKeepBoth) {
@@ -193,7 +273,7 @@ __device__ __forceinline__ uint32_t job_of(const SpPass& P, uint64_t index, bool
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_sp_prune_super(SpPass P)
 {
-    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + thread_id();
     if (s >= P.nsuper) return;
     const SpJob& J = P.jobs[job_of(P, s, true)];
     const uint32_t t = (uint32_t)(s - J.super_base);
@@ -208,10 +288,9 @@ __global__ __launch_bounds__(64) void k_sp_prune_super(SpPass P)
     const double bl[3] = {(double)ex[tx0] - J.pad, (double)ey[ty0] - J.pad, (double)ez[tz0] - J.pad};
     const double bh[3] = {(double)ex[J.ntx + tx1] + J.pad, (double)ey[J.nty + ty1] + J.pad,
                           (double)ez[J.ntz + tz1] + J.pad};
-    __shared__ PruneStack S;
     Iv root;
     const uint32_t len = prune_program(J.ops, nullptr, J.nops, bl, bh,
-                                       P.super_lists + (size_t)s * P.list_stride, root, S);
+                                       P.super_lists + (size_t)s * P.list_stride, root, prune_stack(P));
     const bool one_sign = root.known && (root.lo > 0.0 || root.hi < 0.0);
     P.super_lens[s] = one_sign ? kSingleSign : len;
 }
@@ -222,7 +301,7 @@ __global__ __launch_bounds__(64) void k_sp_prune_super(SpPass P)
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
 {
-    const uint64_t tile = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t tile = (uint64_t)blockIdx.x * blockDim.x + thread_id();
     if (tile >= P.ntiles) return;
     const uint32_t jid = job_of(P, tile, false);
     const SpJob& J = P.jobs[jid];
@@ -247,10 +326,9 @@ __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
                           (double)ez[J.ntz + tz] + J.pad};
     const s2m_op* ops = J.ops;
     uint16_t* L = P.lists + (size_t)tile * P.list_stride;
-    __shared__ PruneStack S;
     Iv root;
     const uint32_t len =
-        prune_program(ops, P.super_lists + (size_t)s * P.list_stride, sl, bl, bh, L, root, S);
+        prune_program(ops, P.super_lists + (size_t)s * P.list_stride, sl, bl, bh, L, root, prune_stack(P));
     const bool one_sign = root.known && (root.lo > 0.0 || root.hi < 0.0);
     uint32_t fl = 0;
     for (uint32_t j = 0; j < len; ++j) fl += J.op_flops[L[j]];
@@ -280,12 +358,12 @@ struct TileAxes {
     float clo[3][8], chi[3][8];
 };
 
-// Entry threadIdx.x (< 24) of A: axis threadIdx.x / 8, cell threadIdx.x % 8.  Cells past
+// Entry thread_id() (< 24) of A: axis thread_id() / 8, cell thread_id() % 8.  Cells past
 // the end of the grid repeat the last cell's entries (their cells are never contoured).
 __device__ __forceinline__ void load_tile_axes(TileAxes& A, const SpJob& J, uint32_t tx, uint32_t ty,
                                                uint32_t tz)
 {
-    const uint32_t ax = threadIdx.x >> 3, c = threadIdx.x & 7u;
+    const uint32_t ax = thread_id() >> 3, c = thread_id() & 7u;
     const uint32_t t = ax == 0 ? tx : ax == 1 ? ty : tz;
     const uint32_t r = ax == 0 ? J.rx : ax == 1 ? J.ry : J.z0 + J.nz;  // cells on the axis
     const uint32_t* lo = ax == 0 ? J.lox : ax == 1 ? J.loy : J.loz;
@@ -329,7 +407,7 @@ constexpr int kLinearPoints = 2;
 __global__ __launch_bounds__(64) void k_sp_bricks(SpPass P)
 {
     const uint32_t nslots = active_slots(P);
-    const uint32_t b = threadIdx.x;
+    const uint32_t b = thread_id();
     for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
         const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
         const uint32_t lens = __builtin_amdgcn_readfirstlane(P.lens[tile]);
@@ -370,88 +448,132 @@ __global__ __launch_bounds__(64) void k_sp_bricks(SpPass P)
 }
 
 // ---------------------------------------------------------------------------------
-// The cells of one tile (all threads of the block; the tile's box values in LDS): thread
-// t owns cells t and t + 256 (x fastest), so the 8 cells of a row segment are 8
-// consecutive lanes.  Per active cell: the vertex (dualcontour.wgsl:72-131) and sign
-// bits (:57-69), staged compactly for k_sp_emit with the cell's rank in its row segment;
-// per row segment: its vertex count.  Every cell's entry of the slot's cell -> vertex box
-// is reset to kNone.  Cells of bricks outside `live` are not contoured (their corner
-// values keep one sign and were not evaluated).  Contains barriers.
+// The cells of one tile (all threads of the block; the tile's box values in LDS), in
+// three steps: the cells whose 8 corners change sign are listed (cells of bricks outside
+// `live` are skipped: their corner values keep one sign and were not evaluated); each
+// listed cell gets its vertex (dualcontour.wgsl:72-131) and sign bits (:57-69); the
+// cells with a vertex are staged compactly for k_sp_emit with their rank in their row
+// segment, their bits go to the slot's `cellbits` and each row segment's count to
+// seg_v.  Contains barriers.
 // ---------------------------------------------------------------------------------
 // Staged cells are allocated in chunks of kStageChunk per block (one counter update per
 // chunk); a slot's cells are contiguous, so a chunk with fewer than the slot's cells left
 // is abandoned.  Hence at most 2 x (active cells) + blocks x kStageChunk entries are used.
 constexpr uint32_t kStageChunk = kStageChunkHost;
 
-struct StageChunk {  // block-shared allocation state
-    uint32_t base, left, nact, slot_base;
+struct ContourLds {  // block-shared state of contour_tile
+    uint16_t cand[512];  // cells with a sign change among their corners
+    uint32_t rows[64];   // row czl * 8 + cyl: bit cxl = the cell has a vertex
+    uint32_t ncand, nact;
+    uint32_t base, left, slot_base;  // staging chunk of the block
 };
+
+// Reset for the next tile (thread_id() < 64; a barrier must follow before contour_tile).
+__device__ __forceinline__ void contour_reset(ContourLds& L)
+{
+    if (thread_id() < 64) L.rows[thread_id()] = 0;
+    if (thread_id() == 0) L.ncand = L.nact = 0;
+}
+
+__device__ __forceinline__ void tile_corners(const float* box, const TileAxes& A, uint32_t bx, uint32_t by,
+                                             uint32_t c, float cv[8])
+{
+    const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
+    const uint32_t xi[2] = {A.lo[0][cxl], A.hi[0][cxl]};
+    const uint32_t yi[2] = {A.lo[1][cyl], A.hi[1][cyl]};
+    const uint32_t zi[2] = {A.lo[2][czl], A.hi[2][czl]};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) cv[k] = box[(zi[(k >> 2) & 1] * by + yi[(k >> 1) & 1]) * bx + xi[k & 1]];
+}
 
 __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, uint32_t slot, uint32_t tx,
                                              uint32_t ty, uint32_t tzl, const float* box, const TileAxes& A,
-                                             uint64_t live, StageChunk& C)
+                                             uint64_t live, ContourLds& L)
 {
-    bool act[2];
-    float pos[2][3];
-    uint32_t meta[2];
+    // 1. cells with a sign change (thread t: cells t and t + 256, x fastest)
 #pragma unroll
     for (uint32_t r = 0; r < 2; ++r) {
-        const uint32_t c = r * 256u + threadIdx.x;
+        const uint32_t c = r * 256u + thread_id();
         const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
-        const uint32_t x = tx * 8u + cxl, y = ty * 8u + cyl, zl = tzl * 8u + czl;
         const uint32_t brick = ((czl >> 1) * 4u + (cyl >> 1)) * 4u + (cxl >> 1);
-        const bool inside = x < J.rx && y < J.ry && zl < J.nz;
-        act[r] = false;
-        uint32_t sg = 0;
-        if (inside && ((live >> brick) & 1ull)) {  // a dead brick has no active cell
+        bool cand = false;
+        if (tx * 8u + cxl < J.rx && ty * 8u + cyl < J.ry && tzl * 8u + czl < J.nz && ((live >> brick) & 1ull)) {
             float cv[8];
-            const uint32_t xi[2] = {A.lo[0][cxl], A.hi[0][cxl]};
-            const uint32_t yi[2] = {A.lo[1][cyl], A.hi[1][cyl]};
-            const uint32_t zi[2] = {A.lo[2][czl], A.hi[2][czl]};
+            tile_corners(box, A, J.bx, J.by, c, cv);
+            uint32_t pm = 0;
 #pragma unroll
-            for (int k = 0; k < 8; ++k)
-                cv[k] = box[(zi[(k >> 2) & 1] * J.by + yi[(k >> 1) & 1]) * J.bx + xi[k & 1]];
+            for (int k = 0; k < 8; ++k) pm |= (uint32_t)(cv[k] > 0.0f) << k;
+            cand = pm != 0u && pm != 0xffu;
+        }
+        const uint64_t w = __ballot(cand);
+        const uint32_t lead = (uint32_t)__builtin_ctzll(w | (1ull << 63));
+        uint32_t wbase = 0;
+        if (w && lane() == lead) wbase = atomicAdd(&L.ncand, (uint32_t)__builtin_popcountll(w));
+        wbase = __shfl(wbase, lead, 64);
+        if (cand) L.cand[wbase + (uint32_t)__builtin_popcountll(w & ((1ull << lane()) - 1ull))] = (uint16_t)c;
+    }
+    __syncthreads();
+    // 2. vertex and sign bits of each listed cell
+    const uint32_t ncand = L.ncand;
+    bool act[2] = {false, false};
+    float pos[2][3];
+    uint32_t cell[2] = {0, 0}, sg[2] = {0, 0}, k_at[2] = {0, 0};
+#pragma unroll
+    for (uint32_t r = 0; r < 2; ++r) {
+        const uint32_t k = r * 256u + thread_id();
+        if (k < ncand) {
+            const uint32_t c = L.cand[k];
+            const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
+            float cv[8];
+            tile_corners(box, A, J.bx, J.by, c, cv);
             const float lo[3] = {A.clo[0][cxl], A.clo[1][cyl], A.clo[2][czl]};
             const float hi[3] = {A.chi[0][cxl], A.chi[1][cyl], A.chi[2][czl]};
             act[r] = contour_cell_sp(cv, lo, hi, pos[r]);
+            cell[r] = c;
             // cell_sign_changes (dualcontour.wgsl:57-69)
-            sg = (uint32_t)(cv[1] > 0.0f) | ((uint32_t)(cv[2] > 0.0f) << 1) |
-                 ((uint32_t)(cv[4] > 0.0f) << 2) | ((uint32_t)(cv[0] > 0.0f) << 3);
+            sg[r] = (uint32_t)(cv[1] > 0.0f) | ((uint32_t)(cv[2] > 0.0f) << 1) |
+                    ((uint32_t)(cv[4] > 0.0f) << 2) | ((uint32_t)(cv[0] > 0.0f) << 3);
         }
-        const uint64_t m = __ballot(act[r]);
-        const uint32_t g0 = lane() & ~7u;
-        const uint32_t gmask = (uint32_t)((m >> g0) & 0xffull);
-        const uint32_t below = (uint32_t)__builtin_popcount(gmask & ((1u << (lane() - g0)) - 1u));
-        if (cxl == 0 && y < J.ry && zl < J.nz)
-            P.seg_v[seg_index(J, zl, y, tx)] = (uint32_t)__builtin_popcount(gmask);
-        meta[r] = c | (below << 9) | (sg << 12);
-        P.vbox[(size_t)slot * 512u + c] = kNone;
+        const uint64_t w = __ballot(act[r]);
+        const uint32_t lead = (uint32_t)__builtin_ctzll(w | (1ull << 63));
+        uint32_t wbase = 0;
+        if (w && lane() == lead) wbase = atomicAdd(&L.nact, (uint32_t)__builtin_popcountll(w));
+        wbase = __shfl(wbase, lead, 64);
+        if (act[r]) {
+            k_at[r] = wbase + (uint32_t)__builtin_popcountll(w & ((1ull << lane()) - 1ull));
+            atomicOr(&L.rows[cell[r] >> 3], 1u << (cell[r] & 7u));
+        }
     }
-    // compact staging of the tile's active cells
-    uint32_t k0 = 0, k1 = 0;
-    if (act[0]) k0 = atomicAdd(&C.nact, 1u);
-    if (act[1]) k1 = atomicAdd(&C.nact, 1u);
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t n = C.nact;
-        if (C.left < n) {
-            C.base = (uint32_t)atomicAdd((unsigned long long*)&P.counters[kStageUsed], (unsigned long long)kStageChunk);
-            C.left = kStageChunk;
+    // 3. staging space, row segment counts and the slot's cell bits
+    if (thread_id() == 0) {
+        const uint32_t n = L.nact;
+        if (L.left < n) {
+            L.base = (uint32_t)atomicAdd((unsigned long long*)&P.counters[kStageUsed], (unsigned long long)kStageChunk);
+            L.left = kStageChunk;
         }
-        C.slot_base = C.base + (kStageChunk - C.left);
-        C.left -= n;
-        C.nact = 0;  // for the next slot (read again only after the next barrier)
-        P.slot_stage[2 * slot] = C.slot_base;
+        L.slot_base = L.base + (kStageChunk - L.left);
+        L.left -= n;
+        P.slot_stage[2 * slot] = L.slot_base;
         P.slot_stage[2 * slot + 1] = n;
     }
+    if (thread_id() < 64) {
+        const uint32_t row = thread_id(), y = ty * 8u + (row & 7u), zl = tzl * 8u + (row >> 3);
+        if (y < J.ry && zl < J.nz) P.seg_v[seg_index(J, zl, y, tx)] = (uint32_t)__builtin_popcount(L.rows[row]);
+        if (row < 16)
+            P.cellbits[(size_t)slot * 16u + row] = L.rows[4 * row] | (L.rows[4 * row + 1] << 8) |
+                                                   (L.rows[4 * row + 2] << 16) | (L.rows[4 * row + 3] << 24);
+    }
     __syncthreads();
-    const uint32_t stage_base = C.slot_base;
+    const uint32_t stage_base = L.slot_base;
 #pragma unroll
     for (uint32_t r = 0; r < 2; ++r) {
         if (!act[r]) continue;
-        const uint64_t at = (uint64_t)stage_base + (r ? k1 : k0);
+        const uint32_t c = cell[r];
+        const uint32_t rank = (uint32_t)__builtin_popcount(L.rows[c >> 3] & ((1u << (c & 7u)) - 1u));
+        const uint64_t at = (uint64_t)stage_base + k_at[r];
         if (at < P.stage_cap)
-            P.stage[at] = make_float4(pos[r][0], pos[r][1], pos[r][2], __uint_as_float(meta[r]));
+            P.stage[at] = make_float4(pos[r][0], pos[r][1], pos[r][2], __uint_as_float(c | (rank << 9) | (sg[r] << 12)));
     }
 }
 
@@ -469,11 +591,13 @@ __global__ __launch_bounds__(256) void k_sp_tiles(SpPass P)
     extern __shared__ float box[];  // P.box_stride floats
     __shared__ StagedTape T;
     __shared__ TileAxes A;
-    __shared__ uint32_t bm[3][32];  // per axis and box index: the bricks (along it) using it
+    __shared__ uint32_t bm[3][32];   // per axis and box index: the bricks (along it) using it
+    __shared__ uint8_t rowm[32 * 32];  // per box row (iz, iy): x-bricks of the live bricks using it
     __shared__ uint16_t list[kChunk];
     __shared__ uint32_t nlist;
-    __shared__ StageChunk C;
-    if (threadIdx.x == 0) C = StageChunk{0u, 0u, 0u, 0u};  // published by the first slot's barriers
+    __shared__ float pc[3][32];  // lattice coordinates of the box per axis
+    __shared__ ContourLds L;
+    if (thread_id() == 0) L.left = 0;  // published by the first slot's barriers
     uint64_t tiles_done = 0, points_done = 0, flops_done = 0;  // thread 0: this block's share
     const uint32_t nslots = active_slots(P);
     for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
@@ -496,53 +620,63 @@ __global__ __launch_bounds__(256) void k_sp_tiles(SpPass P)
         const uint32_t nz = __builtin_amdgcn_readfirstlane(bzr[J.ntz + tz]) - z0 + 1;
         const uint32_t nxy = nx * ny, npts = nxy * nz;
         const uint32_t mx = magic16(nx), mxy = magic16(nxy);
-        const uint16_t* L = P.lists + (size_t)tile * P.list_stride;
+        const uint16_t* Lst = P.lists + (size_t)tile * P.list_stride;
         const uint32_t bx = J.bx, by = J.by;
         const float* cx = J.cx + x0;
         const float* cy = J.cy + y0;
         const float* cz = J.cz + z0;
         const bool staged = n <= kStageOps;
-        __syncthreads();  // the previous slot's readers of T, A, bm, list, box are done
-        if (staged) stage_tape(T, J.ops, L, n);
-        if (threadIdx.x < 24) load_tile_axes(A, J, tx, ty, tz);
-        if (LINEAR && threadIdx.x < 96) bm[threadIdx.x >> 5][threadIdx.x & 31u] = 0;
+        const uint64_t lv = LINEAR ? P.slot_live[slot] : ~0ull;
+        __syncthreads();  // the previous slot's readers of T, A, bm, rowm, list, box, L are done
+        if (staged) stage_tape(T, J.ops, Lst, n);
+        if (thread_id() < 24) load_tile_axes(A, J, tx, ty, tz);
+        if (thread_id() < 96) {
+            const uint32_t a = thread_id() >> 5, i = thread_id() & 31u;
+            const float* c = a == 0 ? cx : a == 1 ? cy : cz;
+            if (i < (a == 0 ? nx : a == 1 ? ny : nz)) pc[a][i] = c[i];
+            if (LINEAR) bm[a][i] = 0;
+        }
+        contour_reset(L);
         __syncthreads();
         if (LINEAR) {
-            if (threadIdx.x < 24) {
-                const uint32_t ax = threadIdx.x >> 3, c = threadIdx.x & 7u;
+            if (thread_id() < 24) {
+                const uint32_t ax = thread_id() >> 3, c = thread_id() & 7u;
                 atomicOr(&bm[ax][A.lo[ax][c]], 1u << (c >> 1));
                 atomicOr(&bm[ax][A.hi[ax][c]], 1u << (c >> 1));
+            }
+            __syncthreads();
+            // box row (iz, iy): the x-bricks of the live bricks whose cells use it
+            for (uint32_t q = thread_id(); q < ny * nz; q += blockDim.x) {
+                const uint32_t iz = div16(q, ny, magic16(ny)), iy = q - iz * ny;
+                uint32_t m = 0;
+                for (uint32_t mz = bm[2][iz]; mz; mz &= mz - 1u)
+                    for (uint32_t my = bm[1][iy]; my; my &= my - 1u)
+                        m |= (uint32_t)(lv >> ((__builtin_ctz(mz) * 4 + __builtin_ctz(my)) * 4)) & 0xfu;
+                rowm[q] = (uint8_t)m;
             }
             __syncthreads();
         }
         auto point = [&](uint32_t q, float& x, float& y, float& z) {
             const uint32_t iz = div16(q, nxy, mxy), r = q - iz * nxy;
             const uint32_t iy = div16(r, nx, mx), ix = r - iy * nx;
-            x = cx[ix];
-            y = cy[iy];
-            z = cz[iz];
+            x = pc[0][ix];
+            y = pc[1][iy];
+            z = pc[2][iz];
             return (iz * by + iy) * bx + ix;
         };
         uint32_t evaluated = npts;
         if (LINEAR) {
-            const uint64_t lv = P.slot_live[slot];
             evaluated = 0;
             for (uint32_t c0 = 0; c0 < npts; c0 += kChunk) {
-                if (threadIdx.x == 0) nlist = 0;
+                if (thread_id() == 0) nlist = 0;
                 __syncthreads();
                 // the chunk's points that some live brick uses, compacted into `list`
                 for (uint32_t k = 0; k < kChunk / 256u; ++k) {
-                    const uint32_t q = c0 + k * 256u + threadIdx.x;
+                    const uint32_t q = c0 + k * 256u + thread_id();
                     bool need = false;
                     if (q < npts) {
-                        const uint32_t iz = div16(q, nxy, mxy), r = q - iz * nxy;
-                        const uint32_t iy = div16(r, nx, mx), ix = r - iy * nx;
-                        const uint32_t mbx = bm[0][ix];
-                        for (uint32_t mz = bm[2][iz]; mz && !need; mz &= mz - 1u)
-                            for (uint32_t my = bm[1][iy]; my && !need; my &= my - 1u) {
-                                const uint32_t row = (uint32_t)(__builtin_ctz(mz) * 4 + __builtin_ctz(my)) * 4u;
-                                need = ((lv >> row) & mbx) != 0;
-                            }
+                        const uint32_t row = div16(q, nx, mx), ix = q - row * nx;  // row = iz * ny + iy
+                        need = (rowm[row] & bm[0][ix]) != 0;
                     }
                     const uint64_t w = __ballot(need);
                     uint32_t wbase = 0;
@@ -555,7 +689,7 @@ __global__ __launch_bounds__(256) void k_sp_tiles(SpPass P)
                 __syncthreads();
                 const uint32_t m = nlist;
                 evaluated += m;
-                for (uint32_t i0 = threadIdx.x; i0 < m; i0 += kLinearPoints * blockDim.x) {
+                for (uint32_t i0 = thread_id(); i0 < m; i0 += kLinearPoints * blockDim.x) {
                     float x[kLinearPoints], y[kLinearPoints], z[kLinearPoints], v[kLinearPoints];
                     uint32_t at[kLinearPoints];
 #pragma unroll
@@ -571,21 +705,21 @@ __global__ __launch_bounds__(256) void k_sp_tiles(SpPass P)
                 __syncthreads();  // list and nlist are reused by the next chunk
             }
         } else {
-            for (uint32_t q = threadIdx.x; q < npts; q += blockDim.x) {
+            for (uint32_t q = thread_id(); q < npts; q += blockDim.x) {
                 float x, y, z;
                 const uint32_t at = point(q, x, y, z);
-                box[at] = staged ? run_general(T, n, x, y, z) : tape_eval_list(J.ops, L, n, x, y, z);
+                box[at] = staged ? run_general(T, n, x, y, z) : tape_eval_list(J.ops, Lst, n, x, y, z);
             }
         }
         __syncthreads();  // the box is complete
-        contour_tile(P, J, slot, tx, ty, tzl, box, A, LINEAR ? P.slot_live[slot] : ~0ull, C);
-        if (threadIdx.x == 0) {
+        contour_tile(P, J, slot, tx, ty, tzl, box, A, lv, L);
+        if (thread_id() == 0) {
             tiles_done += 1;
             points_done += evaluated;
             flops_done += (uint64_t)evaluated * P.flops[tile];
         }
     }
-    if (threadIdx.x == 0 && tiles_done) {  // one update per block, not per tile
+    if (thread_id() == 0 && tiles_done) {  // one update per block, not per tile
         atomicAdd((unsigned long long*)&P.counters[kTilesActive], (unsigned long long)tiles_done);
         atomicAdd((unsigned long long*)&P.counters[kEvalPoints], (unsigned long long)points_done);
         atomicAdd((unsigned long long*)&P.counters[kEvalFlops], (unsigned long long)flops_done);
@@ -593,13 +727,105 @@ __global__ __launch_bounds__(256) void k_sp_tiles(SpPass P)
 }
 
 // ---------------------------------------------------------------------------------
-// k_sp_emit: the staged vertices of one slot per block iteration at their ordered index
-// (row-segment offset + rank), the cell -> vertex box, and the 4-tap normals of the
-// tile's vertices with its pruned program (sdf3d_normal.wgsl:4-9, dualcontour.wgsl:171).
+// Quads of a cell with a vertex (mesh.rs:267-324) from its sign bits s and its look-back
+// neighbours (exists(dx, dy, dz), dx, dy, dz in {-1, 0}; x, y, z: the cell in the job):
+// the mask of valid quads (bit 0: x-edge, 1: y-edge, 2: z-edge).  `invalid` counts the
+// quads with a missing vertex, and with the readback lag the z = 0 ones that the lagged
+// reference looks up (label z - 1) and misses.
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_sp_emit(SpPass P, MeshBufs b)
+template <typename E>
+__device__ __forceinline__ uint32_t cell_quads(uint32_t s, uint32_t x, uint32_t y, uint32_t z, bool lag,
+                                               const E& exists, uint32_t& invalid)
+{
+    const bool c0 = s & 1u, c1 = (s >> 1) & 1u, c2 = (s >> 2) & 1u, c3 = (s >> 3) & 1u;
+    uint32_t nq = 0;
+    if (c0 != c3 && y > 0) {  // x-edge: v(x,y-1,z-1), v(x,y,z-1), v, v(x,y-1,z)
+        if (z > 0) {
+            if (exists(0, -1, -1) && exists(0, 0, -1) && exists(0, -1, 0)) nq |= 1u;
+            else ++invalid;
+        } else if (lag) {
+            ++invalid;
+        }
+    }
+    if (c1 != c3 && x > 0) {  // y-edge: v(x-1,y,z-1), v(x,y,z-1), v, v(x-1,y,z)
+        if (z > 0) {
+            if (exists(-1, 0, -1) && exists(0, 0, -1) && exists(-1, 0, 0)) nq |= 2u;
+            else ++invalid;
+        } else if (lag) {
+            ++invalid;
+        }
+    }
+    if (c2 != c3 && x > 0 && y > 0) {  // z-edge: v(x-1,y-1,z), v(x,y-1,z), v, v(x-1,y,z)
+        if (exists(-1, -1, 0) && exists(0, -1, 0) && exists(-1, 0, 0)) nq |= 4u;
+        else ++invalid;
+    }
+    return nq;
+}
+
+// ---------------------------------------------------------------------------------
+// k_sp_quadcount: per slot (one 64-lane block iteration), the valid quads of the tile's
+// cells with a vertex (their staged entries), two triangles each, summed per row segment
+// into seg_t; the invalid ones are added to the pass's and the job's counts.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_sp_quadcount(SpPass P, MeshBufs b)
+{
+    __shared__ uint32_t ex[81];    // row (k, j): bit i = cell (8tx-1+i, 8ty-1+j, 8tz-1+k) has a vertex
+    __shared__ uint32_t rowt[64];  // triangles of each row segment of the tile
+    const uint32_t nslots = active_slots(P);
+    for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
+        const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
+        const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(P.job_of_slot[slot])];
+        const uint32_t base = __builtin_amdgcn_readfirstlane(P.slot_stage[2 * slot]);
+        const uint32_t n = __builtin_amdgcn_readfirstlane(P.slot_stage[2 * slot + 1]);
+        const bool lag = (J.flags & S2M_COMPAT_DROP_LAST_SLICE) != 0;
+        uint32_t tx, ty, tzl;
+        tile_xyz(J, tile, tx, ty, tzl);
+        __syncthreads();  // the previous slot's readers of ex / rowt are done
+        for (uint32_t t = thread_id(); t < 81u; t += blockDim.x)
+            ex[t] = nb_row<false>(P, J, tx, (int)(ty * 8u + t % 9u) - 1, (int)(J.z0 + tzl * 8u + t / 9u) - 1,
+                                  nullptr);
+        rowt[thread_id()] = 0;
+        __syncthreads();
+        uint32_t invalid = 0;
+        for (uint32_t k = thread_id(); k < n; k += blockDim.x) {
+            if ((uint64_t)base + k >= P.stage_cap) break;
+            const uint32_t meta = __float_as_uint(P.stage[(uint64_t)base + k].w);
+            const uint32_t c = meta & 511u, sg = meta >> 12;
+            const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
+            auto exists = [&](int dx, int dy, int dz) {
+                return ((ex[(czl + 1 + dz) * 9 + (cyl + 1 + dy)] >> (cxl + 1 + dx)) & 1u) != 0;
+            };
+            const uint32_t nq = cell_quads(sg, tx * 8u + cxl, ty * 8u + cyl, J.z0 + tzl * 8u + czl, lag, exists,
+                                           invalid);
+            if (nq) atomicAdd(&rowt[c >> 3], 2u * (uint32_t)__builtin_popcount(nq));
+        }
+        __syncthreads();
+        {
+            const uint32_t row = thread_id(), y = ty * 8u + (row & 7u), zl = tzl * 8u + (row >> 3);
+            if (y < J.ry && zl < J.nz) P.seg_t[seg_index(J, zl, y, tx)] = rowt[row];
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) invalid += __shfl_xor(invalid, d, 64);
+        if (thread_id() == 0 && invalid) {
+            atomicAdd((unsigned long long*)&b.counters[kInvalid], (unsigned long long)invalid);
+            atomicAdd((unsigned long long*)&P.job_out[(size_t)J.out_index * kJobOut + 2], (unsigned long long)invalid);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// k_sp_emit: per slot (one 64-lane block iteration), the staged vertices at their ordered
+// index (row-segment offset + rank) with key, sign bits and the 4-tap normal of the
+// tile's pruned program (sdf3d_normal.wgsl:4-9, dualcontour.wgsl:171), then the
+// triangles of their valid quads at the row segment's triangle offset plus the
+// triangles of the cells before them in the segment (make_triangles, mesh.rs:250-265).
+// Triangle vertex indices are relative to the job's first vertex.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
 {
     __shared__ StagedTape T;
+    __shared__ uint32_t nb[729];  // vertex of cell (8tx-1+i, 8ty-1+j, 8tz-1+k) at nb[(k*9+j)*9+i]
+    __shared__ uint32_t tc[128];  // triangles of cell c in byte c % 4 of tc[c / 4]
     const uint32_t nslots = active_slots(P);
     for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
         const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
@@ -609,28 +835,40 @@ __global__ __launch_bounds__(256) void k_sp_emit(SpPass P, MeshBufs b)
         uint32_t tx, ty, tzl;
         tile_xyz(J, tile, tx, ty, tzl);
         const bool normals = !(J.flags & S2M_NO_NORMALS);
-        const uint16_t* L = P.lists + (size_t)tile * P.list_stride;
+        const bool lag = (J.flags & S2M_COMPAT_DROP_LAST_SLICE) != 0;
+        const uint32_t vbase = (uint32_t)P.job_out[(size_t)J.out_index * kJobOut + 0];
+        const uint16_t* Lst = P.lists + (size_t)tile * P.list_stride;
         const uint32_t lens = __builtin_amdgcn_readfirstlane(P.lens[tile]);
         const uint32_t len = lens & ~kLinearFlag;
         const bool linear = (lens & kLinearFlag) != 0, staged = len <= kStageOps;
-        __syncthreads();  // the previous slot's readers of T are done
-        if (normals && staged) stage_tape(T, J.ops, L, len);
+        __syncthreads();  // the previous slot's readers of T, nb, tc are done
+        if (normals && staged) stage_tape(T, J.ops, Lst, len);
+        for (uint32_t t = thread_id(); t < 81u; t += blockDim.x)
+            (void)nb_row<true>(P, J, tx, (int)(ty * 8u + t % 9u) - 1, (int)(J.z0 + tzl * 8u + t / 9u) - 1,
+                               nb + 9u * t);
+        for (uint32_t w = thread_id(); w < 128u; w += blockDim.x) tc[w] = 0;
         __syncthreads();
-        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        // vertices, and each cell's triangle count
+        for (uint32_t k = thread_id(); k < n; k += blockDim.x) {
             if ((uint64_t)base + k >= P.stage_cap) break;
             const float4 e = P.stage[(uint64_t)base + k];
             const uint32_t meta = __float_as_uint(e.w);
-            const uint32_t c = meta & 511u, rank = (meta >> 9) & 7u, sg = meta >> 12;
-            const uint32_t x = tx * 8u + (c & 7u), y = ty * 8u + ((c >> 3) & 7u), zl = tzl * 8u + (c >> 6);
-            const uint32_t z = J.z0 + zl;
-            const uint64_t vi = (uint64_t)P.seg_off_v[seg_index(J, zl, y, tx)] + rank;
-            P.vbox[(size_t)slot * 512u + c] = (uint32_t)vi;
+            const uint32_t c = meta & 511u, sg = meta >> 12;
+            const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
+            const uint32_t at = ((czl + 1) * 9u + (cyl + 1)) * 9u + (cxl + 1);
+            auto exists = [&](int dx, int dy, int dz) { return nb[(int)at + (dz * 9 + dy) * 9 + dx] != kNone; };
+            uint32_t invalid = 0;
+            const uint32_t nq =
+                cell_quads(sg, tx * 8u + cxl, ty * 8u + cyl, J.z0 + tzl * 8u + czl, lag, exists, invalid);
+            if (nq) atomicOr(&tc[c >> 2], (2u * (uint32_t)__builtin_popcount(nq)) << ((c & 3u) * 8u));
+            const uint64_t vi = nb[at];
             if (vi >= b.vcap) continue;
             float* o = b.vtx + 6 * vi;
             o[0] = e.x;
             o[1] = e.y;
             o[2] = e.z;
-            b.keys[vi] = (uint64_t)x | ((uint64_t)y << 16) | ((uint64_t)z << 32);
+            b.keys[vi] = (uint64_t)(tx * 8u + cxl) | ((uint64_t)(ty * 8u + cyl) << 16) |
+                         ((uint64_t)(J.z0 + tzl * 8u + czl) << 32);
             b.signs[vi] = (uint8_t)sg;
             if (!normals) {
                 o[3] = o[4] = o[5] = 0.0f;
@@ -648,7 +886,7 @@ __global__ __launch_bounds__(256) void k_sp_emit(SpPass P, MeshBufs b)
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
                     d[t] = staged ? run_general(T, len, tx4[t], ty4[t], tz4[t])
-                                  : tape_eval_list(J.ops, L, len, tx4[t], ty4[t], tz4[t]);
+                                  : tape_eval_list(J.ops, Lst, len, tx4[t], ty4[t], tz4[t]);
             }
             const float d1 = d[0], d2 = d[1], d3 = d[2], d4 = d[3];
             const float gx = ((d1 + (-d2)) + (-d3)) + d4;
@@ -658,6 +896,51 @@ __global__ __launch_bounds__(256) void k_sp_emit(SpPass P, MeshBufs b)
             o[3] = gx / ln;
             o[4] = gy / ln;
             o[5] = gz / ln;
+        }
+        __syncthreads();  // every cell's triangle count is in tc
+        // triangles, in vertex order: each quad (q2, q1, q0), (q0, q3, q2)
+        for (uint32_t k = thread_id(); k < n; k += blockDim.x) {
+            if ((uint64_t)base + k >= P.stage_cap) break;
+            const uint32_t meta = __float_as_uint(P.stage[(uint64_t)base + k].w);
+            const uint32_t c = meta & 511u, sg = meta >> 12;
+            const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
+            const uint32_t at = ((czl + 1) * 9u + (cyl + 1)) * 9u + (cxl + 1);
+            auto nbv = [&](int dx, int dy, int dz) { return nb[(int)at + (dz * 9 + dy) * 9 + dx]; };
+            auto exists = [&](int dx, int dy, int dz) { return nbv(dx, dy, dz) != kNone; };
+            uint32_t invalid = 0;
+            const uint32_t y = ty * 8u + cyl, zl = tzl * 8u + czl;
+            const uint32_t nq = cell_quads(sg, tx * 8u + cxl, y, J.z0 + zl, lag, exists, invalid);
+            if (!nq) continue;
+            const uint64_t row2 = ((uint64_t)tc[(c >> 3) * 2u + 1] << 32) | tc[(c >> 3) * 2u];  // 8 counts
+            uint64_t t = P.seg_off_t[seg_index(J, zl, y, tx)];
+            for (uint32_t i = 0; i < cxl; ++i) t += (row2 >> (8u * i)) & 0xffu;
+            const uint32_t v = nbv(0, 0, 0);
+            const bool c0 = sg & 1u, c1 = (sg >> 1) & 1u, c2 = (sg >> 2) & 1u;
+            for (int q = 0; q < 3; ++q) {
+                if (!(nq & (1u << q))) continue;
+                uint32_t f[4];
+                bool rev;
+                if (q == 0) {
+                    f[0] = nbv(0, -1, -1), f[1] = nbv(0, 0, -1), f[3] = nbv(0, -1, 0), rev = c0;
+                } else if (q == 1) {
+                    f[0] = nbv(-1, 0, -1), f[1] = nbv(0, 0, -1), f[3] = nbv(-1, 0, 0), rev = !c1;
+                } else {
+                    f[0] = nbv(-1, -1, 0), f[1] = nbv(0, -1, 0), f[3] = nbv(-1, 0, 0), rev = c2;
+                }
+                f[2] = v;
+                uint32_t qv[4];
+                for (int i = 0; i < 4; ++i) qv[i] = (rev ? f[3 - i] : f[i]) - vbase;
+                if (t + 1 < b.tcap) {
+                    uint32_t* o = b.tri + 3 * t;
+                    o[0] = qv[2];
+                    o[1] = qv[1];
+                    o[2] = qv[0];
+                    o[3] = qv[0];
+                    o[4] = qv[3];
+                    o[5] = qv[2];
+                }
+                t += 2;
+            }
         }
     }
 }
@@ -725,208 +1008,114 @@ struct SegCursor {
     }
 };
 
-template <int WHICH>
+// Per block: the sums of kScanChunk consecutive segments' vertex and triangle counts
+// (bsum[block], bsum[nblocks + block]).
 __global__ __launch_bounds__(256) void k_sp_reduce(SpPass P, uint32_t* __restrict__ bsum)
 {
-    __shared__ uint32_t lds[4];
-    const uint32_t* in = WHICH ? P.seg_t : P.seg_v;
-    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
-    uint32_t s = 0;
+    __shared__ uint32_t lds[2][4];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + thread_id() * kScanPer;
+    uint32_t sv = 0, st = 0;
     if (base < P.nseg) {
         SegCursor c;
         c.load(P, base);
         for (uint32_t k = 0; k < kScanPer && base + k < P.nseg; ++k) {
-            if (c.active(P)) s += in[base + k];
+            if (c.active(P)) {
+                sv += P.seg_v[base + k];
+                st += P.seg_t[base + k];
+            }
             c.next(P, base + k + 1);
         }
     }
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) s += __shfl_xor(s, d, 64);
-    if (lane() == 0) lds[threadIdx.x >> 6] = s;
+    for (int d = 32; d >= 1; d >>= 1) {
+        sv += __shfl_xor(sv, d, 64);
+        st += __shfl_xor(st, d, 64);
+    }
+    if (lane() == 0) {
+        lds[0][thread_id() >> 6] = sv;
+        lds[1][thread_id() >> 6] = st;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) bsum[blockIdx.x] = lds[0] + lds[1] + lds[2] + lds[3];
+    if (thread_id() < 2)
+        bsum[thread_id() * gridDim.x + blockIdx.x] =
+            lds[thread_id()][0] + lds[thread_id()][1] + lds[thread_id()][2] + lds[thread_id()][3];
 }
 
-template <int WHICH>
+// Exclusive offsets of the segments of tiles with a slot (and of each job's first
+// segment): the block's offsets boff[block] (vertices), boff[nblocks + block]
+// (triangles) plus the block-local scan.
 __global__ __launch_bounds__(256) void k_sp_downsweep(SpPass P, const uint32_t* __restrict__ boff)
 {
-    __shared__ uint32_t lds[4];
-    const uint32_t* in = WHICH ? P.seg_t : P.seg_v;
-    uint32_t* out = WHICH ? P.seg_off_t : P.seg_off_v;
-    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + threadIdx.x * kScanPer;
-    uint32_t v[kScanPer] = {}, s = 0, act = 0, first = 0;
+    __shared__ uint32_t lds[2][4];
+    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + thread_id() * kScanPer;
+    uint32_t v[kScanPer], t[kScanPer], sv = 0, st = 0, act = 0, first = 0;
     if (base < P.nseg) {
         SegCursor c;
         c.load(P, base);
         for (uint32_t k = 0; k < kScanPer; ++k) {
-            v[k] = 0;
+            v[k] = t[k] = 0;
             if (base + k >= P.nseg) continue;
             if (c.active(P)) {
-                v[k] = in[base + k];
+                v[k] = P.seg_v[base + k];
+                t[k] = P.seg_t[base + k];
                 act |= 1u << k;
             }
             if (c.first()) first |= 1u << k;
-            s += v[k];
+            sv += v[k];
+            st += t[k];
             c.next(P, base + k + 1);
         }
     }
-    uint32_t inc = s;
+    uint32_t iv = sv, it = st;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t t = __shfl_up(inc, d, 64);
-        if (lane() >= (uint32_t)d) inc += t;
+        const uint32_t a = __shfl_up(iv, d, 64), b = __shfl_up(it, d, 64);
+        if (lane() >= (uint32_t)d) {
+            iv += a;
+            it += b;
+        }
     }
-    if (lane() == 63) lds[threadIdx.x >> 6] = inc;
+    if (lane() == 63) {
+        lds[0][thread_id() >> 6] = iv;
+        lds[1][thread_id() >> 6] = it;
+    }
     __syncthreads();
-    uint32_t run = boff[blockIdx.x] + inc - s;
-    for (uint32_t w = 0; w < (threadIdx.x >> 6); ++w) run += lds[w];
+    uint32_t rv = boff[blockIdx.x] + iv - sv, rt = boff[gridDim.x + blockIdx.x] + it - st;
+    for (uint32_t w = 0; w < (thread_id() >> 6); ++w) {
+        rv += lds[0][w];
+        rt += lds[1][w];
+    }
+    if (base >= P.nseg) return;
     for (uint32_t k = 0; k < kScanPer; ++k) {
-        if ((act | first) & (1u << k)) out[base + k] = run;
-        run += v[k];
+        if ((act | first) & (1u << k)) {
+            P.seg_off_v[base + k] = rv;
+            P.seg_off_t[base + k] = rt;
+        }
+        rv += v[k];
+        rt += t[k];
     }
 }
 
-// First vertex (WHICH = 0) or triangle (WHICH = 1) index of every job whose first slab
-// this pass holds: the exclusive offset of its first row segment.
-template <int WHICH>
+// First vertex and first triangle index of every job whose first slab this pass holds:
+// the exclusive offsets of its first row segment.
 __global__ void k_sp_ranges(SpPass P)
 {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t j = blockIdx.x * blockDim.x + thread_id();
     if (j >= P.njobs) return;
     const SpJob& J = P.jobs[j];
     if (J.z0 != 0) return;
-    const uint32_t* off = WHICH ? P.seg_off_t : P.seg_off_v;
-    const uint64_t* total = P.counters + (WHICH ? kTriPrev : kVtxPrev);
     // a job with no row segment (empty grid) starts where the pass's output starts
-    P.job_out[(size_t)J.out_index * kJobOut + (WHICH ? 1 : 0)] =
-        (J.nz * J.ry * J.ntx) ? (uint64_t)off[J.seg_base] : *total;
-}
-
-// ---------------------------------------------------------------------------------
-// Quads of one slot (mesh.rs:267-324): per cell with a vertex, the x-, y- and z-edge
-// quads; invalid ones (a missing vertex) are counted in the EMIT = 0 pass.  Triangle
-// vertex indices are relative to the job's first vertex.
-// ---------------------------------------------------------------------------------
-template <int EMIT>
-__global__ __launch_bounds__(256) void k_sp_quads(SpPass P, MeshBufs b)
-{
-    // vertex of cell (x0 - 1 + i, y0 - 1 + j, z0 - 1 + k) at nb[(k * 9 + j) * 9 + i]: the
-    // tile and the look-back layer the quads reach, gathered once per tile
-    __shared__ uint32_t nb[9 * 9 * 9];
-    __shared__ uint8_t sgn[512];  // sign bits of the tile's cells with a vertex
-    const uint32_t nslots = active_slots(P);
-    for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
-        const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
-        const uint32_t jid = __builtin_amdgcn_readfirstlane(P.job_of_slot[slot]);
-        const SpJob& J = P.jobs[jid];
-        const bool lag = (J.flags & S2M_COMPAT_DROP_LAST_SLICE) != 0;
-        const uint32_t vbase = EMIT ? (uint32_t)P.job_out[(size_t)J.out_index * kJobOut + 0] : 0u;
-        uint32_t tx, ty, tzl;
-        tile_xyz(J, tile, tx, ty, tzl);
-        const uint32_t* own = P.vbox + (size_t)slot * 512u;
-        __syncthreads();  // the previous slot's readers of nb / sgn are done
-        for (uint32_t q = threadIdx.x; q < 729u; q += blockDim.x) {
-            const uint32_t i = q % 9u, j = (q / 9u) % 9u, k = q / 81u;
-            uint32_t v;
-            if (i && j && k) {
-                v = own[((k - 1) * 8u + (j - 1)) * 8u + (i - 1)];
-            } else {
-                const int x = (int)(tx * 8u + i) - 1, y = (int)(ty * 8u + j) - 1;
-                const int z = (int)(J.z0 + tzl * 8u + k) - 1;
-                // cells outside the grid have no vertex (and no entry in the boundary map)
-                v = (x < 0 || y < 0 || z < 0 || (uint32_t)x >= J.rx || (uint32_t)y >= J.ry)
-                        ? kNone
-                        : vertex_at(P, J, (uint32_t)x, (uint32_t)y, (uint32_t)z);
-            }
-            nb[q] = v;
-        }
-        for (uint32_t c = threadIdx.x; c < 512u; c += blockDim.x) {
-            const uint32_t v = own[c];
-            sgn[c] = (v != kNone && v < b.vcap) ? b.signs[v] : (uint8_t)0;
-        }
-        __syncthreads();
-        uint32_t invalid = 0;
-        for (uint32_t r = 0; r < 2; ++r) {
-            const uint32_t c = r * 256u + threadIdx.x;
-            const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
-            const uint32_t x = tx * 8u + cxl, y = ty * 8u + cyl, zl = tzl * 8u + czl;
-            const uint32_t z = J.z0 + zl;
-            const uint32_t at = ((czl + 1) * 9u + (cyl + 1)) * 9u + (cxl + 1);
-            auto nbv = [&](int dx, int dy, int dz) { return nb[(int)at + (dz * 9 + dy) * 9 + dx]; };
-            const uint32_t v = nb[at];
-            uint32_t q[3][4];
-            uint32_t nq = 0;
-            if (v != kNone) {  // implies the cell is inside the grid
-                const uint32_t s = sgn[c];
-                const bool c0 = s & 1, c1 = (s >> 1) & 1, c2 = (s >> 2) & 1, c3 = (s >> 3) & 1;
-                if (c0 != c3 && y > 0) {  // x-edge: Quad(v(x,y-1,z-1), v(x,y,z-1), v, v(x,y-1,z)).swap(c.0)
-                    if (z > 0) {
-                        const uint32_t t[4] = {nbv(0, -1, -1), nbv(0, 0, -1), v, nbv(0, -1, 0)};
-                        for (int k = 0; k < 4; ++k) q[0][k] = c0 ? t[3 - k] : t[k];
-                        if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 1;
-                        else ++invalid;
-                    } else if (lag) {
-                        ++invalid;  // the lagged reference looks up label z-1 and misses
-                    }
-                }
-                if (c1 != c3 && x > 0) {  // y-edge: Quad(v(x-1,y,z-1), v(x,y,z-1), v, v(x-1,y,z)).swap(!c.1)
-                    if (z > 0) {
-                        const uint32_t t[4] = {nbv(-1, 0, -1), nbv(0, 0, -1), v, nbv(-1, 0, 0)};
-                        for (int k = 0; k < 4; ++k) q[1][k] = !c1 ? t[3 - k] : t[k];
-                        if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 2;
-                        else ++invalid;
-                    } else if (lag) {
-                        ++invalid;
-                    }
-                }
-                if (c2 != c3 && x > 0 && y > 0) {  // z-edge: Quad(v(x-1,y-1,z), v(x,y-1,z), v, v(x-1,y,z)).swap(c.2)
-                    const uint32_t t[4] = {nbv(-1, -1, 0), nbv(0, -1, 0), v, nbv(-1, 0, 0)};
-                    for (int k = 0; k < 4; ++k) q[2][k] = c2 ? t[3 - k] : t[k];
-                    if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 4;
-                    else ++invalid;
-                }
-            }
-            const uint32_t ntri = 2u * (uint32_t)__builtin_popcount(nq);
-            uint32_t inc = ntri;  // inclusive sum over the 8 lanes of the row segment
-#pragma unroll
-            for (int d = 1; d < 8; d <<= 1) {
-                const uint32_t t = __shfl_up(inc, d, 8);
-                if ((lane() & 7u) >= (uint32_t)d) inc += t;
-            }
-            const uint32_t seg_total = __shfl(inc, 7, 8);
-            const bool row_ok = y < J.ry && zl < J.nz;
-            if (!EMIT) {
-                if (cxl == 0 && row_ok) P.seg_t[seg_index(J, zl, y, tx)] = seg_total;
-            } else if (nq) {
-                uint64_t t = (uint64_t)P.seg_off_t[seg_index(J, zl, y, tx)] + (inc - ntri);
-                for (int e = 0; e < 3; ++e) {
-                    if (!(nq & (1u << e))) continue;
-                    if (t + 1 < b.tcap) {  // make_triangles: (q2, q1, q0), (q0, q3, q2)
-                        uint32_t* o = b.tri + 3 * t;
-                        o[0] = q[e][2] - vbase;
-                        o[1] = q[e][1] - vbase;
-                        o[2] = q[e][0] - vbase;
-                        o[3] = q[e][0] - vbase;
-                        o[4] = q[e][3] - vbase;
-                        o[5] = q[e][2] - vbase;
-                    }
-                    t += 2;
-                }
-            }
-        }
-        if (!EMIT && invalid) {
-            atomicAdd((unsigned long long*)&b.counters[kInvalid], (unsigned long long)invalid);
-            atomicAdd((unsigned long long*)&P.job_out[(size_t)J.out_index * kJobOut + 2],
-                      (unsigned long long)invalid);
-        }
-    }
+    const bool empty = (uint64_t)J.nz * J.ry * J.ntx == 0;
+    uint64_t* o = P.job_out + (size_t)J.out_index * kJobOut;
+    o[0] = empty ? P.counters[kVtxPrev] : (uint64_t)P.seg_off_v[J.seg_base];
+    o[1] = empty ? P.counters[kTriPrev] : (uint64_t)P.seg_off_t[J.seg_base];
 }
 
 // Cell -> vertex map of the slab's last plane, for the next slab (single job).
 __global__ __launch_bounds__(256) void k_sp_boundary(SpPass P)
 {
     const SpJob& J = P.jobs[0];
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + thread_id();
     if (i >= J.rx * J.ry) return;
     J.boundary[i] = vertex_at(P, J, i % J.rx, i / J.rx, J.z0 + J.nz - 1);
 }
@@ -935,7 +1124,7 @@ __global__ __launch_bounds__(256) void k_sp_boundary(SpPass P)
 // capacity check (a kernel rather than a memset, so a pass can be graph-captured).
 __global__ void k_sp_pass_begin(uint64_t* counters)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+    if (thread_id() == 0 && blockIdx.x == 0) {
         const uint64_t used = counters[kSlotsUsed];
         if (used > counters[kSlotsMax]) counters[kSlotsMax] = used;
         counters[kSlotsUsed] = 0;
@@ -950,45 +1139,59 @@ hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters)
     return hipGetLastError();
 }
 
-hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b, int persistent_blocks,
-                              int boundary_plane, hipEvent_t* marks)
+// Blocks of a persistent launch: as many as the device holds at once (at most `cap`),
+// so no block starts only when others end.
+template <typename K>
+static uint32_t resident_blocks(K kernel, int threads, size_t smem, int cus, uint32_t cap)
+{
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), threads,
+                                                     smem) != hipSuccess ||
+        per_cu < 1)
+        per_cu = 1;
+    return std::min<uint32_t>(cap, (uint32_t)cus * (uint32_t)per_cu);
+}
+
+hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b, int cus,
+                              uint32_t tile_blocks, int boundary_plane, hipEvent_t* marks)
 {
     if (P.ntiles == 0) return hipSuccess;
     const uint64_t nsb64 = (P.nseg + kScanChunk - 1) / kScanChunk;
     if (nsb64 > kMaxScanBlocks) return hipErrorInvalidValue;
     const uint32_t nsb = (uint32_t)nsb64;
-    const uint32_t pb = (uint32_t)persistent_blocks;
     const size_t smem = (size_t)P.box_stride * sizeof(float);
     const uint32_t jb = (P.njobs + 63) / 64;
+    const uint32_t big = 1u << 20;
     auto mark = [&](int i) {
         if (marks && marks[i]) (void)hipEventRecord(marks[i], s);
     };
+    const size_t prune_smem = (size_t)P.prune_depth * 64u * (2 * sizeof(double) + sizeof(uint16_t));
     mark(0);
     if (P.nsuper)
-        hipLaunchKernelGGL(k_sp_prune_super, dim3((uint32_t)((P.nsuper + 63) / 64)), dim3(64), 0, s, P);
-    hipLaunchKernelGGL(k_sp_prune, dim3((uint32_t)((P.ntiles + 63) / 64)), dim3(64), 0, s, P);
+        hipLaunchKernelGGL(k_sp_prune_super, dim3((uint32_t)((P.nsuper + 63) / 64)), dim3(64), prune_smem, s, P);
+    hipLaunchKernelGGL(k_sp_prune, dim3((uint32_t)((P.ntiles + 63) / 64)), dim3(64), prune_smem, s, P);
     mark(1);
-    hipLaunchKernelGGL(k_sp_bricks, dim3(pb), dim3(64), 0, s, P);
-    hipLaunchKernelGGL((k_sp_tiles<1>), dim3(pb), dim3(256), smem, s, P);
-    hipLaunchKernelGGL((k_sp_tiles<0>), dim3(pb), dim3(256), smem, s, P);
+    hipLaunchKernelGGL(k_sp_bricks, dim3(resident_blocks(k_sp_bricks, 64, 0, cus, big)), dim3(64), 0, s, P);
+    hipLaunchKernelGGL((k_sp_tiles<1>), dim3(resident_blocks(k_sp_tiles<1>, 256, smem, cus, tile_blocks)),
+                       dim3(256), smem, s, P);
+    hipLaunchKernelGGL((k_sp_tiles<0>), dim3(resident_blocks(k_sp_tiles<0>, 256, smem, cus, tile_blocks)),
+                       dim3(256), smem, s, P);
     mark(2);
-    if (nsb) {
-        hipLaunchKernelGGL((k_sp_reduce<0>), dim3(nsb), dim3(256), 0, s, P, b.bcount);
-        (void)launch_scan(s, b, nsb, 0);
-        hipLaunchKernelGGL((k_sp_downsweep<0>), dim3(nsb), dim3(256), 0, s, P, b.boffset);
-    }
-    hipLaunchKernelGGL((k_sp_ranges<0>), dim3(jb), dim3(64), 0, s, P);
+    hipLaunchKernelGGL(k_sp_quadcount, dim3(resident_blocks(k_sp_quadcount, 64, 0, cus, big)), dim3(64), 0, s,
+                       P, b);
     mark(3);
-    hipLaunchKernelGGL(k_sp_emit, dim3(pb), dim3(256), 0, s, P, b);
-    mark(4);
-    hipLaunchKernelGGL((k_sp_quads<0>), dim3(pb), dim3(256), 0, s, P, b);
     if (nsb) {
-        hipLaunchKernelGGL((k_sp_reduce<1>), dim3(nsb), dim3(256), 0, s, P, b.bcount);
-        (void)launch_scan(s, b, nsb, 1);
-        hipLaunchKernelGGL((k_sp_downsweep<1>), dim3(nsb), dim3(256), 0, s, P, b.boffset);
+        hipLaunchKernelGGL(k_sp_reduce, dim3(nsb), dim3(256), 0, s, P, b.bcount);
+        MeshBufs bt = b;  // triangle block sums and offsets follow the vertex ones
+        bt.bcount += nsb;
+        bt.boffset += nsb;
+        (void)launch_scan(s, b, nsb, 0);
+        (void)launch_scan(s, bt, nsb, 1);
+        hipLaunchKernelGGL(k_sp_downsweep, dim3(nsb), dim3(256), 0, s, P, b.boffset);
     }
-    hipLaunchKernelGGL((k_sp_ranges<1>), dim3(jb), dim3(64), 0, s, P);
-    hipLaunchKernelGGL((k_sp_quads<1>), dim3(pb), dim3(256), 0, s, P, b);
+    hipLaunchKernelGGL(k_sp_ranges, dim3(jb), dim3(64), 0, s, P);
+    mark(4);
+    hipLaunchKernelGGL(k_sp_emit, dim3(resident_blocks(k_sp_emit, 64, 0, cus, big)), dim3(64), 0, s, P, b);
     if (boundary_plane) {
         hipLaunchKernelGGL(k_sp_boundary, dim3((uint32_t)((P.boundary_cells + 255) / 256)), dim3(256), 0,
                            s, P);
