@@ -99,7 +99,7 @@ struct s2m_ctx {
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_seg, sp_boundary;
     DevBuf sp_stage, sp_slot_stage, sp_slot_live;
-    DevBuf sp_job_out, sp_super_lists, sp_super_lens;
+    DevBuf sp_job_out, sp_queues, sp_super_iv, sp_super_lists, sp_super_lens;
     uint64_t slot_cap = 0;
     int cus = 256;
     std::vector<uint8_t> host_img;  // staging of the last pass's tables (kept until its sync)
@@ -407,12 +407,12 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         tab_off[i] = tab_bytes;
         tab_bytes += (tabs[i].bytes.size() + 255) & ~size_t(255);
         list_stride = std::max(list_stride, progs[i]->n);
+        box_stride = std::max(box_stride, tabs[i].bmax[0] * tabs[i].bmax[1] * tabs[i].bmax[2]);
         uint32_t depth = 0;  // operand stack entries below the top (k_sp_prune's LDS stack)
         for (const s2m_op& o : progs[i]->ops) {
             if (o.code < S2M_OP_UNION) prune_depth = std::max(prune_depth, depth++);
             else --depth;
         }
-        box_stride = std::max(box_stride, tabs[i].bmax[0] * tabs[i].bmax[1] * tabs[i].bmax[2]);
         ntiles_all += (uint64_t)tabs[i].nt[0] * tabs[i].nt[1] * tabs[i].nt[2];
         nseg_all += (uint64_t)jobs[i].nz_cells * jobs[i].res[1] * tabs[i].nt[0];
         plane_max = std::max<uint64_t>(plane_max, (uint64_t)jobs[i].res[0] * jobs[i].res[1]);
@@ -519,8 +519,6 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     const SpJob* ddesc = reinterpret_cast<const SpJob*>(dimg + desc_off);
 
     HIP_TRY(ctx->tile_lists.ensure(tiles_pass * list_stride * sizeof(uint16_t)));
-    HIP_TRY(ctx->sp_super_lists.ensure(std::max<uint64_t>(nsuper_pass, 1) * list_stride * sizeof(uint16_t)));
-    HIP_TRY(ctx->sp_super_lens.ensure(std::max<uint64_t>(nsuper_pass, 1) * sizeof(uint32_t)));
     HIP_TRY(ctx->tile_lens.ensure(tiles_pass * sizeof(uint32_t)));
     HIP_TRY(ctx->tile_flops.ensure(tiles_pass * sizeof(uint32_t)));
     HIP_TRY(ctx->sp_slot_of_tile.ensure(tiles_pass * sizeof(int32_t)));
@@ -530,13 +528,19 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     HIP_TRY(ctx->boffset.ensure((size_t)2 * nsb_max * sizeof(uint32_t)));
     HIP_TRY(ctx->counters.ensure(kNumCounters * sizeof(uint64_t)));
     HIP_TRY(ctx->sp_job_out.ensure((size_t)n * kJobOut * sizeof(uint64_t)));
+    HIP_TRY(ctx->sp_queues.ensure((size_t)kQueues * kQueueGroups * 32 * sizeof(uint32_t)));
+    const uint64_t nsup_alloc = std::max<uint64_t>(nsuper_pass, 1);
+    HIP_TRY(ctx->sp_super_iv.ensure(nsup_alloc * list_stride * 2 * sizeof(double)));
+    HIP_TRY(ctx->sp_super_lists.ensure(nsup_alloc * list_stride * sizeof(uint16_t)));
+    HIP_TRY(ctx->sp_super_lens.ensure(nsup_alloc * sizeof(uint32_t)));
 
     // a closed surface crosses O(res^2) cells: start from 6 per row-plane cell
     uint64_t want_v = std::min<uint64_t>(cells_all, std::max<uint64_t>(65536, 6 * planes_sum));
     want_v = std::max<uint64_t>(want_v, ctx->vcap);
     uint64_t want_t = std::max<uint64_t>(3 * want_v, ctx->tcap);
     uint64_t want_s = std::min<uint64_t>(tiles_pass, std::max<uint64_t>({4096, tiles_pass / 4, ctx->slot_cap}));
-    const int pblocks = std::max(64, ctx->cus * 8);
+    const uint64_t tile_blocks = (uint64_t)std::max(1, ctx->cus) * kTileBlocksPerCU;
+    uint64_t stage_cap = 0;
 
     for (int attempt = 0; attempt < 4; ++attempt) {
         HIP_TRY(ctx->vtx.ensure(want_v * 6 * sizeof(float)));
@@ -546,9 +550,9 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         HIP_TRY(ctx->sp_tile_of_slot.ensure(want_s * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_job_of_slot.ensure(want_s * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_cellbits.ensure(want_s * 16 * sizeof(uint32_t)));
-        // staged cells are allocated in per-block chunks: at most 2 x vertices + blocks x chunk
-        // (two launches of pblocks blocks each)
-        const uint64_t stage_cap = 2 * want_v + 2ull * pblocks * kStageChunkHost;
+        // staged cells are allocated in per-block chunks: about 2 x vertices + blocks x chunk
+        // (two launches); the device count is checked below
+        stage_cap = std::max(stage_cap, 2 * want_v + 2 * tile_blocks * kStageChunkHost);
         HIP_TRY(ctx->sp_stage.ensure(stage_cap * sizeof(float4)));
         HIP_TRY(ctx->sp_slot_stage.ensure(want_s * 2 * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_slot_live.ensure(want_s * sizeof(uint64_t)));
@@ -601,8 +605,6 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             } else {
                 P.nsuper = nsuper_pass;
             }
-            P.super_lists = ctx->sp_super_lists.as<uint16_t>();
-            P.super_lens = ctx->sp_super_lens.as<uint32_t>();
             P.lists = ctx->tile_lists.as<uint16_t>();
             P.lens = ctx->tile_lens.as<uint32_t>();
             P.flops = ctx->tile_flops.as<uint32_t>();
@@ -613,7 +615,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.box_stride = box_stride;
             P.cellbits = ctx->sp_cellbits.as<uint32_t>();
             P.stage = ctx->sp_stage.as<float4>();
-            P.stage_cap = 2 * want_v + 2ull * pblocks * kStageChunkHost;
+            P.stage_cap = stage_cap;
             P.slot_stage = ctx->sp_slot_stage.as<uint32_t>();
             P.slot_live = ctx->sp_slot_live.as<uint64_t>();
             P.seg_v = ctx->sp_seg.as<uint32_t>();
@@ -621,18 +623,22 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.seg_off_v = P.seg_t + P.nseg;
             P.seg_off_t = P.seg_off_v + P.nseg;
             P.counters = b.counters;
+            P.queues = ctx->sp_queues.as<uint32_t>();
+            P.super_iv = ctx->sp_super_iv.as<double2>();
+            P.super_lists = ctx->sp_super_lists.as<uint16_t>();
+            P.super_lens = ctx->sp_super_lens.as<uint32_t>();
             P.job_out = ctx->sp_job_out.as<uint64_t>();
             hipEvent_t m[kSparseMarks] = {};
             if (ctx->profiling)
                 for (auto& e : m) e = ctx->next_event();
-            HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters));
+            HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters, ctx->sp_queues.as<uint32_t>()));
             const int boundary = (n == 1 && pass + 1 < npass) ? 1 : 0;
-            HIP_TRY(launch_sparse_pass(ctx->stream, P, b, ctx->cus, (uint32_t)pblocks, boundary,
+            HIP_TRY(launch_sparse_pass(ctx->stream, P, b, ctx->cus, (uint32_t)tile_blocks, boundary,
                                          ctx->profiling ? m : nullptr));
             if (ctx->profiling) ev.insert(ev.end(), m, m + kSparseMarks);
             tiles_total += P.ntiles;
         }
-        HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters));  // fold the last pass
+        HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters, ctx->sp_queues.as<uint32_t>()));  // fold the last pass
         if (ctx->profiling) {
             ev_end = ctx->next_event();
             HIP_TRY(hipEventRecord(ev_end, ctx->stream));
@@ -669,7 +675,8 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
                         (unsigned long long)cnt[kVtxTotal], (unsigned long long)cnt[kTriTotal]);
         const bool ok_v = cnt[kVtxTotal] <= want_v, ok_t = cnt[kTriTotal] <= want_t;
         const bool ok_s = cnt[kSlotsMax] <= want_s;
-        if (ok_v && ok_t && ok_s) {
+        const bool ok_stage = cnt[kStageMax] <= stage_cap;
+        if (ok_v && ok_t && ok_s && ok_stage) {
             for (uint32_t i = 0; i < n; ++i) {
                 SpRange& r = out[i];
                 r.v0 = jo[(size_t)i * kJobOut + 0];
@@ -686,6 +693,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             return S2M_OK;
         }
         if (!ok_s) want_s = std::min<uint64_t>(tiles_pass, cnt[kSlotsMax] + cnt[kSlotsMax] / 8 + 64);
+        if (!ok_stage) stage_cap = cnt[kStageMax] + cnt[kStageMax] / 8 + tile_blocks * kStageChunkHost;
         if (!ok_v) want_v = cnt[kVtxTotal] + cnt[kVtxTotal] / 8 + 1024;
         if (!ok_t) want_t = cnt[kTriTotal] + cnt[kTriTotal] / 8 + 2048;
     }
@@ -1017,7 +1025,8 @@ void s2m_destroy(s2m_ctx* ctx)
                       &ctx->tile_tab, &ctx->tile_lists, &ctx->tile_lens,
                       &ctx->tile_flops, &ctx->sp_slot_of_tile, &ctx->sp_tile_of_slot,
                       &ctx->sp_job_of_slot, &ctx->sp_cellbits, &ctx->sp_seg,
-                      &ctx->sp_boundary, &ctx->sp_job_out, &ctx->sp_super_lists, &ctx->sp_super_lens,
+                      &ctx->sp_boundary, &ctx->sp_job_out, &ctx->sp_queues, &ctx->sp_super_iv,
+                      &ctx->sp_super_lists, &ctx->sp_super_lens,
                       &ctx->sp_stage, &ctx->sp_slot_stage, &ctx->sp_slot_live})
         b->release();
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);

@@ -1,5 +1,4 @@
-// device_staged.h - a tile's pruned program staged in LDS and run with wave-uniform
-// dispatch.
+// device_staged.h - a tile's pruned program, staged in LDS, run with wave-uniform dispatch.
 //
 // Every lane of a block evaluates the same pruned program (one tile per block), so the
 // opcode is read once per op for the whole wave (readfirstlane) and the dispatch is a
@@ -16,18 +15,26 @@ namespace s2m {
 constexpr uint32_t kStageOps = 128;            // longest program staged in LDS
 constexpr uint32_t kLinearFlag = 0x80000000u;  // SpPass::lens: the program is a left fold
 
-struct StagedTape {
-    uint32_t code[kStageOps];
-    uint32_t iarg[kStageOps];
-    float f[kStageOps][8];  // primitives use f[0..6], operators f[0]
-};
+template <uint32_t CAP>
+struct StagedTapeN {
+    uint32_t code[CAP];
+    uint32_t iarg[CAP];
+    float f[CAP][8];  // primitives use f[0..6], operators f[0]
 
-// Copy ops[L[k]], k < n <= kStageOps, into T (all threads of the block take part; the
-// caller synchronises before and after).
-__device__ __forceinline__ void stage_tape(StagedTape& T, const s2m_op* __restrict__ ops,
-                                           const uint16_t* __restrict__ L, uint32_t n)
+    __device__ __forceinline__ uint32_t code_at(uint32_t k) const { return __builtin_amdgcn_readfirstlane(code[k]); }
+    __device__ __forceinline__ uint32_t iarg_at(uint32_t k) const { return __builtin_amdgcn_readfirstlane(iarg[k]); }
+    __device__ __forceinline__ const float* f_at(uint32_t k) const { return f[k]; }
+};
+using StagedTape = StagedTapeN<kStageOps>;
+
+// Copy ops[L[k]], k < n <= CAP, into T: worker t of nt copies entries t, t + nt, ... (by
+// default the threads of the block; the caller synchronises before and after).
+template <uint32_t CAP>
+__device__ __forceinline__ void stage_tape(StagedTapeN<CAP>& T, const s2m_op* __restrict__ ops,
+                                           const uint16_t* __restrict__ L, uint32_t n,
+                                           uint32_t t = threadIdx.x, uint32_t nt = blockDim.x)
 {
-    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+    for (uint32_t k = t; k < n; k += nt) {
         const s2m_op& o = ops[L[k]];
         T.code[k] = o.code;
         T.iarg[k] = o.iarg;
@@ -36,19 +43,20 @@ __device__ __forceinline__ void stage_tape(StagedTape& T, const s2m_op* __restri
     }
 }
 
-__device__ __forceinline__ uint32_t uniform_code(const StagedTape& T, uint32_t k)
+template <class TT>
+__device__ __forceinline__ uint32_t uniform_code(const TT& T, uint32_t k)
 {
-    return __builtin_amdgcn_readfirstlane(T.code[k]);
+    return T.code_at(k);
 }
 
-// Primitive k of T at NP points (code is wave-uniform).  MANDEL = false leaves the
-// Mandelbulb out of the code (left folds never contain one).
-template <int NP, bool MANDEL>
-__device__ __forceinline__ void staged_prim(const StagedTape& T, uint32_t k, uint32_t code,
-                                            const float (&x)[NP], const float (&y)[NP],
-                                            const float (&z)[NP], float (&d)[NP])
+// Primitive `code` with operands f (iarg: Mandelbulb only) at NP points (code is
+// wave-uniform).  MANDEL = false leaves the Mandelbulb out of the code (left folds never
+// contain one).
+template <int NP, bool MANDEL, class TT>
+__device__ __forceinline__ void staged_prim(const TT& T, uint32_t k, uint32_t code, const float (&x)[NP],
+                                            const float (&y)[NP], const float (&z)[NP], float (&d)[NP])
 {
-    const float* f = T.f[k];
+    const float* f = T.f_at(k);
     switch (code) {
     case S2M_OP_SPHERE: {
         const float c0 = f[0], c1 = f[1], c2 = f[2], s = f[3];
@@ -83,7 +91,7 @@ __device__ __forceinline__ void staged_prim(const StagedTape& T, uint32_t k, uin
     default:
         if (MANDEL) {
             const float pw = f[0], bo = f[1], off = f[2];
-            const uint32_t ia = __builtin_amdgcn_readfirstlane(T.iarg[k]);
+            const uint32_t ia = T.iarg_at(k);
 #pragma unroll
             for (int p = 0; p < NP; ++p) d[p] = sd_mandelbulb(x[p], y[p], z[p], pw, bo, off, ia);
         }
@@ -124,20 +132,21 @@ __device__ __forceinline__ void staged_op(uint32_t code, float k, float (&acc)[N
 }
 
 // A left-fold program (kLinearFlag): acc = P0; acc = op_i(acc, P_i) for i = 1, 2, ...
-template <int NP>
-__device__ __forceinline__ void run_linear(const StagedTape& T, uint32_t n, const float (&x)[NP],
-                                           const float (&y)[NP], const float (&z)[NP], float (&acc)[NP])
+template <int NP, class TT>
+__device__ __forceinline__ void run_linear(const TT& T, uint32_t n, const float (&x)[NP], const float (&y)[NP],
+                                           const float (&z)[NP], float (&acc)[NP])
 {
     staged_prim<NP, false>(T, 0, uniform_code(T, 0), x, y, z, acc);
     for (uint32_t k = 1; k + 1 < n; k += 2) {
         float d[NP];
         staged_prim<NP, false>(T, k, uniform_code(T, k), x, y, z, d);
-        staged_op<NP>(uniform_code(T, k + 1), T.f[k + 1][0], acc, d);
+        staged_op<NP>(uniform_code(T, k + 1), T.f_at(k + 1)[0], acc, d);
     }
 }
 
 // Any valid program: the postfix stack machine of tape_eval with a wave-uniform depth.
-__device__ __forceinline__ float run_general(const StagedTape& T, uint32_t n, float px, float py, float pz)
+template <class TT>
+__device__ __forceinline__ float run_general(const TT& T, uint32_t n, float px, float py, float pz)
 {
     float st[S2M_MAX_STACK];
     float acc = 0.0f;
@@ -153,7 +162,7 @@ __device__ __forceinline__ float run_general(const StagedTape& T, uint32_t n, fl
         } else {
             float a[1] = {st[--sp]};
             const float d[1] = {acc};
-            staged_op<1>(code, T.f[k][0], a, d);
+            staged_op<1>(code, T.f_at(k)[0], a, d);
             acc = a[0];
         }
     }

@@ -35,6 +35,8 @@ enum Counter : uint32_t {
     kSlotsUsed = 8,    // active tiles of the current slab (sparse path)
     kSlotsMax = 9,     // largest kSlotsUsed of the job so far
     kStageUsed = 10,   // staged active cells of the current pass (sparse path)
+    kLinearSlots = 11, // slots of the current pass whose program is a left fold
+    kStageMax = 12,    // largest kStageUsed of the job so far
     kNumCounters = 16
 };
 
@@ -66,6 +68,9 @@ constexpr uint32_t kTile = 8;  // cells per tile edge (sparse pipeline)
 constexpr uint32_t kSuper = 4;  // tiles per super-tile edge (first pruning level)
 constexpr uint32_t kSingleSign = 0xffffffffu;  // super_lens: the SDF keeps one sign there
 constexpr uint32_t kStageChunkHost = 1024;  // staged-cell allocation chunk (sparse.hip)
+constexpr uint32_t kTileBlocksPerCU = 8;    // most k_sp_tiles blocks per compute unit
+constexpr uint32_t kQueues = 2;        // work queues of a pass (k_sp_tiles<1>, k_sp_tiles<0>)
+constexpr uint32_t kQueueGroups = 64;  // counters per queue, 32 u32 (128 bytes) apart
 // Profiling marks of one sparse pass: prune | eval+contour | quad count | scan | emission
 constexpr int kSparseMarks = 6;
 
@@ -112,6 +117,7 @@ struct SpPass {
     uint64_t nseg;             // row segments of the pass
     uint64_t boundary_cells;   // rx * ry of jobs[0] when a boundary plane is written
     uint64_t nsuper;           // super-tiles of the pass
+    double2* super_iv;         // [super-tile][list_stride] primitive intervals (lo, hi)
     uint16_t* super_lists;     // [super-tile][list_stride] pruned op indices
     uint32_t* super_lens;      // [super-tile] list length or kSingleSign
     uint16_t* lists;           // [tile][list_stride] pruned op indices
@@ -133,6 +139,7 @@ struct SpPass {
     uint32_t* seg_off_v;       // exclusive offsets: global vertex index
     uint32_t* seg_off_t;       // exclusive offsets: global triangle index
     uint64_t* counters;
+    uint32_t* queues;          // [kQueues][kQueueGroups][32] work queue counters (word 0)
     uint64_t* job_out;         // [job][kJobOut]
 };
 
@@ -157,9 +164,9 @@ hipError_t launch_scan(hipStream_t s, const MeshBufs& b, uint32_t nblocks, uint3
 hipError_t launch_normals(hipStream_t s, const s2m_op* ops, uint32_t nops, float eps,
                           const MeshBufs& b);
 hipError_t launch_quads(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int emit);
-hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters);
-// cus: compute units of the device; tile_blocks: most blocks of a k_sp_tiles launch (the
-// staging bound).
+hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t* queues);
+// cus: compute units of the device; tile_blocks: most blocks of a k_sp_tiles launch (each
+// holds a staging chunk).
 hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b, int cus,
                               uint32_t tile_blocks, int boundary_plane, hipEvent_t* marks);
 // Contour corner values given per cell (test hook): cells of the slab read their 8

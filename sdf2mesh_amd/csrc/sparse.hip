@@ -6,19 +6,19 @@
 // are evaluated and contoured.  A pass covers the tiles of one slab of one job, or of
 // every job of a batch (each job whole); tiles, super-tiles and row segments of the pass
 // are numbered job after job.  In stream order:
-//   k_sp_prune_super  one thread per super-tile: interval bounds of the job's program on
-//                   its padded box; single-signed super-tiles are marked, the others get
-//                   an exactly pruned op list
-//   k_sp_prune      one thread per tile: the super-tile's list pruned on the tile's padded
-//                   box; single-signed tiles are dropped, the others get a slot in a pool
-//                   shared by all jobs (and a flag when their list is a left fold)
+//   k_sp_prune_prims / k_sp_prune_super / k_sp_prune   exact interval pruning: the
+//                   job's program on each super-tile's padded box (primitive intervals in
+//                   parallel, then one lane per super-tile combines them in program order),
+//                   then the super-tile's list on each of its tiles (one lane per tile);
+//                   single-signed tiles are dropped, the others get a slot in a pool shared
+//                   by all jobs (and a flag when their list is a left fold)
 //   k_sp_bricks     per slot, one lane per brick of 2x2x2 cells: interval bounds of a
 //                   left fold on the brick; bricks that keep one sign are not evaluated
-//   k_sp_tiles<1/0> per slot (persistent): the pruned program, staged in LDS, at the
+//   k_sp_tiles<1/0> per slot (one block): the pruned program, staged in LDS, at the
 //                   lattice points the tile's live bricks use (left folds) or at all
-//                   points of its box (other programs), into an LDS box; then the
-//                   tile's cells: vertex and sign bits of each active cell, staged
-//                   compactly, the slot's cell bits and each row segment's vertex count
+//                   points of its box (other programs), into an LDS box; then the tile's
+//                   cells: vertex and sign bits of each active cell, staged compactly, the
+//                   slot's cell bits and each row segment's vertex count
 //   k_sp_quadcount  per slot: valid quads of the staged cells (neighbour cells' bits),
 //                   two triangles each, per row segment; invalid quads counted
 //   reduce / k_scan / downsweep   exclusive scans of the row segments' vertex and
@@ -29,7 +29,8 @@
 //                   4-tap normals (the tile's pruned program), then the triangles of
 //                   their quads, vertex indices relative to the job's first vertex
 //   k_sp_boundary   (multi-slab single job) cell -> vertex map of the slab's last plane
-// A cell's vertex index is never stored: it is the first index of its row segment plus
+// The per-slot kernels are persistent (as many blocks as the device holds) and draw slots
+// from a work queue.  A cell's vertex index is never stored: it is the first index of its row segment plus
 // the number of cells with a vertex before it in the segment (the slot's cell bits).
 // Loops over slots read their bound on the device: no host round trip inside a pass.
 #include <algorithm>
@@ -59,6 +60,19 @@ __device__ __forceinline__ uint32_t active_slots(const SpPass& P)
 {
     const uint64_t n = P.counters[kSlotsUsed];
     return (uint32_t)(n < P.slot_cap ? n : P.slot_cap);
+}
+
+// Work queues of a persistent launch: worker b of nid draws slots g, g + G, g + 2G, ... (g =
+// b mod G, G = min(kQueueGroups, nid)) one at a time from its group's counter, so blocks that
+// draw cheap tiles take more of them.  One counter per group, each on its own 128-byte
+// line: updates of one address are serialised by the memory system, so a single counter
+// for all blocks would throttle them.
+__device__ __forceinline__ uint32_t queue_take(const SpPass& P, uint32_t q, uint32_t id, uint32_t nid)
+{
+    const uint32_t G = min(kQueueGroups, nid), g = id % G;
+    const uint32_t j = atomicAdd(&P.queues[(q * kQueueGroups + g) * 32u], 1u);
+    const uint64_t slot = (uint64_t)j * G + g;
+    return (uint32_t)(slot < 0xffffffffull ? slot : 0xffffffffull);
 }
 
 // Row segment of row (zl, y) and tile column tx of job J.
@@ -187,72 +201,64 @@ __device__ __forceinline__ bool contour_cell_sp(const float c[8], const float lo
 
 namespace {
 
-// Exact pruning of a postfix program on the box [bl, bh] (device_interval.h): `in` lists
-// the program's op indices (nullptr: every op of `ops` in order).  The kept op indices
-// are written to `out` in order; returns their count, and the interval of the result in
-// `root`.  The top of the interval stack stays in registers; deeper entries live in the
-// block's dynamic LDS, P.prune_depth entries per lane (one column per lane: a thread's
-// depth differs from its neighbours' when they prune different jobs).  Blocks of 64.
+// Exact pruning of postfix programs on boxes (device_interval.h).  The kept op indices
+// are written in order; the interval of the result is `top` when the walk ends.  The top
+// of the interval stack is kept in registers, deeper entries in LDS columns (entry d of
+// lane l at [d * 64 + l]).
 struct PruneStack {
     double* lo;       // [depth][64]
     double* hi;       // [depth][64]
     uint16_t* start;  // [depth][64]
 };
 
-__device__ __forceinline__ PruneStack prune_stack(const SpPass& P)
-{
-    extern __shared__ double prune_lds[];
-    const uint32_t d = P.prune_depth;
-    return PruneStack{prune_lds, prune_lds + 64u * d, reinterpret_cast<uint16_t*>(prune_lds + 128u * d)};
-}
-
-__device__ uint32_t prune_program(const s2m_op* __restrict__ ops, const uint16_t* __restrict__ in,
-                                  uint32_t n, const double bl[3], const double bh[3],
-                                  uint16_t* __restrict__ out, Iv& root, const PruneStack& S)
-{
-    const uint32_t ln = thread_id() & 63u;
+struct PruneWalk {
     Iv top = iv_unknown();
     uint32_t top_start = 0, depth = 0, len = 0;  // depth: entries below `top`
-    for (uint32_t q = 0; q < n; ++q) {
-        const uint32_t i = in ? in[q] : q;
-        const uint32_t code = ops[i].code;
-        if (code < S2M_OP_UNION) {
-            if (q != 0) {  // push the previous top
-                S.lo[depth * 64u + ln] = top.lo;
-                S.hi[depth * 64u + ln] = top.hi;
-                S.start[depth * 64u + ln] = (uint16_t)top_start;
-                ++depth;
-            }
-            top = iv_primitive(code, ops[i].f, bl, bh);
-            top_start = len;
-            out[len++] = (uint16_t)i;
-        } else {
-            const Iv B = top;
-            const uint32_t sB = top_start;
-            --depth;
-            const double alo = S.lo[depth * 64u + ln], ahi = S.hi[depth * 64u + ln];
-            const Iv A{alo, ahi, alo > -kHuge};
-            const uint32_t sA = S.start[depth * 64u + ln];
-            int keep;
-            const Iv r = iv_op(code, A, B, ops[i].f[0], keep);
-            if (keep == kKeepBoth) {
-                out[len++] = (uint16_t)i;
-                top = r;
-            } else if (keep == kKeepA) {
-                len = sB;  // drop B's subtree and the operator
-                top = A;
-            } else {       // drop A's subtree and the operator
-                const uint32_t nb = len - sB;
-                for (uint32_t j = 0; j < nb; ++j) out[sA + j] = out[sB + j];
-                len = sA + nb;
-                top = B;
-            }
-            top_start = sA;
+
+    // The primitive with op index i, interval v (q: its position in the program).
+    __device__ __forceinline__ void prim(uint32_t q, uint32_t i, const Iv& v, uint16_t* out, const PruneStack& S,
+                                         uint32_t ln)
+    {
+        if (q != 0) {  // push the previous top
+            S.lo[depth * 64u + ln] = top.lo;
+            S.hi[depth * 64u + ln] = top.hi;
+            S.start[depth * 64u + ln] = (uint16_t)top_start;
+            ++depth;
         }
+        top = v;
+        top_start = len;
+        out[len++] = (uint16_t)i;
     }
-    root = top;
-    return len;
-}
+
+    // Operator i (code, smoothing radius k) on the two topmost values.
+    __device__ __forceinline__ void op(uint32_t i, uint32_t code, float k, uint16_t* out, const PruneStack& S,
+                                       uint32_t ln)
+    {
+        const Iv B = top;
+        const uint32_t sB = top_start;
+        --depth;
+        const double alo = S.lo[depth * 64u + ln], ahi = S.hi[depth * 64u + ln];
+        const Iv A{alo, ahi, alo > -kHuge};  // iv_make / iv_unknown: known iff lo > -kHuge
+        const uint32_t sA = S.start[depth * 64u + ln];
+        int keep;
+        const Iv r = iv_op(code, A, B, k, keep);
+        if (keep == kKeepBoth) {
+            out[len++] = (uint16_t)i;
+            top = r;
+        } else if (keep == kKeepA) {
+            len = sB;  // drop B's subtree and the operator
+            top = A;
+        } else {       // drop A's subtree and the operator
+            const uint32_t nb = len - sB;
+            for (uint32_t j = 0; j < nb; ++j) out[sA + j] = out[sB + j];
+            len = sA + nb;
+            top = B;
+        }
+        top_start = sA;
+    }
+
+    __device__ __forceinline__ bool one_sign() const { return top.known && (top.lo > 0.0 || top.hi < 0.0); }
+};
 
 __device__ __forceinline__ uint32_t job_of(const SpPass& P, uint64_t index, bool super)
 {
@@ -267,88 +273,172 @@ __device__ __forceinline__ uint32_t job_of(const SpPass& P, uint64_t index, bool
 
 }  // namespace
 
-// ---------------------------------------------------------------------------------
-// One thread per super-tile (kSuper^3 tiles): the job's program pruned on the super-tile's
-// padded box.  A super-tile whose SDF keeps one sign gets kSingleSign and no list.
-// ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_sp_prune_super(SpPass P)
+// Padded box of the tiles [t0, t1] (per axis, t1 inclusive; z: tile layers of the job).
+__device__ __forceinline__ void tiles_box(const SpJob& J, const uint32_t t0[3], const uint32_t t1[3], double bl[3],
+                                          double bh[3])
 {
-    const uint64_t s = (uint64_t)blockIdx.x * blockDim.x + thread_id();
-    if (s >= P.nsuper) return;
-    const SpJob& J = P.jobs[job_of(P, s, true)];
+    const float* e[3] = {J.ext, J.ext + 2 * J.ntx, J.ext + 2 * J.ntx + 2 * J.nty};
+    const uint32_t n[3] = {J.ntx, J.nty, J.ntz};
+    for (int a = 0; a < 3; ++a) {
+        bl[a] = (double)e[a][t0[a]] - J.pad;
+        bh[a] = (double)e[a][n[a] + t1[a]] + J.pad;
+    }
+}
+
+// Tiles of super-tile s of job J: first and last tile per axis (z: tile layer of the job).
+__device__ __forceinline__ void super_tiles(const SpJob& J, uint64_t s, uint32_t t0[3], uint32_t t1[3])
+{
     const uint32_t t = (uint32_t)(s - J.super_base);
     const uint32_t sx = t % J.nsx, sy = (t / J.nsx) % J.nsy, szl = t / (J.nsx * J.nsy);
-    const uint32_t tx0 = sx * kSuper, tx1 = min(tx0 + kSuper, J.ntx) - 1;
-    const uint32_t ty0 = sy * kSuper, ty1 = min(ty0 + kSuper, J.nty) - 1;
     const uint32_t layers = (J.nz + kTile - 1) / kTile;  // tile layers of this pass
-    const uint32_t tz0 = J.tz0 + szl * kSuper, tz1 = J.tz0 + min(szl * kSuper + kSuper, layers) - 1;
-    const float* ex = J.ext;
-    const float* ey = ex + 2 * J.ntx;
-    const float* ez = ey + 2 * J.nty;
-    const double bl[3] = {(double)ex[tx0] - J.pad, (double)ey[ty0] - J.pad, (double)ez[tz0] - J.pad};
-    const double bh[3] = {(double)ex[J.ntx + tx1] + J.pad, (double)ey[J.nty + ty1] + J.pad,
-                          (double)ez[J.ntz + tz1] + J.pad};
-    Iv root;
-    const uint32_t len = prune_program(J.ops, nullptr, J.nops, bl, bh,
-                                       P.super_lists + (size_t)s * P.list_stride, root, prune_stack(P));
-    const bool one_sign = root.known && (root.lo > 0.0 || root.hi < 0.0);
-    P.super_lens[s] = one_sign ? kSingleSign : len;
+    t0[0] = sx * kSuper;
+    t1[0] = min(t0[0] + kSuper, J.ntx) - 1;
+    t0[1] = sy * kSuper;
+    t1[1] = min(t0[1] + kSuper, J.nty) - 1;
+    t0[2] = J.tz0 + szl * kSuper;
+    t1[2] = J.tz0 + min(szl * kSuper + kSuper, layers) - 1;
 }
 
 // ---------------------------------------------------------------------------------
-// One thread per tile: its super-tile's program pruned on the tile's padded box; tiles
-// that keep one sign are dropped, the others get a slot in the pool shared by all jobs.
+// Pruning, in three launches:
+//  k_sp_prune_prims  one lane per (super-tile, primitive): the primitive's interval on the
+//                    super-tile's padded box (super_iv)
+//  k_sp_prune_super  one lane per super-tile: the job's program combined in order from
+//                    those intervals, exactly pruned (super_lists), or kSingleSign when
+//                    the SDF keeps one sign on the super-tile (super_lens)
+//  k_sp_prune        one wave per super-tile, lane l = its tile (l % 4, l / 4 % 4, l / 16):
+//                    the super-tile's list pruned on the tile's padded box (every lane walks
+//                    the same list: wave-uniform op dispatch).  Tiles that keep one sign are
+//                    dropped, the others get a slot in the pool shared by all jobs (and a
+//                    flag when their list is a left fold).
+// Dynamic LDS of the last two: the prune stack (P.prune_depth entries per lane).
 // ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_sp_prune_prims(SpPass P, uint32_t chunks)
+{
+    const uint64_t s = blockIdx.x / chunks;
+    const uint32_t k = (blockIdx.x % chunks) * 64u + (thread_id() & 63u);
+    const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(job_of(P, s, true))];
+    if (k >= J.nops || J.ops[k].code >= S2M_OP_UNION) return;
+    uint32_t t0[3], t1[3];
+    super_tiles(J, s, t0, t1);
+    double bl[3], bh[3];
+    tiles_box(J, t0, t1, bl, bh);
+    const Iv v = iv_primitive(J.ops[k].code, J.ops[k].f, bl, bh);
+    P.super_iv[s * P.list_stride + k] = make_double2(v.lo, v.hi);
+}
+
+constexpr uint32_t kSuperChunk = 16;  // ops of a lane's program staged in LDS at a time
+
+__global__ __launch_bounds__(64) void k_sp_prune_super(SpPass P)
+{
+    extern __shared__ double prune_lds[];
+    __shared__ double2 civ[kSuperChunk][64];   // primitive intervals
+    __shared__ uint32_t ccode[kSuperChunk][64];
+    __shared__ float ck[kSuperChunk][64];      // operator smoothing radius
+    const uint32_t D = P.prune_depth, ln = thread_id() & 63u;
+    const PruneStack S{prune_lds, prune_lds + 64u * D, reinterpret_cast<uint16_t*>(prune_lds + 128u * D)};
+    const uint64_t s = (uint64_t)blockIdx.x * 64u + ln;
+    if (s >= P.nsuper) return;
+    const SpJob& J = P.jobs[job_of(P, s, true)];
+    const s2m_op* ops = J.ops;
+    const uint32_t nops = J.nops;
+    const double2* iv = P.super_iv + s * P.list_stride;
+    uint16_t* out = P.super_lists + s * P.list_stride;
+    PruneWalk w;
+    for (uint32_t q0 = 0; q0 < nops; q0 += kSuperChunk) {
+        // the lane's next ops and intervals: all loads in flight at once (each lane reads
+        // back only its own column)
+        const uint32_t m = min(kSuperChunk, nops - q0);
+        for (uint32_t j = 0; j < m; ++j) {
+            ccode[j][ln] = ops[q0 + j].code;
+            ck[j][ln] = ops[q0 + j].f[0];
+            civ[j][ln] = iv[q0 + j];  // operators' entries are unused
+        }
+        for (uint32_t j = 0; j < m; ++j) {
+            const uint32_t q = q0 + j, code = ccode[j][ln];
+            if (code < S2M_OP_UNION) {
+                const double2 v = civ[j][ln];
+                w.prim(q, q, Iv{v.x, v.y, v.x > -kHuge}, out, S, ln);  // known iff lo > -kHuge
+            } else {
+                w.op(q, code, ck[j][ln], out, S, ln);
+            }
+        }
+    }
+    P.super_lens[s] = w.one_sign() ? kSingleSign : w.len;
+}
+
 __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
 {
-    const uint64_t tile = (uint64_t)blockIdx.x * blockDim.x + thread_id();
-    if (tile >= P.ntiles) return;
-    const uint32_t jid = job_of(P, tile, false);
+    extern __shared__ double prune_lds[];
+    __shared__ StagedTape T;          // the super-tile's list (at most kStageOps ops)
+    __shared__ uint16_t idx[kStageOps];  // op index of each staged op
+    const uint32_t D = P.prune_depth, ln = thread_id() & 63u;
+    const PruneStack S{prune_lds, prune_lds + 64u * D, reinterpret_cast<uint16_t*>(prune_lds + 128u * D)};
+    const uint64_t s = blockIdx.x;
+    const uint32_t jid = __builtin_amdgcn_readfirstlane(job_of(P, s, true));
     const SpJob& J = P.jobs[jid];
-    uint32_t tx, ty, tzl;
-    tile_xyz(J, tile, tx, ty, tzl);
-    const uint64_t s =
-        J.super_base + ((uint64_t)(tzl / kSuper) * J.nsy + ty / kSuper) * J.nsx + tx / kSuper;
-    const uint32_t sl = P.super_lens[s];
-    int32_t slot = -1;
-    if (sl == kSingleSign) {  // no surface anywhere in the super-tile
+    const uint32_t sl = __builtin_amdgcn_readfirstlane(P.super_lens[s]);
+    uint32_t t0[3], t1[3];
+    super_tiles(J, s, t0, t1);
+    const uint32_t tx = t0[0] + (ln & 3u), ty = t0[1] + ((ln >> 2) & 3u), tz = t0[2] + (ln >> 4);
+    const bool valid = tx <= t1[0] && ty <= t1[1] && tz <= t1[2];
+    const uint64_t tile = J.tile_base + ((uint64_t)(tz - J.tz0) * J.nty + ty) * J.ntx + tx;
+    const s2m_op* ops = J.ops;
+    const uint16_t* in = P.super_lists + s * P.list_stride;
+    const bool staged = sl != kSingleSign && sl <= kStageOps;
+    if (staged) {
+        stage_tape(T, ops, in, sl);
+        for (uint32_t k = ln; k < sl; k += 64u) idx[k] = in[k];
+    }
+    __syncthreads();
+    bool surface = false;
+    if (valid && sl == kSingleSign) {  // no surface anywhere in the super-tile
         P.lens[tile] = 0;
         P.flops[tile] = 0;
-        P.slot_of_tile[tile] = slot;
-        return;
+    } else if (valid) {
+        const uint32_t tt[3] = {tx, ty, tz};
+        double bl[3], bh[3];
+        tiles_box(J, tt, tt, bl, bh);
+        uint16_t* L = P.lists + (size_t)tile * P.list_stride;
+        PruneWalk w;
+        if (staged) {  // every lane walks the same list: wave-uniform dispatch
+            for (uint32_t q = 0; q < sl; ++q) {
+                const uint32_t code = uniform_code(T, q), i = idx[q];
+                if (code < S2M_OP_UNION) w.prim(q, i, iv_primitive(code, T.f[q], bl, bh), L, S, ln);
+                else w.op(i, code, T.f[q][0], L, S, ln);
+            }
+        } else {
+            for (uint32_t q = 0; q < sl; ++q) {
+                const uint32_t i = __builtin_amdgcn_readfirstlane(in[q]);
+                const uint32_t code = ops[i].code;
+                if (code < S2M_OP_UNION) w.prim(q, i, iv_primitive(code, ops[i].f, bl, bh), L, S, ln);
+                else w.op(i, code, ops[i].f[0], L, S, ln);
+            }
+        }
+        surface = !w.one_sign();
+        uint32_t fl = 0;
+        for (uint32_t j = 0; j < w.len; ++j) fl += J.op_flops[L[j]];
+        P.lens[tile] = w.len | (surface && is_left_fold(ops, L, w.len) ? kLinearFlag : 0u);
+        P.flops[tile] = fl;
     }
-    const uint32_t tz = J.tz0 + tzl;
-    const float* ex = J.ext;
-    const float* ey = ex + 2 * J.ntx;
-    const float* ez = ey + 2 * J.nty;
-    const double bl[3] = {(double)ex[tx] - J.pad, (double)ey[ty] - J.pad, (double)ez[tz] - J.pad};
-    const double bh[3] = {(double)ex[J.ntx + tx] + J.pad, (double)ey[J.nty + ty] + J.pad,
-                          (double)ez[J.ntz + tz] + J.pad};
-    const s2m_op* ops = J.ops;
-    uint16_t* L = P.lists + (size_t)tile * P.list_stride;
-    Iv root;
-    const uint32_t len =
-        prune_program(ops, P.super_lists + (size_t)s * P.list_stride, sl, bl, bh, L, root, prune_stack(P));
-    const bool one_sign = root.known && (root.lo > 0.0 || root.hi < 0.0);
-    uint32_t fl = 0;
-    for (uint32_t j = 0; j < len; ++j) fl += J.op_flops[L[j]];
-    P.lens[tile] = len | (!one_sign && is_left_fold(ops, L, len) ? kLinearFlag : 0u);
-    P.flops[tile] = fl;
-    // slots for the wave's surviving tiles: one counter update per wave
-    const uint64_t w = __ballot(!one_sign);
+    // slots for the surviving tiles: one counter update per wave
+    const uint64_t m = __ballot(surface);
     uint64_t wbase = 0;
-    if (lane() == __builtin_ctzll(w | (1ull << 63)) && w)
-        wbase = atomicAdd((unsigned long long*)&P.counters[kSlotsUsed], (unsigned long long)__builtin_popcountll(w));
-    wbase = __shfl(wbase, __builtin_ctzll(w | (1ull << 63)), 64);
-    if (!one_sign) {
-        const uint64_t sidx = wbase + (uint64_t)__builtin_popcountll(w & ((1ull << lane()) - 1ull));
+    if (ln == 0 && m) wbase = atomicAdd((unsigned long long*)&P.counters[kSlotsUsed], (unsigned long long)__builtin_popcountll(m));
+    wbase = __shfl(wbase, 0, 64);
+    int32_t slot = -1;
+    if (surface) {
+        const uint64_t sidx = wbase + (uint64_t)__builtin_popcountll(m & ((1ull << ln) - 1ull));
         if (sidx < P.slot_cap) {
             slot = (int32_t)sidx;
             P.tile_of_slot[sidx] = (uint32_t)tile;
             P.job_of_slot[sidx] = jid;
         }
     }
-    P.slot_of_tile[tile] = slot;
+    if (valid) P.slot_of_tile[tile] = slot;
+    const uint64_t lin = __ballot(slot >= 0 && (P.lens[tile] & kLinearFlag));
+    if (lin && ln == (uint32_t)__builtin_ctzll(lin))
+        atomicAdd((unsigned long long*)&P.counters[kLinearSlots], (unsigned long long)__builtin_popcountll(lin));
 }
 
 // Per-axis corner tables of a tile's 8 cells: box-relative lattice index and coordinate
@@ -358,12 +448,12 @@ struct TileAxes {
     float clo[3][8], chi[3][8];
 };
 
-// Entry thread_id() (< 24) of A: axis thread_id() / 8, cell thread_id() % 8.  Cells past
-// the end of the grid repeat the last cell's entries (their cells are never contoured).
+// Entry e (< 24) of A: axis e / 8, cell e % 8.  Cells past the end of the grid repeat the
+// last cell's entries (their cells are never contoured).
 __device__ __forceinline__ void load_tile_axes(TileAxes& A, const SpJob& J, uint32_t tx, uint32_t ty,
-                                               uint32_t tz)
+                                               uint32_t tz, uint32_t e)
 {
-    const uint32_t ax = thread_id() >> 3, c = thread_id() & 7u;
+    const uint32_t ax = e >> 3, c = e & 7u;
     const uint32_t t = ax == 0 ? tx : ax == 1 ? ty : tz;
     const uint32_t r = ax == 0 ? J.rx : ax == 1 ? J.ry : J.z0 + J.nz;  // cells on the axis
     const uint32_t* lo = ax == 0 ? J.lox : ax == 1 ? J.loy : J.loz;
@@ -390,12 +480,6 @@ __device__ __forceinline__ uint32_t div16(uint32_t n, uint32_t d, uint32_t m)
     return d <= 1u ? n : __umulhi(n, m);
 }
 
-// Per slot (persistent, grid-stride): the tile's pruned program at every lattice point
-// of its closed box, staged in LDS.  LINEAR = 1 runs the slots whose program is a left
-// fold (two points per lane), LINEAR = 0 the others; both launches walk all slots and
-// skip the other kind.
-constexpr int kLinearPoints = 2;
-
 // ---------------------------------------------------------------------------------
 // Bricks: the 4x4x4 blocks of 2x2x2 cells of a tile, brick (bx, by, bz) = bit
 // (bz * 4 + by) * 4 + bx of the slot's `slot_live`.  Per slot (one 64-lane block
@@ -406,17 +490,20 @@ constexpr int kLinearPoints = 2;
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_sp_bricks(SpPass P)
 {
+    __shared__ StagedTape T;
     const uint32_t nslots = active_slots(P);
-    const uint32_t b = thread_id();
     for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
+        const uint32_t b = thread_id();
         const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
         const uint32_t lens = __builtin_amdgcn_readfirstlane(P.lens[tile]);
-        if (!(lens & kLinearFlag)) {
+        const uint32_t n = lens & ~kLinearFlag;
+        if (!(lens & kLinearFlag) || n > kStageOps) {  // every brick is kept
             if (b == 0) P.slot_live[slot] = ~0ull;
             continue;
         }
-        const uint32_t n = lens & ~kLinearFlag;
         const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(P.job_of_slot[slot])];
+        __syncthreads();  // the previous slot's readers of T are done
+        stage_tape(T, J.ops, P.lists + (size_t)tile * P.list_stride, n);
         uint32_t tx, ty, tzl;
         tile_xyz(J, tile, tx, ty, tzl);
         const uint32_t t[3] = {tx, ty, J.tz0 + tzl};
@@ -431,15 +518,12 @@ __global__ __launch_bounds__(64) void k_sp_bricks(SpPass P)
             bl[a] = (double)co[a][lo[a][g0]];
             bh[a] = (double)co[a][hi[a][g1]];
         }
-        const uint16_t* L = P.lists + (size_t)tile * P.list_stride;
-        const s2m_op* ops = J.ops;
-        Iv acc = iv_primitive(ops[L[0]].code, ops[L[0]].f, bl, bh);
+        __syncthreads();
+        Iv acc = iv_primitive(uniform_code(T, 0), T.f[0], bl, bh);
         for (uint32_t k = 1; k + 1 < n; k += 2) {
-            const s2m_op& pr = ops[L[k]];
-            const s2m_op& op = ops[L[k + 1]];
-            const Iv d = iv_primitive(pr.code, pr.f, bl, bh);
+            const Iv d = iv_primitive(uniform_code(T, k), T.f[k], bl, bh);
             int keep;
-            acc = iv_op(op.code, acc, d, op.f[0], keep);
+            acc = iv_op(uniform_code(T, k + 1), acc, d, T.f[k + 1][0], keep);
         }
         const bool may_cross = !(acc.known && (acc.lo > 0.0 || acc.hi < 0.0));
         const uint64_t m = __ballot(may_cross);
@@ -460,6 +544,8 @@ __global__ __launch_bounds__(64) void k_sp_bricks(SpPass P)
 // chunk); a slot's cells are contiguous, so a chunk with fewer than the slot's cells left
 // is abandoned.  Hence at most 2 x (active cells) + blocks x kStageChunk entries are used.
 constexpr uint32_t kStageChunk = kStageChunkHost;
+constexpr uint32_t kTileThreads = 256;  // threads of a k_sp_tiles block (one tile at a time)
+constexpr int kLinearPoints = 2;        // points per lane sharing each op's decode (left folds)
 
 struct ContourLds {  // block-shared state of contour_tile
     uint16_t cand[512];  // cells with a sign change among their corners
@@ -493,7 +579,7 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
     // 1. cells with a sign change (thread t: cells t and t + 256, x fastest)
 #pragma unroll
     for (uint32_t r = 0; r < 2; ++r) {
-        const uint32_t c = r * 256u + thread_id();
+        const uint32_t c = r * kTileThreads + thread_id();
         const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
         const uint32_t brick = ((czl >> 1) * 4u + (cyl >> 1)) * 4u + (cxl >> 1);
         bool cand = false;
@@ -520,7 +606,7 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
     uint32_t cell[2] = {0, 0}, sg[2] = {0, 0}, k_at[2] = {0, 0};
 #pragma unroll
     for (uint32_t r = 0; r < 2; ++r) {
-        const uint32_t k = r * 256u + thread_id();
+        const uint32_t k = r * kTileThreads + thread_id();
         if (k < ncand) {
             const uint32_t c = L.cand[k];
             const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
@@ -577,16 +663,17 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
     }
 }
 
-constexpr uint32_t kChunk = 2048;  // box points examined per compaction round
+constexpr uint32_t kChunk = 1024;  // box points examined per compaction round
 
-// k_sp_tiles: per slot (persistent, grid-stride), the tile's pruned program, staged in
+// k_sp_tiles: per slot (persistent, slots drawn from a work queue), the tile's pruned
+// program, staged in
 // LDS, at the lattice points of its closed box that the tile's cells need (all points for
 // programs that are not left folds), into an LDS box; then the tile's cells are contoured
 // from it (contour_tile).  LINEAR = 1 runs the slots whose program is a left fold (two
 // points per lane), LINEAR = 0 the others; both launches walk all slots and skip the
 // other kind.
 template <int LINEAR>
-__global__ __launch_bounds__(256) void k_sp_tiles(SpPass P)
+__global__ __launch_bounds__(kTileThreads) void k_sp_tiles(SpPass P)
 {
     extern __shared__ float box[];  // P.box_stride floats
     __shared__ StagedTape T;
@@ -597,10 +684,20 @@ __global__ __launch_bounds__(256) void k_sp_tiles(SpPass P)
     __shared__ uint32_t nlist;
     __shared__ float pc[3][32];  // lattice coordinates of the box per axis
     __shared__ ContourLds L;
+    __shared__ uint32_t sh_slot;
+    const uint32_t nslots = active_slots(P), nlinear = (uint32_t)P.counters[kLinearSlots];
+    if (LINEAR ? nlinear == 0 : nlinear >= nslots) return;  // no slot of this kind
+    constexpr uint32_t kQueue = LINEAR ? 0u : 1u;
     if (thread_id() == 0) L.left = 0;  // published by the first slot's barriers
     uint64_t tiles_done = 0, points_done = 0, flops_done = 0;  // thread 0: this block's share
-    const uint32_t nslots = active_slots(P);
-    for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
+    uint32_t next = thread_id() == 0 ? queue_take(P, kQueue, blockIdx.x, gridDim.x) : 0u;
+    for (;;) {
+        __syncthreads();  // the previous slot's readers of sh_slot, T, A, bm, rowm, list, pc, box, L are done
+        if (thread_id() == 0) sh_slot = next;
+        __syncthreads();
+        const uint32_t slot = sh_slot;
+        if (slot >= nslots) break;
+        if (thread_id() == 0) next = queue_take(P, kQueue, blockIdx.x, gridDim.x);  // prefetched
         const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
         const uint32_t lens = __builtin_amdgcn_readfirstlane(P.lens[tile]);
         if (((lens & kLinearFlag) != 0) != (LINEAR != 0)) continue;  // same for the whole block
@@ -627,9 +724,8 @@ __global__ __launch_bounds__(256) void k_sp_tiles(SpPass P)
         const float* cz = J.cz + z0;
         const bool staged = n <= kStageOps;
         const uint64_t lv = LINEAR ? P.slot_live[slot] : ~0ull;
-        __syncthreads();  // the previous slot's readers of T, A, bm, rowm, list, box, L are done
         if (staged) stage_tape(T, J.ops, Lst, n);
-        if (thread_id() < 24) load_tile_axes(A, J, tx, ty, tz);
+        if (thread_id() < 24) load_tile_axes(A, J, tx, ty, tz, thread_id());
         if (thread_id() < 96) {
             const uint32_t a = thread_id() >> 5, i = thread_id() & 31u;
             const float* c = a == 0 ? cx : a == 1 ? cy : cz;
@@ -671,10 +767,11 @@ __global__ __launch_bounds__(256) void k_sp_tiles(SpPass P)
                 if (thread_id() == 0) nlist = 0;
                 __syncthreads();
                 // the chunk's points that some live brick uses, compacted into `list`
-                for (uint32_t k = 0; k < kChunk / 256u; ++k) {
-                    const uint32_t q = c0 + k * 256u + thread_id();
+                const uint32_t cn = min(kChunk, npts - c0);
+                for (uint32_t k = 0; k < cn; k += kTileThreads) {
+                    const uint32_t q = c0 + k + thread_id();
                     bool need = false;
-                    if (q < npts) {
+                    if (q < c0 + cn) {
                         const uint32_t row = div16(q, nx, mx), ix = q - row * nx;  // row = iz * ny + iy
                         need = (rowm[row] & bm[0][ix]) != 0;
                     }
@@ -1122,20 +1219,23 @@ __global__ __launch_bounds__(256) void k_sp_boundary(SpPass P)
 
 // Per-pass reset of the slot counter; the largest count seen is kept for the host's
 // capacity check (a kernel rather than a memset, so a pass can be graph-captured).
-__global__ void k_sp_pass_begin(uint64_t* counters)
+__global__ void k_sp_pass_begin(uint64_t* counters, uint32_t* queues)
 {
     if (thread_id() == 0 && blockIdx.x == 0) {
         const uint64_t used = counters[kSlotsUsed];
         if (used > counters[kSlotsMax]) counters[kSlotsMax] = used;
         counters[kSlotsUsed] = 0;
+        if (counters[kStageUsed] > counters[kStageMax]) counters[kStageMax] = counters[kStageUsed];
         counters[kStageUsed] = 0;
+        counters[kLinearSlots] = 0;
     }
+    for (uint32_t i = thread_id(); i < kQueues * kQueueGroups; i += blockDim.x) queues[i * 32u] = 0;
 }
 
 // ---------------------------------------------------------------------------------
-hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters)
+hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t* queues)
 {
-    hipLaunchKernelGGL(k_sp_pass_begin, dim3(1), dim3(64), 0, s, counters);
+    hipLaunchKernelGGL(k_sp_pass_begin, dim3(1), dim3(64), 0, s, counters, queues);
     return hipGetLastError();
 }
 
@@ -1159,23 +1259,25 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
     const uint64_t nsb64 = (P.nseg + kScanChunk - 1) / kScanChunk;
     if (nsb64 > kMaxScanBlocks) return hipErrorInvalidValue;
     const uint32_t nsb = (uint32_t)nsb64;
-    const size_t smem = (size_t)P.box_stride * sizeof(float);
     const uint32_t jb = (P.njobs + 63) / 64;
     const uint32_t big = 1u << 20;
     auto mark = [&](int i) {
         if (marks && marks[i]) (void)hipEventRecord(marks[i], s);
     };
     const size_t prune_smem = (size_t)P.prune_depth * 64u * (2 * sizeof(double) + sizeof(uint16_t));
+    const uint32_t chunks = (P.list_stride + 63) / 64;
+    if (P.nsuper * chunks > 0x7fffffffull) return hipErrorInvalidValue;
     mark(0);
-    if (P.nsuper)
-        hipLaunchKernelGGL(k_sp_prune_super, dim3((uint32_t)((P.nsuper + 63) / 64)), dim3(64), prune_smem, s, P);
-    hipLaunchKernelGGL(k_sp_prune, dim3((uint32_t)((P.ntiles + 63) / 64)), dim3(64), prune_smem, s, P);
+    hipLaunchKernelGGL(k_sp_prune_prims, dim3((uint32_t)(P.nsuper * chunks)), dim3(64), 0, s, P, chunks);
+    hipLaunchKernelGGL(k_sp_prune_super, dim3((uint32_t)((P.nsuper + 63) / 64)), dim3(64), prune_smem, s, P);
+    hipLaunchKernelGGL(k_sp_prune, dim3((uint32_t)P.nsuper), dim3(64), prune_smem, s, P);
     mark(1);
     hipLaunchKernelGGL(k_sp_bricks, dim3(resident_blocks(k_sp_bricks, 64, 0, cus, big)), dim3(64), 0, s, P);
-    hipLaunchKernelGGL((k_sp_tiles<1>), dim3(resident_blocks(k_sp_tiles<1>, 256, smem, cus, tile_blocks)),
-                       dim3(256), smem, s, P);
-    hipLaunchKernelGGL((k_sp_tiles<0>), dim3(resident_blocks(k_sp_tiles<0>, 256, smem, cus, tile_blocks)),
-                       dim3(256), smem, s, P);
+    const size_t smem = (size_t)P.box_stride * sizeof(float);
+    hipLaunchKernelGGL((k_sp_tiles<1>), dim3(resident_blocks(k_sp_tiles<1>, kTileThreads, smem, cus, tile_blocks)),
+                       dim3(kTileThreads), smem, s, P);
+    hipLaunchKernelGGL((k_sp_tiles<0>), dim3(resident_blocks(k_sp_tiles<0>, kTileThreads, smem, cus, tile_blocks)),
+                       dim3(kTileThreads), smem, s, P);
     mark(2);
     hipLaunchKernelGGL(k_sp_quadcount, dim3(resident_blocks(k_sp_quadcount, 64, 0, cus, big)), dim3(64), 0, s,
                        P, b);
