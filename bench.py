@@ -7,12 +7,15 @@ A step meshes one job (SDF lattice evaluation, contouring, normals, quads) with 
 program and parameters already on the device; the mesh stays in HBM.  Mvoxels/s =
 res^3 / 1e6 / seconds per step.  Prints one JSON line with:
 
-  roofline      the SDF evaluation kernel (k_sp_eval on the tile-pruned path, one
-                launch per pass; k_eval_lattice on the dense path, one per slab):
-                algorithmic fp32 flops executed per launch (static op count of the
-                program each tile runs x its lattice points) / mean launch time from
-                HIP events recorded around that kernel alone on the library's stream,
-                against the fp32 peak (vector ALU: the path has no matrix work)
+  roofline      the SDF evaluation kernel (k_sp_tiles on the tile-pruned path: the
+                pruned program at the tile's lattice points fused with contouring, one
+                launch per pass for left-fold programs; k_eval_lattice on the dense path,
+                one per slab): algorithmic fp32 flops executed per launch (static op
+                count of the program each tile runs x the lattice points it evaluates) /
+                mean launch time from HIP events recorded around the tile launches on the
+                library's stream, against the fp32 peak (vector ALU: the path has no
+                matrix work).  traffic: HBM bytes per launch from the rocprofv3
+                FETCH_SIZE / WRITE_SIZE summary in profiles/hbm_traffic.json
   cpu_baseline  the oracle (CPU restatement of the reference algorithm: 8 corner
                 evaluations per cell, OpenMP) on a bounded sample of z-slices of the
                 same job

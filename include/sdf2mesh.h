@@ -183,7 +183,7 @@ typedef struct s2m_timing {
     uint64_t tiles_total;    /* tiles of 8^3 cells (pruned path; 0 when dense)       */
     uint64_t tiles_active;   /* tiles whose lattice points were evaluated            */
     uint64_t eval_flops;     /* algorithmic flops executed on the lattice            */
-    double prune_ms;         /* interval pruning of tiles (pruned path)              */
+    double prune_ms;         /* interval pruning of tiles and bricks (pruned path)   */
 } s2m_timing;
 
 typedef struct s2m_ctx s2m_ctx;

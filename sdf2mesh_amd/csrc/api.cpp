@@ -390,6 +390,20 @@ struct SpRange {
     uint64_t v0, nv, t0, nt, invalid;
 };
 
+// Is the program a left fold of library primitives (P0 P1 op1 P2 op2 ...) short enough to
+// be staged (sparse.hip, kStageOps = 128)?  Every pruned sub-program of one is one too:
+// pruning keeps either the accumulated value or the right operand, a single primitive.
+bool left_fold_program(const std::vector<s2m_op>& ops)
+{
+    const size_t n = ops.size();
+    if (n == 0 || n > 128 || (n & 1u) == 0) return false;
+    for (size_t k = 0; k < n; ++k) {
+        const bool prim = ops[k].code < S2M_OP_UNION;
+        if (prim != (k == 0 || (k & 1u)) || ops[k].code == S2M_OP_MANDELBULB) return false;
+    }
+    return true;
+}
+
 // Mesh `n` jobs with the sparse pipeline.  n == 1 may be cut into z-slabs; a batch runs
 // as one pass with every job whole.  Capacities (slots, vertices, triangles) start from
 // estimates and grow if the device counters report more.
@@ -401,12 +415,14 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     std::vector<size_t> tab_off(n);
     size_t tab_bytes = 0;
     uint32_t list_stride = 1, box_stride = 1, prune_depth = 1;
+    bool general_tiles = false;  // a tile's pruned program may not be a left fold
     uint64_t ntiles_all = 0, nseg_all = 0, plane_max = 0, cells_all = 0, planes_sum = 0;
     for (uint32_t i = 0; i < n; ++i) {
         build_sp_tables(jobs[i], progs[i], ps[i]->eps, tabs[i]);
         tab_off[i] = tab_bytes;
         tab_bytes += (tabs[i].bytes.size() + 255) & ~size_t(255);
         list_stride = std::max(list_stride, progs[i]->n);
+        if (!left_fold_program(progs[i]->ops)) general_tiles = true;
         box_stride = std::max(box_stride, tabs[i].bmax[0] * tabs[i].bmax[1] * tabs[i].bmax[2]);
         uint32_t depth = 0;  // operand stack entries below the top (k_sp_prune's LDS stack)
         for (const s2m_op& o : progs[i]->ops) {
@@ -633,7 +649,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
                 for (auto& e : m) e = ctx->next_event();
             HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters, ctx->sp_queues.as<uint32_t>()));
             const int boundary = (n == 1 && pass + 1 < npass) ? 1 : 0;
-            HIP_TRY(launch_sparse_pass(ctx->stream, P, b, ctx->cus, (uint32_t)tile_blocks, boundary,
+            HIP_TRY(launch_sparse_pass(ctx->stream, P, b, ctx->cus, (uint32_t)tile_blocks, general_tiles, boundary,
                                          ctx->profiling ? m : nullptr));
             if (ctx->profiling) ev.insert(ev.end(), m, m + kSparseMarks);
             tiles_total += P.ntiles;

@@ -71,7 +71,8 @@ constexpr uint32_t kStageChunkHost = 1024;  // staged-cell allocation chunk (spa
 constexpr uint32_t kTileBlocksPerCU = 8;    // most k_sp_tiles blocks per compute unit
 constexpr uint32_t kQueues = 2;        // work queues of a pass (k_sp_tiles<1>, k_sp_tiles<0>)
 constexpr uint32_t kQueueGroups = 64;  // counters per queue, 32 u32 (128 bytes) apart
-// Profiling marks of one sparse pass: prune | eval+contour | quad count | scan | emission
+// Profiling marks of one sparse pass: prune (tiles, bricks) | eval+contour | quad count |
+// scan | emission
 constexpr int kSparseMarks = 6;
 
 // Tile-sparse pipeline (sparse.hip).  One job of a pass: a slab of a single job, or a
@@ -166,9 +167,11 @@ hipError_t launch_normals(hipStream_t s, const s2m_op* ops, uint32_t nops, float
 hipError_t launch_quads(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int emit);
 hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t* queues);
 // cus: compute units of the device; tile_blocks: most blocks of a k_sp_tiles launch (each
-// holds a staging chunk).
+// holds a staging chunk); general_tiles: some tile's pruned program may not be a left fold
+// (false when every job's whole program is one: its pruned programs are then left folds).
 hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b, int cus,
-                              uint32_t tile_blocks, int boundary_plane, hipEvent_t* marks);
+                              uint32_t tile_blocks, bool general_tiles, int boundary_plane,
+                              hipEvent_t* marks);
 // Contour corner values given per cell (test hook): cells of the slab read their 8
 // corners from `corners` (8 floats per cell, cells in (z,y,x) order from plane z0).
 hipError_t launch_cells_from_corners(hipStream_t s, const SlabArgs& a, const float* corners,

@@ -1253,7 +1253,7 @@ static uint32_t resident_blocks(K kernel, int threads, size_t smem, int cus, uin
 }
 
 hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b, int cus,
-                              uint32_t tile_blocks, int boundary_plane, hipEvent_t* marks)
+                              uint32_t tile_blocks, bool general_tiles, int boundary_plane, hipEvent_t* marks)
 {
     if (P.ntiles == 0) return hipSuccess;
     const uint64_t nsb64 = (P.nseg + kScanChunk - 1) / kScanChunk;
@@ -1271,13 +1271,14 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
     hipLaunchKernelGGL(k_sp_prune_prims, dim3((uint32_t)(P.nsuper * chunks)), dim3(64), 0, s, P, chunks);
     hipLaunchKernelGGL(k_sp_prune_super, dim3((uint32_t)((P.nsuper + 63) / 64)), dim3(64), prune_smem, s, P);
     hipLaunchKernelGGL(k_sp_prune, dim3((uint32_t)P.nsuper), dim3(64), prune_smem, s, P);
-    mark(1);
     hipLaunchKernelGGL(k_sp_bricks, dim3(resident_blocks(k_sp_bricks, 64, 0, cus, big)), dim3(64), 0, s, P);
+    mark(1);
     const size_t smem = (size_t)P.box_stride * sizeof(float);
     hipLaunchKernelGGL((k_sp_tiles<1>), dim3(resident_blocks(k_sp_tiles<1>, kTileThreads, smem, cus, tile_blocks)),
                        dim3(kTileThreads), smem, s, P);
-    hipLaunchKernelGGL((k_sp_tiles<0>), dim3(resident_blocks(k_sp_tiles<0>, kTileThreads, smem, cus, tile_blocks)),
-                       dim3(kTileThreads), smem, s, P);
+    if (general_tiles)
+        hipLaunchKernelGGL((k_sp_tiles<0>), dim3(resident_blocks(k_sp_tiles<0>, kTileThreads, smem, cus, tile_blocks)),
+                           dim3(kTileThreads), smem, s, P);
     mark(2);
     hipLaunchKernelGGL(k_sp_quadcount, dim3(resident_blocks(k_sp_quadcount, 64, 0, cus, big)), dim3(64), 0, s,
                        P, b);
