@@ -70,6 +70,35 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(p); }
 };
 
+// Page-locked host memory (hipHostMalloc), grown on demand: copies from and to it are
+// DMA transfers that do not stage through a driver buffer.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need)
+    {
+        if (need <= bytes) return hipSuccess;
+        if (p) {
+            hipError_t e = hipHostFree(p);
+            p = nullptr;
+            bytes = 0;
+            if (e != hipSuccess) return e;
+        }
+        size_t alloc = std::max<size_t>(need, 4096);
+        hipError_t e = hipHostMalloc(&p, alloc, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = alloc;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    template <class T>
+    T* as() const { return static_cast<T*>(p); }
+};
+
 // Per-axis corner coordinate table (host side).
 struct AxisTable {
     std::vector<float> coord;
@@ -99,10 +128,11 @@ struct s2m_ctx {
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_seg, sp_boundary;
     DevBuf sp_stage, sp_slot_stage, sp_slot_live;
-    DevBuf sp_job_out, sp_queues, sp_super_iv, sp_super_lists, sp_super_lens;
+    DevBuf sp_job_out, sp_queues, sp_super_iv, sp_super_lists, sp_super_lens, sp_super_act;
     uint64_t slot_cap = 0;
     int cus = 256;
-    std::vector<uint8_t> host_img;  // staging of the last pass's tables (kept until its sync)
+    PinnedBuf host_img;   // staging of the last pass's tables (kept until its sync)
+    PinnedBuf host_back;  // device counters and per-job ranges read back after a pass
     uint64_t vcap = 0, tcap = 0;
     uint64_t last_nvtx = 0, last_ntri = 0, last_invalid = 0;
     std::vector<hipEvent_t> events;
@@ -393,6 +423,23 @@ struct SpRange {
 // Is the program a left fold of library primitives (P0 P1 op1 P2 op2 ...) short enough to
 // be staged (sparse.hip, kStageOps = 128)?  Every pruned sub-program of one is one too:
 // pruning keeps either the accumulated value or the right operand, a single primitive.
+// Is the program a left fold of unions and smooth unions (k > 0) of library primitives
+// other than the Mandelbulb (k_sp_prune_fold)?
+bool union_fold_program(const std::vector<s2m_op>& ops)
+{
+    const size_t n = ops.size();
+    if (n == 0 || (n & 1u) == 0) return false;
+    for (size_t k = 0; k < n; ++k) {
+        const uint32_t c = ops[k].code;
+        if (k == 0 || (k & 1u)) {
+            if (c >= S2M_OP_UNION || c == S2M_OP_MANDELBULB) return false;
+        } else if (!(c == S2M_OP_UNION || (c == S2M_OP_SMOOTH_UNION && ops[k].f[0] > 0.0f))) {
+            return false;
+        }
+    }
+    return true;
+}
+
 bool left_fold_program(const std::vector<s2m_op>& ops)
 {
     const size_t n = ops.size();
@@ -416,6 +463,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     size_t tab_bytes = 0;
     uint32_t list_stride = 1, box_stride = 1, prune_depth = 1;
     bool general_tiles = false;  // a tile's pruned program may not be a left fold
+    bool fold_jobs = false, other_jobs = false;  // programs that are / are not union folds
     uint64_t ntiles_all = 0, nseg_all = 0, plane_max = 0, cells_all = 0, planes_sum = 0;
     for (uint32_t i = 0; i < n; ++i) {
         build_sp_tables(jobs[i], progs[i], ps[i]->eps, tabs[i]);
@@ -423,6 +471,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         tab_bytes += (tabs[i].bytes.size() + 255) & ~size_t(255);
         list_stride = std::max(list_stride, progs[i]->n);
         if (!left_fold_program(progs[i]->ops)) general_tiles = true;
+        (union_fold_program(progs[i]->ops) ? fold_jobs : other_jobs) = true;
         box_stride = std::max(box_stride, tabs[i].bmax[0] * tabs[i].bmax[1] * tabs[i].bmax[2]);
         uint32_t depth = 0;  // operand stack entries below the top (k_sp_prune's LDS stack)
         for (const s2m_op& o : progs[i]->ops) {
@@ -461,13 +510,15 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         return fail(S2M_E_ARG, "grid too large for one pass; mesh the jobs separately");
 
     // device tables + job descriptors (one per pass for a single job's slabs, else per job)
-    std::vector<uint8_t>& img = ctx->host_img;
     const size_t njob_desc = n == 1 ? npass : n;
     const size_t desc_off = (tab_bytes + 255) & ~size_t(255);
-    img.assign(desc_off + njob_desc * sizeof(SpJob), 0);
+    const size_t img_bytes = desc_off + njob_desc * sizeof(SpJob);
+    HIP_TRY(ctx->host_img.ensure(img_bytes));
+    uint8_t* img = ctx->host_img.as<uint8_t>();
+    std::memset(img, 0, img_bytes);
     for (uint32_t i = 0; i < n; ++i)
-        std::memcpy(img.data() + tab_off[i], tabs[i].bytes.data(), tabs[i].bytes.size());
-    HIP_TRY(ctx->tile_tab.ensure(img.size()));
+        std::memcpy(img + tab_off[i], tabs[i].bytes.data(), tabs[i].bytes.size());
+    HIP_TRY(ctx->tile_tab.ensure(img_bytes));
     uint8_t* dimg = ctx->tile_tab.as<uint8_t>();
     std::vector<SpJob> desc(njob_desc);
     uint64_t tile_base = 0, seg_base = 0, super_base = 0, nsuper_pass = 0;
@@ -506,6 +557,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         super_base += nsup;
         nsuper_pass = n == 1 ? std::max(nsuper_pass, nsup) : super_base;
         J.nops = progs[i]->n;
+        J.union_fold = union_fold_program(progs[i]->ops) ? 1u : 0u;
         J.flags = ps[i]->flags;
         J.bx = T.bmax[0];
         J.by = T.bmax[1];
@@ -530,8 +582,8 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     }
     HIP_TRY(ctx->sp_boundary.ensure(std::max<uint64_t>(plane_max, 1) * sizeof(uint32_t)));
     for (auto& J : desc) J.boundary = ctx->sp_boundary.as<uint32_t>();
-    std::memcpy(img.data() + desc_off, desc.data(), desc.size() * sizeof(SpJob));
-    HIP_TRY(hipMemcpyAsync(dimg, img.data(), img.size(), hipMemcpyHostToDevice, ctx->stream));
+    std::memcpy(img + desc_off, desc.data(), desc.size() * sizeof(SpJob));
+    HIP_TRY(hipMemcpyAsync(dimg, img, img_bytes, hipMemcpyHostToDevice, ctx->stream));
     const SpJob* ddesc = reinterpret_cast<const SpJob*>(dimg + desc_off);
 
     HIP_TRY(ctx->tile_lists.ensure(tiles_pass * list_stride * sizeof(uint16_t)));
@@ -549,6 +601,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     HIP_TRY(ctx->sp_super_iv.ensure(nsup_alloc * list_stride * 2 * sizeof(double)));
     HIP_TRY(ctx->sp_super_lists.ensure(nsup_alloc * list_stride * sizeof(uint16_t)));
     HIP_TRY(ctx->sp_super_lens.ensure(nsup_alloc * sizeof(uint32_t)));
+    HIP_TRY(ctx->sp_super_act.ensure(nsup_alloc * sizeof(uint64_t)));
 
     // a closed surface crosses O(res^2) cells: start from 6 per row-plane cell
     uint64_t want_v = std::min<uint64_t>(cells_all, std::max<uint64_t>(65536, 6 * planes_sum));
@@ -643,13 +696,15 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.super_iv = ctx->sp_super_iv.as<double2>();
             P.super_lists = ctx->sp_super_lists.as<uint16_t>();
             P.super_lens = ctx->sp_super_lens.as<uint32_t>();
+            P.super_act = ctx->sp_super_act.as<uint64_t>();
             P.job_out = ctx->sp_job_out.as<uint64_t>();
             hipEvent_t m[kSparseMarks] = {};
             if (ctx->profiling)
                 for (auto& e : m) e = ctx->next_event();
             HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters, ctx->sp_queues.as<uint32_t>()));
             const int boundary = (n == 1 && pass + 1 < npass) ? 1 : 0;
-            HIP_TRY(launch_sparse_pass(ctx->stream, P, b, ctx->cus, (uint32_t)tile_blocks, general_tiles, boundary,
+            HIP_TRY(launch_sparse_pass(ctx->stream, P, b, ctx->cus, (uint32_t)tile_blocks, general_tiles, fold_jobs,
+                                       other_jobs, boundary,
                                          ctx->profiling ? m : nullptr));
             if (ctx->profiling) ev.insert(ev.end(), m, m + kSparseMarks);
             tiles_total += P.ntiles;
@@ -659,11 +714,13 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             ev_end = ctx->next_event();
             HIP_TRY(hipEventRecord(ev_end, ctx->stream));
         }
-        uint64_t cnt[kNumCounters];
-        std::vector<uint64_t> jo((size_t)n * kJobOut);
-        HIP_TRY(hipMemcpyAsync(cnt, b.counters, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(jo.data(), ctx->sp_job_out.p, jo.size() * 8, hipMemcpyDeviceToHost,
-                               ctx->stream));
+        HIP_TRY(ctx->host_back.ensure((kNumCounters + (size_t)n * kJobOut) * sizeof(uint64_t)));
+        const uint64_t* cnt = ctx->host_back.as<uint64_t>();
+        const uint64_t* jo = cnt + kNumCounters;
+        HIP_TRY(hipMemcpyAsync(ctx->host_back.p, b.counters, kNumCounters * sizeof(uint64_t),
+                               hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(ctx->host_back.as<uint64_t>() + kNumCounters, ctx->sp_job_out.p,
+                               (size_t)n * kJobOut * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         if (ctx->profiling) {
             float ms = 0;
@@ -1042,9 +1099,11 @@ void s2m_destroy(s2m_ctx* ctx)
                       &ctx->tile_flops, &ctx->sp_slot_of_tile, &ctx->sp_tile_of_slot,
                       &ctx->sp_job_of_slot, &ctx->sp_cellbits, &ctx->sp_seg,
                       &ctx->sp_boundary, &ctx->sp_job_out, &ctx->sp_queues, &ctx->sp_super_iv,
-                      &ctx->sp_super_lists, &ctx->sp_super_lens,
+                      &ctx->sp_super_lists, &ctx->sp_super_lens, &ctx->sp_super_act,
                       &ctx->sp_stage, &ctx->sp_slot_stage, &ctx->sp_slot_live})
         b->release();
+    ctx->host_img.release();
+    ctx->host_back.release();
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

@@ -85,6 +85,7 @@ struct SpJob {
     uint32_t ntx, nty, ntz;    // tiles of the whole job per axis
     uint32_t tz0;              // tile layer of plane z0
     uint32_t nops, flags;
+    uint32_t union_fold;       // the program is a left fold of unions / smooth unions (k > 0)
     uint32_t bx, by;           // lattice box dims of the job's tiles (slot box indexing)
     uint32_t out_index;        // row of the pass's job_out table
     float eps;
@@ -121,6 +122,7 @@ struct SpPass {
     double2* super_iv;         // [super-tile][list_stride] primitive intervals (lo, hi)
     uint16_t* super_lists;     // [super-tile][list_stride] pruned op indices
     uint32_t* super_lens;      // [super-tile] list length or kSingleSign
+    uint64_t* super_act;       // [super-tile] bit (z * 4 + y) * 4 + x: that tile has a slot
     uint16_t* lists;           // [tile][list_stride] pruned op indices
     uint32_t* lens;            // [tile]
     uint32_t* flops;           // [tile] flops per point of the pruned tape
@@ -169,9 +171,11 @@ hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t*
 // cus: compute units of the device; tile_blocks: most blocks of a k_sp_tiles launch (each
 // holds a staging chunk); general_tiles: some tile's pruned program may not be a left fold
 // (false when every job's whole program is one: its pruned programs are then left folds).
+// fold_jobs / other_jobs: some job's program is / is not a left fold of unions (pruned at
+// the super-tile level without a sequential walk).
 hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b, int cus,
-                              uint32_t tile_blocks, bool general_tiles, int boundary_plane,
-                              hipEvent_t* marks);
+                              uint32_t tile_blocks, bool general_tiles, bool fold_jobs, bool other_jobs,
+                              int boundary_plane, hipEvent_t* marks);
 // Contour corner values given per cell (test hook): cells of the slab read their 8
 // corners from `corners` (8 floats per cell, cells in (z,y,x) order from plane z0).
 hipError_t launch_cells_from_corners(hipStream_t s, const SlabArgs& a, const float* corners,

@@ -34,6 +34,8 @@
 // the number of cells with a vertex before it in the segment (the slot's cell bits).
 // Loops over slots read their bound on the device: no host round trip inside a pass.
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "device_interval.h"
 #include "device_sdf.h"
@@ -206,14 +208,26 @@ namespace {
 // of the interval stack is kept in registers, deeper entries in LDS columns (entry d of
 // lane l at [d * 64 + l]).
 struct PruneStack {
-    double* lo;       // [depth][64]
-    double* hi;       // [depth][64]
-    uint16_t* start;  // [depth][64]
+    double* lo;        // [depth][64]
+    double* hi;        // [depth][64]
+    uint16_t* start;   // [depth][64]
+    uint16_t* single;  // [depth][64] op index of a one-primitive subtree, else kComposite
+
+    // LDS bytes of a stack of `depth` entries per lane, and its layout from `base`.
+    static constexpr size_t bytes(uint32_t depth) { return (size_t)depth * 64u * (2 * sizeof(double) + 2 * sizeof(uint16_t)); }
+    __device__ static PruneStack at(double* base, uint32_t depth)
+    {
+        uint16_t* s16 = reinterpret_cast<uint16_t*>(base + 128u * depth);
+        return PruneStack{base, base + 64u * depth, s16, s16 + 64u * depth};
+    }
 };
+
+constexpr uint32_t kComposite = 0xffffu;
 
 struct PruneWalk {
     Iv top = iv_unknown();
     uint32_t top_start = 0, depth = 0, len = 0;  // depth: entries below `top`
+    uint32_t top_single = kComposite;  // op index when `top` is one primitive
 
     // The primitive with op index i, interval v (q: its position in the program).
     __device__ __forceinline__ void prim(uint32_t q, uint32_t i, const Iv& v, uint16_t* out, const PruneStack& S,
@@ -223,10 +237,12 @@ struct PruneWalk {
             S.lo[depth * 64u + ln] = top.lo;
             S.hi[depth * 64u + ln] = top.hi;
             S.start[depth * 64u + ln] = (uint16_t)top_start;
+            S.single[depth * 64u + ln] = (uint16_t)top_single;
             ++depth;
         }
         top = v;
         top_start = len;
+        top_single = i;
         out[len++] = (uint16_t)i;
     }
 
@@ -239,19 +255,26 @@ struct PruneWalk {
         --depth;
         const double alo = S.lo[depth * 64u + ln], ahi = S.hi[depth * 64u + ln];
         const Iv A{alo, ahi, alo > -kHuge};  // iv_make / iv_unknown: known iff lo > -kHuge
-        const uint32_t sA = S.start[depth * 64u + ln];
+        const uint32_t sA = S.start[depth * 64u + ln], singleA = S.single[depth * 64u + ln];
         int keep;
         const Iv r = iv_op(code, A, B, k, keep);
         if (keep == kKeepBoth) {
             out[len++] = (uint16_t)i;
             top = r;
+            top_single = kComposite;
         } else if (keep == kKeepA) {
             len = sB;  // drop B's subtree and the operator
             top = A;
+            top_single = singleA;
         } else {       // drop A's subtree and the operator
-            const uint32_t nb = len - sB;
-            for (uint32_t j = 0; j < nb; ++j) out[sA + j] = out[sB + j];
-            len = sA + nb;
+            if (top_single != kComposite) {  // B is one primitive: no read-back of the list
+                out[sA] = (uint16_t)top_single;
+                len = sA + 1;
+            } else {
+                const uint32_t nb = len - sB;
+                for (uint32_t j = 0; j < nb; ++j) out[sA + j] = out[sB + j];
+                len = sA + nb;
+            }
             top = B;
         }
         top_start = sA;
@@ -318,13 +341,120 @@ __global__ __launch_bounds__(64) void k_sp_prune_prims(SpPass P, uint32_t chunks
     const uint64_t s = blockIdx.x / chunks;
     const uint32_t k = (blockIdx.x % chunks) * 64u + (thread_id() & 63u);
     const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(job_of(P, s, true))];
-    if (k >= J.nops || J.ops[k].code >= S2M_OP_UNION) return;
+    if (J.union_fold || k >= J.nops || J.ops[k].code >= S2M_OP_UNION) return;  // folds: k_sp_prune_fold
     uint32_t t0[3], t1[3];
     super_tiles(J, s, t0, t1);
     double bl[3], bh[3];
     tiles_box(J, t0, t1, bl, bh);
     const Iv v = iv_primitive(J.ops[k].code, J.ops[k].f, bl, bh);
     P.super_iv[s * P.list_stride + k] = make_double2(v.lo, v.hi);
+}
+
+// ---------------------------------------------------------------------------------
+// k_sp_prune_fold: the super-tile lists of programs that are left folds of unions and
+// smooth unions (P0 P1 op1 P2 op2 ..., op in {UNION, SMOOTH_UNION with k > 0}), one wave
+// per super-tile, without a sequential walk: lane q evaluates op q's interval, and
+// primitive P_m (m >= 1) is dropped with its operator when
+//     lo(P_m) >= U_{m-1} + k_m + slack,   U_{m-1} = min over j < m of hi(P_j),
+// a prefix minimum over the lanes.  The accumulated value after m - 1 steps never exceeds
+// min_j p_j by more than the rounding of its fp32 operations (smooth union <= min of its
+// operands), which the slack bounds: kMargin (64 ulps) x the magnitude of every step so
+// far, each bounded by |p_j| + k_j + max_i |p_i| + sum_i k_i / 4 (|acc| never exceeds the
+// last two terms); so p_m >= acc + k_m everywhere on the box, h = 1 and
+// the operator returns acc bit for bit (union: p_m >= acc returns acc).  Dropping never
+// changes acc, so later bounds stay valid.  The super-tile keeps one sign when
+// min_j hi(P_j) + slack < 0 (acc < 0), or when min over the kept P_j of lo(P_j) - the kept
+// smooth radii / 4 - slack > 0 (smooth union >= min of its operands - k / 4).
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_sp_prune_fold(SpPass P)
+{
+    const uint64_t s = blockIdx.x;
+    const uint32_t ln = thread_id() & 63u;
+    const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(job_of(P, s, true))];
+    if (!J.union_fold) return;
+    uint32_t t0[3], t1[3];
+    super_tiles(J, s, t0, t1);
+    double bl[3], bh[3];
+    tiles_box(J, t0, t1, bl, bh);
+    const s2m_op* ops = J.ops;
+    const uint32_t nops = J.nops;
+    uint16_t* out = P.super_lists + s * P.list_stride;
+    double carry_u = kHuge;    // min hi(P_j) over the primitives so far
+    double carry_mag = 0.0;    // sum of |P_j| + k_j over the primitives so far
+    double carry_max = 0.0;    // max of |P_j| + k_j so far
+    double carry_ka = 0.0;     // sum of k_j / 4 so far
+    double carry_cnt = 0.0;    // primitives so far
+    double carry_lo = kHuge;   // min lo(P_j) over the kept primitives
+    double carry_k4 = 0.0;     // sum of the kept smooth radii / 4
+    uint32_t nkept = 0;
+    bool prev_kept = true;     // was the last primitive of the previous chunk kept?
+    for (uint32_t c0 = 0; c0 < nops; c0 += 64u) {
+        const uint32_t q = c0 + ln;
+        const bool valid = q < nops;
+        const bool prim = valid && (q == 0 || (q & 1u));  // a left fold: P0, then P_m at odd q
+        // the operator after primitive q (q >= 1) and its smoothing radius
+        double kk = 0.0;
+        if (prim && q >= 1 && ops[q + 1].code == S2M_OP_SMOOTH_UNION) kk = (double)ops[q + 1].f[0];
+        Iv B = iv_unknown();
+        if (prim) B = iv_primitive(ops[q].code, ops[q].f, bl, bh);
+        const double mag = prim ? fmax(fabs(B.lo), fabs(B.hi)) + kk : 0.0;
+        // prefix scans over the lanes (inclusive): min hi, sum and max of the magnitudes,
+        // sum of k / 4, count of primitives
+        double u_ex = prim ? B.hi : kHuge, m_in = mag, x_in = mag, ka_in = 0.25 * kk, c_in = prim ? 1.0 : 0.0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const double uo = __shfl_up(u_ex, d, 64), mo = __shfl_up(m_in, d, 64);
+            const double xo = __shfl_up(x_in, d, 64), ko = __shfl_up(ka_in, d, 64), co = __shfl_up(c_in, d, 64);
+            if (ln >= (uint32_t)d) {
+                u_ex = fmin(u_ex, uo);
+                m_in += mo;
+                x_in = fmax(x_in, xo);
+                ka_in += ko;
+                c_in += co;
+            }
+        }
+        const double u_in = u_ex;  // inclusive
+        u_ex = __shfl_up(u_in, 1, 64);
+        if (ln == 0) u_ex = kHuge;
+        u_ex = fmin(u_ex, carry_u);
+        m_in += carry_mag;
+        x_in = fmax(x_in, carry_max);
+        ka_in += carry_ka;
+        c_in += carry_cnt;
+        const double slack = kMargin * (m_in + c_in * (x_in + ka_in));
+        const double tiny = 1e-12 * (fabs(u_ex) + fabs(B.lo) + fabs(B.hi) + kk);
+        const bool drop = prim && q >= 1 && B.known && u_ex < kHuge && B.lo >= u_ex + kk + tiny + slack;
+        // an operator (even q >= 2) is kept with its primitive q - 1
+        const uint64_t pk = __ballot(prim && !drop);
+        const bool prev_prim_kept = ln == 0 ? prev_kept : ((pk >> (ln - 1)) & 1ull) != 0;
+        const bool kept = valid && (prim ? !drop : prev_prim_kept);
+        const uint64_t km = __ballot(kept);
+        if (kept) out[nkept + (uint32_t)__builtin_popcountll(km & ((1ull << ln) - 1ull))] = (uint16_t)q;
+        nkept += (uint32_t)__builtin_popcountll(km);
+        // one-sign bounds over the kept primitives
+        double lo_k = (prim && !drop) ? B.lo : kHuge;
+        double k4 = (prim && !drop) ? 0.25 * kk : 0.0;
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            lo_k = fmin(lo_k, __shfl_xor(lo_k, d, 64));
+            k4 += __shfl_xor(k4, d, 64);
+        }
+        carry_lo = fmin(carry_lo, lo_k);
+        carry_k4 += k4;
+        carry_u = fmin(carry_u, __shfl(u_in, 63, 64));
+        carry_mag = __shfl(m_in, 63, 64);
+        carry_max = __shfl(x_in, 63, 64);
+        carry_ka = __shfl(ka_in, 63, 64);
+        carry_cnt = __shfl(c_in, 63, 64);
+        // the chunk's last primitive: at the highest odd q of the chunk (q = c0 + 63 when
+        // the chunk is full; 64 is even, so lane 63 is odd)
+        const uint32_t last = min(nops - c0, 64u) - 1u;
+        const uint32_t lastp = (c0 + last) & 1u ? last : (last ? last - 1u : 0u);
+        prev_kept = ((pk >> lastp) & 1ull) != 0;
+    }
+    const double slack = kMargin * (carry_mag + carry_cnt * (carry_max + carry_ka));
+    const bool one_sign = (carry_u + slack < 0.0) || (carry_lo > -kHuge && carry_lo - carry_k4 - slack > 0.0);
+    if (ln == 0) P.super_lens[s] = one_sign ? kSingleSign : nkept;
 }
 
 constexpr uint32_t kSuperChunk = 16;  // ops of a lane's program staged in LDS at a time
@@ -336,10 +466,11 @@ __global__ __launch_bounds__(64) void k_sp_prune_super(SpPass P)
     __shared__ uint32_t ccode[kSuperChunk][64];
     __shared__ float ck[kSuperChunk][64];      // operator smoothing radius
     const uint32_t D = P.prune_depth, ln = thread_id() & 63u;
-    const PruneStack S{prune_lds, prune_lds + 64u * D, reinterpret_cast<uint16_t*>(prune_lds + 128u * D)};
+    const PruneStack S = PruneStack::at(prune_lds, D);
     const uint64_t s = (uint64_t)blockIdx.x * 64u + ln;
     if (s >= P.nsuper) return;
     const SpJob& J = P.jobs[job_of(P, s, true)];
+    if (J.union_fold) return;  // k_sp_prune_fold
     const s2m_op* ops = J.ops;
     const uint32_t nops = J.nops;
     const double2* iv = P.super_iv + s * P.list_stride;
@@ -373,7 +504,7 @@ __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
     __shared__ StagedTape T;          // the super-tile's list (at most kStageOps ops)
     __shared__ uint16_t idx[kStageOps];  // op index of each staged op
     const uint32_t D = P.prune_depth, ln = thread_id() & 63u;
-    const PruneStack S{prune_lds, prune_lds + 64u * D, reinterpret_cast<uint16_t*>(prune_lds + 128u * D)};
+    const PruneStack S = PruneStack::at(prune_lds, D);
     const uint64_t s = blockIdx.x;
     const uint32_t jid = __builtin_amdgcn_readfirstlane(job_of(P, s, true));
     const SpJob& J = P.jobs[jid];
@@ -436,6 +567,8 @@ __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
         }
     }
     if (valid) P.slot_of_tile[tile] = slot;
+    const uint64_t act = __ballot(slot >= 0);  // bit l: tile l of the super-tile has a slot
+    if (ln == 0) P.super_act[s] = act;
     const uint64_t lin = __ballot(slot >= 0 && (P.lens[tile] & kLinearFlag));
     if (lin && ln == (uint32_t)__builtin_ctzll(lin))
         atomicAdd((unsigned long long*)&P.counters[kLinearSlots], (unsigned long long)__builtin_popcountll(lin));
@@ -1057,10 +1190,18 @@ constexpr uint32_t kScanChunk = 256 * kScanPer;
 // geometry is kept in registers: the walk reloads it only when it enters the next job).
 struct SegCursor {
     uint32_t tx, y, zl;                  // segment (zl, y, tx) of the current job
-    uint32_t ntx, nty, ry, nz;           // the current job's geometry
-    uint64_t tile_row;                   // tile of (zl >> 3, y >> 3, 0)
-    uint64_t tile_base;
+    uint32_t ntx, ry, nz, nsx, nsy;      // the current job's geometry
+    uint64_t super_base;
+    uint64_t super_row;                  // super-tile of tiles (0, y >> 3, zl >> 3)
+    uint32_t bit_row;                    // bit of tile (0, y >> 3, zl >> 3) in its super word
+    uint64_t word, word_at;              // cached super_act word and its super-tile
 
+    __device__ void set_row()
+    {
+        const uint32_t ty = y >> 3, tzl = zl >> 3;
+        super_row = super_base + ((uint64_t)(tzl / kSuper) * nsy + ty / kSuper) * nsx;
+        bit_row = ((tzl % kSuper) * kSuper + ty % kSuper) * kSuper;
+    }
     __device__ void load(const SpPass& P, uint64_t i)
     {
         uint32_t lo = 0, hi = P.njobs - 1;  // last job with seg_base <= i
@@ -1071,10 +1212,11 @@ struct SegCursor {
         }
         const SpJob& J = P.jobs[lo];
         ntx = J.ntx;
-        nty = J.nty;
         ry = J.ry;
         nz = J.nz;
-        tile_base = J.tile_base;
+        nsx = J.nsx;
+        nsy = J.nsy;
+        super_base = J.super_base;
         tx = y = zl = 0;
         if (i < P.nseg && ntx != 0 && ry != 0) {  // else: past the end, never used
             const uint32_t li = (uint32_t)(i - J.seg_base);
@@ -1083,22 +1225,31 @@ struct SegCursor {
             y = row % ry;
             zl = row / ry;
         }
-        tile_row = tile_base + ((uint64_t)(zl >> 3) * nty + (y >> 3)) * ntx;
+        set_row();
+        word_at = ~0ull;
     }
     // does the tile of the cursor's segment have a slot (and so a written count)?
-    __device__ bool active(const SpPass& P) const { return P.slot_of_tile[tile_row + tx] >= 0; }
+    __device__ bool active(const SpPass& P)
+    {
+        const uint64_t s = super_row + tx / kSuper;
+        if (s != word_at) {
+            word = P.super_act[s];
+            word_at = s;
+        }
+        return (word >> (bit_row + tx % kSuper)) & 1ull;
+    }
     __device__ bool first() const { return tx == 0 && y == 0 && zl == 0; }
     __device__ void next(const SpPass& P, uint64_t i_next)
     {
         if (++tx < ntx) return;
         tx = 0;
         if (++y < ry) {
-            if ((y & 7u) == 0) tile_row += ntx;
+            if ((y & 7u) == 0) set_row();
             return;
         }
         y = 0;
         if (++zl < nz) {
-            tile_row = tile_base + ((uint64_t)(zl >> 3) * nty) * ntx;
+            set_row();
             return;
         }
         if (i_next < P.nseg) load(P, i_next);  // next job
@@ -1240,20 +1391,38 @@ hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t*
 }
 
 // Blocks of a persistent launch: as many as the device holds at once (at most `cap`),
-// so no block starts only when others end.
+// so no block starts only when others end.  The occupancy query is slow (tens of
+// microseconds), so its answer is cached per (kernel, block size, LDS bytes).
 template <typename K>
 static uint32_t resident_blocks(K kernel, int threads, size_t smem, int cus, uint32_t cap)
 {
+    struct Entry {
+        const void* f;
+        int threads;
+        size_t smem;
+        int per_cu;
+    };
+    static std::mutex mu;
+    static std::vector<Entry> cache;
+    const void* f = reinterpret_cast<const void*>(kernel);
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kernel), threads,
-                                                     smem) != hipSuccess ||
-        per_cu < 1)
-        per_cu = 1;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        for (const Entry& e : cache)
+            if (e.f == f && e.threads == threads && e.smem == smem) per_cu = e.per_cu;
+    }
+    if (per_cu == 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, f, threads, smem) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        std::lock_guard<std::mutex> lock(mu);
+        cache.push_back(Entry{f, threads, smem, per_cu});
+    }
     return std::min<uint32_t>(cap, (uint32_t)cus * (uint32_t)per_cu);
 }
 
 hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b, int cus,
-                              uint32_t tile_blocks, bool general_tiles, int boundary_plane, hipEvent_t* marks)
+                              uint32_t tile_blocks, bool general_tiles, bool fold_jobs, bool other_jobs,
+                              int boundary_plane, hipEvent_t* marks)
 {
     if (P.ntiles == 0) return hipSuccess;
     const uint64_t nsb64 = (P.nseg + kScanChunk - 1) / kScanChunk;
@@ -1264,12 +1433,15 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
     auto mark = [&](int i) {
         if (marks && marks[i]) (void)hipEventRecord(marks[i], s);
     };
-    const size_t prune_smem = (size_t)P.prune_depth * 64u * (2 * sizeof(double) + sizeof(uint16_t));
+    const size_t prune_smem = PruneStack::bytes(P.prune_depth);
     const uint32_t chunks = (P.list_stride + 63) / 64;
     if (P.nsuper * chunks > 0x7fffffffull) return hipErrorInvalidValue;
     mark(0);
-    hipLaunchKernelGGL(k_sp_prune_prims, dim3((uint32_t)(P.nsuper * chunks)), dim3(64), 0, s, P, chunks);
-    hipLaunchKernelGGL(k_sp_prune_super, dim3((uint32_t)((P.nsuper + 63) / 64)), dim3(64), prune_smem, s, P);
+    if (fold_jobs) hipLaunchKernelGGL(k_sp_prune_fold, dim3((uint32_t)P.nsuper), dim3(64), 0, s, P);
+    if (other_jobs) {
+        hipLaunchKernelGGL(k_sp_prune_prims, dim3((uint32_t)(P.nsuper * chunks)), dim3(64), 0, s, P, chunks);
+        hipLaunchKernelGGL(k_sp_prune_super, dim3((uint32_t)((P.nsuper + 63) / 64)), dim3(64), prune_smem, s, P);
+    }
     hipLaunchKernelGGL(k_sp_prune, dim3((uint32_t)P.nsuper), dim3(64), prune_smem, s, P);
     hipLaunchKernelGGL(k_sp_bricks, dim3(resident_blocks(k_sp_bricks, 64, 0, cus, big)), dim3(64), 0, s, P);
     mark(1);
