@@ -128,7 +128,7 @@ struct s2m_ctx {
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_seg, sp_boundary;
     DevBuf sp_stage, sp_slot_stage, sp_slot_live;
-    DevBuf sp_job_out, sp_queues, sp_super_iv, sp_super_lists, sp_super_lens, sp_super_act;
+    DevBuf sp_job_out, sp_queues, sp_super_iv, sp_super_lists, sp_super_lens, sp_super_act, sp_rows;
     uint64_t slot_cap = 0;
     int cus = 256;
     PinnedBuf host_img;   // staging of the last pass's tables (kept until its sync)
@@ -521,7 +521,8 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     HIP_TRY(ctx->tile_tab.ensure(img_bytes));
     uint8_t* dimg = ctx->tile_tab.as<uint8_t>();
     std::vector<SpJob> desc(njob_desc);
-    uint64_t tile_base = 0, seg_base = 0, super_base = 0, nsuper_pass = 0;
+    uint64_t tile_base = 0, seg_base = 0, super_base = 0, nsuper_pass = 0, row_base = 0, trow_base = 0;
+    uint64_t nrows_pass = 0, ntrows_pass = 0;
     for (size_t d = 0; d < njob_desc; ++d) {
         const uint32_t i = n == 1 ? 0 : (uint32_t)d;
         const Job& j = jobs[i];
@@ -536,14 +537,25 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             J.tz0 = (uint32_t)d * layers;
             J.tile_base = 0;
             J.seg_base = 0;
+            J.row_base = 0;
+            J.trow_base = 0;
         } else {
             J.z0 = 0;
             J.nz = j.nz_cells;
             J.tz0 = 0;
             J.tile_base = tile_base;
             J.seg_base = seg_base;
+            J.row_base = row_base;
+            J.trow_base = trow_base;
             tile_base += (uint64_t)T.nt[0] * T.nt[1] * T.nt[2];
             seg_base += (uint64_t)j.nz_cells * j.res[1] * T.nt[0];
+            row_base += (uint64_t)j.nz_cells * j.res[1];
+            trow_base += (uint64_t)T.nt[2] * T.nt[1];
+        }
+        {   // rows and tile rows of this job's pass (per pass for a single job's slabs)
+            const uint64_t r = (uint64_t)J.nz * J.ry, tr = (uint64_t)((J.nz + kTile - 1) / kTile) * T.nt[1];
+            nrows_pass = n == 1 ? std::max(nrows_pass, r) : row_base;
+            ntrows_pass = n == 1 ? std::max(ntrows_pass, tr) : trow_base;
         }
         J.ntx = T.nt[0];
         J.nty = T.nt[1];
@@ -591,7 +603,8 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     HIP_TRY(ctx->tile_flops.ensure(tiles_pass * sizeof(uint32_t)));
     HIP_TRY(ctx->sp_slot_of_tile.ensure(tiles_pass * sizeof(int32_t)));
     HIP_TRY(ctx->sp_seg.ensure(std::max<uint64_t>(seg_pass, 1) * 4 * sizeof(uint32_t)));
-    const uint32_t nsb_max = (uint32_t)((seg_pass + 4095) / 4096) + 1;  // vertex and triangle sums
+    HIP_TRY(ctx->sp_rows.ensure(std::max<uint64_t>(nrows_pass, 1) * 4 * sizeof(uint32_t)));
+    const uint32_t nsb_max = (uint32_t)((nrows_pass + 4095) / 4096) + 1;  // vertex and triangle sums
     HIP_TRY(ctx->bcount.ensure((size_t)2 * nsb_max * sizeof(uint32_t)));
     HIP_TRY(ctx->boffset.ensure((size_t)2 * nsb_max * sizeof(uint32_t)));
     HIP_TRY(ctx->counters.ensure(kNumCounters * sizeof(uint64_t)));
@@ -663,10 +676,14 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
                 const SpJob& J = desc[pass];
                 P.ntiles = (uint64_t)J.ntx * J.nty * ((J.nz + kTile - 1) / kTile);
                 P.nseg = (uint64_t)J.nz * J.ry * J.ntx;
+                P.nrows = (uint64_t)J.nz * J.ry;
+                P.ntrows = (uint64_t)((J.nz + kTile - 1) / kTile) * J.nty;
                 P.boundary_cells = (uint64_t)J.rx * J.ry;
             } else {
                 P.ntiles = ntiles_all;
                 P.nseg = nseg_all;
+                P.nrows = nrows_pass;
+                P.ntrows = ntrows_pass;
             }
             if (n == 1) {
                 const SpJob& J = desc[pass];
@@ -691,6 +708,8 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.seg_t = P.seg_v + P.nseg;
             P.seg_off_v = P.seg_t + P.nseg;
             P.seg_off_t = P.seg_off_v + P.nseg;
+            P.row_sum = ctx->sp_rows.as<uint32_t>();
+            P.row_off = P.row_sum + 2 * P.nrows;
             P.counters = b.counters;
             P.queues = ctx->sp_queues.as<uint32_t>();
             P.super_iv = ctx->sp_super_iv.as<double2>();
@@ -1099,7 +1118,7 @@ void s2m_destroy(s2m_ctx* ctx)
                       &ctx->tile_flops, &ctx->sp_slot_of_tile, &ctx->sp_tile_of_slot,
                       &ctx->sp_job_of_slot, &ctx->sp_cellbits, &ctx->sp_seg,
                       &ctx->sp_boundary, &ctx->sp_job_out, &ctx->sp_queues, &ctx->sp_super_iv,
-                      &ctx->sp_super_lists, &ctx->sp_super_lens, &ctx->sp_super_act,
+                      &ctx->sp_super_lists, &ctx->sp_super_lens, &ctx->sp_super_act, &ctx->sp_rows,
                       &ctx->sp_stage, &ctx->sp_slot_stage, &ctx->sp_slot_live})
         b->release();
     ctx->host_img.release();

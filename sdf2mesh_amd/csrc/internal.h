@@ -92,6 +92,8 @@ struct SpJob {
     double pad;                // tile box padding (normal taps, rounding)
     uint64_t tile_base;        // first tile of the job in the pass's numbering
     uint64_t seg_base;         // first row segment of the job
+    uint64_t row_base;         // first row (zl, y) of the job
+    uint64_t trow_base;        // first tile row (tile layer, ty) of the job
     uint64_t super_base;       // first super-tile of the job
     uint32_t nsx, nsy, nsz;    // super-tiles of the job in this pass per axis
     const s2m_op* ops;
@@ -117,6 +119,8 @@ struct SpPass {
     uint32_t prune_depth;      // >= 1, >= the deepest operand stack of any job's program - 1
     uint64_t ntiles;           // tiles of the pass
     uint64_t nseg;             // row segments of the pass
+    uint64_t nrows;            // rows (z, y) of the pass
+    uint64_t ntrows;           // tile rows (tile layer, ty) of the pass
     uint64_t boundary_cells;   // rx * ry of jobs[0] when a boundary plane is written
     uint64_t nsuper;           // super-tiles of the pass
     double2* super_iv;         // [super-tile][list_stride] primitive intervals (lo, hi)
@@ -139,8 +143,10 @@ struct SpPass {
     uint64_t* slot_live;       // [slot] bricks (2x2x2 cells) that may hold an active cell
     uint32_t* seg_v;           // [nseg] vertices per row segment
     uint32_t* seg_t;           // triangles per row segment
-    uint32_t* seg_off_v;       // exclusive offsets: global vertex index
-    uint32_t* seg_off_t;       // exclusive offsets: global triangle index
+    uint32_t* seg_off_v;       // first vertex of the segment within its row
+    uint32_t* seg_off_t;       // first triangle of the segment within its row
+    uint32_t* row_sum;         // [2][nrows] vertices, triangles per row
+    uint32_t* row_off;         // [2][nrows] exclusive offsets: global vertex / triangle index
     uint64_t* counters;
     uint32_t* queues;          // [kQueues][kQueueGroups][32] work queue counters (word 0)
     uint64_t* job_out;         // [job][kJobOut]

@@ -21,11 +21,11 @@
 //                   slot's cell bits and each row segment's vertex count
 //   k_sp_quadcount  per slot: valid quads of the staged cells (neighbour cells' bits),
 //                   two triangles each, per row segment; invalid quads counted
-//   reduce / k_scan / downsweep   exclusive scans of the row segments' vertex and
-//                   triangle counts, job after job and (z, y, tile x) inside a job: the
-//                   reference's VertexList order
+//   k_sp_rowsum / rowreduce / k_scan / rowdown   offsets of the row segments in the
+//                   reference's VertexList order (job after job, then z, y, tile x):
+//                   per row, the segments of its active tiles; then a scan over rows
 //   k_sp_ranges     each job's first vertex and triangle index
-//   k_sp_emit       per slot: the staged vertices at their ordered index with their
+//   k_sp_emit<1/0>  per slot: the staged vertices at their ordered index with their
 //                   4-tap normals (the tile's pruned program), then the triangles of
 //                   their quads, vertex indices relative to the job's first vertex
 //   k_sp_boundary   (multi-slab single job) cell -> vertex map of the slab's last plane
@@ -83,6 +83,12 @@ __device__ __forceinline__ size_t seg_index(const SpJob& J, uint32_t zl, uint32_
     return J.seg_base + ((size_t)zl * J.ry + y) * J.ntx + tx;
 }
 
+// Row (zl, y) of job J in the pass's row numbering (rows job after job, zl then y).
+__device__ __forceinline__ size_t row_index(const SpJob& J, uint32_t zl, uint32_t y)
+{
+    return J.row_base + (size_t)zl * J.ry + y;
+}
+
 // Tile coordinates (tx, ty, tile layer within the pass) of pass tile `tile` of job J.
 __device__ __forceinline__ void tile_xyz(const SpJob& J, uint64_t tile, uint32_t& tx, uint32_t& ty,
                                          uint32_t& tzl)
@@ -134,7 +140,7 @@ __device__ uint32_t nb_row(const SpPass& P, const SpJob& J, uint32_t tx, int y, 
         if (c ? !(b8 & 0x80u) : !b8) continue;
         bits |= c ? 1u : b8 << 1;
         if (INDEX) {
-            const uint32_t first = P.seg_off_v[seg_index(J, zl, (uint32_t)y, t)];
+            const uint32_t first = P.seg_off_v[seg_index(J, zl, (uint32_t)y, t)] + P.row_off[row_index(J, zl, (uint32_t)y)];
             if (c) {
                 idx[0] = first + (uint32_t)__builtin_popcount(b8 & 0x7fu);
             } else {
@@ -158,7 +164,8 @@ __device__ __forceinline__ uint32_t vertex_at(const SpPass& P, const SpJob& J, u
     if (slot < 0) return kNone;
     const uint32_t b8 = slot_row_bits(P, slot, (z & 7u) * 8u + (y & 7u)), i = x & 7u;
     if (!((b8 >> i) & 1u)) return kNone;
-    return P.seg_off_v[seg_index(J, z - J.z0, y, x >> 3)] + (uint32_t)__builtin_popcount(b8 & ((1u << i) - 1u));
+    return P.seg_off_v[seg_index(J, z - J.z0, y, x >> 3)] + P.row_off[row_index(J, z - J.z0, y)] +
+           (uint32_t)__builtin_popcount(b8 & ((1u << i) - 1u));
 }
 
 // dualcontour.wgsl:72-131: crossings on the 12 edges in the reference's order, counted
@@ -1049,8 +1056,11 @@ __global__ __launch_bounds__(64) void k_sp_quadcount(SpPass P, MeshBufs b)
 // tile's pruned program (sdf3d_normal.wgsl:4-9, dualcontour.wgsl:171), then the
 // triangles of their valid quads at the row segment's triangle offset plus the
 // triangles of the cells before them in the segment (make_triangles, mesh.rs:250-265).
-// Triangle vertex indices are relative to the job's first vertex.
+// Triangle vertex indices are relative to the job's first vertex.  LINEAR = 1 handles the
+// slots whose program is a left fold, LINEAR = 0 the others (launched only when a program
+// of the pass is not a left fold; its stack interpreter needs more registers).
 // ---------------------------------------------------------------------------------
+template <int LINEAR>
 __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
 {
     __shared__ StagedTape T;
@@ -1071,6 +1081,7 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
         const uint32_t lens = __builtin_amdgcn_readfirstlane(P.lens[tile]);
         const uint32_t len = lens & ~kLinearFlag;
         const bool linear = (lens & kLinearFlag) != 0, staged = len <= kStageOps;
+        if (linear != (LINEAR != 0)) continue;  // the other launch's slot
         __syncthreads();  // the previous slot's readers of T, nb, tc are done
         if (normals && staged) stage_tape(T, J.ops, Lst, len);
         for (uint32_t t = thread_id(); t < 81u; t += blockDim.x)
@@ -1110,7 +1121,7 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
             const float ty4[4] = {py + (-eps), py + (-eps), py + eps, py + eps};
             const float tz4[4] = {pz + (-eps), pz + eps, pz + (-eps), pz + eps};
             float d[4];
-            if (linear) {
+            if (LINEAR) {
                 run_linear<4>(T, len, tx4, ty4, tz4, d);
             } else {
 #pragma unroll
@@ -1142,7 +1153,7 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
             const uint32_t nq = cell_quads(sg, tx * 8u + cxl, y, J.z0 + zl, lag, exists, invalid);
             if (!nq) continue;
             const uint64_t row2 = ((uint64_t)tc[(c >> 3) * 2u + 1] << 32) | tc[(c >> 3) * 2u];  // 8 counts
-            uint64_t t = P.seg_off_t[seg_index(J, zl, y, tx)];
+            uint64_t t = (uint64_t)P.seg_off_t[seg_index(J, zl, y, tx)] + P.row_off[P.nrows + row_index(J, zl, y)];
             for (uint32_t i = 0; i < cxl; ++i) t += (row2 >> (8u * i)) & 0xffu;
             const uint32_t v = nbv(0, 0, 0);
             const bool c0 = sg & 1u, c1 = (sg >> 1) & 1u, c2 = (sg >> 2) & 1u;
@@ -1176,103 +1187,86 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
 }
 
 // ---------------------------------------------------------------------------------
-// Exclusive scan of the pass's row-segment counts (vertices or triangles) in segment
-// order: per-block sums of kScanChunk segments, k_scan over the sums (adds and advances
-// the running total), block-local scan plus the block offset.  Only tiles with a slot
-// write counts: a segment of a tile without one counts 0 and is never read, and only
-// segments of tiles with a slot (plus each job's first segment, its start) receive an
-// offset.
+// Offsets in the reference's VertexList order: job after job, then (z, y, tile x).
+//  k_sp_rowsum    one wave per tile row (job, tile layer, ty), lane = row (z % 8, y % 8):
+//                 the row segments of the tile row's active tiles (the super-tiles'
+//                 activity words) in x order: each segment's offset within its row
+//                 (seg_off_v / seg_off_t) and the row's totals (row_sum)
+//  k_sp_rowreduce / k_scan / k_sp_rowdown   exclusive scan of the rows' totals (row_off)
+// A segment's first vertex (triangle) is row_off[row] + seg_off[segment].
 // ---------------------------------------------------------------------------------
-constexpr uint32_t kScanPer = 16;                  // consecutive segments per thread
+constexpr uint32_t kScanPer = 16;                  // consecutive rows per thread
 constexpr uint32_t kScanChunk = 256 * kScanPer;
 
-// Walks consecutive pass segments, tracking the job and the tile of each (the job's
-// geometry is kept in registers: the walk reloads it only when it enters the next job).
-struct SegCursor {
-    uint32_t tx, y, zl;                  // segment (zl, y, tx) of the current job
-    uint32_t ntx, ry, nz, nsx, nsy;      // the current job's geometry
-    uint64_t super_base;
-    uint64_t super_row;                  // super-tile of tiles (0, y >> 3, zl >> 3)
-    uint32_t bit_row;                    // bit of tile (0, y >> 3, zl >> 3) in its super word
-    uint64_t word, word_at;              // cached super_act word and its super-tile
+__global__ __launch_bounds__(64) void k_sp_rowsum(SpPass P)
+{
+    const uint64_t tr = blockIdx.x;
+    uint32_t lo = 0, hi = P.njobs - 1;  // last job whose first tile row <= tr
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (P.jobs[mid].trow_base <= tr) lo = mid;
+        else hi = mid - 1;
+    }
+    const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(lo)];
+    const uint32_t local = (uint32_t)(tr - J.trow_base), ty = local % J.nty, tzl = local / J.nty;
+    const uint32_t zl = tzl * 8u + (lane() >> 3), y = ty * 8u + (lane() & 7u);
+    const bool row_ok = zl < J.nz && y < J.ry;
+    const uint64_t srow = J.super_base + ((uint64_t)(tzl / kSuper) * J.nsy + ty / kSuper) * J.nsx;
+    const uint32_t bit0 = ((tzl % kSuper) * kSuper + ty % kSuper) * kSuper;
+    const size_t seg0 = row_ok ? seg_index(J, zl, y, 0) : 0;
+    uint32_t sv = 0, st = 0;
+    for (uint32_t sx0 = 0; sx0 < J.nsx; sx0 += 16u) {
+        // active tiles 4 * sx0 .. 4 * sx0 + 63: lane l < 16 reads super-tile sx0 + l's word
+        uint64_t m = 0;
+        if (lane() < 16u && sx0 + lane() < J.nsx)
+            m = ((P.super_act[srow + sx0 + lane()] >> bit0) & 0xfull) << (4u * lane());
+#pragma unroll
+        for (int d = 8; d >= 1; d >>= 1) m |= __shfl_xor(m, d, 64);
+        m = __shfl(m, 0, 64);
+        // the active tiles' counts, 8 at a time (independent loads), then the running sums
+        while (m) {
+            uint32_t txs[8], v[8], t[8];
+            uint32_t nb = 0;
+            for (; nb < 8u && m; ++nb) {
+                txs[nb] = 4u * sx0 + (uint32_t)__builtin_ctzll(m);
+                m &= m - 1ull;
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; ++k) {
+                v[k] = t[k] = 0;
+                if (k < nb && row_ok) {
+                    v[k] = P.seg_v[seg0 + txs[k]];
+                    t[k] = P.seg_t[seg0 + txs[k]];
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; ++k) {
+                if (k < nb && row_ok) {
+                    P.seg_off_v[seg0 + txs[k]] = sv;
+                    P.seg_off_t[seg0 + txs[k]] = st;
+                }
+                sv += v[k];
+                st += t[k];
+            }
+        }
+    }
+    if (row_ok) {
+        const size_t row = row_index(J, zl, y);
+        P.row_sum[row] = sv;
+        P.row_sum[P.nrows + row] = st;
+    }
+}
 
-    __device__ void set_row()
-    {
-        const uint32_t ty = y >> 3, tzl = zl >> 3;
-        super_row = super_base + ((uint64_t)(tzl / kSuper) * nsy + ty / kSuper) * nsx;
-        bit_row = ((tzl % kSuper) * kSuper + ty % kSuper) * kSuper;
-    }
-    __device__ void load(const SpPass& P, uint64_t i)
-    {
-        uint32_t lo = 0, hi = P.njobs - 1;  // last job with seg_base <= i
-        while (lo < hi) {
-            const uint32_t mid = (lo + hi + 1) / 2;
-            if (P.jobs[mid].seg_base <= i) lo = mid;
-            else hi = mid - 1;
-        }
-        const SpJob& J = P.jobs[lo];
-        ntx = J.ntx;
-        ry = J.ry;
-        nz = J.nz;
-        nsx = J.nsx;
-        nsy = J.nsy;
-        super_base = J.super_base;
-        tx = y = zl = 0;
-        if (i < P.nseg && ntx != 0 && ry != 0) {  // else: past the end, never used
-            const uint32_t li = (uint32_t)(i - J.seg_base);
-            tx = li % ntx;
-            const uint32_t row = li / ntx;
-            y = row % ry;
-            zl = row / ry;
-        }
-        set_row();
-        word_at = ~0ull;
-    }
-    // does the tile of the cursor's segment have a slot (and so a written count)?
-    __device__ bool active(const SpPass& P)
-    {
-        const uint64_t s = super_row + tx / kSuper;
-        if (s != word_at) {
-            word = P.super_act[s];
-            word_at = s;
-        }
-        return (word >> (bit_row + tx % kSuper)) & 1ull;
-    }
-    __device__ bool first() const { return tx == 0 && y == 0 && zl == 0; }
-    __device__ void next(const SpPass& P, uint64_t i_next)
-    {
-        if (++tx < ntx) return;
-        tx = 0;
-        if (++y < ry) {
-            if ((y & 7u) == 0) set_row();
-            return;
-        }
-        y = 0;
-        if (++zl < nz) {
-            set_row();
-            return;
-        }
-        if (i_next < P.nseg) load(P, i_next);  // next job
-    }
-};
-
-// Per block: the sums of kScanChunk consecutive segments' vertex and triangle counts
+// Per block: the sums of kScanChunk consecutive rows' vertex and triangle totals
 // (bsum[block], bsum[nblocks + block]).
-__global__ __launch_bounds__(256) void k_sp_reduce(SpPass P, uint32_t* __restrict__ bsum)
+__global__ __launch_bounds__(256) void k_sp_rowreduce(SpPass P, uint32_t* __restrict__ bsum)
 {
     __shared__ uint32_t lds[2][4];
     const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + thread_id() * kScanPer;
     uint32_t sv = 0, st = 0;
-    if (base < P.nseg) {
-        SegCursor c;
-        c.load(P, base);
-        for (uint32_t k = 0; k < kScanPer && base + k < P.nseg; ++k) {
-            if (c.active(P)) {
-                sv += P.seg_v[base + k];
-                st += P.seg_t[base + k];
-            }
-            c.next(P, base + k + 1);
-        }
+    for (uint32_t k = 0; k < kScanPer && base + k < P.nrows; ++k) {
+        sv += P.row_sum[base + k];
+        st += P.row_sum[P.nrows + base + k];
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
@@ -1289,30 +1283,19 @@ __global__ __launch_bounds__(256) void k_sp_reduce(SpPass P, uint32_t* __restric
             lds[thread_id()][0] + lds[thread_id()][1] + lds[thread_id()][2] + lds[thread_id()][3];
 }
 
-// Exclusive offsets of the segments of tiles with a slot (and of each job's first
-// segment): the block's offsets boff[block] (vertices), boff[nblocks + block]
-// (triangles) plus the block-local scan.
-__global__ __launch_bounds__(256) void k_sp_downsweep(SpPass P, const uint32_t* __restrict__ boff)
+// Exclusive offsets of the rows: the block's offsets boff[block] (vertices),
+// boff[nblocks + block] (triangles) plus the block-local scan.
+__global__ __launch_bounds__(256) void k_sp_rowdown(SpPass P, const uint32_t* __restrict__ boff)
 {
     __shared__ uint32_t lds[2][4];
     const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + thread_id() * kScanPer;
-    uint32_t v[kScanPer], t[kScanPer], sv = 0, st = 0, act = 0, first = 0;
-    if (base < P.nseg) {
-        SegCursor c;
-        c.load(P, base);
-        for (uint32_t k = 0; k < kScanPer; ++k) {
-            v[k] = t[k] = 0;
-            if (base + k >= P.nseg) continue;
-            if (c.active(P)) {
-                v[k] = P.seg_v[base + k];
-                t[k] = P.seg_t[base + k];
-                act |= 1u << k;
-            }
-            if (c.first()) first |= 1u << k;
-            sv += v[k];
-            st += t[k];
-            c.next(P, base + k + 1);
-        }
+    uint32_t v[kScanPer], t[kScanPer], sv = 0, st = 0;
+    for (uint32_t k = 0; k < kScanPer; ++k) {
+        const bool in = base + k < P.nrows;
+        v[k] = in ? P.row_sum[base + k] : 0u;
+        t[k] = in ? P.row_sum[P.nrows + base + k] : 0u;
+        sv += v[k];
+        st += t[k];
     }
     uint32_t iv = sv, it = st;
 #pragma unroll
@@ -1333,12 +1316,9 @@ __global__ __launch_bounds__(256) void k_sp_downsweep(SpPass P, const uint32_t* 
         rv += lds[0][w];
         rt += lds[1][w];
     }
-    if (base >= P.nseg) return;
-    for (uint32_t k = 0; k < kScanPer; ++k) {
-        if ((act | first) & (1u << k)) {
-            P.seg_off_v[base + k] = rv;
-            P.seg_off_t[base + k] = rt;
-        }
+    for (uint32_t k = 0; k < kScanPer && base + k < P.nrows; ++k) {
+        P.row_off[base + k] = rv;
+        P.row_off[P.nrows + base + k] = rt;
         rv += v[k];
         rt += t[k];
     }
@@ -1352,11 +1332,11 @@ __global__ void k_sp_ranges(SpPass P)
     if (j >= P.njobs) return;
     const SpJob& J = P.jobs[j];
     if (J.z0 != 0) return;
-    // a job with no row segment (empty grid) starts where the pass's output starts
-    const bool empty = (uint64_t)J.nz * J.ry * J.ntx == 0;
+    // a job with no row (empty grid) starts where the pass's output starts
+    const bool empty = (uint64_t)J.nz * J.ry == 0;
     uint64_t* o = P.job_out + (size_t)J.out_index * kJobOut;
-    o[0] = empty ? P.counters[kVtxPrev] : (uint64_t)P.seg_off_v[J.seg_base];
-    o[1] = empty ? P.counters[kTriPrev] : (uint64_t)P.seg_off_t[J.seg_base];
+    o[0] = empty ? P.counters[kVtxPrev] : (uint64_t)P.row_off[J.row_base];
+    o[1] = empty ? P.counters[kTriPrev] : (uint64_t)P.row_off[P.nrows + J.row_base];
 }
 
 // Cell -> vertex map of the slab's last plane, for the next slab (single job).
@@ -1425,7 +1405,7 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
                               int boundary_plane, hipEvent_t* marks)
 {
     if (P.ntiles == 0) return hipSuccess;
-    const uint64_t nsb64 = (P.nseg + kScanChunk - 1) / kScanChunk;
+    const uint64_t nsb64 = (P.nrows + kScanChunk - 1) / kScanChunk;
     if (nsb64 > kMaxScanBlocks) return hipErrorInvalidValue;
     const uint32_t nsb = (uint32_t)nsb64;
     const uint32_t jb = (P.njobs + 63) / 64;
@@ -1455,18 +1435,21 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
     hipLaunchKernelGGL(k_sp_quadcount, dim3(resident_blocks(k_sp_quadcount, 64, 0, cus, big)), dim3(64), 0, s,
                        P, b);
     mark(3);
+    if (P.ntrows) hipLaunchKernelGGL(k_sp_rowsum, dim3((uint32_t)P.ntrows), dim3(64), 0, s, P);
     if (nsb) {
-        hipLaunchKernelGGL(k_sp_reduce, dim3(nsb), dim3(256), 0, s, P, b.bcount);
+        hipLaunchKernelGGL(k_sp_rowreduce, dim3(nsb), dim3(256), 0, s, P, b.bcount);
         MeshBufs bt = b;  // triangle block sums and offsets follow the vertex ones
         bt.bcount += nsb;
         bt.boffset += nsb;
         (void)launch_scan(s, b, nsb, 0);
         (void)launch_scan(s, bt, nsb, 1);
-        hipLaunchKernelGGL(k_sp_downsweep, dim3(nsb), dim3(256), 0, s, P, b.boffset);
+        hipLaunchKernelGGL(k_sp_rowdown, dim3(nsb), dim3(256), 0, s, P, b.boffset);
     }
     hipLaunchKernelGGL(k_sp_ranges, dim3(jb), dim3(64), 0, s, P);
     mark(4);
-    hipLaunchKernelGGL(k_sp_emit, dim3(resident_blocks(k_sp_emit, 64, 0, cus, big)), dim3(64), 0, s, P, b);
+    hipLaunchKernelGGL((k_sp_emit<1>), dim3(resident_blocks(k_sp_emit<1>, 64, 0, cus, big)), dim3(64), 0, s, P, b);
+    if (general_tiles)
+        hipLaunchKernelGGL((k_sp_emit<0>), dim3(resident_blocks(k_sp_emit<0>, 64, 0, cus, big)), dim3(64), 0, s, P, b);
     if (boundary_plane) {
         hipLaunchKernelGGL(k_sp_boundary, dim3((uint32_t)((P.boundary_cells + 255) / 256)), dim3(256), 0,
                            s, P);
