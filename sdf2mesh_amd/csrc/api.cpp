@@ -126,7 +126,7 @@ struct s2m_ctx {
     DevBuf lattice, vidx, bcount, boffset, counters, tables;
     DevBuf vtx, keys, signs, tri, corners;
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
-    DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_seg, sp_boundary;
+    DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_slot_rows, sp_boundary;
     DevBuf sp_stage, sp_slot_stage, sp_slot_live;
     DevBuf sp_job_out, sp_queues, sp_super_iv, sp_super_lists, sp_super_lens, sp_super_act, sp_rows;
     uint64_t slot_cap = 0;
@@ -464,7 +464,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     uint32_t list_stride = 1, box_stride = 1, prune_depth = 1;
     bool general_tiles = false;  // a tile's pruned program may not be a left fold
     bool fold_jobs = false, other_jobs = false;  // programs that are / are not union folds
-    uint64_t ntiles_all = 0, nseg_all = 0, plane_max = 0, cells_all = 0, planes_sum = 0;
+    uint64_t ntiles_all = 0, nrows_all = 0, plane_max = 0, cells_all = 0, planes_sum = 0;
     for (uint32_t i = 0; i < n; ++i) {
         build_sp_tables(jobs[i], progs[i], ps[i]->eps, tabs[i]);
         tab_off[i] = tab_bytes;
@@ -479,7 +479,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             else --depth;
         }
         ntiles_all += (uint64_t)tabs[i].nt[0] * tabs[i].nt[1] * tabs[i].nt[2];
-        nseg_all += (uint64_t)jobs[i].nz_cells * jobs[i].res[1] * tabs[i].nt[0];
+        nrows_all += (uint64_t)jobs[i].nz_cells * jobs[i].res[1];
         plane_max = std::max<uint64_t>(plane_max, (uint64_t)jobs[i].res[0] * jobs[i].res[1]);
         cells_all += (uint64_t)jobs[i].res[0] * jobs[i].res[1] * jobs[i].nz_cells;
         planes_sum += (uint64_t)jobs[i].res[0] * jobs[i].res[1];
@@ -503,10 +503,9 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     }
     const uint32_t npass = n == 1 ? (tabs[0].nt[2] + layers - 1) / std::max(layers, 1u) : 1;
     const uint64_t tiles_pass = n == 1 ? (uint64_t)tabs[0].nt[0] * tabs[0].nt[1] * layers : ntiles_all;
-    const uint64_t seg_pass =
-        n == 1 ? (uint64_t)layers * kTile * jobs[0].res[1] * tabs[0].nt[0] : nseg_all;
-    if (tiles_pass > 0x7fffffffull || seg_pass > 0xffffffffull ||
-        (seg_pass + 4095) / 4096 > kMaxScanBlocks)
+    const uint64_t rows_pass = n == 1 ? (uint64_t)layers * kTile * jobs[0].res[1] : nrows_all;
+    if (tiles_pass > 0x7fffffffull || rows_pass > 0xffffffffull ||
+        (rows_pass + 4095) / 4096 > kMaxScanBlocks)
         return fail(S2M_E_ARG, "grid too large for one pass; mesh the jobs separately");
 
     // device tables + job descriptors (one per pass for a single job's slabs, else per job)
@@ -521,7 +520,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     HIP_TRY(ctx->tile_tab.ensure(img_bytes));
     uint8_t* dimg = ctx->tile_tab.as<uint8_t>();
     std::vector<SpJob> desc(njob_desc);
-    uint64_t tile_base = 0, seg_base = 0, super_base = 0, nsuper_pass = 0, row_base = 0, trow_base = 0;
+    uint64_t tile_base = 0, super_base = 0, nsuper_pass = 0, row_base = 0, trow_base = 0;
     uint64_t nrows_pass = 0, ntrows_pass = 0;
     for (size_t d = 0; d < njob_desc; ++d) {
         const uint32_t i = n == 1 ? 0 : (uint32_t)d;
@@ -536,7 +535,6 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             J.nz = std::min(j.nz_cells - std::min(j.nz_cells, J.z0), layers * kTile);
             J.tz0 = (uint32_t)d * layers;
             J.tile_base = 0;
-            J.seg_base = 0;
             J.row_base = 0;
             J.trow_base = 0;
         } else {
@@ -544,11 +542,9 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             J.nz = j.nz_cells;
             J.tz0 = 0;
             J.tile_base = tile_base;
-            J.seg_base = seg_base;
             J.row_base = row_base;
             J.trow_base = trow_base;
             tile_base += (uint64_t)T.nt[0] * T.nt[1] * T.nt[2];
-            seg_base += (uint64_t)j.nz_cells * j.res[1] * T.nt[0];
             row_base += (uint64_t)j.nz_cells * j.res[1];
             trow_base += (uint64_t)T.nt[2] * T.nt[1];
         }
@@ -602,7 +598,6 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     HIP_TRY(ctx->tile_lens.ensure(tiles_pass * sizeof(uint32_t)));
     HIP_TRY(ctx->tile_flops.ensure(tiles_pass * sizeof(uint32_t)));
     HIP_TRY(ctx->sp_slot_of_tile.ensure(tiles_pass * sizeof(int32_t)));
-    HIP_TRY(ctx->sp_seg.ensure(std::max<uint64_t>(seg_pass, 1) * 4 * sizeof(uint32_t)));
     HIP_TRY(ctx->sp_rows.ensure(std::max<uint64_t>(nrows_pass, 1) * 4 * sizeof(uint32_t)));
     const uint32_t nsb_max = (uint32_t)((nrows_pass + 4095) / 4096) + 1;  // vertex and triangle sums
     HIP_TRY(ctx->bcount.ensure((size_t)2 * nsb_max * sizeof(uint32_t)));
@@ -632,6 +627,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         HIP_TRY(ctx->sp_tile_of_slot.ensure(want_s * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_job_of_slot.ensure(want_s * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_cellbits.ensure(want_s * 16 * sizeof(uint32_t)));
+        HIP_TRY(ctx->sp_slot_rows.ensure(want_s * 4 * 64 * sizeof(uint32_t)));
         // staged cells are allocated in per-block chunks: about 2 x vertices + blocks x chunk
         // (two launches); the device count is checked below
         stage_cap = std::max(stage_cap, 2 * want_v + 2 * tile_blocks * kStageChunkHost);
@@ -675,13 +671,11 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             if (n == 1) {
                 const SpJob& J = desc[pass];
                 P.ntiles = (uint64_t)J.ntx * J.nty * ((J.nz + kTile - 1) / kTile);
-                P.nseg = (uint64_t)J.nz * J.ry * J.ntx;
                 P.nrows = (uint64_t)J.nz * J.ry;
                 P.ntrows = (uint64_t)((J.nz + kTile - 1) / kTile) * J.nty;
                 P.boundary_cells = (uint64_t)J.rx * J.ry;
             } else {
                 P.ntiles = ntiles_all;
-                P.nseg = nseg_all;
                 P.nrows = nrows_pass;
                 P.ntrows = ntrows_pass;
             }
@@ -704,10 +698,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.stage_cap = stage_cap;
             P.slot_stage = ctx->sp_slot_stage.as<uint32_t>();
             P.slot_live = ctx->sp_slot_live.as<uint64_t>();
-            P.seg_v = ctx->sp_seg.as<uint32_t>();
-            P.seg_t = P.seg_v + P.nseg;
-            P.seg_off_v = P.seg_t + P.nseg;
-            P.seg_off_t = P.seg_off_v + P.nseg;
+            P.slot_rows = ctx->sp_slot_rows.as<uint32_t>();
             P.row_sum = ctx->sp_rows.as<uint32_t>();
             P.row_off = P.row_sum + 2 * P.nrows;
             P.counters = b.counters;
@@ -1116,7 +1107,7 @@ void s2m_destroy(s2m_ctx* ctx)
                       &ctx->tables, &ctx->vtx, &ctx->keys, &ctx->signs, &ctx->tri, &ctx->corners,
                       &ctx->tile_tab, &ctx->tile_lists, &ctx->tile_lens,
                       &ctx->tile_flops, &ctx->sp_slot_of_tile, &ctx->sp_tile_of_slot,
-                      &ctx->sp_job_of_slot, &ctx->sp_cellbits, &ctx->sp_seg,
+                      &ctx->sp_job_of_slot, &ctx->sp_cellbits, &ctx->sp_slot_rows,
                       &ctx->sp_boundary, &ctx->sp_job_out, &ctx->sp_queues, &ctx->sp_super_iv,
                       &ctx->sp_super_lists, &ctx->sp_super_lens, &ctx->sp_super_act, &ctx->sp_rows,
                       &ctx->sp_stage, &ctx->sp_slot_stage, &ctx->sp_slot_live})

@@ -91,7 +91,6 @@ struct SpJob {
     float eps;
     double pad;                // tile box padding (normal taps, rounding)
     uint64_t tile_base;        // first tile of the job in the pass's numbering
-    uint64_t seg_base;         // first row segment of the job
     uint64_t row_base;         // first row (zl, y) of the job
     uint64_t trow_base;        // first tile row (tile layer, ty) of the job
     uint64_t super_base;       // first super-tile of the job
@@ -118,7 +117,6 @@ struct SpPass {
     uint32_t list_stride;      // >= nops of every job
     uint32_t prune_depth;      // >= 1, >= the deepest operand stack of any job's program - 1
     uint64_t ntiles;           // tiles of the pass
-    uint64_t nseg;             // row segments of the pass
     uint64_t nrows;            // rows (z, y) of the pass
     uint64_t ntrows;           // tile rows (tile layer, ty) of the pass
     uint64_t boundary_cells;   // rx * ry of jobs[0] when a boundary plane is written
@@ -141,10 +139,8 @@ struct SpPass {
     uint64_t stage_cap;
     uint32_t* slot_stage;      // [slot][2] first staged cell, staged cells
     uint64_t* slot_live;       // [slot] bricks (2x2x2 cells) that may hold an active cell
-    uint32_t* seg_v;           // [nseg] vertices per row segment
-    uint32_t* seg_t;           // triangles per row segment
-    uint32_t* seg_off_v;       // first vertex of the segment within its row
-    uint32_t* seg_off_t;       // first triangle of the segment within its row
+    uint32_t* slot_rows;       // [4][slot_cap][64] per row of each slot's tile: vertices,
+                               // triangles, first vertex / triangle within the row
     uint32_t* row_sum;         // [2][nrows] vertices, triangles per row
     uint32_t* row_off;         // [2][nrows] exclusive offsets: global vertex / triangle index
     uint64_t* counters;

@@ -78,9 +78,13 @@ __device__ __forceinline__ uint32_t queue_take(const SpPass& P, uint32_t q, uint
 }
 
 // Row segment of row (zl, y) and tile column tx of job J.
-__device__ __forceinline__ size_t seg_index(const SpJob& J, uint32_t zl, uint32_t y, uint32_t tx)
+// Per-slot row data: array k (kRowVtx, kRowTri: counts; kRowVtxOff, kRowTriOff: first
+// vertex / triangle of the row segment within its row) of slot `slot`, row r = (z % 8) * 8
+// + y % 8 of the tile.
+enum RowArray : uint32_t { kRowVtx = 0, kRowTri = 1, kRowVtxOff = 2, kRowTriOff = 3 };
+__device__ __forceinline__ uint32_t* slot_row(const SpPass& P, uint32_t k, uint32_t slot)
 {
-    return J.seg_base + ((size_t)zl * J.ry + y) * J.ntx + tx;
+    return P.slot_rows + ((size_t)k * P.slot_cap + slot) * 64u;
 }
 
 // Row (zl, y) of job J in the pass's row numbering (rows job after job, zl then y).
@@ -140,7 +144,7 @@ __device__ uint32_t nb_row(const SpPass& P, const SpJob& J, uint32_t tx, int y, 
         if (c ? !(b8 & 0x80u) : !b8) continue;
         bits |= c ? 1u : b8 << 1;
         if (INDEX) {
-            const uint32_t first = P.seg_off_v[seg_index(J, zl, (uint32_t)y, t)] + P.row_off[row_index(J, zl, (uint32_t)y)];
+            const uint32_t first = slot_row(P, kRowVtxOff, (uint32_t)slot)[r] + P.row_off[row_index(J, zl, (uint32_t)y)];
             if (c) {
                 idx[0] = first + (uint32_t)__builtin_popcount(b8 & 0x7fu);
             } else {
@@ -164,7 +168,7 @@ __device__ __forceinline__ uint32_t vertex_at(const SpPass& P, const SpJob& J, u
     if (slot < 0) return kNone;
     const uint32_t b8 = slot_row_bits(P, slot, (z & 7u) * 8u + (y & 7u)), i = x & 7u;
     if (!((b8 >> i) & 1u)) return kNone;
-    return P.seg_off_v[seg_index(J, z - J.z0, y, x >> 3)] + P.row_off[row_index(J, z - J.z0, y)] +
+    return slot_row(P, kRowVtxOff, (uint32_t)slot)[(z & 7u) * 8u + (y & 7u)] + P.row_off[row_index(J, z - J.z0, y)] +
            (uint32_t)__builtin_popcount(b8 & ((1u << i) - 1u));
 }
 
@@ -677,8 +681,8 @@ __global__ __launch_bounds__(64) void k_sp_bricks(SpPass P)
 // `live` are skipped: their corner values keep one sign and were not evaluated); each
 // listed cell gets its vertex (dualcontour.wgsl:72-131) and sign bits (:57-69); the
 // cells with a vertex are staged compactly for k_sp_emit with their rank in their row
-// segment, their bits go to the slot's `cellbits` and each row segment's count to
-// seg_v.  Contains barriers.
+// segment, their bits go to the slot's `cellbits` and each row segment's count to the
+// slot's kRowVtx row.  Contains barriers.
 // ---------------------------------------------------------------------------------
 // Staged cells are allocated in chunks of kStageChunk per block (one counter update per
 // chunk); a slot's cells are contiguous, so a chunk with fewer than the slot's cells left
@@ -784,8 +788,8 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
         P.slot_stage[2 * slot + 1] = n;
     }
     if (thread_id() < 64) {
-        const uint32_t row = thread_id(), y = ty * 8u + (row & 7u), zl = tzl * 8u + (row >> 3);
-        if (y < J.ry && zl < J.nz) P.seg_v[seg_index(J, zl, y, tx)] = (uint32_t)__builtin_popcount(L.rows[row]);
+        const uint32_t row = thread_id();
+        slot_row(P, kRowVtx, slot)[row] = (uint32_t)__builtin_popcount(L.rows[row]);  // 0 outside the grid
         if (row < 16)
             P.cellbits[(size_t)slot * 16u + row] = L.rows[4 * row] | (L.rows[4 * row + 1] << 8) |
                                                    (L.rows[4 * row + 2] << 16) | (L.rows[4 * row + 3] << 24);
@@ -1002,7 +1006,8 @@ __device__ __forceinline__ uint32_t cell_quads(uint32_t s, uint32_t x, uint32_t 
 // ---------------------------------------------------------------------------------
 // k_sp_quadcount: per slot (one 64-lane block iteration), the valid quads of the tile's
 // cells with a vertex (their staged entries), two triangles each, summed per row segment
-// into seg_t; the invalid ones are added to the pass's and the job's counts.
+// into the slot's kRowTri row; the invalid ones are added to the pass's and the job's
+// counts.
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void k_sp_quadcount(SpPass P, MeshBufs b)
 {
@@ -1038,8 +1043,8 @@ __global__ __launch_bounds__(64) void k_sp_quadcount(SpPass P, MeshBufs b)
         }
         __syncthreads();
         {
-            const uint32_t row = thread_id(), y = ty * 8u + (row & 7u), zl = tzl * 8u + (row >> 3);
-            if (y < J.ry && zl < J.nz) P.seg_t[seg_index(J, zl, y, tx)] = rowt[row];
+            const uint32_t row = thread_id();
+            slot_row(P, kRowTri, slot)[row] = rowt[row];  // 0 outside the grid
         }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) invalid += __shfl_xor(invalid, d, 64);
@@ -1153,7 +1158,7 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
             const uint32_t nq = cell_quads(sg, tx * 8u + cxl, y, J.z0 + zl, lag, exists, invalid);
             if (!nq) continue;
             const uint64_t row2 = ((uint64_t)tc[(c >> 3) * 2u + 1] << 32) | tc[(c >> 3) * 2u];  // 8 counts
-            uint64_t t = (uint64_t)P.seg_off_t[seg_index(J, zl, y, tx)] + P.row_off[P.nrows + row_index(J, zl, y)];
+            uint64_t t = (uint64_t)slot_row(P, kRowTriOff, slot)[c >> 3] + P.row_off[P.nrows + row_index(J, zl, y)];
             for (uint32_t i = 0; i < cxl; ++i) t += (row2 >> (8u * i)) & 0xffu;
             const uint32_t v = nbv(0, 0, 0);
             const bool c0 = sg & 1u, c1 = (sg >> 1) & 1u, c2 = (sg >> 2) & 1u;
@@ -1191,9 +1196,9 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
 //  k_sp_rowsum    one wave per tile row (job, tile layer, ty), lane = row (z % 8, y % 8):
 //                 the row segments of the tile row's active tiles (the super-tiles'
 //                 activity words) in x order: each segment's offset within its row
-//                 (seg_off_v / seg_off_t) and the row's totals (row_sum)
+//                 (the slot's kRowVtxOff / kRowTriOff rows) and the row's totals (row_sum)
 //  k_sp_rowreduce / k_scan / k_sp_rowdown   exclusive scan of the rows' totals (row_off)
-// A segment's first vertex (triangle) is row_off[row] + seg_off[segment].
+// A segment's first vertex (triangle) is row_off[row] + its offset within the row.
 // ---------------------------------------------------------------------------------
 constexpr uint32_t kScanPer = 16;                  // consecutive rows per thread
 constexpr uint32_t kScanChunk = 256 * kScanPer;
@@ -1213,7 +1218,7 @@ __global__ __launch_bounds__(64) void k_sp_rowsum(SpPass P)
     const bool row_ok = zl < J.nz && y < J.ry;
     const uint64_t srow = J.super_base + ((uint64_t)(tzl / kSuper) * J.nsy + ty / kSuper) * J.nsx;
     const uint32_t bit0 = ((tzl % kSuper) * kSuper + ty % kSuper) * kSuper;
-    const size_t seg0 = row_ok ? seg_index(J, zl, y, 0) : 0;
+    const uint64_t trow_tile = J.tile_base + ((uint64_t)tzl * J.nty + ty) * J.ntx;  // tile (0, ty, tzl)
     uint32_t sv = 0, st = 0;
     for (uint32_t sx0 = 0; sx0 < J.nsx; sx0 += 16u) {
         // active tiles 4 * sx0 .. 4 * sx0 + 63: lane l < 16 reads super-tile sx0 + l's word
@@ -1223,27 +1228,33 @@ __global__ __launch_bounds__(64) void k_sp_rowsum(SpPass P)
 #pragma unroll
         for (int d = 8; d >= 1; d >>= 1) m |= __shfl_xor(m, d, 64);
         m = __shfl(m, 0, 64);
-        // the active tiles' counts, 8 at a time (independent loads), then the running sums
+        // the active tiles' row counts (one 256-byte row of each slot), 8 tiles at a time
         while (m) {
-            uint32_t txs[8], v[8], t[8];
+            uint32_t slots[8], v[8], t[8];
             uint32_t nb = 0;
-            for (; nb < 8u && m; ++nb) {
-                txs[nb] = 4u * sx0 + (uint32_t)__builtin_ctzll(m);
-                m &= m - 1ull;
-            }
 #pragma unroll
             for (uint32_t k = 0; k < 8u; ++k) {
-                v[k] = t[k] = 0;
-                if (k < nb && row_ok) {
-                    v[k] = P.seg_v[seg0 + txs[k]];
-                    t[k] = P.seg_t[seg0 + txs[k]];
+                slots[k] = 0;
+                if (m) {
+                    const uint32_t tx = 4u * sx0 + (uint32_t)__builtin_ctzll(m);
+                    m &= m - 1ull;
+                    slots[k] = (uint32_t)__builtin_amdgcn_readfirstlane(P.slot_of_tile[trow_tile + tx]);
+                    nb = k + 1;
                 }
             }
 #pragma unroll
             for (uint32_t k = 0; k < 8u; ++k) {
-                if (k < nb && row_ok) {
-                    P.seg_off_v[seg0 + txs[k]] = sv;
-                    P.seg_off_t[seg0 + txs[k]] = st;
+                v[k] = t[k] = 0;
+                if (k < nb) {
+                    v[k] = slot_row(P, kRowVtx, slots[k])[lane()];
+                    t[k] = slot_row(P, kRowTri, slots[k])[lane()];
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < 8u; ++k) {
+                if (k < nb) {
+                    slot_row(P, kRowVtxOff, slots[k])[lane()] = sv;
+                    slot_row(P, kRowTriOff, slots[k])[lane()] = st;
                 }
                 sv += v[k];
                 st += t[k];
@@ -1325,7 +1336,7 @@ __global__ __launch_bounds__(256) void k_sp_rowdown(SpPass P, const uint32_t* __
 }
 
 // First vertex and first triangle index of every job whose first slab this pass holds:
-// the exclusive offsets of its first row segment.
+// the exclusive offsets of its first row.
 __global__ void k_sp_ranges(SpPass P)
 {
     const uint32_t j = blockIdx.x * blockDim.x + thread_id();
