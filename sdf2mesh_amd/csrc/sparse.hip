@@ -808,6 +808,7 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
 }
 
 constexpr uint32_t kChunk = 1024;  // box points examined per compaction round
+constexpr uint32_t kList = 2048;   // needed points held before they are evaluated
 
 // k_sp_tiles: per slot (persistent, slots drawn from a work queue), the tile's pruned
 // program, staged in
@@ -824,7 +825,7 @@ __global__ __launch_bounds__(kTileThreads) void k_sp_tiles(SpPass P)
     __shared__ TileAxes A;
     __shared__ uint32_t bm[3][32];   // per axis and box index: the bricks (along it) using it
     __shared__ uint8_t rowm[32 * 32];  // per box row (iz, iy): x-bricks of the live bricks using it
-    __shared__ uint16_t list[kChunk];
+    __shared__ uint16_t list[kList];
     __shared__ uint32_t nlist;
     __shared__ float pc[3][32];  // lattice coordinates of the box per axis
     __shared__ ContourLds L;
@@ -906,11 +907,13 @@ __global__ __launch_bounds__(kTileThreads) void k_sp_tiles(SpPass P)
         };
         uint32_t evaluated = npts;
         if (LINEAR) {
+            // the points that some live brick uses, compacted into `list` chunk by chunk and
+            // evaluated when the list holds more than a chunk's worth (or at the end), so
+            // evaluation rounds run with full waves
             evaluated = 0;
+            if (thread_id() == 0) nlist = 0;
+            __syncthreads();
             for (uint32_t c0 = 0; c0 < npts; c0 += kChunk) {
-                if (thread_id() == 0) nlist = 0;
-                __syncthreads();
-                // the chunk's points that some live brick uses, compacted into `list`
                 const uint32_t cn = min(kChunk, npts - c0);
                 for (uint32_t k = 0; k < cn; k += kTileThreads) {
                     const uint32_t q = c0 + k + thread_id();
@@ -924,11 +927,12 @@ __global__ __launch_bounds__(kTileThreads) void k_sp_tiles(SpPass P)
                     if (lane() == 0 && w) wbase = atomicAdd(&nlist, (uint32_t)__builtin_popcountll(w));
                     wbase = __shfl(wbase, 0, 64);
                     if (need)
-                        list[wbase + (uint32_t)__builtin_popcountll(w & ((1ull << lane()) - 1ull))] =
-                            (uint16_t)(q - c0);
+                        list[wbase + (uint32_t)__builtin_popcountll(w & ((1ull << lane()) - 1ull))] = (uint16_t)q;
                 }
                 __syncthreads();
                 const uint32_t m = nlist;
+                __syncthreads();  // every wave has read nlist before the next chunk appends to it
+                if (m <= kList - kChunk && c0 + kChunk < npts) continue;  // room for another chunk
                 evaluated += m;
                 for (uint32_t i0 = thread_id(); i0 < m; i0 += kLinearPoints * blockDim.x) {
                     float x[kLinearPoints], y[kLinearPoints], z[kLinearPoints], v[kLinearPoints];
@@ -936,14 +940,16 @@ __global__ __launch_bounds__(kTileThreads) void k_sp_tiles(SpPass P)
 #pragma unroll
                     for (int p = 0; p < kLinearPoints; ++p) {
                         const uint32_t i = i0 + p * blockDim.x;
-                        at[p] = point(c0 + list[i < m ? i : i0], x[p], y[p], z[p]);
+                        at[p] = point(list[i < m ? i : i0], x[p], y[p], z[p]);
                     }
                     run_linear<kLinearPoints>(T, n, x, y, z, v);
 #pragma unroll
                     for (int p = 0; p < kLinearPoints; ++p)
                         if (i0 + p * blockDim.x < m) box[at[p]] = v[p];
                 }
-                __syncthreads();  // list and nlist are reused by the next chunk
+                __syncthreads();  // every lane's reads of the list and of nlist are done
+                if (thread_id() == 0) nlist = 0;
+                __syncthreads();
             }
         } else {
             for (uint32_t q = thread_id(); q < npts; q += blockDim.x) {
