@@ -111,6 +111,14 @@ struct SpJob {
     uint32_t* boundary;        // [ry][rx] vertex of cell plane z0 - 1 (multi-slab job)
 };
 
+// Per slot, written by k_sp_bricks for the kernels after it.
+struct SlotHdr {
+    uint32_t tile, job, lens;
+    uint32_t txy;    // tile x | tile y << 16
+    uint32_t tzl;    // tile layer within the pass
+    uint32_t box[3]; // per axis: first lattice index of the tile's box | points << 24
+};
+
 struct SpPass {
     const SpJob* jobs;
     uint32_t njobs;
@@ -139,6 +147,7 @@ struct SpPass {
     uint64_t stage_cap;
     uint32_t* slot_stage;      // [slot][2] first staged cell, staged cells
     uint64_t* slot_live;       // [slot] bricks (2x2x2 cells) that may hold an active cell
+    SlotHdr* slot_hdr;         // [slot]
     uint32_t* slot_rows;       // [4][slot_cap][64] per row of each slot's tile: vertices,
                                // triangles, first vertex / triangle within the row
     uint32_t* row_sum;         // [2][nrows] vertices, triangles per row

@@ -641,15 +641,31 @@ __global__ __launch_bounds__(64) void k_sp_bricks(SpPass P)
         const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
         const uint32_t lens = __builtin_amdgcn_readfirstlane(P.lens[tile]);
         const uint32_t n = lens & ~kLinearFlag;
+        const uint32_t jid = __builtin_amdgcn_readfirstlane(P.job_of_slot[slot]);
+        const SpJob& J = P.jobs[jid];
+        uint32_t tx, ty, tzl;
+        tile_xyz(J, tile, tx, ty, tzl);
+        if (b < 3) {  // the slot's header: lane a writes axis a's box origin and size
+            const uint32_t a = b, t = a == 0 ? tx : a == 1 ? ty : J.tz0 + tzl;
+            const uint32_t nt = a == 0 ? J.ntx : a == 1 ? J.nty : J.ntz;
+            const uint32_t* ib = J.ibox + 2 * (a == 0 ? 0 : a == 1 ? J.ntx : J.ntx + J.nty);
+            const uint32_t o = ib[t], e = ib[nt + t];
+            SlotHdr& H = P.slot_hdr[slot];
+            H.box[a] = o | ((e - o + 1) << 24);
+            if (a == 0) {
+                H.tile = (uint32_t)tile;
+                H.job = jid;
+                H.lens = lens;
+                H.txy = tx | (ty << 16);
+                H.tzl = tzl;
+            }
+        }
         if (!(lens & kLinearFlag) || n > kStageOps) {  // every brick is kept
             if (b == 0) P.slot_live[slot] = ~0ull;
             continue;
         }
-        const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(P.job_of_slot[slot])];
         __syncthreads();  // the previous slot's readers of T are done
         stage_tape(T, J.ops, P.lists + (size_t)tile * P.list_stride, n);
-        uint32_t tx, ty, tzl;
-        tile_xyz(J, tile, tx, ty, tzl);
         const uint32_t t[3] = {tx, ty, J.tz0 + tzl};
         const uint32_t r[3] = {J.rx, J.ry, J.z0 + J.nz};  // cells on each axis (z: of the job)
         const uint32_t* lo[3] = {J.lox, J.loy, J.loz};
@@ -843,23 +859,19 @@ __global__ __launch_bounds__(kTileThreads) void k_sp_tiles(SpPass P)
         const uint32_t slot = sh_slot;
         if (slot >= nslots) break;
         if (thread_id() == 0) next = queue_take(P, kQueue, blockIdx.x, gridDim.x);  // prefetched
-        const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
-        const uint32_t lens = __builtin_amdgcn_readfirstlane(P.lens[tile]);
+        const SlotHdr& H = P.slot_hdr[__builtin_amdgcn_readfirstlane(slot)];
+        const uint32_t lens = __builtin_amdgcn_readfirstlane(H.lens);
         if (((lens & kLinearFlag) != 0) != (LINEAR != 0)) continue;  // same for the whole block
+        const uint64_t tile = __builtin_amdgcn_readfirstlane(H.tile);
         const uint32_t n = lens & ~kLinearFlag;
-        const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(P.job_of_slot[slot])];
-        uint32_t tx, ty, tzl;
-        tile_xyz(J, tile, tx, ty, tzl);
+        const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(H.job)];
+        const uint32_t txy = __builtin_amdgcn_readfirstlane(H.txy);
+        const uint32_t tx = txy & 0xffffu, ty = txy >> 16, tzl = __builtin_amdgcn_readfirstlane(H.tzl);
         const uint32_t tz = J.tz0 + tzl;
-        const uint32_t* bxr = J.ibox;
-        const uint32_t* byr = bxr + 2 * J.ntx;
-        const uint32_t* bzr = byr + 2 * J.nty;
-        const uint32_t x0 = __builtin_amdgcn_readfirstlane(bxr[tx]);
-        const uint32_t y0 = __builtin_amdgcn_readfirstlane(byr[ty]);
-        const uint32_t z0 = __builtin_amdgcn_readfirstlane(bzr[tz]);
-        const uint32_t nx = __builtin_amdgcn_readfirstlane(bxr[J.ntx + tx]) - x0 + 1;
-        const uint32_t ny = __builtin_amdgcn_readfirstlane(byr[J.nty + ty]) - y0 + 1;
-        const uint32_t nz = __builtin_amdgcn_readfirstlane(bzr[J.ntz + tz]) - z0 + 1;
+        const uint32_t bxh = __builtin_amdgcn_readfirstlane(H.box[0]), byh = __builtin_amdgcn_readfirstlane(H.box[1]);
+        const uint32_t bzh = __builtin_amdgcn_readfirstlane(H.box[2]);
+        const uint32_t x0 = bxh & 0xffffffu, y0 = byh & 0xffffffu, z0 = bzh & 0xffffffu;
+        const uint32_t nx = bxh >> 24, ny = byh >> 24, nz = bzh >> 24;
         const uint32_t nxy = nx * ny, npts = nxy * nz;
         const uint32_t mx = magic16(nx), mxy = magic16(nxy);
         const uint16_t* Lst = P.lists + (size_t)tile * P.list_stride;
