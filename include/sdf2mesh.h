@@ -166,7 +166,10 @@ typedef struct s2m_device_mesh {
 } s2m_device_mesh;
 
 /* Per-stage device time of the last meshing call, from HIP events recorded on the
- * context's stream (milliseconds).  Filled only when profiling is on. */
+ * context's stream (milliseconds).  Filled only when profiling is on: level 1 records the
+ * whole call and the evaluation launches (total_ms, eval_ms; the pruned path's other
+ * stage fields stay 0), level 2 every stage (each event adds a marker between kernels,
+ * so level 2 costs a little device time).  Counts are filled at every level. */
 typedef struct s2m_timing {
     double total_ms;    /* first kernel start to last kernel end                   */
     double eval_ms;     /* lattice evaluation kernels                               */
@@ -193,7 +196,7 @@ typedef struct s2m_program s2m_program;
 s2m_ctx* s2m_create(int device);
 void s2m_destroy(s2m_ctx* ctx);
 const char* s2m_last_error(void);
-int s2m_set_profiling(s2m_ctx* ctx, int on);
+int s2m_set_profiling(s2m_ctx* ctx, int level);  /* 0 off, 1 total + evaluation, 2 every stage */
 int s2m_last_timing(const s2m_ctx* ctx, s2m_timing* out);
 
 /* ---- programs ------------------------------------------------------------------- */

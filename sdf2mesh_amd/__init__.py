@@ -371,8 +371,9 @@ class Context:
         except Exception:
             pass
 
-    def set_profiling(self, on: bool = True) -> None:
-        _check(lib().s2m_set_profiling(C.c_void_p(self.handle), int(on)))
+    def set_profiling(self, level: int = 1) -> None:
+        """0 off; 1 (or True): whole call and evaluation launches; 2: every stage."""
+        _check(lib().s2m_set_profiling(C.c_void_p(self.handle), int(level)))
 
     def last_timing(self) -> dict:
         t = _Timing()

@@ -121,7 +121,7 @@ struct s2m_program {
 struct s2m_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    bool profiling = false;
+    int profiling = 0;  // 0 off, 1 total + evaluation kernels, 2 every stage
     s2m_timing timing{};
     DevBuf lattice, vidx, bcount, boffset, counters, tables;
     DevBuf vtx, keys, signs, tri, corners;
@@ -711,8 +711,10 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.super_act = ctx->sp_super_act.as<uint64_t>();
             P.job_out = ctx->sp_job_out.as<uint64_t>();
             hipEvent_t m[kSparseMarks] = {};
-            if (ctx->profiling)
+            if (ctx->profiling >= 2)
                 for (auto& e : m) e = ctx->next_event();
+            else if (ctx->profiling)  // the evaluation launches only: fewer markers between kernels
+                m[1] = ctx->next_event(), m[2] = ctx->next_event();
             HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters, ctx->sp_queues.as<uint32_t>()));
             const int boundary = (n == 1 && pass + 1 < npass) ? 1 : 0;
             HIP_TRY(launch_sparse_pass(ctx->stream, P, b, ctx->cus, (uint32_t)tile_blocks, general_tiles, fold_jobs,
@@ -739,9 +741,9 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_end));
             ctx->timing.total_ms = ms;
             for (size_t q = 0; q + kSparseMarks - 1 < ev.size(); q += kSparseMarks) {
-                float t[kSparseMarks - 1];
+                float t[kSparseMarks - 1] = {};
                 for (int k = 0; k + 1 < kSparseMarks; ++k)
-                    HIP_TRY(hipEventElapsedTime(&t[k], ev[q + k], ev[q + k + 1]));
+                    if (ev[q + k] && ev[q + k + 1]) HIP_TRY(hipEventElapsedTime(&t[k], ev[q + k], ev[q + k + 1]));
                 ctx->timing.prune_ms += t[0];
                 ctx->timing.eval_ms += t[1];     // evaluation + contouring
                 ctx->timing.quad_ms += t[2];     // quad count
@@ -1124,7 +1126,7 @@ void s2m_destroy(s2m_ctx* ctx)
 int s2m_set_profiling(s2m_ctx* ctx, int on)
 {
     if (!ctx) return fail(S2M_E_ARG, "ctx is NULL");
-    ctx->profiling = on != 0;
+    ctx->profiling = on < 0 ? 0 : on > 2 ? 2 : on;
     return S2M_OK;
 }
 
