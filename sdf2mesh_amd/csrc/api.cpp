@@ -461,7 +461,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     std::vector<SpTables> tabs(n);
     std::vector<size_t> tab_off(n);
     size_t tab_bytes = 0;
-    uint32_t list_stride = 1, box_stride = 1, prune_depth = 1;
+    uint32_t list_stride = 1, box_stride = 1, box_rows = 1, prune_depth = 1;
     bool general_tiles = false;  // a tile's pruned program may not be a left fold
     bool fold_jobs = false, other_jobs = false;  // programs that are / are not union folds
     uint64_t ntiles_all = 0, nrows_all = 0, plane_max = 0, cells_all = 0, planes_sum = 0;
@@ -473,6 +473,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         if (!left_fold_program(progs[i]->ops)) general_tiles = true;
         (union_fold_program(progs[i]->ops) ? fold_jobs : other_jobs) = true;
         box_stride = std::max(box_stride, tabs[i].bmax[0] * tabs[i].bmax[1] * tabs[i].bmax[2]);
+        box_rows = std::max(box_rows, tabs[i].bmax[1] * tabs[i].bmax[2]);
         uint32_t depth = 0;  // operand stack entries below the top (k_sp_prune's LDS stack)
         for (const s2m_op& o : progs[i]->ops) {
             if (o.code < S2M_OP_UNION) prune_depth = std::max(prune_depth, depth++);
@@ -694,6 +695,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.job_of_slot = ctx->sp_job_of_slot.as<uint32_t>();
             P.slot_cap = (uint32_t)want_s;
             P.box_stride = box_stride;
+            P.box_rows = box_rows;
             P.cellbits = ctx->sp_cellbits.as<uint32_t>();
             P.stage = ctx->sp_stage.as<float4>();
             P.stage_cap = stage_cap;

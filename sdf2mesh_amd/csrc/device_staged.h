@@ -27,6 +27,24 @@ struct StagedTapeN {
 };
 using StagedTape = StagedTapeN<kStageOps>;
 
+// A left fold (P0 P1 op1 P2 op2 ...) staged in 44 % of StagedTapeN's space: entry j holds
+// primitive 2j - 1 (primitive 0 for j = 0) in f[j][0..6] and operator 2j in f[j][7]
+// (primitives read at most 7 operands, operators one).  Left folds hold no Mandelbulb, so
+// there is no integer operand.
+template <uint32_t CAP>
+struct StagedFoldN {
+    uint8_t code[CAP];
+    float f[CAP / 2 + 1][8];
+
+    __device__ __forceinline__ uint32_t code_at(uint32_t k) const { return __builtin_amdgcn_readfirstlane(code[k]); }
+    __device__ __forceinline__ uint32_t iarg_at(uint32_t) const { return 0u; }
+    __device__ __forceinline__ const float* f_at(uint32_t k) const
+    {
+        return (k & 1u) || k == 0 ? f[(k + 1u) >> 1] : &f[k >> 1][7];
+    }
+};
+using StagedFold = StagedFoldN<kStageOps>;
+
 // Copy ops[L[k]], k < n <= CAP, into T: worker t of nt copies entries t, t + nt, ... (by
 // default the threads of the block; the caller synchronises before and after).
 template <uint32_t CAP>
@@ -40,6 +58,24 @@ __device__ __forceinline__ void stage_tape(StagedTapeN<CAP>& T, const s2m_op* __
         T.iarg[k] = o.iarg;
 #pragma unroll
         for (int j = 0; j < 8; ++j) T.f[k][j] = o.f[j];
+    }
+}
+
+// The same for a left fold (n <= CAP, n odd, primitives at 0 and the odd positions).
+template <uint32_t CAP>
+__device__ __forceinline__ void stage_tape(StagedFoldN<CAP>& T, const s2m_op* __restrict__ ops,
+                                           const uint16_t* __restrict__ L, uint32_t n,
+                                           uint32_t t = threadIdx.x, uint32_t nt = blockDim.x)
+{
+    for (uint32_t k = t; k < n; k += nt) {
+        const s2m_op& o = ops[L[k]];
+        T.code[k] = (uint8_t)o.code;
+        if ((k & 1u) || k == 0) {
+#pragma unroll
+            for (int j = 0; j < 7; ++j) T.f[(k + 1u) >> 1][j] = o.f[j];
+        } else {
+            T.f[k >> 1][7] = o.f[0];
+        }
     }
 }
 

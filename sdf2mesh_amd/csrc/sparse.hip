@@ -707,8 +707,8 @@ constexpr uint32_t kStageChunk = kStageChunkHost;
 constexpr uint32_t kTileThreads = 256;  // threads of a k_sp_tiles block (one tile at a time)
 constexpr int kLinearPoints = 2;        // points per lane sharing each op's decode (left folds)
 
-struct ContourLds {  // block-shared state of contour_tile
-    uint16_t cand[512];  // cells with a sign change among their corners
+struct ContourLds {  // block-shared state of contour_tile (and `cand`, 512 entries: the cells
+                     // with a sign change among their corners)
     uint32_t rows[64];   // row czl * 8 + cyl: bit cxl = the cell has a vertex
     uint32_t ncand, nact;
     uint32_t base, left, slot_base;  // staging chunk of the block
@@ -734,7 +734,7 @@ __device__ __forceinline__ void tile_corners(const float* box, const TileAxes& A
 
 __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, uint32_t slot, uint32_t tx,
                                              uint32_t ty, uint32_t tzl, const float* box, const TileAxes& A,
-                                             uint64_t live, ContourLds& L)
+                                             uint64_t live, uint16_t* cand, ContourLds& L)
 {
     // 1. cells with a sign change (thread t: cells t and t + 256, x fastest)
 #pragma unroll
@@ -742,21 +742,21 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
         const uint32_t c = r * kTileThreads + thread_id();
         const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
         const uint32_t brick = ((czl >> 1) * 4u + (cyl >> 1)) * 4u + (cxl >> 1);
-        bool cand = false;
+        bool cand_c = false;
         if (tx * 8u + cxl < J.rx && ty * 8u + cyl < J.ry && tzl * 8u + czl < J.nz && ((live >> brick) & 1ull)) {
             float cv[8];
             tile_corners(box, A, J.bx, J.by, c, cv);
             uint32_t pm = 0;
 #pragma unroll
             for (int k = 0; k < 8; ++k) pm |= (uint32_t)(cv[k] > 0.0f) << k;
-            cand = pm != 0u && pm != 0xffu;
+            cand_c = pm != 0u && pm != 0xffu;
         }
-        const uint64_t w = __ballot(cand);
+        const uint64_t w = __ballot(cand_c);
         const uint32_t lead = (uint32_t)__builtin_ctzll(w | (1ull << 63));
         uint32_t wbase = 0;
         if (w && lane() == lead) wbase = atomicAdd(&L.ncand, (uint32_t)__builtin_popcountll(w));
         wbase = __shfl(wbase, lead, 64);
-        if (cand) L.cand[wbase + (uint32_t)__builtin_popcountll(w & ((1ull << lane()) - 1ull))] = (uint16_t)c;
+        if (cand_c) cand[wbase + (uint32_t)__builtin_popcountll(w & ((1ull << lane()) - 1ull))] = (uint16_t)c;
     }
     __syncthreads();
     // 2. vertex and sign bits of each listed cell
@@ -768,7 +768,7 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
     for (uint32_t r = 0; r < 2; ++r) {
         const uint32_t k = r * kTileThreads + thread_id();
         if (k < ncand) {
-            const uint32_t c = L.cand[k];
+            const uint32_t c = cand[k];
             const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
             float cv[8];
             tile_corners(box, A, J.bx, J.by, c, cv);
@@ -825,6 +825,7 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
 
 constexpr uint32_t kChunk = 1024;  // box points examined per compaction round
 constexpr uint32_t kList = 2048;   // needed points held before they are evaluated
+static_assert(kList >= 8 * 8 * 8, "the point list doubles as the contour's candidate cells");
 
 // k_sp_tiles: per slot (persistent, slots drawn from a work queue), the tile's pruned
 // program, staged in
@@ -836,12 +837,13 @@ constexpr uint32_t kList = 2048;   // needed points held before they are evaluat
 template <int LINEAR>
 __global__ __launch_bounds__(kTileThreads) void k_sp_tiles(SpPass P)
 {
-    extern __shared__ float box[];  // P.box_stride floats
-    __shared__ StagedTape T;
+    extern __shared__ float box[];  // P.box_stride floats, then P.box_rows bytes (rowm)
+    // left folds in the compact layout (left-fold slots only), other programs in the full one
+    __shared__ std::conditional_t<LINEAR != 0, StagedFold, StagedTape> T;
     __shared__ TileAxes A;
     __shared__ uint32_t bm[3][32];   // per axis and box index: the bricks (along it) using it
-    __shared__ uint8_t rowm[32 * 32];  // per box row (iz, iy): x-bricks of the live bricks using it
-    __shared__ uint16_t list[kList];
+    // points to evaluate; then the contour's candidate cells (contour_tile's `cand`)
+    __shared__ uint16_t list[LINEAR ? kList : 8 * 8 * 8];
     __shared__ uint32_t nlist;
     __shared__ float pc[3][32];  // lattice coordinates of the box per axis
     __shared__ ContourLds L;
@@ -849,6 +851,7 @@ __global__ __launch_bounds__(kTileThreads) void k_sp_tiles(SpPass P)
     const uint32_t nslots = active_slots(P), nlinear = (uint32_t)P.counters[kLinearSlots];
     if (LINEAR ? nlinear == 0 : nlinear >= nslots) return;  // no slot of this kind
     constexpr uint32_t kQueue = LINEAR ? 0u : 1u;
+    uint8_t* rowm = (uint8_t*)(box + P.box_stride);  // per box row (iz, iy): x-bricks of the live bricks using it
     if (thread_id() == 0) L.left = 0;  // published by the first slot's barriers
     uint64_t tiles_done = 0, points_done = 0, flops_done = 0;  // thread 0: this block's share
     uint32_t next = thread_id() == 0 ? queue_take(P, kQueue, blockIdx.x, gridDim.x) : 0u;
@@ -971,7 +974,7 @@ __global__ __launch_bounds__(kTileThreads) void k_sp_tiles(SpPass P)
             }
         }
         __syncthreads();  // the box is complete
-        contour_tile(P, J, slot, tx, ty, tzl, box, A, lv, L);
+        contour_tile(P, J, slot, tx, ty, tzl, box, A, lv, list, L);
         if (thread_id() == 0) {
             tiles_done += 1;
             points_done += evaluated;
@@ -1086,7 +1089,7 @@ __global__ __launch_bounds__(64) void k_sp_quadcount(SpPass P, MeshBufs b)
 template <int LINEAR>
 __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
 {
-    __shared__ StagedTape T;
+    __shared__ std::conditional_t<LINEAR != 0, StagedFold, StagedTape> T;
     __shared__ uint32_t nb[729];  // vertex of cell (8tx-1+i, 8ty-1+j, 8tz-1+k) at nb[(k*9+j)*9+i]
     __shared__ uint32_t tc[128];  // triangles of cell c in byte c % 4 of tc[c / 4]
     const uint32_t nslots = active_slots(P);
@@ -1454,7 +1457,7 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
     hipLaunchKernelGGL(k_sp_prune, dim3((uint32_t)P.nsuper), dim3(64), prune_smem, s, P);
     hipLaunchKernelGGL(k_sp_bricks, dim3(resident_blocks(k_sp_bricks, 64, 0, cus, big)), dim3(64), 0, s, P);
     mark(1);
-    const size_t smem = (size_t)P.box_stride * sizeof(float);
+    const size_t smem = (size_t)P.box_stride * sizeof(float) + ((P.box_rows + 3u) & ~3u);
     hipLaunchKernelGGL((k_sp_tiles<1>), dim3(resident_blocks(k_sp_tiles<1>, kTileThreads, smem, cus, tile_blocks)),
                        dim3(kTileThreads), smem, s, P);
     if (general_tiles)
