@@ -461,7 +461,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     std::vector<SpTables> tabs(n);
     std::vector<size_t> tab_off(n);
     size_t tab_bytes = 0;
-    uint32_t list_stride = 1, box_stride = 1, box_rows = 1, prune_depth = 1;
+    uint32_t list_stride = 1, box_stride = 1, prune_depth = 1;
     bool general_tiles = false;  // a tile's pruned program may not be a left fold
     bool fold_jobs = false, other_jobs = false;  // programs that are / are not union folds
     uint64_t ntiles_all = 0, nrows_all = 0, plane_max = 0, cells_all = 0, planes_sum = 0;
@@ -473,7 +473,6 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         if (!left_fold_program(progs[i]->ops)) general_tiles = true;
         (union_fold_program(progs[i]->ops) ? fold_jobs : other_jobs) = true;
         box_stride = std::max(box_stride, tabs[i].bmax[0] * tabs[i].bmax[1] * tabs[i].bmax[2]);
-        box_rows = std::max(box_rows, tabs[i].bmax[1] * tabs[i].bmax[2]);
         uint32_t depth = 0;  // operand stack entries below the top (k_sp_prune's LDS stack)
         for (const s2m_op& o : progs[i]->ops) {
             if (o.code < S2M_OP_UNION) prune_depth = std::max(prune_depth, depth++);
@@ -487,8 +486,11 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     }
     if (box_stride * sizeof(float) > 40u * 1024u)
         return fail(S2M_E_ARG, "tile box of %u points exceeds the LDS staging limit", box_stride);
-    for (uint32_t i = 0; i < n; ++i)  // a tile spans at most 9 lattice positions, 2 coordinates each
-        if (tabs[i].bmax[0] > 32 || tabs[i].bmax[1] > 32 || tabs[i].bmax[2] > 32)
+    // a tile's box spans lattice positions c0 .. c0 + 8: L of the first, H of the last and at
+    // most both of each of the 7 between, so at most 16 coordinates per axis (k_sp_tiles
+    // gives each box row (z, y) a thread)
+    for (uint32_t i = 0; i < n; ++i)
+        if (tabs[i].bmax[0] > 16 || tabs[i].bmax[1] > 16 || tabs[i].bmax[2] > 16)
             return fail(S2M_E_ARG, "tile box of %u x %u x %u lattice points", tabs[i].bmax[0], tabs[i].bmax[1],
                         tabs[i].bmax[2]);
     // slabs (single job only): whole tile layers, per-pass buffers bounded to ~1 GiB
@@ -695,7 +697,6 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.job_of_slot = ctx->sp_job_of_slot.as<uint32_t>();
             P.slot_cap = (uint32_t)want_s;
             P.box_stride = box_stride;
-            P.box_rows = box_rows;
             P.cellbits = ctx->sp_cellbits.as<uint32_t>();
             P.stage = ctx->sp_stage.as<float4>();
             P.stage_cap = stage_cap;

@@ -141,7 +141,6 @@ struct SpPass {
     uint32_t* job_of_slot;     // [slot]
     uint32_t slot_cap;
     uint32_t box_stride;       // floats of a tile's lattice box (LDS)
-    uint32_t box_rows;         // rows (z, y) of a tile's lattice box
     uint32_t* cellbits;        // [slot][16] bit c: cell c of the tile has a vertex (row r = c / 8
                                // in byte r % 4 of word r / 4)
     float4* stage;             // [stage_cap] active cells: position, cell | rank << 9 | signs << 12

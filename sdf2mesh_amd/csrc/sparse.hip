@@ -823,9 +823,10 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
     }
 }
 
-constexpr uint32_t kChunk = 1024;  // box points examined per compaction round
-constexpr uint32_t kList = 2048;   // needed points held before they are evaluated
+constexpr uint32_t kList = 1024;   // needed points held before they are evaluated
+constexpr uint32_t kBoxAxis = 16;  // most lattice indices of a tile box per axis
 static_assert(kList >= 8 * 8 * 8, "the point list doubles as the contour's candidate cells");
+static_assert(kTileThreads >= kBoxAxis * kBoxAxis, "one box row per thread");
 
 // k_sp_tiles: per slot (persistent, slots drawn from a work queue), the tile's pruned
 // program, staged in
@@ -835,13 +836,14 @@ static_assert(kList >= 8 * 8 * 8, "the point list doubles as the contour's candi
 // points per lane), LINEAR = 0 the others; both launches walk all slots and skip the
 // other kind.
 template <int LINEAR>
-__global__ __launch_bounds__(kTileThreads) void k_sp_tiles(SpPass P)
+__device__ __forceinline__ void tiles_body(const SpPass& P)
 {
-    extern __shared__ float box[];  // P.box_stride floats, then P.box_rows bytes (rowm)
+    extern __shared__ float box[];  // P.box_stride floats
     // left folds in the compact layout (left-fold slots only), other programs in the full one
     __shared__ std::conditional_t<LINEAR != 0, StagedFold, StagedTape> T;
     __shared__ TileAxes A;
-    __shared__ uint32_t bm[3][32];   // per axis and box index: the bricks (along it) using it
+    __shared__ uint32_t bm[3][32];   // per axis (y, z) and box index: the bricks (along it) using it
+    __shared__ uint32_t xb[4];       // per x-brick: the box x indices its cells use
     // points to evaluate; then the contour's candidate cells (contour_tile's `cand`)
     __shared__ uint16_t list[LINEAR ? kList : 8 * 8 * 8];
     __shared__ uint32_t nlist;
@@ -851,12 +853,11 @@ __global__ __launch_bounds__(kTileThreads) void k_sp_tiles(SpPass P)
     const uint32_t nslots = active_slots(P), nlinear = (uint32_t)P.counters[kLinearSlots];
     if (LINEAR ? nlinear == 0 : nlinear >= nslots) return;  // no slot of this kind
     constexpr uint32_t kQueue = LINEAR ? 0u : 1u;
-    uint8_t* rowm = (uint8_t*)(box + P.box_stride);  // per box row (iz, iy): x-bricks of the live bricks using it
     if (thread_id() == 0) L.left = 0;  // published by the first slot's barriers
     uint64_t tiles_done = 0, points_done = 0, flops_done = 0;  // thread 0: this block's share
     uint32_t next = thread_id() == 0 ? queue_take(P, kQueue, blockIdx.x, gridDim.x) : 0u;
     for (;;) {
-        __syncthreads();  // the previous slot's readers of sh_slot, T, A, bm, rowm, list, pc, box, L are done
+        __syncthreads();  // the previous slot's readers of sh_slot, T, A, bm, xb, list, pc, box, L are done
         if (thread_id() == 0) sh_slot = next;
         __syncthreads();
         const uint32_t slot = sh_slot;
@@ -894,24 +895,6 @@ __global__ __launch_bounds__(kTileThreads) void k_sp_tiles(SpPass P)
         }
         contour_reset(L);
         __syncthreads();
-        if (LINEAR) {
-            if (thread_id() < 24) {
-                const uint32_t ax = thread_id() >> 3, c = thread_id() & 7u;
-                atomicOr(&bm[ax][A.lo[ax][c]], 1u << (c >> 1));
-                atomicOr(&bm[ax][A.hi[ax][c]], 1u << (c >> 1));
-            }
-            __syncthreads();
-            // box row (iz, iy): the x-bricks of the live bricks whose cells use it
-            for (uint32_t q = thread_id(); q < ny * nz; q += blockDim.x) {
-                const uint32_t iz = div16(q, ny, magic16(ny)), iy = q - iz * ny;
-                uint32_t m = 0;
-                for (uint32_t mz = bm[2][iz]; mz; mz &= mz - 1u)
-                    for (uint32_t my = bm[1][iy]; my; my &= my - 1u)
-                        m |= (uint32_t)(lv >> ((__builtin_ctz(mz) * 4 + __builtin_ctz(my)) * 4)) & 0xfu;
-                rowm[q] = (uint8_t)m;
-            }
-            __syncthreads();
-        }
         auto point = [&](uint32_t q, float& x, float& y, float& z) {
             const uint32_t iz = div16(q, nxy, mxy), r = q - iz * nxy;
             const uint32_t iy = div16(r, nx, mx), ix = r - iy * nx;
@@ -922,49 +905,74 @@ __global__ __launch_bounds__(kTileThreads) void k_sp_tiles(SpPass P)
         };
         uint32_t evaluated = npts;
         if (LINEAR) {
-            // the points that some live brick uses, compacted into `list` chunk by chunk and
-            // evaluated when the list holds more than a chunk's worth (or at the end), so
-            // evaluation rounds run with full waves
-            evaluated = 0;
+            // The points that some live brick uses.  Per box row (iz, iy): the x-bricks of the
+            // live bricks whose cells use it (rm), hence the row's needed x positions (the
+            // union of those x-bricks' positions, xb).  One row per thread; each wave reserves
+            // its rows' points in `list` with one atomic; entries are iz << 10 | iy << 5 | ix.
+            // The list is evaluated window by window (one window unless more than kList points).
+            if (thread_id() >= 8 && thread_id() < 24) {
+                const uint32_t ax = thread_id() >> 3, c = thread_id() & 7u;
+                atomicOr(&bm[ax][A.lo[ax][c]], 1u << (c >> 1));
+                atomicOr(&bm[ax][A.hi[ax][c]], 1u << (c >> 1));
+            }
+            if (thread_id() < 4) {
+                const uint32_t c = 2u * thread_id();
+                xb[thread_id()] = (1u << A.lo[0][c]) | (1u << A.hi[0][c]) | (1u << A.lo[0][c + 1]) | (1u << A.hi[0][c + 1]);
+            }
             if (thread_id() == 0) nlist = 0;
             __syncthreads();
-            for (uint32_t c0 = 0; c0 < npts; c0 += kChunk) {
-                const uint32_t cn = min(kChunk, npts - c0);
-                for (uint32_t k = 0; k < cn; k += kTileThreads) {
-                    const uint32_t q = c0 + k + thread_id();
-                    bool need = false;
-                    if (q < c0 + cn) {
-                        const uint32_t row = div16(q, nx, mx), ix = q - row * nx;  // row = iz * ny + iy
-                        need = (rowm[row] & bm[0][ix]) != 0;
-                    }
-                    const uint64_t w = __ballot(need);
-                    uint32_t wbase = 0;
-                    if (lane() == 0 && w) wbase = atomicAdd(&nlist, (uint32_t)__builtin_popcountll(w));
-                    wbase = __shfl(wbase, 0, 64);
-                    if (need)
-                        list[wbase + (uint32_t)__builtin_popcountll(w & ((1ull << lane()) - 1ull))] = (uint16_t)q;
+            const uint32_t q = thread_id();
+            uint32_t M = 0, code = 0;
+            if (q < ny * nz) {
+                const uint32_t iz = div16(q, ny, magic16(ny)), iy = q - iz * ny;
+                uint32_t rm = 0;
+                for (uint32_t mz = bm[2][iz]; mz; mz &= mz - 1u)
+                    for (uint32_t my = bm[1][iy]; my; my &= my - 1u)
+                        rm |= (uint32_t)(lv >> ((__builtin_ctz(mz) * 4 + __builtin_ctz(my)) * 4)) & 0xfu;
+                M = ((rm & 1u) ? xb[0] : 0u) | ((rm & 2u) ? xb[1] : 0u) | ((rm & 4u) ? xb[2] : 0u) |
+                    ((rm & 8u) ? xb[3] : 0u);
+                code = (iz << 10) | (iy << 5);
+            }
+            const uint32_t cnt = (uint32_t)__builtin_popcount(M);
+            uint32_t inc = cnt;  // inclusive prefix sum over the wave's lanes
+#pragma unroll
+            for (int d = 1; d < 64; d <<= 1) {
+                const uint32_t v = __shfl_up(inc, (unsigned)d, 64);
+                if (lane() >= (uint32_t)d) inc += v;
+            }
+            uint32_t wbase = 0;
+            if (lane() == 63 && inc) wbase = atomicAdd(&nlist, inc);
+            const uint32_t off = __shfl(wbase, 63, 64) + inc - cnt;
+            __syncthreads();
+            const uint32_t total = nlist;
+            evaluated = total;
+            for (uint32_t w0 = 0; w0 < total; w0 += kList) {
+                if (w0) __syncthreads();  // the previous window's readers of the list are done
+                if (off < w0 + kList && off + cnt > w0) {
+                    uint32_t at = off - w0;  // wraps below w0: skipped by the bound test
+                    for (uint32_t mm = M; mm; mm &= mm - 1u, ++at)
+                        if (at < kList) list[at] = (uint16_t)(code | (uint32_t)__builtin_ctz(mm));
                 }
                 __syncthreads();
-                const uint32_t m = nlist;
-                __syncthreads();  // every wave has read nlist before the next chunk appends to it
-                if (m <= kList - kChunk && c0 + kChunk < npts) continue;  // room for another chunk
-                evaluated += m;
+                const uint32_t m = min(kList, total - w0);
                 for (uint32_t i0 = thread_id(); i0 < m; i0 += kLinearPoints * blockDim.x) {
                     float x[kLinearPoints], y[kLinearPoints], z[kLinearPoints], v[kLinearPoints];
                     uint32_t at[kLinearPoints];
 #pragma unroll
                     for (int p = 0; p < kLinearPoints; ++p) {
                         const uint32_t i = i0 + p * blockDim.x;
-                        at[p] = point(list[i < m ? i : i0], x[p], y[p], z[p]);
+                        const uint32_t e = list[i < m ? i : i0];
+                        const uint32_t ix = e & 31u, iy = (e >> 5) & 31u, iz = e >> 10;
+                        x[p] = pc[0][ix];
+                        y[p] = pc[1][iy];
+                        z[p] = pc[2][iz];
+                        at[p] = (iz * by + iy) * bx + ix;
                     }
                     run_linear<kLinearPoints>(T, n, x, y, z, v);
 #pragma unroll
                     for (int p = 0; p < kLinearPoints; ++p)
                         if (i0 + p * blockDim.x < m) box[at[p]] = v[p];
                 }
-                __syncthreads();  // every lane's reads of the list and of nlist are done
-                if (thread_id() == 0) nlist = 0;
-                __syncthreads();
             }
         } else {
             for (uint32_t q = thread_id(); q < npts; q += blockDim.x) {
@@ -986,6 +994,19 @@ __global__ __launch_bounds__(kTileThreads) void k_sp_tiles(SpPass P)
         atomicAdd((unsigned long long*)&P.counters[kEvalPoints], (unsigned long long)points_done);
         atomicAdd((unsigned long long*)&P.counters[kEvalFlops], (unsigned long long)flops_done);
     }
+}
+
+template <int LINEAR>
+__global__ void k_sp_tiles(SpPass P);
+template <>
+__global__ __launch_bounds__(kTileThreads) void k_sp_tiles<1>(SpPass P)
+{
+    tiles_body<1>(P);
+}
+template <>
+__global__ __launch_bounds__(kTileThreads) void k_sp_tiles<0>(SpPass P)
+{
+    tiles_body<0>(P);
 }
 
 // ---------------------------------------------------------------------------------
@@ -1457,7 +1478,7 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
     hipLaunchKernelGGL(k_sp_prune, dim3((uint32_t)P.nsuper), dim3(64), prune_smem, s, P);
     hipLaunchKernelGGL(k_sp_bricks, dim3(resident_blocks(k_sp_bricks, 64, 0, cus, big)), dim3(64), 0, s, P);
     mark(1);
-    const size_t smem = (size_t)P.box_stride * sizeof(float) + ((P.box_rows + 3u) & ~3u);
+    const size_t smem = (size_t)P.box_stride * sizeof(float);
     hipLaunchKernelGGL((k_sp_tiles<1>), dim3(resident_blocks(k_sp_tiles<1>, kTileThreads, smem, cus, tile_blocks)),
                        dim3(kTileThreads), smem, s, P);
     if (general_tiles)
