@@ -127,7 +127,7 @@ struct s2m_ctx {
     DevBuf vtx, keys, signs, tri, corners;
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_slot_rows, sp_boundary;
-    DevBuf sp_stage, sp_slot_stage, sp_slot_live, sp_slot_hdr;
+    DevBuf sp_stage, sp_slot_stage, sp_slot_live, sp_slot_hdr, sp_slot_rec;
     DevBuf sp_job_out, sp_queues, sp_super_iv, sp_super_lists, sp_super_lens, sp_super_act, sp_rows;
     uint64_t slot_cap = 0;
     int cus = 256;
@@ -638,6 +638,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         HIP_TRY(ctx->sp_slot_stage.ensure(want_s * 2 * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_slot_live.ensure(want_s * sizeof(uint64_t)));
         HIP_TRY(ctx->sp_slot_hdr.ensure(want_s * sizeof(SlotHdr)));
+        HIP_TRY(ctx->sp_slot_rec.ensure(want_s * kTileRecBytes));
         ctx->vcap = std::max(ctx->vcap, want_v);
         ctx->tcap = std::max(ctx->tcap, want_t);
         ctx->slot_cap = std::max(ctx->slot_cap, want_s);
@@ -703,6 +704,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.slot_stage = ctx->sp_slot_stage.as<uint32_t>();
             P.slot_live = ctx->sp_slot_live.as<uint64_t>();
             P.slot_hdr = ctx->sp_slot_hdr.as<SlotHdr>();
+            P.slot_rec = ctx->sp_slot_rec.as<uint4>();
             P.slot_rows = ctx->sp_slot_rows.as<uint32_t>();
             P.row_sum = ctx->sp_rows.as<uint32_t>();
             P.row_off = P.row_sum + 2 * P.nrows;
@@ -1117,7 +1119,7 @@ void s2m_destroy(s2m_ctx* ctx)
                       &ctx->sp_job_of_slot, &ctx->sp_cellbits, &ctx->sp_slot_rows,
                       &ctx->sp_boundary, &ctx->sp_job_out, &ctx->sp_queues, &ctx->sp_super_iv,
                       &ctx->sp_super_lists, &ctx->sp_super_lens, &ctx->sp_super_act, &ctx->sp_rows,
-                      &ctx->sp_stage, &ctx->sp_slot_stage, &ctx->sp_slot_live, &ctx->sp_slot_hdr})
+                      &ctx->sp_stage, &ctx->sp_slot_stage, &ctx->sp_slot_live, &ctx->sp_slot_hdr, &ctx->sp_slot_rec})
         b->release();
     ctx->host_img.release();
     ctx->host_back.release();

@@ -69,6 +69,7 @@ constexpr uint32_t kSuper = 4;  // tiles per super-tile edge (first pruning leve
 constexpr uint32_t kSingleSign = 0xffffffffu;  // super_lens: the SDF keeps one sign there
 constexpr uint32_t kStageChunkHost = 1024;  // staged-cell allocation chunk (sparse.hip)
 constexpr uint32_t kTileBlocksPerCU = 8;    // most k_sp_tiles blocks per compute unit
+constexpr uint32_t kTileRecBytes = 2832;    // per slot: k_sp_tiles' staged record (sparse.hip)
 constexpr uint32_t kQueues = 2;        // work queues of a pass (k_sp_tiles<1>, k_sp_tiles<0>)
 constexpr uint32_t kQueueGroups = 64;  // counters per queue, 32 u32 (128 bytes) apart
 // Profiling marks of one sparse pass: prune (tiles, bricks) | eval+contour | quad count |
@@ -148,6 +149,7 @@ struct SpPass {
     uint32_t* slot_stage;      // [slot][2] first staged cell, staged cells
     uint64_t* slot_live;       // [slot] bricks (2x2x2 cells) that may hold an active cell
     SlotHdr* slot_hdr;         // [slot]
+    uint4* slot_rec;           // [slot] kTileRecBytes: k_sp_tiles<1>'s staged record
     uint32_t* slot_rows;       // [4][slot_cap][64] per row of each slot's tile: vertices,
                                // triangles, first vertex / triangle within the row
     uint32_t* row_sum;         // [2][nrows] vertices, triangles per row

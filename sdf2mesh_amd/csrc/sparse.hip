@@ -612,6 +612,30 @@ __device__ __forceinline__ void load_tile_axes(TileAxes& A, const SpJob& J, uint
     A.chi[ax][c] = co[hi[gi]];
 }
 
+// Per slot, written by k_sp_bricks: what k_sp_tiles<1> stages for the slot, in its LDS
+// layout, so that one read of the record (made while the block contours the slot before)
+// replaces the dependent loads of header, job, pruned list, ops, axis tables and
+// coordinates.  Header and live bricks are written for every slot, the rest for slots
+// whose program is a left fold.
+struct alignas(16) TileRec {
+    SlotHdr H;
+    uint64_t live;              // bricks (2x2x2 cells) that may hold an active cell
+    uint32_t pad[2];
+    TileAxes A;
+    float pc[3][16];            // lattice coordinates of the box per axis
+    StagedFold T;               // the pruned program (the used rows are read)
+};
+static_assert(sizeof(TileRec) == kTileRecBytes, "record size shared with the host");
+constexpr uint32_t kRecVec = kTileRecBytes / 16;   // 16-byte words per record
+constexpr uint32_t kRecPre = 64;                    // words read ahead: header .. codes, ops 0..14
+constexpr uint32_t kRecRowsAt = (uint32_t)offsetof(TileRec, T) + (uint32_t)offsetof(StagedFold, f);
+static_assert(kRecRowsAt + 32u * 8u <= 16u * kRecPre, "ops 0..14 are read ahead");
+struct TileGen {  // k_sp_tiles<0>: the same parts, staged per slot
+    StagedTape T;
+    TileAxes A;
+    float pc[3][16];
+};
+
 // ---------------------------------------------------------------------------------
 // n / d for n, d < 2^16 with m = ceil(2^32 / d): the error term n * (m*d - 2^32) / 2^32 / d
 // stays below 1/d, so it never reaches the next integer.
@@ -632,9 +656,10 @@ __device__ __forceinline__ uint32_t div16(uint32_t n, uint32_t d, uint32_t m)
 // are not contoured and its points are not evaluated.  Left-fold programs only (other
 // programs keep every brick).
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_sp_bricks(SpPass P)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_sp_bricks(SpPass P)
 {
-    __shared__ StagedTape T;
+    __shared__ StagedFold T;
+    __shared__ TileAxes A;
     const uint32_t nslots = active_slots(P);
     for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
         const uint32_t b = thread_id();
@@ -645,49 +670,66 @@ __global__ __launch_bounds__(64) void k_sp_bricks(SpPass P)
         const SpJob& J = P.jobs[jid];
         uint32_t tx, ty, tzl;
         tile_xyz(J, tile, tx, ty, tzl);
-        if (b < 3) {  // the slot's header: lane a writes axis a's box origin and size
-            const uint32_t a = b, t = a == 0 ? tx : a == 1 ? ty : J.tz0 + tzl;
+        TileRec& RG = *reinterpret_cast<TileRec*>(P.slot_rec + (size_t)slot * kRecVec);
+        if (b < 48) {  // the slot's header (lane a < 3: axis a's box origin and size) and coordinates
+            const uint32_t a = b >> 4, i = b & 15u, t = a == 0 ? tx : a == 1 ? ty : J.tz0 + tzl;
             const uint32_t nt = a == 0 ? J.ntx : a == 1 ? J.nty : J.ntz;
             const uint32_t* ib = J.ibox + 2 * (a == 0 ? 0 : a == 1 ? J.ntx : J.ntx + J.nty);
             const uint32_t o = ib[t], e = ib[nt + t];
-            SlotHdr& H = P.slot_hdr[slot];
-            H.box[a] = o | ((e - o + 1) << 24);
-            if (a == 0) {
+            const bool linear = (lens & kLinearFlag) && n <= kStageOps;
+            if (linear && i <= e - o) RG.pc[a][i] = (a == 0 ? J.cx : a == 1 ? J.cy : J.cz)[o + i];
+            if (i == 0) {
+                const uint32_t box = o | ((e - o + 1) << 24);
+                P.slot_hdr[slot].box[a] = box;
+                RG.H.box[a] = box;
+            }
+            if (b == 0) {
+                SlotHdr H;
                 H.tile = (uint32_t)tile;
                 H.job = jid;
                 H.lens = lens;
                 H.txy = tx | (ty << 16);
                 H.tzl = tzl;
+                for (SlotHdr* d : {&P.slot_hdr[slot], &RG.H}) {
+                    d->tile = H.tile;
+                    d->job = H.job;
+                    d->lens = H.lens;
+                    d->txy = H.txy;
+                    d->tzl = H.tzl;
+                }
             }
         }
         if (!(lens & kLinearFlag) || n > kStageOps) {  // every brick is kept
-            if (b == 0) P.slot_live[slot] = ~0ull;
+            if (b == 0) P.slot_live[slot] = RG.live = ~0ull;
             continue;
         }
-        __syncthreads();  // the previous slot's readers of T are done
+        __syncthreads();  // the previous slot's readers of T and A are done
         stage_tape(T, J.ops, P.lists + (size_t)tile * P.list_stride, n);
-        const uint32_t t[3] = {tx, ty, J.tz0 + tzl};
-        const uint32_t r[3] = {J.rx, J.ry, J.z0 + J.nz};  // cells on each axis (z: of the job)
-        const uint32_t* lo[3] = {J.lox, J.loy, J.loz};
-        const uint32_t* hi[3] = {J.hix, J.hiy, J.hiz};
-        const float* co[3] = {J.cx, J.cy, J.cz};
+        if (b < 24) load_tile_axes(A, J, tx, ty, J.tz0 + tzl, b);
+        __syncthreads();
+        {  // the record's axes and program (codes and the rows in use)
+            const uint32_t words = (uint32_t)(offsetof(StagedFold, f) + 32u * ((n + 1u) >> 1)) / 4u;
+            for (uint32_t k = b; k < words; k += 64u)
+                reinterpret_cast<uint32_t*>(&RG.T)[k] = reinterpret_cast<const uint32_t*>(&T)[k];
+            for (uint32_t k = b; k < (uint32_t)sizeof(TileAxes) / 4u; k += 64u)
+                reinterpret_cast<uint32_t*>(&RG.A)[k] = reinterpret_cast<const uint32_t*>(&A)[k];
+        }
+        // brick b: cells c, c + 1 per axis; the first cell's lo corner, the second's hi corner
         const uint32_t c[3] = {(b & 3u) * 2u, ((b >> 2) & 3u) * 2u, (b >> 4) * 2u};
         double bl[3], bh[3];
-        for (int a = 0; a < 3; ++a) {  // first cell's lo corner, second cell's hi corner
-            const uint32_t g0 = min(t[a] * 8u + c[a], r[a] - 1), g1 = min(t[a] * 8u + c[a] + 1, r[a] - 1);
-            bl[a] = (double)co[a][lo[a][g0]];
-            bh[a] = (double)co[a][hi[a][g1]];
+        for (int a = 0; a < 3; ++a) {
+            bl[a] = (double)A.clo[a][c[a]];
+            bh[a] = (double)A.chi[a][c[a] + 1];
         }
-        __syncthreads();
-        Iv acc = iv_primitive(uniform_code(T, 0), T.f[0], bl, bh);
+        Iv acc = iv_primitive(uniform_code(T, 0), T.f_at(0), bl, bh);
         for (uint32_t k = 1; k + 1 < n; k += 2) {
-            const Iv d = iv_primitive(uniform_code(T, k), T.f[k], bl, bh);
+            const Iv d = iv_primitive(uniform_code(T, k), T.f_at(k), bl, bh);
             int keep;
-            acc = iv_op(uniform_code(T, k + 1), acc, d, T.f[k + 1][0], keep);
+            acc = iv_op(uniform_code(T, k + 1), acc, d, T.f_at(k + 1)[0], keep);
         }
         const bool may_cross = !(acc.known && (acc.lo > 0.0 || acc.hi < 0.0));
         const uint64_t m = __ballot(may_cross);
-        if (b == 0) P.slot_live[slot] = m;
+        if (b == 0) P.slot_live[slot] = RG.live = m;
     }
 }
 
@@ -839,78 +881,81 @@ template <int LINEAR>
 __device__ __forceinline__ void tiles_body(const SpPass& P)
 {
     extern __shared__ float box[];  // P.box_stride floats
-    // left folds in the compact layout (left-fold slots only), other programs in the full one
-    __shared__ std::conditional_t<LINEAR != 0, StagedFold, StagedTape> T;
-    __shared__ TileAxes A;
+    // left folds: the slot's record (k_sp_bricks); other programs: staged here
+    __shared__ std::conditional_t<LINEAR != 0, TileRec, TileGen> R;
     __shared__ uint32_t bm[3][32];   // per axis (y, z) and box index: the bricks (along it) using it
     __shared__ uint32_t xb[4];       // per x-brick: the box x indices its cells use
     // points to evaluate; then the contour's candidate cells (contour_tile's `cand`)
     __shared__ uint16_t list[LINEAR ? kList : 8 * 8 * 8];
     __shared__ uint32_t nlist;
-    __shared__ float pc[3][32];  // lattice coordinates of the box per axis
     __shared__ ContourLds L;
-    __shared__ uint32_t sh_slot;
+    __shared__ uint32_t sh_slot, sh_next;
     const uint32_t nslots = active_slots(P), nlinear = (uint32_t)P.counters[kLinearSlots];
     if (LINEAR ? nlinear == 0 : nlinear >= nslots) return;  // no slot of this kind
     constexpr uint32_t kQueue = LINEAR ? 0u : 1u;
     if (thread_id() == 0) L.left = 0;  // published by the first slot's barriers
     uint64_t tiles_done = 0, points_done = 0, flops_done = 0;  // thread 0: this block's share
     uint32_t next = thread_id() == 0 ? queue_take(P, kQueue, blockIdx.x, gridDim.x) : 0u;
+    // Left folds: threads < kRecPre hold the first kRecPre x 16 bytes of the next slot's
+    // record, read while the current slot is contoured and stored to R at the loop's top.
+    uint4 pre = make_uint4(0u, 0u, 0u, 0u);
+    auto prefetch = [&]() {  // all threads; `next` published in sh_next before a barrier
+        const uint32_t s = sh_next;
+        if (thread_id() < kRecPre && s < nslots) pre = P.slot_rec[(size_t)s * kRecVec + thread_id()];
+    };
+    if constexpr (LINEAR != 0) {
+        if (thread_id() < 64) bm[1 + (thread_id() >> 5)][thread_id() & 31u] = 0;
+        if (thread_id() == 0) sh_next = next;
+        __syncthreads();
+        prefetch();
+    }
     for (;;) {
-        __syncthreads();  // the previous slot's readers of sh_slot, T, A, bm, xb, list, pc, box, L are done
+        __syncthreads();  // the previous slot's readers of sh_slot, R, xb, list, box, L are done
         if (thread_id() == 0) sh_slot = next;
+        if (LINEAR && thread_id() < kRecPre) reinterpret_cast<uint4*>(&R)[thread_id()] = pre;
         __syncthreads();
         const uint32_t slot = sh_slot;
         if (slot >= nslots) break;
         if (thread_id() == 0) next = queue_take(P, kQueue, blockIdx.x, gridDim.x);  // prefetched
-        const SlotHdr& H = P.slot_hdr[__builtin_amdgcn_readfirstlane(slot)];
+        const SlotHdr& H = [&]() -> const SlotHdr& {
+            if constexpr (LINEAR != 0) return R.H;
+            else return P.slot_hdr[__builtin_amdgcn_readfirstlane(slot)];
+        }();
         const uint32_t lens = __builtin_amdgcn_readfirstlane(H.lens);
-        if (((lens & kLinearFlag) != 0) != (LINEAR != 0)) continue;  // same for the whole block
+        if (((lens & kLinearFlag) != 0) != (LINEAR != 0)) {  // same for the whole block
+            if (LINEAR) {
+                if (thread_id() == 0) sh_next = next;
+                __syncthreads();
+                prefetch();
+            }
+            continue;
+        }
         const uint64_t tile = __builtin_amdgcn_readfirstlane(H.tile);
         const uint32_t n = lens & ~kLinearFlag;
         const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(H.job)];
         const uint32_t txy = __builtin_amdgcn_readfirstlane(H.txy);
         const uint32_t tx = txy & 0xffffu, ty = txy >> 16, tzl = __builtin_amdgcn_readfirstlane(H.tzl);
-        const uint32_t tz = J.tz0 + tzl;
         const uint32_t bxh = __builtin_amdgcn_readfirstlane(H.box[0]), byh = __builtin_amdgcn_readfirstlane(H.box[1]);
         const uint32_t bzh = __builtin_amdgcn_readfirstlane(H.box[2]);
-        const uint32_t x0 = bxh & 0xffffffu, y0 = byh & 0xffffffu, z0 = bzh & 0xffffffu;
         const uint32_t nx = bxh >> 24, ny = byh >> 24, nz = bzh >> 24;
-        const uint32_t nxy = nx * ny, npts = nxy * nz;
-        const uint32_t mx = magic16(nx), mxy = magic16(nxy);
-        const uint16_t* Lst = P.lists + (size_t)tile * P.list_stride;
         const uint32_t bx = J.bx, by = J.by;
-        const float* cx = J.cx + x0;
-        const float* cy = J.cy + y0;
-        const float* cz = J.cz + z0;
-        const bool staged = n <= kStageOps;
-        const uint64_t lv = LINEAR ? P.slot_live[slot] : ~0ull;
-        if (staged) stage_tape(T, J.ops, Lst, n);
-        if (thread_id() < 24) load_tile_axes(A, J, tx, ty, tz, thread_id());
-        if (thread_id() < 96) {
-            const uint32_t a = thread_id() >> 5, i = thread_id() & 31u;
-            const float* c = a == 0 ? cx : a == 1 ? cy : cz;
-            if (i < (a == 0 ? nx : a == 1 ? ny : nz)) pc[a][i] = c[i];
-            if (LINEAR) bm[a][i] = 0;
-        }
+        const uint16_t* Lst = P.lists + (size_t)tile * P.list_stride;
+        uint64_t lv = ~0ull;
+        uint32_t evaluated = 0;
         contour_reset(L);
-        __syncthreads();
-        auto point = [&](uint32_t q, float& x, float& y, float& z) {
-            const uint32_t iz = div16(q, nxy, mxy), r = q - iz * nxy;
-            const uint32_t iy = div16(r, nx, mx), ix = r - iy * nx;
-            x = pc[0][ix];
-            y = pc[1][iy];
-            z = pc[2][iz];
-            return (iz * by + iy) * bx + ix;
-        };
-        uint32_t evaluated = npts;
-        if (LINEAR) {
+        if constexpr (LINEAR != 0) {
+            lv = R.live;
+            // the rest of a long program's record (rows past the prefetched part)
+            const uint32_t need = (kRecRowsAt + 32u * ((n + 1u) >> 1) + 15u) >> 4;
+            for (uint32_t k = kRecPre + thread_id(); k < need; k += blockDim.x)
+                reinterpret_cast<uint4*>(&R)[k] = P.slot_rec[(size_t)slot * kRecVec + k];
             // The points that some live brick uses.  Per box row (iz, iy): the x-bricks of the
             // live bricks whose cells use it (rm), hence the row's needed x positions (the
             // union of those x-bricks' positions, xb).  One row per thread; each wave reserves
             // its rows' points in `list` with one atomic; entries are iz << 10 | iy << 5 | ix.
             // The list is evaluated window by window (one window unless more than kList points).
-            if (thread_id() >= 8 && thread_id() < 24) {
+            const TileAxes& A = R.A;
+            if (thread_id() >= 8 && thread_id() < 24) {  // bm is zero here (cleared below)
                 const uint32_t ax = thread_id() >> 3, c = thread_id() & 7u;
                 atomicOr(&bm[ax][A.lo[ax][c]], 1u << (c >> 1));
                 atomicOr(&bm[ax][A.hi[ax][c]], 1u << (c >> 1));
@@ -944,6 +989,7 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
             if (lane() == 63 && inc) wbase = atomicAdd(&nlist, inc);
             const uint32_t off = __shfl(wbase, 63, 64) + inc - cnt;
             __syncthreads();
+            if (thread_id() < 64) bm[1 + (thread_id() >> 5)][thread_id() & 31u] = 0;  // for the next slot
             const uint32_t total = nlist;
             evaluated = total;
             for (uint32_t w0 = 0; w0 < total; w0 += kList) {
@@ -963,26 +1009,43 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
                         const uint32_t i = i0 + p * blockDim.x;
                         const uint32_t e = list[i < m ? i : i0];
                         const uint32_t ix = e & 31u, iy = (e >> 5) & 31u, iz = e >> 10;
-                        x[p] = pc[0][ix];
-                        y[p] = pc[1][iy];
-                        z[p] = pc[2][iz];
+                        x[p] = R.pc[0][ix];
+                        y[p] = R.pc[1][iy];
+                        z[p] = R.pc[2][iz];
                         at[p] = (iz * by + iy) * bx + ix;
                     }
-                    run_linear<kLinearPoints>(T, n, x, y, z, v);
+                    run_linear<kLinearPoints>(R.T, n, x, y, z, v);
 #pragma unroll
                     for (int p = 0; p < kLinearPoints; ++p)
                         if (i0 + p * blockDim.x < m) box[at[p]] = v[p];
                 }
             }
+            if (thread_id() == 0) sh_next = next;
         } else {
-            for (uint32_t q = thread_id(); q < npts; q += blockDim.x) {
-                float x, y, z;
-                const uint32_t at = point(q, x, y, z);
-                box[at] = staged ? run_general(T, n, x, y, z) : tape_eval_list(J.ops, Lst, n, x, y, z);
+            const uint32_t x0 = bxh & 0xffffffu, y0 = byh & 0xffffffu, z0 = bzh & 0xffffffu;
+            const uint32_t nxy = nx * ny, npts = nxy * nz;
+            const uint32_t mx = magic16(nx), mxy = magic16(nxy);
+            const bool staged = n <= kStageOps;
+            if (staged) stage_tape(R.T, J.ops, Lst, n);
+            if (thread_id() < 24) load_tile_axes(R.A, J, tx, ty, J.tz0 + tzl, thread_id());
+            if (thread_id() < 48) {
+                const uint32_t a = thread_id() >> 4, i = thread_id() & 15u;
+                const float* c = a == 0 ? J.cx + x0 : a == 1 ? J.cy + y0 : J.cz + z0;
+                if (i < (a == 0 ? nx : a == 1 ? ny : nz)) R.pc[a][i] = c[i];
             }
+            __syncthreads();
+            for (uint32_t q = thread_id(); q < npts; q += blockDim.x) {
+                const uint32_t iz = div16(q, nxy, mxy), r = q - iz * nxy;
+                const uint32_t iy = div16(r, nx, mx), ix = r - iy * nx;
+                const float x = R.pc[0][ix], y = R.pc[1][iy], z = R.pc[2][iz];
+                box[(iz * by + iy) * bx + ix] =
+                    staged ? run_general(R.T, n, x, y, z) : tape_eval_list(J.ops, Lst, n, x, y, z);
+            }
+            evaluated = npts;
         }
         __syncthreads();  // the box is complete
-        contour_tile(P, J, slot, tx, ty, tzl, box, A, lv, list, L);
+        if (LINEAR) prefetch();  // the next slot's record, while this one is contoured
+        contour_tile(P, J, slot, tx, ty, tzl, box, R.A, lv, list, L);
         if (thread_id() == 0) {
             tiles_done += 1;
             points_done += evaluated;
@@ -999,7 +1062,7 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
 template <int LINEAR>
 __global__ void k_sp_tiles(SpPass P);
 template <>
-__global__ __launch_bounds__(kTileThreads) void k_sp_tiles<1>(SpPass P)
+__global__ __launch_bounds__(kTileThreads) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_sp_tiles<1>(SpPass P)
 {
     tiles_body<1>(P);
 }
