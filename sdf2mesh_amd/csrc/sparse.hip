@@ -922,12 +922,15 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
             else return P.slot_hdr[__builtin_amdgcn_readfirstlane(slot)];
         }();
         const uint32_t lens = __builtin_amdgcn_readfirstlane(H.lens);
-        if (((lens & kLinearFlag) != 0) != (LINEAR != 0)) {  // same for the whole block
+        auto skip = [&]() {  // to the next slot (left folds: its record is read first)
             if (LINEAR) {
                 if (thread_id() == 0) sh_next = next;
                 __syncthreads();
                 prefetch();
             }
+        };
+        if (((lens & kLinearFlag) != 0) != (LINEAR != 0)) {  // same for the whole block
+            skip();
             continue;
         }
         const uint64_t tile = __builtin_amdgcn_readfirstlane(H.tile);
@@ -945,6 +948,17 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
         contour_reset(L);
         if constexpr (LINEAR != 0) {
             lv = R.live;
+            if (lv == 0) {  // no brick can hold an active cell: empty rows, no cell bits
+                if (thread_id() < 64) slot_row(P, kRowVtx, slot)[thread_id()] = 0;
+                if (thread_id() < 16) P.cellbits[(size_t)slot * 16u + thread_id()] = 0;
+                if (thread_id() == 0) {
+                    P.slot_stage[2 * slot] = 0;
+                    P.slot_stage[2 * slot + 1] = 0;
+                    tiles_done += 1;
+                }
+                skip();
+                continue;
+            }
             // the rest of a long program's record (rows past the prefetched part)
             const uint32_t need = (kRecRowsAt + 32u * ((n + 1u) >> 1) + 15u) >> 4;
             for (uint32_t k = kRecPre + thread_id(); k < need; k += blockDim.x)
@@ -1120,13 +1134,13 @@ __global__ __launch_bounds__(64) void k_sp_quadcount(SpPass P, MeshBufs b)
     __shared__ uint32_t rowt[64];  // triangles of each row segment of the tile
     const uint32_t nslots = active_slots(P);
     for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
-        const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
-        const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(P.job_of_slot[slot])];
+        const SlotHdr& H = P.slot_hdr[slot];
+        const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(H.job)];
         const uint32_t base = __builtin_amdgcn_readfirstlane(P.slot_stage[2 * slot]);
         const uint32_t n = __builtin_amdgcn_readfirstlane(P.slot_stage[2 * slot + 1]);
         const bool lag = (J.flags & S2M_COMPAT_DROP_LAST_SLICE) != 0;
-        uint32_t tx, ty, tzl;
-        tile_xyz(J, tile, tx, ty, tzl);
+        const uint32_t txy = __builtin_amdgcn_readfirstlane(H.txy);
+        const uint32_t tx = txy & 0xffffu, ty = txy >> 16, tzl = __builtin_amdgcn_readfirstlane(H.tzl);
         __syncthreads();  // the previous slot's readers of ex / rowt are done
         for (uint32_t t = thread_id(); t < 81u; t += blockDim.x)
             ex[t] = nb_row<false>(P, J, tx, (int)(ty * 8u + t % 9u) - 1, (int)(J.z0 + tzl * 8u + t / 9u) - 1,
@@ -1178,22 +1192,32 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
     __shared__ uint32_t tc[128];  // triangles of cell c in byte c % 4 of tc[c / 4]
     const uint32_t nslots = active_slots(P);
     for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
-        const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
-        const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(P.job_of_slot[slot])];
+        const SlotHdr& H = P.slot_hdr[slot];
+        const uint64_t tile = __builtin_amdgcn_readfirstlane(H.tile);
+        const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(H.job)];
         const uint32_t base = __builtin_amdgcn_readfirstlane(P.slot_stage[2 * slot]);
         const uint32_t n = __builtin_amdgcn_readfirstlane(P.slot_stage[2 * slot + 1]);
-        uint32_t tx, ty, tzl;
-        tile_xyz(J, tile, tx, ty, tzl);
+        const uint32_t txy = __builtin_amdgcn_readfirstlane(H.txy);
+        const uint32_t tx = txy & 0xffffu, ty = txy >> 16, tzl = __builtin_amdgcn_readfirstlane(H.tzl);
         const bool normals = !(J.flags & S2M_NO_NORMALS);
         const bool lag = (J.flags & S2M_COMPAT_DROP_LAST_SLICE) != 0;
         const uint32_t vbase = (uint32_t)P.job_out[(size_t)J.out_index * kJobOut + 0];
         const uint16_t* Lst = P.lists + (size_t)tile * P.list_stride;
-        const uint32_t lens = __builtin_amdgcn_readfirstlane(P.lens[tile]);
+        const uint32_t lens = __builtin_amdgcn_readfirstlane(H.lens);
         const uint32_t len = lens & ~kLinearFlag;
         const bool linear = (lens & kLinearFlag) != 0, staged = len <= kStageOps;
         if (linear != (LINEAR != 0)) continue;  // the other launch's slot
         __syncthreads();  // the previous slot's readers of T, nb, tc are done
-        if (normals && staged) stage_tape(T, J.ops, Lst, len);
+        if (normals) {
+            if constexpr (LINEAR != 0) {  // the program as staged in the slot's record
+                const uint32_t words = (uint32_t)(offsetof(StagedFold, f) + 32u * ((len + 1u) >> 1)) / 4u;
+                const uint32_t* src =
+                    reinterpret_cast<const uint32_t*>(P.slot_rec + (size_t)slot * kRecVec) + offsetof(TileRec, T) / 4u;
+                for (uint32_t k = thread_id(); k < words; k += blockDim.x) reinterpret_cast<uint32_t*>(&T)[k] = src[k];
+            } else if (staged) {
+                stage_tape(T, J.ops, Lst, len);
+            }
+        }
         for (uint32_t t = thread_id(); t < 81u; t += blockDim.x)
             (void)nb_row<true>(P, J, tx, (int)(ty * 8u + t % 9u) - 1, (int)(J.z0 + tzl * 8u + t / 9u) - 1,
                                nb + 9u * t);
