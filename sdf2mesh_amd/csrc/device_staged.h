@@ -205,16 +205,4 @@ __device__ __forceinline__ float run_general(const TT& T, uint32_t n, float px, 
     return acc;
 }
 
-// Is the postfix program of these codes a left fold of library primitives?
-__device__ __forceinline__ bool is_left_fold(const s2m_op* ops, const uint16_t* L, uint32_t n)
-{
-    if (n == 0 || n > kStageOps || (n & 1u) == 0) return false;
-    for (uint32_t k = 0; k < n; ++k) {
-        const uint32_t c = ops[L[k]].code;
-        const bool prim = c < S2M_OP_UNION;
-        if (prim != (k == 0 || (k & 1u)) || c == S2M_OP_MANDELBULB) return false;
-    }
-    return true;
-}
-
 }  // namespace s2m

@@ -221,32 +221,45 @@ namespace {
 struct PruneStack {
     double* lo;        // [depth][64]
     double* hi;        // [depth][64]
+    uint32_t* sum;     // [depth][64] PruneWalk::top_sum of the entry
     uint16_t* start;   // [depth][64]
     uint16_t* single;  // [depth][64] op index of a one-primitive subtree, else kComposite
 
     // LDS bytes of a stack of `depth` entries per lane, and its layout from `base`.
-    static constexpr size_t bytes(uint32_t depth) { return (size_t)depth * 64u * (2 * sizeof(double) + 2 * sizeof(uint16_t)); }
+    static constexpr size_t bytes(uint32_t depth)
+    {
+        return (size_t)depth * 64u * (2 * sizeof(double) + sizeof(uint32_t) + 2 * sizeof(uint16_t));
+    }
     __device__ static PruneStack at(double* base, uint32_t depth)
     {
-        uint16_t* s16 = reinterpret_cast<uint16_t*>(base + 128u * depth);
-        return PruneStack{base, base + 64u * depth, s16, s16 + 64u * depth};
+        uint32_t* s32 = reinterpret_cast<uint32_t*>(base + 128u * depth);
+        uint16_t* s16 = reinterpret_cast<uint16_t*>(s32 + 64u * depth);
+        return PruneStack{base, base + 64u * depth, s32, s16, s16 + 64u * depth};
     }
 };
 
 constexpr uint32_t kComposite = 0xffffu;
+constexpr uint32_t kFoldBit = 0x80000000u;  // PruneWalk sums: the subtree is a left fold
 
 struct PruneWalk {
     Iv top = iv_unknown();
     uint32_t top_start = 0, depth = 0, len = 0;  // depth: entries below `top`
     uint32_t top_single = kComposite;  // op index when `top` is one primitive
+    // flops of the kept ops of the top subtree (op_flops), | kFoldBit when that subtree is a
+    // left fold of library primitives (a primitive other than the Mandelbulb, or a left
+    // fold, such a primitive and an operator): the tile's flops and left-fold flag without
+    // reading its list back
+    uint32_t top_sum = 0;
 
-    // The primitive with op index i, interval v (q: its position in the program).
+    // The primitive with op index i, interval v (q: its position in the program; sum: its
+    // flops | kFoldBit unless it is a Mandelbulb).
     __device__ __forceinline__ void prim(uint32_t q, uint32_t i, const Iv& v, uint16_t* out, const PruneStack& S,
-                                         uint32_t ln)
+                                         uint32_t ln, uint32_t sum = 0)
     {
         if (q != 0) {  // push the previous top
             S.lo[depth * 64u + ln] = top.lo;
             S.hi[depth * 64u + ln] = top.hi;
+            S.sum[depth * 64u + ln] = top_sum;
             S.start[depth * 64u + ln] = (uint16_t)top_start;
             S.single[depth * 64u + ln] = (uint16_t)top_single;
             ++depth;
@@ -254,12 +267,13 @@ struct PruneWalk {
         top = v;
         top_start = len;
         top_single = i;
+        top_sum = sum;
         out[len++] = (uint16_t)i;
     }
 
-    // Operator i (code, smoothing radius k) on the two topmost values.
+    // Operator i (code, smoothing radius k, flops fl) on the two topmost values.
     __device__ __forceinline__ void op(uint32_t i, uint32_t code, float k, uint16_t* out, const PruneStack& S,
-                                       uint32_t ln)
+                                       uint32_t ln, uint32_t fl = 0)
     {
         const Iv B = top;
         const uint32_t sB = top_start;
@@ -267,16 +281,20 @@ struct PruneWalk {
         const double alo = S.lo[depth * 64u + ln], ahi = S.hi[depth * 64u + ln];
         const Iv A{alo, ahi, alo > -kHuge};  // iv_make / iv_unknown: known iff lo > -kHuge
         const uint32_t sA = S.start[depth * 64u + ln], singleA = S.single[depth * 64u + ln];
+        const uint32_t sumA = S.sum[depth * 64u + ln];
         int keep;
         const Iv r = iv_op(code, A, B, k, keep);
         if (keep == kKeepBoth) {
             out[len++] = (uint16_t)i;
+            const bool fold = (sumA & kFoldBit) && top_single != kComposite && (top_sum & kFoldBit);
+            top_sum = ((sumA & ~kFoldBit) + (top_sum & ~kFoldBit) + fl) | (fold ? kFoldBit : 0u);
             top = r;
             top_single = kComposite;
         } else if (keep == kKeepA) {
             len = sB;  // drop B's subtree and the operator
             top = A;
             top_single = singleA;
+            top_sum = sumA;
         } else {       // drop A's subtree and the operator
             if (top_single != kComposite) {  // B is one primitive: no read-back of the list
                 out[sA] = (uint16_t)top_single;
@@ -514,6 +532,7 @@ __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
     extern __shared__ double prune_lds[];
     __shared__ StagedTape T;          // the super-tile's list (at most kStageOps ops)
     __shared__ uint16_t idx[kStageOps];  // op index of each staged op
+    __shared__ uint32_t sums[kStageOps];  // its flops | kFoldBit for primitives other than the Mandelbulb
     const uint32_t D = P.prune_depth, ln = thread_id() & 63u;
     const PruneStack S = PruneStack::at(prune_lds, D);
     const uint64_t s = blockIdx.x;
@@ -528,12 +547,20 @@ __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
     const s2m_op* ops = J.ops;
     const uint16_t* in = P.super_lists + s * P.list_stride;
     const bool staged = sl != kSingleSign && sl <= kStageOps;
+    auto op_sum = [&](uint32_t i) {
+        const uint32_t c = ops[i].code;
+        return J.op_flops[i] | (c < S2M_OP_UNION && c != S2M_OP_MANDELBULB ? kFoldBit : 0u);
+    };
     if (staged) {
         stage_tape(T, ops, in, sl);
-        for (uint32_t k = ln; k < sl; k += 64u) idx[k] = in[k];
+        for (uint32_t k = ln; k < sl; k += 64u) {
+            idx[k] = in[k];
+            sums[k] = op_sum(in[k]);
+        }
     }
     __syncthreads();
     bool surface = false;
+    uint32_t lens = 0;
     if (valid && sl == kSingleSign) {  // no surface anywhere in the super-tile
         P.lens[tile] = 0;
         P.flops[tile] = 0;
@@ -545,23 +572,24 @@ __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
         PruneWalk w;
         if (staged) {  // every lane walks the same list: wave-uniform dispatch
             for (uint32_t q = 0; q < sl; ++q) {
-                const uint32_t code = uniform_code(T, q), i = idx[q];
-                if (code < S2M_OP_UNION) w.prim(q, i, iv_primitive(code, T.f[q], bl, bh), L, S, ln);
-                else w.op(i, code, T.f[q][0], L, S, ln);
+                const uint32_t code = uniform_code(T, q), i = idx[q], sum = sums[q];
+                if (code < S2M_OP_UNION) w.prim(q, i, iv_primitive(code, T.f[q], bl, bh), L, S, ln, sum);
+                else w.op(i, code, T.f[q][0], L, S, ln, sum);
             }
         } else {
             for (uint32_t q = 0; q < sl; ++q) {
                 const uint32_t i = __builtin_amdgcn_readfirstlane(in[q]);
-                const uint32_t code = ops[i].code;
-                if (code < S2M_OP_UNION) w.prim(q, i, iv_primitive(code, ops[i].f, bl, bh), L, S, ln);
-                else w.op(i, code, ops[i].f[0], L, S, ln);
+                const uint32_t code = ops[i].code, sum = op_sum(i);
+                if (code < S2M_OP_UNION) w.prim(q, i, iv_primitive(code, ops[i].f, bl, bh), L, S, ln, sum);
+                else w.op(i, code, ops[i].f[0], L, S, ln, sum);
             }
         }
         surface = !w.one_sign();
-        uint32_t fl = 0;
-        for (uint32_t j = 0; j < w.len; ++j) fl += J.op_flops[L[j]];
-        P.lens[tile] = w.len | (surface && is_left_fold(ops, L, w.len) ? kLinearFlag : 0u);
-        P.flops[tile] = fl;
+        // the pruned program is one subtree: its flops and whether it is a left fold
+        const bool fold = (w.top_sum & kFoldBit) && w.len <= kStageOps;
+        lens = w.len | (surface && fold ? kLinearFlag : 0u);
+        P.lens[tile] = lens;
+        P.flops[tile] = w.top_sum & ~kFoldBit;
     }
     // slots for the surviving tiles: one counter update per wave
     const uint64_t m = __ballot(surface);
@@ -580,7 +608,7 @@ __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
     if (valid) P.slot_of_tile[tile] = slot;
     const uint64_t act = __ballot(slot >= 0);  // bit l: tile l of the super-tile has a slot
     if (ln == 0) P.super_act[s] = act;
-    const uint64_t lin = __ballot(slot >= 0 && (P.lens[tile] & kLinearFlag));
+    const uint64_t lin = __ballot(slot >= 0 && (lens & kLinearFlag));
     if (lin && ln == (uint32_t)__builtin_ctzll(lin))
         atomicAdd((unsigned long long*)&P.counters[kLinearSlots], (unsigned long long)__builtin_popcountll(lin));
 }
