@@ -6,15 +6,18 @@
 // are evaluated and contoured.  A pass covers the tiles of one slab of one job, or of
 // every job of a batch (each job whole); tiles, super-tiles and row segments of the pass
 // are numbered job after job.  In stream order:
-//   k_sp_prune_prims / k_sp_prune_super / k_sp_prune   exact interval pruning: the
-//                   job's program on each super-tile's padded box (primitive intervals in
-//                   parallel, then one lane per super-tile combines them in program order),
-//                   then the super-tile's list on each of its tiles (one lane per tile);
-//                   single-signed tiles are dropped, the others get a slot in a pool shared
-//                   by all jobs (and a flag when their list is a left fold)
+//   k_sp_prune_fold (left folds of unions / smooth unions) or k_sp_prune_prims +
+//   k_sp_prune_super (other programs), then k_sp_prune   exact interval pruning: the
+//                   job's program on each super-tile's padded box, then the super-tile's
+//                   list on each of its tiles (one lane per tile); single-signed tiles are
+//                   dropped, the others get a slot in a pool shared by all jobs (and a
+//                   flag when their list is a left fold)
 //   k_sp_bricks     per slot, one lane per brick of 2x2x2 cells: interval bounds of a
-//                   left fold on the brick; bricks that keep one sign are not evaluated
-//   k_sp_tiles<1/0> per slot (one block): the pruned program, staged in LDS, at the
+//                   left fold on the brick (bricks that keep one sign are not evaluated),
+//                   and the slot's record: header, live bricks, axis tables, box
+//                   coordinates and the program in k_sp_tiles' LDS layout
+//   k_sp_tiles<1/0> per slot (one block): the pruned program in LDS (left folds: from the
+//                   record, read while the block contours the slot before), at the
 //                   lattice points the tile's live bricks use (left folds) or at all
 //                   points of its box (other programs), into an LDS box; then the tile's
 //                   cells: vertex and sign bits of each active cell, staged compactly, the
