@@ -45,6 +45,28 @@ struct StagedFoldN {
 };
 using StagedFold = StagedFoldN<kStageOps>;
 
+// A left fold in StagedFoldN's layout read in place from global memory: its address and
+// the op index are wave-uniform, so the reads are scalar loads (through the scalar data
+// cache, into SGPRs) and no LDS copy is made.  The memory is written by an earlier kernel.
+struct ScalarFold {
+    using cu32 = const __attribute__((address_space(4))) uint32_t;
+    using cf32 = const __attribute__((address_space(4))) float;
+    cu32* code4;  // the code bytes, four per word
+    cf32* f;      // the rows of 8 floats
+
+    __device__ static ScalarFold at(const void* staged_fold)
+    {
+        const char* base = static_cast<const char*>(staged_fold);
+        return ScalarFold{(cu32*)base, (cf32*)(base + offsetof(StagedFold, f))};
+    }
+    __device__ __forceinline__ uint32_t code_at(uint32_t k) const { return (code4[k >> 2] >> ((k & 3u) * 8u)) & 0xffu; }
+    __device__ __forceinline__ uint32_t iarg_at(uint32_t) const { return 0u; }
+    __device__ __forceinline__ cf32* f_at(uint32_t k) const
+    {
+        return (k & 1u) || k == 0 ? f + ((k + 1u) >> 1) * 8u : f + (k >> 1) * 8u + 7u;
+    }
+};
+
 // Copy ops[L[k]], k < n <= CAP, into T: worker t of nt copies entries t, t + nt, ... (by
 // default the threads of the block; the caller synchronises before and after).
 template <uint32_t CAP>
@@ -92,7 +114,7 @@ template <int NP, bool MANDEL, class TT>
 __device__ __forceinline__ void staged_prim(const TT& T, uint32_t k, uint32_t code, const float (&x)[NP],
                                             const float (&y)[NP], const float (&z)[NP], float (&d)[NP])
 {
-    const float* f = T.f_at(k);
+    const auto* f = T.f_at(k);
     switch (code) {
     case S2M_OP_SPHERE: {
         const float c0 = f[0], c1 = f[1], c2 = f[2], s = f[3];

@@ -1218,7 +1218,7 @@ __global__ __launch_bounds__(64) void k_sp_quadcount(SpPass P, MeshBufs b)
 template <int LINEAR>
 __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
 {
-    __shared__ std::conditional_t<LINEAR != 0, StagedFold, StagedTape> T;
+    __shared__ StagedTape T;      // programs that are not left folds (left folds: read in place)
     __shared__ uint32_t nb[729];  // vertex of cell (8tx-1+i, 8ty-1+j, 8tz-1+k) at nb[(k*9+j)*9+i]
     __shared__ uint32_t tc[128];  // triangles of cell c in byte c % 4 of tc[c / 4]
     const uint32_t nslots = active_slots(P);
@@ -1239,16 +1239,10 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
         const bool linear = (lens & kLinearFlag) != 0, staged = len <= kStageOps;
         if (linear != (LINEAR != 0)) continue;  // the other launch's slot
         __syncthreads();  // the previous slot's readers of T, nb, tc are done
-        if (normals) {
-            if constexpr (LINEAR != 0) {  // the program as staged in the slot's record
-                const uint32_t words = (uint32_t)(offsetof(StagedFold, f) + 32u * ((len + 1u) >> 1)) / 4u;
-                const uint32_t* src =
-                    reinterpret_cast<const uint32_t*>(P.slot_rec + (size_t)slot * kRecVec) + offsetof(TileRec, T) / 4u;
-                for (uint32_t k = thread_id(); k < words; k += blockDim.x) reinterpret_cast<uint32_t*>(&T)[k] = src[k];
-            } else if (staged) {
-                stage_tape(T, J.ops, Lst, len);
-            }
-        }
+        if (!LINEAR && normals && staged) stage_tape(T, J.ops, Lst, len);
+        // left folds: the program as staged in the slot's record, read in place
+        const ScalarFold TS =
+            ScalarFold::at(reinterpret_cast<const char*>(P.slot_rec + (size_t)slot * kRecVec) + offsetof(TileRec, T));
         for (uint32_t t = thread_id(); t < 81u; t += blockDim.x)
             (void)nb_row<true>(P, J, tx, (int)(ty * 8u + t % 9u) - 1, (int)(J.z0 + tzl * 8u + t / 9u) - 1,
                                nb + 9u * t);
@@ -1286,8 +1280,8 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
             const float ty4[4] = {py + (-eps), py + (-eps), py + eps, py + eps};
             const float tz4[4] = {pz + (-eps), pz + eps, pz + (-eps), pz + eps};
             float d[4];
-            if (LINEAR) {
-                run_linear<4>(T, len, tx4, ty4, tz4, d);
+            if constexpr (LINEAR != 0) {
+                run_linear<4>(TS, len, tx4, ty4, tz4, d);
             } else {
 #pragma unroll
                 for (int t = 0; t < 4; ++t)
