@@ -807,22 +807,27 @@ __device__ __forceinline__ void tile_corners(const float* box, const TileAxes& A
 
 __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, uint32_t slot, uint32_t tx,
                                              uint32_t ty, uint32_t tzl, const float* box, const TileAxes& A,
-                                             uint64_t live, uint16_t* cand, ContourLds& L)
+                                             uint32_t nlive, const uint8_t* lb, uint16_t* cand, ContourLds& L)
 {
-    // 1. cells with a sign change (thread t: cells t and t + 256, x fastest)
-#pragma unroll
-    for (uint32_t r = 0; r < 2; ++r) {
-        const uint32_t c = r * kTileThreads + thread_id();
-        const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
-        const uint32_t brick = ((czl >> 1) * 4u + (cyl >> 1)) * 4u + (cxl >> 1);
+    // 1. cells with a sign change: the 8 cells of each live brick (lb: the live bricks,
+    // nlive of them), one per thread
+    for (uint32_t k0 = 0; k0 < nlive * 8u; k0 += kTileThreads) {
+        const uint32_t k = k0 + thread_id();
+        uint32_t c = 0;
         bool cand_c = false;
-        if (tx * 8u + cxl < J.rx && ty * 8u + cyl < J.ry && tzl * 8u + czl < J.nz && ((live >> brick) & 1ull)) {
-            float cv[8];
-            tile_corners(box, A, J.bx, J.by, c, cv);
-            uint32_t pm = 0;
+        if (k < nlive * 8u) {
+            const uint32_t b = lb[k >> 3], j = k & 7u;  // brick (x fastest), cell within it
+            const uint32_t cxl = (b & 3u) * 2u + (j & 1u), cyl = ((b >> 2) & 3u) * 2u + ((j >> 1) & 1u);
+            const uint32_t czl = (b >> 4) * 2u + (j >> 2);
+            c = (czl * 8u + cyl) * 8u + cxl;
+            if (tx * 8u + cxl < J.rx && ty * 8u + cyl < J.ry && tzl * 8u + czl < J.nz) {
+                float cv[8];
+                tile_corners(box, A, J.bx, J.by, c, cv);
+                uint32_t pm = 0;
 #pragma unroll
-            for (int k = 0; k < 8; ++k) pm |= (uint32_t)(cv[k] > 0.0f) << k;
-            cand_c = pm != 0u && pm != 0xffu;
+                for (int i = 0; i < 8; ++i) pm |= (uint32_t)(cv[i] > 0.0f) << i;
+                cand_c = pm != 0u && pm != 0xffu;
+            }
         }
         const uint64_t w = __ballot(cand_c);
         const uint32_t lead = (uint32_t)__builtin_ctzll(w | (1ull << 63));
@@ -921,6 +926,7 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
     __shared__ uint32_t nlist;
     __shared__ ContourLds L;
     __shared__ uint32_t sh_slot, sh_next;
+    __shared__ uint8_t lb[64];  // the live bricks of the slot (contour_tile)
     const uint32_t nslots = active_slots(P), nlinear = (uint32_t)P.counters[kLinearSlots];
     if (LINEAR ? nlinear == 0 : nlinear >= nslots) return;  // no slot of this kind
     constexpr uint32_t kQueue = LINEAR ? 0u : 1u;
@@ -1088,9 +1094,11 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
             }
             evaluated = npts;
         }
+        if (thread_id() < 64 && ((lv >> thread_id()) & 1ull))  // the live bricks, in order
+            lb[__builtin_popcountll(lv & ((1ull << thread_id()) - 1ull))] = (uint8_t)thread_id();
         __syncthreads();  // the box is complete
         if (LINEAR) prefetch();  // the next slot's record, while this one is contoured
-        contour_tile(P, J, slot, tx, ty, tzl, box, R.A, lv, list, L);
+        contour_tile(P, J, slot, tx, ty, tzl, box, R.A, (uint32_t)__builtin_popcountll(lv), lb, list, L);
         if (thread_id() == 0) {
             tiles_done += 1;
             points_done += evaluated;
