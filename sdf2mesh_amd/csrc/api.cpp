@@ -130,6 +130,7 @@ struct s2m_ctx {
     DevBuf sp_stage, sp_slot_stage, sp_slot_live, sp_slot_hdr, sp_slot_rec;
     DevBuf sp_job_out, sp_queues, sp_super_iv, sp_super_lists, sp_super_lens, sp_super_act, sp_rows;
     uint64_t slot_cap = 0;
+    uint64_t stage_cap = 0;  // staged-cell entries of sp_stage
     int cus = 256;
     PinnedBuf host_img;   // staging of the last pass's tables (kept until its sync)
     PinnedBuf host_back;  // device counters and per-job ranges read back after a pass
@@ -620,7 +621,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     uint64_t want_t = std::max<uint64_t>(3 * want_v, ctx->tcap);
     uint64_t want_s = std::min<uint64_t>(tiles_pass, std::max<uint64_t>({4096, tiles_pass / 4, ctx->slot_cap}));
     const uint64_t tile_blocks = (uint64_t)std::max(1, ctx->cus) * kTileBlocksPerCU;
-    uint64_t stage_cap = 0;
+    uint64_t stage_cap = ctx->stage_cap;
 
     for (int attempt = 0; attempt < 4; ++attempt) {
         HIP_TRY(ctx->vtx.ensure(want_v * 6 * sizeof(float)));
@@ -631,10 +632,11 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         HIP_TRY(ctx->sp_job_of_slot.ensure(want_s * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_cellbits.ensure(want_s * 16 * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_slot_rows.ensure(want_s * 4 * 64 * sizeof(uint32_t)));
-        // staged cells are allocated in per-block chunks: about 2 x vertices + blocks x chunk
-        // (two launches); the device count is checked below
+        // staged cells are reserved 8 per live brick of a slot, in per-block chunks: about
+        // 2 x vertices + blocks x chunk (two launches); the device count is checked below
         stage_cap = std::max(stage_cap, 2 * want_v + 2 * tile_blocks * kStageChunkHost);
         HIP_TRY(ctx->sp_stage.ensure(stage_cap * sizeof(float4)));
+        ctx->stage_cap = stage_cap;
         HIP_TRY(ctx->sp_slot_stage.ensure(want_s * 2 * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_slot_live.ensure(want_s * sizeof(uint64_t)));
         HIP_TRY(ctx->sp_slot_hdr.ensure(want_s * sizeof(SlotHdr)));

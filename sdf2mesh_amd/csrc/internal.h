@@ -69,7 +69,7 @@ constexpr uint32_t kSuper = 4;  // tiles per super-tile edge (first pruning leve
 constexpr uint32_t kSingleSign = 0xffffffffu;  // super_lens: the SDF keeps one sign there
 constexpr uint32_t kStageChunkHost = 1024;  // staged-cell allocation chunk (sparse.hip)
 constexpr uint32_t kTileBlocksPerCU = 8;    // most k_sp_tiles blocks per compute unit
-constexpr uint32_t kTileRecBytes = 2832;    // per slot: k_sp_tiles' staged record (sparse.hip)
+constexpr uint32_t kTileRecBytes = 2720;    // per slot: k_sp_tiles' staged record (sparse.hip)
 constexpr uint32_t kQueues = 2;        // work queues of a pass (k_sp_tiles<1>, k_sp_tiles<0>)
 constexpr uint32_t kQueueGroups = 64;  // counters per queue, 32 u32 (128 bytes) apart
 // Profiling marks of one sparse pass: prune (tiles, bricks) | eval+contour | quad count |

@@ -619,7 +619,7 @@ __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
 // Per-axis corner tables of a tile's 8 cells: box-relative lattice index and coordinate
 // of each cell's lo / hi corner.
 struct TileAxes {
-    uint32_t lo[3][8], hi[3][8];
+    uint8_t lo[3][8], hi[3][8];  // box indices (< 16)
     float clo[3][8], chi[3][8];
 };
 
@@ -651,16 +651,17 @@ __device__ __forceinline__ void load_tile_axes(TileAxes& A, const SpJob& J, uint
 struct alignas(16) TileRec {
     SlotHdr H;
     uint64_t live;              // bricks (2x2x2 cells) that may hold an active cell
-    uint32_t pad[2];
+    uint16_t xb[4];             // per x-brick: the box x indices its cells use (bit mask)
     TileAxes A;
+    uint8_t bm[2][16];          // per box y (z) index: the y (z) bricks whose cells use it
     float pc[3][16];            // lattice coordinates of the box per axis
     StagedFold T;               // the pruned program (the used rows are read)
 };
 static_assert(sizeof(TileRec) == kTileRecBytes, "record size shared with the host");
 constexpr uint32_t kRecVec = kTileRecBytes / 16;   // 16-byte words per record
-constexpr uint32_t kRecPre = 64;                    // words read ahead: header .. codes, ops 0..14
+constexpr uint32_t kRecPre = 64;                    // words read ahead: header .. codes, ops 0..22
 constexpr uint32_t kRecRowsAt = (uint32_t)offsetof(TileRec, T) + (uint32_t)offsetof(StagedFold, f);
-static_assert(kRecRowsAt + 32u * 8u <= 16u * kRecPre, "ops 0..14 are read ahead");
+static_assert(kRecRowsAt + 32u * 12u <= 16u * kRecPre, "ops 0..22 are read ahead");
 struct TileGen {  // k_sp_tiles<0>: the same parts, staged per slot
     StagedTape T;
     TileAxes A;
@@ -744,6 +745,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
                 reinterpret_cast<uint32_t*>(&RG.T)[k] = reinterpret_cast<const uint32_t*>(&T)[k];
             for (uint32_t k = b; k < (uint32_t)sizeof(TileAxes) / 4u; k += 64u)
                 reinterpret_cast<uint32_t*>(&RG.A)[k] = reinterpret_cast<const uint32_t*>(&A)[k];
+            if (b < 4) {
+                const uint32_t c = 2u * b;
+                RG.xb[b] = (uint16_t)((1u << A.lo[0][c]) | (1u << A.hi[0][c]) | (1u << A.lo[0][c + 1]) |
+                                      (1u << A.hi[0][c + 1]));
+            } else if (b >= 32) {  // lane 32 + 16 (a - 1) + i: axis a (y, z), box index i
+                const uint32_t a = 1u + ((b - 32u) >> 4), i = b & 15u;
+                uint32_t m = 0;
+                for (uint32_t c = 0; c < 8u; ++c)
+                    if (A.lo[a][c] == i || A.hi[a][c] == i) m |= 1u << (c >> 1);
+                RG.bm[a - 1u][i] = (uint8_t)m;
+            }
         }
         // brick b: cells c, c + 1 per axis; the first cell's lo corner, the second's hi corner
         const uint32_t c[3] = {(b & 3u) * 2u, ((b >> 2) & 3u) * 2u, (b >> 4) * 2u};
@@ -792,6 +804,18 @@ __device__ __forceinline__ void contour_reset(ContourLds& L)
 {
     if (thread_id() < 64) L.rows[thread_id()] = 0;
     if (thread_id() == 0) L.ncand = L.nact = 0;
+}
+
+// Thread 0: staging space for the slot's cells with a vertex, at most `cells` (8 per live
+// brick), from the block's chunk; published by the barriers before contour_tile.
+__device__ __forceinline__ void contour_reserve(const SpPass& P, ContourLds& L, uint32_t cells)
+{
+    if (L.left < cells) {
+        L.base = (uint32_t)atomicAdd((unsigned long long*)&P.counters[kStageUsed], (unsigned long long)kStageChunk);
+        L.left = kStageChunk;
+    }
+    L.slot_base = L.base + (kStageChunk - L.left);
+    L.left -= cells;
 }
 
 __device__ __forceinline__ void tile_corners(const float* box, const TileAxes& A, uint32_t bx, uint32_t by,
@@ -869,17 +893,11 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
         }
     }
     __syncthreads();
-    // 3. staging space, row segment counts and the slot's cell bits
+    // 3. the staged cells (at the space reserved by contour_reserve), row segment counts
+    // and the slot's cell bits
     if (thread_id() == 0) {
-        const uint32_t n = L.nact;
-        if (L.left < n) {
-            L.base = (uint32_t)atomicAdd((unsigned long long*)&P.counters[kStageUsed], (unsigned long long)kStageChunk);
-            L.left = kStageChunk;
-        }
-        L.slot_base = L.base + (kStageChunk - L.left);
-        L.left -= n;
         P.slot_stage[2 * slot] = L.slot_base;
-        P.slot_stage[2 * slot + 1] = n;
+        P.slot_stage[2 * slot + 1] = L.nact;
     }
     if (thread_id() < 64) {
         const uint32_t row = thread_id();
@@ -888,7 +906,6 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
             P.cellbits[(size_t)slot * 16u + row] = L.rows[4 * row] | (L.rows[4 * row + 1] << 8) |
                                                    (L.rows[4 * row + 2] << 16) | (L.rows[4 * row + 3] << 24);
     }
-    __syncthreads();
     const uint32_t stage_base = L.slot_base;
 #pragma unroll
     for (uint32_t r = 0; r < 2; ++r) {
@@ -919,8 +936,6 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
     extern __shared__ float box[];  // P.box_stride floats
     // left folds: the slot's record (k_sp_bricks); other programs: staged here
     __shared__ std::conditional_t<LINEAR != 0, TileRec, TileGen> R;
-    __shared__ uint32_t bm[3][32];   // per axis (y, z) and box index: the bricks (along it) using it
-    __shared__ uint32_t xb[4];       // per x-brick: the box x indices its cells use
     // points to evaluate; then the contour's candidate cells (contour_tile's `cand`)
     __shared__ uint16_t list[LINEAR ? kList : 8 * 8 * 8];
     __shared__ uint32_t nlist;
@@ -941,13 +956,13 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
         if (thread_id() < kRecPre && s < nslots) pre = P.slot_rec[(size_t)s * kRecVec + thread_id()];
     };
     if constexpr (LINEAR != 0) {
-        if (thread_id() < 64) bm[1 + (thread_id() >> 5)][thread_id() & 31u] = 0;
+        if (thread_id() == 0) nlist = 0;
         if (thread_id() == 0) sh_next = next;
         __syncthreads();
         prefetch();
     }
     for (;;) {
-        __syncthreads();  // the previous slot's readers of sh_slot, R, xb, list, box, L are done
+        __syncthreads();  // the previous slot's readers of sh_slot, R, list, box, L are done
         if (thread_id() == 0) sh_slot = next;
         if (LINEAR && thread_id() < kRecPre) reinterpret_cast<uint4*>(&R)[thread_id()] = pre;
         __syncthreads();
@@ -1005,28 +1020,17 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
             // union of those x-bricks' positions, xb).  One row per thread; each wave reserves
             // its rows' points in `list` with one atomic; entries are iz << 10 | iy << 5 | ix.
             // The list is evaluated window by window (one window unless more than kList points).
-            const TileAxes& A = R.A;
-            if (thread_id() >= 8 && thread_id() < 24) {  // bm is zero here (cleared below)
-                const uint32_t ax = thread_id() >> 3, c = thread_id() & 7u;
-                atomicOr(&bm[ax][A.lo[ax][c]], 1u << (c >> 1));
-                atomicOr(&bm[ax][A.hi[ax][c]], 1u << (c >> 1));
-            }
-            if (thread_id() < 4) {
-                const uint32_t c = 2u * thread_id();
-                xb[thread_id()] = (1u << A.lo[0][c]) | (1u << A.hi[0][c]) | (1u << A.lo[0][c + 1]) | (1u << A.hi[0][c + 1]);
-            }
-            if (thread_id() == 0) nlist = 0;
-            __syncthreads();
+            if (thread_id() == 0) contour_reserve(P, L, 8u * (uint32_t)__builtin_popcountll(lv));
             const uint32_t q = thread_id();
             uint32_t M = 0, code = 0;
             if (q < ny * nz) {
                 const uint32_t iz = div16(q, ny, magic16(ny)), iy = q - iz * ny;
                 uint32_t rm = 0;
-                for (uint32_t mz = bm[2][iz]; mz; mz &= mz - 1u)
-                    for (uint32_t my = bm[1][iy]; my; my &= my - 1u)
+                for (uint32_t mz = R.bm[1][iz]; mz; mz &= mz - 1u)
+                    for (uint32_t my = R.bm[0][iy]; my; my &= my - 1u)
                         rm |= (uint32_t)(lv >> ((__builtin_ctz(mz) * 4 + __builtin_ctz(my)) * 4)) & 0xfu;
-                M = ((rm & 1u) ? xb[0] : 0u) | ((rm & 2u) ? xb[1] : 0u) | ((rm & 4u) ? xb[2] : 0u) |
-                    ((rm & 8u) ? xb[3] : 0u);
+                M = ((rm & 1u) ? R.xb[0] : 0u) | ((rm & 2u) ? R.xb[1] : 0u) | ((rm & 4u) ? R.xb[2] : 0u) |
+                    ((rm & 8u) ? R.xb[3] : 0u);
                 code = (iz << 10) | (iy << 5);
             }
             const uint32_t cnt = (uint32_t)__builtin_popcount(M);
@@ -1039,11 +1043,10 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
             uint32_t wbase = 0;
             if (lane() == 63 && inc) wbase = atomicAdd(&nlist, inc);
             const uint32_t off = __shfl(wbase, 63, 64) + inc - cnt;
-            __syncthreads();
-            if (thread_id() < 64) bm[1 + (thread_id() >> 5)][thread_id() & 31u] = 0;  // for the next slot
-            const uint32_t total = nlist;
-            evaluated = total;
-            for (uint32_t w0 = 0; w0 < total; w0 += kList) {
+            // window 0 is filled at once (every wave has its offsets); the total is read
+            // after the barrier that completes it
+            uint32_t total = 0;
+            for (uint32_t w0 = 0;; w0 += kList) {
                 if (w0) __syncthreads();  // the previous window's readers of the list are done
                 if (off < w0 + kList && off + cnt > w0) {
                     uint32_t at = off - w0;  // wraps below w0: skipped by the bound test
@@ -1051,6 +1054,7 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
                         if (at < kList) list[at] = (uint16_t)(code | (uint32_t)__builtin_ctz(mm));
                 }
                 __syncthreads();
+                if (w0 == 0) total = nlist;  // reset for the next slot after the box is complete
                 const uint32_t m = min(kList, total - w0);
                 for (uint32_t i0 = thread_id(); i0 < m; i0 += kLinearPoints * blockDim.x) {
                     float x[kLinearPoints], y[kLinearPoints], z[kLinearPoints], v[kLinearPoints];
@@ -1070,13 +1074,16 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
                     for (int p = 0; p < kLinearPoints; ++p)
                         if (i0 + p * blockDim.x < m) box[at[p]] = v[p];
                 }
+                if (w0 + kList >= total) break;
             }
+            evaluated = total;
             if (thread_id() == 0) sh_next = next;
         } else {
             const uint32_t x0 = bxh & 0xffffffu, y0 = byh & 0xffffffu, z0 = bzh & 0xffffffu;
             const uint32_t nxy = nx * ny, npts = nxy * nz;
             const uint32_t mx = magic16(nx), mxy = magic16(nxy);
             const bool staged = n <= kStageOps;
+            if (thread_id() == 0) contour_reserve(P, L, 8u * 8u * 8u);
             if (staged) stage_tape(R.T, J.ops, Lst, n);
             if (thread_id() < 24) load_tile_axes(R.A, J, tx, ty, J.tz0 + tzl, thread_id());
             if (thread_id() < 48) {
@@ -1097,7 +1104,10 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
         if (thread_id() < 64 && ((lv >> thread_id()) & 1ull))  // the live bricks, in order
             lb[__builtin_popcountll(lv & ((1ull << thread_id()) - 1ull))] = (uint8_t)thread_id();
         __syncthreads();  // the box is complete
-        if (LINEAR) prefetch();  // the next slot's record, while this one is contoured
+        if (LINEAR) {
+            if (thread_id() == 0) nlist = 0;  // every thread has read the total
+            prefetch();  // the next slot's record, while this one is contoured
+        }
         contour_tile(P, J, slot, tx, ty, tzl, box, R.A, (uint32_t)__builtin_popcountll(lv), lb, list, L);
         if (thread_id() == 0) {
             tiles_done += 1;
