@@ -128,11 +128,13 @@ struct s2m_ctx {
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_slot_rows, sp_boundary;
     DevBuf sp_stage, sp_slot_stage, sp_slot_live, sp_slot_hdr, sp_slot_rec;
-    DevBuf sp_job_out, sp_queues, sp_super_iv, sp_super_lists, sp_super_lens, sp_super_act, sp_rows;
+    DevBuf sp_queues, sp_super_iv, sp_super_lists, sp_super_lens, sp_super_act, sp_rows;
     uint64_t slot_cap = 0;
     uint64_t stage_cap = 0;  // staged-cell entries of sp_stage
     int cus = 256;
     PinnedBuf host_img;   // staging of the last pass's tables (kept until its sync)
+    std::vector<uint8_t> dev_img;  // the tables image in tile_tab (an identical image is not re-sent)
+    const void* dev_img_at = nullptr;  // ... at this device address
     PinnedBuf host_back;  // device counters and per-job ranges read back after a pass
     uint64_t vcap = 0, tcap = 0;
     uint64_t last_nvtx = 0, last_ntri = 0, last_invalid = 0;
@@ -595,7 +597,14 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     HIP_TRY(ctx->sp_boundary.ensure(std::max<uint64_t>(plane_max, 1) * sizeof(uint32_t)));
     for (auto& J : desc) J.boundary = ctx->sp_boundary.as<uint32_t>();
     std::memcpy(img + desc_off, desc.data(), desc.size() * sizeof(SpJob));
-    HIP_TRY(hipMemcpyAsync(dimg, img, img_bytes, hipMemcpyHostToDevice, ctx->stream));
+    // the tables of consecutive calls with the same jobs are identical: send a changed image only
+    if (ctx->dev_img_at != dimg || ctx->dev_img.size() != img_bytes ||
+        std::memcmp(ctx->dev_img.data(), img, img_bytes) != 0) {
+        ctx->dev_img_at = nullptr;  // until the copy is queued
+        HIP_TRY(hipMemcpyAsync(dimg, img, img_bytes, hipMemcpyHostToDevice, ctx->stream));
+        ctx->dev_img.assign(img, img + img_bytes);
+        ctx->dev_img_at = dimg;
+    }
     const SpJob* ddesc = reinterpret_cast<const SpJob*>(dimg + desc_off);
 
     HIP_TRY(ctx->tile_lists.ensure(tiles_pass * list_stride * sizeof(uint16_t)));
@@ -606,8 +615,8 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     const uint32_t nsb_max = (uint32_t)((nrows_pass + 4095) / 4096) + 1;  // vertex and triangle sums
     HIP_TRY(ctx->bcount.ensure((size_t)2 * nsb_max * sizeof(uint32_t)));
     HIP_TRY(ctx->boffset.ensure((size_t)2 * nsb_max * sizeof(uint32_t)));
-    HIP_TRY(ctx->counters.ensure(kNumCounters * sizeof(uint64_t)));
-    HIP_TRY(ctx->sp_job_out.ensure((size_t)n * kJobOut * sizeof(uint64_t)));
+    // the counters, then per job kJobOut words (first vertex, first triangle, invalid quads)
+    HIP_TRY(ctx->counters.ensure((kNumCounters + (size_t)n * kJobOut) * sizeof(uint64_t)));
     HIP_TRY(ctx->sp_queues.ensure((size_t)kQueues * kQueueGroups * 32 * sizeof(uint32_t)));
     const uint64_t nsup_alloc = std::max<uint64_t>(nsuper_pass, 1);
     HIP_TRY(ctx->sp_super_iv.ensure(nsup_alloc * list_stride * 2 * sizeof(double)));
@@ -656,8 +665,9 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         b.bcount = ctx->bcount.as<uint32_t>();
         b.boffset = ctx->boffset.as<uint32_t>();
 
-        HIP_TRY(hipMemsetAsync(b.counters, 0, kNumCounters * sizeof(uint64_t), ctx->stream));
-        HIP_TRY(hipMemsetAsync(ctx->sp_job_out.p, 0, (size_t)n * kJobOut * sizeof(uint64_t), ctx->stream));
+        uint64_t* job_out = b.counters + kNumCounters;
+        HIP_TRY(launch_sparse_job_begin(ctx->stream, b.counters, kNumCounters + n * kJobOut,
+                                        ctx->sp_queues.as<uint32_t>()));
         if (n == 1 && npass > 1)
             HIP_TRY(hipMemsetAsync(ctx->sp_boundary.p, 0xff, plane_max * sizeof(uint32_t), ctx->stream));
 
@@ -716,13 +726,13 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.super_lists = ctx->sp_super_lists.as<uint16_t>();
             P.super_lens = ctx->sp_super_lens.as<uint32_t>();
             P.super_act = ctx->sp_super_act.as<uint64_t>();
-            P.job_out = ctx->sp_job_out.as<uint64_t>();
+            P.job_out = job_out;
             hipEvent_t m[kSparseMarks] = {};
             if (ctx->profiling >= 2)
                 for (auto& e : m) e = ctx->next_event();
             else if (ctx->profiling)  // the evaluation launches only: fewer markers between kernels
                 m[1] = ctx->next_event(), m[2] = ctx->next_event();
-            HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters, ctx->sp_queues.as<uint32_t>()));
+            if (pass > 0) HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters, ctx->sp_queues.as<uint32_t>()));
             const int boundary = (n == 1 && pass + 1 < npass) ? 1 : 0;
             HIP_TRY(launch_sparse_pass(ctx->stream, P, b, ctx->cus, (uint32_t)tile_blocks, general_tiles, fold_jobs,
                                        other_jobs, boundary,
@@ -738,10 +748,8 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         HIP_TRY(ctx->host_back.ensure((kNumCounters + (size_t)n * kJobOut) * sizeof(uint64_t)));
         const uint64_t* cnt = ctx->host_back.as<uint64_t>();
         const uint64_t* jo = cnt + kNumCounters;
-        HIP_TRY(hipMemcpyAsync(ctx->host_back.p, b.counters, kNumCounters * sizeof(uint64_t),
+        HIP_TRY(hipMemcpyAsync(ctx->host_back.p, b.counters, (kNumCounters + (size_t)n * kJobOut) * sizeof(uint64_t),
                                hipMemcpyDeviceToHost, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(ctx->host_back.as<uint64_t>() + kNumCounters, ctx->sp_job_out.p,
-                               (size_t)n * kJobOut * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         if (ctx->profiling) {
             float ms = 0;
@@ -1119,7 +1127,7 @@ void s2m_destroy(s2m_ctx* ctx)
                       &ctx->tile_tab, &ctx->tile_lists, &ctx->tile_lens,
                       &ctx->tile_flops, &ctx->sp_slot_of_tile, &ctx->sp_tile_of_slot,
                       &ctx->sp_job_of_slot, &ctx->sp_cellbits, &ctx->sp_slot_rows,
-                      &ctx->sp_boundary, &ctx->sp_job_out, &ctx->sp_queues, &ctx->sp_super_iv,
+                      &ctx->sp_boundary, &ctx->sp_queues, &ctx->sp_super_iv,
                       &ctx->sp_super_lists, &ctx->sp_super_lens, &ctx->sp_super_act, &ctx->sp_rows,
                       &ctx->sp_stage, &ctx->sp_slot_stage, &ctx->sp_slot_live, &ctx->sp_slot_hdr, &ctx->sp_slot_rec})
         b->release();

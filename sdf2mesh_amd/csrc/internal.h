@@ -181,6 +181,9 @@ hipError_t launch_normals(hipStream_t s, const s2m_op* ops, uint32_t nops, float
                           const MeshBufs& b);
 hipError_t launch_quads(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int emit);
 hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t* queues);
+// A job's first pass: counters[0 .. n) (the counters and the job output rows) and the
+// work queues cleared.
+hipError_t launch_sparse_job_begin(hipStream_t s, uint64_t* counters, uint32_t n, uint32_t* queues);
 // cus: compute units of the device; tile_blocks: most blocks of a k_sp_tiles launch (each
 // holds a staging chunk); general_tiles: some tile's pruned program may not be a left fold
 // (false when every job's whole program is one: its pruned programs are then left folds).

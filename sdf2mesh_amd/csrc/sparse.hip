@@ -1546,10 +1546,22 @@ __global__ void k_sp_pass_begin(uint64_t* counters, uint32_t* queues)
     for (uint32_t i = thread_id(); i < kQueues * kQueueGroups; i += blockDim.x) queues[i * 32u] = 0;
 }
 
+__global__ __launch_bounds__(256) void k_sp_job_begin(uint64_t* counters, uint32_t n, uint32_t* queues)
+{
+    for (uint32_t i = thread_id(); i < n; i += blockDim.x) counters[i] = 0;
+    for (uint32_t i = thread_id(); i < kQueues * kQueueGroups; i += blockDim.x) queues[i * 32u] = 0;
+}
+
 // ---------------------------------------------------------------------------------
 hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t* queues)
 {
     hipLaunchKernelGGL(k_sp_pass_begin, dim3(1), dim3(64), 0, s, counters, queues);
+    return hipGetLastError();
+}
+
+hipError_t launch_sparse_job_begin(hipStream_t s, uint64_t* counters, uint32_t n, uint32_t* queues)
+{
+    hipLaunchKernelGGL(k_sp_job_begin, dim3(1), dim3(256), 0, s, counters, n, queues);
     return hipGetLastError();
 }
 
