@@ -612,7 +612,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     HIP_TRY(ctx->tile_flops.ensure(tiles_pass * sizeof(uint32_t)));
     HIP_TRY(ctx->sp_slot_of_tile.ensure(tiles_pass * sizeof(int32_t)));
     HIP_TRY(ctx->sp_rows.ensure(std::max<uint64_t>(nrows_pass, 1) * 4 * sizeof(uint32_t)));
-    const uint32_t nsb_max = (uint32_t)((nrows_pass + 4095) / 4096) + 1;  // vertex and triangle sums
+    const uint32_t nsb_max = (uint32_t)((nrows_pass + kRowChunk - 1) / kRowChunk) + 1;  // vertex and triangle sums
     HIP_TRY(ctx->bcount.ensure((size_t)2 * nsb_max * sizeof(uint32_t)));
     HIP_TRY(ctx->boffset.ensure((size_t)2 * nsb_max * sizeof(uint32_t)));
     // the counters, then per job kJobOut words (first vertex, first triangle, invalid quads)
@@ -667,7 +667,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
 
         uint64_t* job_out = b.counters + kNumCounters;
         HIP_TRY(launch_sparse_job_begin(ctx->stream, b.counters, kNumCounters + n * kJobOut,
-                                        ctx->sp_queues.as<uint32_t>()));
+                                        ctx->sp_queues.as<uint32_t>(), b.bcount, 2 * nsb_max));
         if (n == 1 && npass > 1)
             HIP_TRY(hipMemsetAsync(ctx->sp_boundary.p, 0xff, plane_max * sizeof(uint32_t), ctx->stream));
 
@@ -732,7 +732,9 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
                 for (auto& e : m) e = ctx->next_event();
             else if (ctx->profiling)  // the evaluation launches only: fewer markers between kernels
                 m[1] = ctx->next_event(), m[2] = ctx->next_event();
-            if (pass > 0) HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters, ctx->sp_queues.as<uint32_t>()));
+            if (pass > 0)
+                HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters, ctx->sp_queues.as<uint32_t>(), b.bcount,
+                                                 2 * nsb_max));
             const int boundary = (n == 1 && pass + 1 < npass) ? 1 : 0;
             HIP_TRY(launch_sparse_pass(ctx->stream, P, b, ctx->cus, (uint32_t)tile_blocks, general_tiles, fold_jobs,
                                        other_jobs, boundary,
@@ -740,7 +742,8 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             if (ctx->profiling) ev.insert(ev.end(), m, m + kSparseMarks);
             tiles_total += P.ntiles;
         }
-        HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters, ctx->sp_queues.as<uint32_t>()));  // fold the last pass
+        HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters, ctx->sp_queues.as<uint32_t>(), nullptr,
+                                         0));  // fold the last pass
         if (ctx->profiling) {
             ev_end = ctx->next_event();
             HIP_TRY(hipEventRecord(ev_end, ctx->stream));

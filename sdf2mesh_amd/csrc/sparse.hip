@@ -24,7 +24,7 @@
 //                   slot's cell bits and each row segment's vertex count
 //   k_sp_quadcount  per slot: valid quads of the staged cells (neighbour cells' bits),
 //                   two triangles each, per row segment; invalid quads counted
-//   k_sp_rowsum / rowreduce / k_scan / rowdown   offsets of the row segments in the
+//   k_sp_rowsum / k_sp_rowdown   offsets of the row segments in the
 //                   reference's VertexList order (job after job, then z, y, tile x):
 //                   per row, the segments of its active tiles; then a scan over rows
 //   k_sp_ranges     each job's first vertex and triangle index
@@ -1369,13 +1369,16 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
 //                 the row segments of the tile row's active tiles (the super-tiles'
 //                 activity words) in x order: each segment's offset within its row
 //                 (the slot's kRowVtxOff / kRowTriOff rows) and the row's totals (row_sum)
-//  k_sp_rowreduce / k_scan / k_sp_rowdown   exclusive scan of the rows' totals (row_off)
+//                 and each 4096-row chunk's sums (one atomic per wave and chunk)
+//  k_sp_rowdown   exclusive scan of the rows' totals (row_off), block b adding the sums of
+//                 the chunks before it
 // A segment's first vertex (triangle) is row_off[row] + its offset within the row.
 // ---------------------------------------------------------------------------------
 constexpr uint32_t kScanPer = 16;                  // consecutive rows per thread
 constexpr uint32_t kScanChunk = 256 * kScanPer;
+static_assert(kScanChunk == kRowChunk, "rows per chunk of the offset scan");
 
-__global__ __launch_bounds__(64) void k_sp_rowsum(SpPass P)
+__global__ __launch_bounds__(64) void k_sp_rowsum(SpPass P, uint32_t* __restrict__ bsum)
 {
     const uint64_t tr = blockIdx.x;
     uint32_t lo = 0, hi = P.njobs - 1;  // last job whose first tile row <= tr
@@ -1433,44 +1436,40 @@ __global__ __launch_bounds__(64) void k_sp_rowsum(SpPass P)
             }
         }
     }
+    const uint64_t row = row_ok ? row_index(J, zl, y) : 0;
     if (row_ok) {
-        const size_t row = row_index(J, zl, y);
         P.row_sum[row] = sv;
         P.row_sum[P.nrows + row] = st;
     }
-}
-
-// Per block: the sums of kScanChunk consecutive rows' vertex and triangle totals
-// (bsum[block], bsum[nblocks + block]).
-__global__ __launch_bounds__(256) void k_sp_rowreduce(SpPass P, uint32_t* __restrict__ bsum)
-{
-    __shared__ uint32_t lds[2][4];
-    const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + thread_id() * kScanPer;
-    uint32_t sv = 0, st = 0;
-    for (uint32_t k = 0; k < kScanPer && base + k < P.nrows; ++k) {
-        sv += P.row_sum[base + k];
-        st += P.row_sum[P.nrows + base + k];
-    }
+    // the rows' totals into their chunks' sums (bsum: vertices, then triangles), one atomic
+    // per chunk the wave's rows fall in
+    const uint32_t nsb = (uint32_t)((P.nrows + kScanChunk - 1) / kScanChunk);
+    const uint32_t chunk = (uint32_t)(row / kScanChunk);
+    uint64_t rem = __ballot(row_ok && (sv | st));
+    while (rem) {
+        const uint32_t c = __shfl(chunk, (int)__builtin_ctzll(rem), 64);
+        const bool mine = ((rem >> lane()) & 1ull) && chunk == c;
+        uint32_t v = mine ? sv : 0u, t = mine ? st : 0u;
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-        sv += __shfl_xor(sv, d, 64);
-        st += __shfl_xor(st, d, 64);
+        for (int d = 32; d >= 1; d >>= 1) {
+            v += __shfl_xor(v, d, 64);
+            t += __shfl_xor(t, d, 64);
+        }
+        if (lane() == (uint32_t)__builtin_ctzll(rem)) {
+            if (v) atomicAdd(&bsum[c], v);
+            if (t) atomicAdd(&bsum[nsb + c], t);
+        }
+        rem &= ~__ballot(mine);
     }
-    if (lane() == 0) {
-        lds[0][thread_id() >> 6] = sv;
-        lds[1][thread_id() >> 6] = st;
-    }
-    __syncthreads();
-    if (thread_id() < 2)
-        bsum[thread_id() * gridDim.x + blockIdx.x] =
-            lds[thread_id()][0] + lds[thread_id()][1] + lds[thread_id()][2] + lds[thread_id()][3];
 }
 
-// Exclusive offsets of the rows: the block's offsets boff[block] (vertices),
-// boff[nblocks + block] (triangles) plus the block-local scan.
-__global__ __launch_bounds__(256) void k_sp_rowdown(SpPass P, const uint32_t* __restrict__ boff)
+// Exclusive offsets of the rows, block b = rows b * kScanChunk ...: the pass's first index
+// (kVtxPrev / kTriPrev) plus the chunk sums of the blocks before it (bsum) plus the
+// block-local scan.  The last block writes the totals after the pass (kVtxTotal / kTriTotal).
+__global__ __launch_bounds__(256) void k_sp_rowdown(SpPass P, const uint32_t* __restrict__ bsum)
 {
     __shared__ uint32_t lds[2][4];
+    __shared__ uint32_t pre[2][4];
     const uint64_t base = (uint64_t)blockIdx.x * kScanChunk + thread_id() * kScanPer;
     uint32_t v[kScanPer], t[kScanPer], sv = 0, st = 0;
     for (uint32_t k = 0; k < kScanPer; ++k) {
@@ -1493,8 +1492,30 @@ __global__ __launch_bounds__(256) void k_sp_rowdown(SpPass P, const uint32_t* __
         lds[0][thread_id() >> 6] = iv;
         lds[1][thread_id() >> 6] = it;
     }
+    {  // the chunks before this block
+        uint32_t cv = 0, ct = 0;
+        for (uint32_t c = thread_id(); c < blockIdx.x; c += blockDim.x) {
+            cv += bsum[c];
+            ct += bsum[gridDim.x + c];
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) {
+            cv += __shfl_xor(cv, d, 64);
+            ct += __shfl_xor(ct, d, 64);
+        }
+        if (lane() == 0) {
+            pre[0][thread_id() >> 6] = cv;
+            pre[1][thread_id() >> 6] = ct;
+        }
+    }
     __syncthreads();
-    uint32_t rv = boff[blockIdx.x] + iv - sv, rt = boff[gridDim.x + blockIdx.x] + it - st;
+    const uint32_t bv = (uint32_t)P.counters[kVtxPrev] + pre[0][0] + pre[0][1] + pre[0][2] + pre[0][3];
+    const uint32_t bt = (uint32_t)P.counters[kTriPrev] + pre[1][0] + pre[1][1] + pre[1][2] + pre[1][3];
+    if (blockIdx.x + 1 == gridDim.x && thread_id() == 0) {  // after the pass
+        P.counters[kVtxTotal] = bv + lds[0][0] + lds[0][1] + lds[0][2] + lds[0][3];
+        P.counters[kTriTotal] = bt + lds[1][0] + lds[1][1] + lds[1][2] + lds[1][3];
+    }
+    uint32_t rv = bv + iv - sv, rt = bt + it - st;
     for (uint32_t w = 0; w < (thread_id() >> 6); ++w) {
         rv += lds[0][w];
         rt += lds[1][w];
@@ -1533,7 +1554,7 @@ __global__ __launch_bounds__(256) void k_sp_boundary(SpPass P)
 
 // Per-pass reset of the slot counter; the largest count seen is kept for the host's
 // capacity check (a kernel rather than a memset, so a pass can be graph-captured).
-__global__ void k_sp_pass_begin(uint64_t* counters, uint32_t* queues)
+__global__ void k_sp_pass_begin(uint64_t* counters, uint32_t* queues, uint32_t* bsum, uint32_t nbsum)
 {
     if (thread_id() == 0 && blockIdx.x == 0) {
         const uint64_t used = counters[kSlotsUsed];
@@ -1542,26 +1563,33 @@ __global__ void k_sp_pass_begin(uint64_t* counters, uint32_t* queues)
         if (counters[kStageUsed] > counters[kStageMax]) counters[kStageMax] = counters[kStageUsed];
         counters[kStageUsed] = 0;
         counters[kLinearSlots] = 0;
+        counters[kVtxPrev] = counters[kVtxTotal];  // the next pass's first vertex and triangle
+        counters[kTriPrev] = counters[kTriTotal];
     }
     for (uint32_t i = thread_id(); i < kQueues * kQueueGroups; i += blockDim.x) queues[i * 32u] = 0;
+    for (uint32_t i = thread_id(); i < nbsum; i += blockDim.x) bsum[i] = 0;
 }
 
-__global__ __launch_bounds__(256) void k_sp_job_begin(uint64_t* counters, uint32_t n, uint32_t* queues)
+__global__ __launch_bounds__(256) void k_sp_job_begin(uint64_t* counters, uint32_t n, uint32_t* queues,
+                                                      uint32_t* bsum, uint32_t nbsum)
 {
     for (uint32_t i = thread_id(); i < n; i += blockDim.x) counters[i] = 0;
     for (uint32_t i = thread_id(); i < kQueues * kQueueGroups; i += blockDim.x) queues[i * 32u] = 0;
+    for (uint32_t i = thread_id(); i < nbsum; i += blockDim.x) bsum[i] = 0;
 }
 
 // ---------------------------------------------------------------------------------
-hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t* queues)
+hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t* queues, uint32_t* bsum,
+                                    uint32_t nbsum)
 {
-    hipLaunchKernelGGL(k_sp_pass_begin, dim3(1), dim3(64), 0, s, counters, queues);
+    hipLaunchKernelGGL(k_sp_pass_begin, dim3(1), dim3(256), 0, s, counters, queues, bsum, nbsum);
     return hipGetLastError();
 }
 
-hipError_t launch_sparse_job_begin(hipStream_t s, uint64_t* counters, uint32_t n, uint32_t* queues)
+hipError_t launch_sparse_job_begin(hipStream_t s, uint64_t* counters, uint32_t n, uint32_t* queues,
+                                   uint32_t* bsum, uint32_t nbsum)
 {
-    hipLaunchKernelGGL(k_sp_job_begin, dim3(1), dim3(256), 0, s, counters, n, queues);
+    hipLaunchKernelGGL(k_sp_job_begin, dim3(1), dim3(256), 0, s, counters, n, queues, bsum, nbsum);
     return hipGetLastError();
 }
 
@@ -1630,16 +1658,9 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
     hipLaunchKernelGGL(k_sp_quadcount, dim3(resident_blocks(k_sp_quadcount, 64, 0, cus, big)), dim3(64), 0, s,
                        P, b);
     mark(3);
-    if (P.ntrows) hipLaunchKernelGGL(k_sp_rowsum, dim3((uint32_t)P.ntrows), dim3(64), 0, s, P);
-    if (nsb) {
-        hipLaunchKernelGGL(k_sp_rowreduce, dim3(nsb), dim3(256), 0, s, P, b.bcount);
-        MeshBufs bt = b;  // triangle block sums and offsets follow the vertex ones
-        bt.bcount += nsb;
-        bt.boffset += nsb;
-        (void)launch_scan(s, b, nsb, 0);
-        (void)launch_scan(s, bt, nsb, 1);
-        hipLaunchKernelGGL(k_sp_rowdown, dim3(nsb), dim3(256), 0, s, P, b.boffset);
-    }
+    // b.bcount: the chunk sums (2 x nsb, cleared by the pass's begin kernel)
+    if (P.ntrows) hipLaunchKernelGGL(k_sp_rowsum, dim3((uint32_t)P.ntrows), dim3(64), 0, s, P, b.bcount);
+    if (nsb) hipLaunchKernelGGL(k_sp_rowdown, dim3(nsb), dim3(256), 0, s, P, b.bcount);
     hipLaunchKernelGGL(k_sp_ranges, dim3(jb), dim3(64), 0, s, P);
     mark(4);
     hipLaunchKernelGGL((k_sp_emit<1>), dim3(resident_blocks(k_sp_emit<1>, 64, 0, cus, big)), dim3(64), 0, s, P, b);
