@@ -742,17 +742,18 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             if (ctx->profiling) ev.insert(ev.end(), m, m + kSparseMarks);
             tiles_total += P.ntiles;
         }
-        HIP_TRY(launch_sparse_pass_begin(ctx->stream, b.counters, ctx->sp_queues.as<uint32_t>(), nullptr,
-                                         0));  // fold the last pass
+        // the last pass folded and the counters and job rows written to mapped host memory
+        const uint32_t nback = kNumCounters + n * kJobOut;
+        HIP_TRY(ctx->host_back.ensure(nback * sizeof(uint64_t)));
+        void* back_dev = nullptr;
+        HIP_TRY(hipHostGetDevicePointer(&back_dev, ctx->host_back.p, 0));
+        HIP_TRY(launch_sparse_job_end(ctx->stream, b.counters, nback, static_cast<uint64_t*>(back_dev)));
         if (ctx->profiling) {
             ev_end = ctx->next_event();
             HIP_TRY(hipEventRecord(ev_end, ctx->stream));
         }
-        HIP_TRY(ctx->host_back.ensure((kNumCounters + (size_t)n * kJobOut) * sizeof(uint64_t)));
         const uint64_t* cnt = ctx->host_back.as<uint64_t>();
         const uint64_t* jo = cnt + kNumCounters;
-        HIP_TRY(hipMemcpyAsync(ctx->host_back.p, b.counters, (kNumCounters + (size_t)n * kJobOut) * sizeof(uint64_t),
-                               hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         if (ctx->profiling) {
             float ms = 0;

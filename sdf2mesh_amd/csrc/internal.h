@@ -186,6 +186,9 @@ hipError_t launch_quads(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int
 // offset scan's chunk sums bsum[0 .. nbsum) cleared.
 hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t* queues, uint32_t* bsum,
                                     uint32_t nbsum);
+// After a job's last pass: slot and staging counters folded into their maxima, then
+// counters[0 .. n) copied to `out` (a device address of mapped host memory).
+hipError_t launch_sparse_job_end(hipStream_t s, uint64_t* counters, uint32_t n, uint64_t* out);
 // A job's first pass: counters[0 .. n) (the counters and the job output rows), the work
 // queues and bsum[0 .. nbsum) cleared.
 hipError_t launch_sparse_job_begin(hipStream_t s, uint64_t* counters, uint32_t n, uint32_t* queues,

@@ -1578,7 +1578,25 @@ __global__ __launch_bounds__(256) void k_sp_job_begin(uint64_t* counters, uint32
     for (uint32_t i = thread_id(); i < nbsum; i += blockDim.x) bsum[i] = 0;
 }
 
+// After a job's last pass: the pass folded into the maxima as k_sp_pass_begin does, then
+// counters[0 .. n) copied to `out` (host memory mapped for the device).
+__global__ __launch_bounds__(256) void k_sp_job_end(uint64_t* counters, uint32_t n, uint64_t* out)
+{
+    if (thread_id() == 0) {
+        if (counters[kSlotsUsed] > counters[kSlotsMax]) counters[kSlotsMax] = counters[kSlotsUsed];
+        if (counters[kStageUsed] > counters[kStageMax]) counters[kStageMax] = counters[kStageUsed];
+    }
+    __syncthreads();
+    for (uint32_t i = thread_id(); i < n; i += blockDim.x) out[i] = counters[i];
+}
+
 // ---------------------------------------------------------------------------------
+hipError_t launch_sparse_job_end(hipStream_t s, uint64_t* counters, uint32_t n, uint64_t* out)
+{
+    hipLaunchKernelGGL(k_sp_job_end, dim3(1), dim3(256), 0, s, counters, n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t* queues, uint32_t* bsum,
                                     uint32_t nbsum)
 {
