@@ -35,7 +35,6 @@ enum Counter : uint32_t {
     kSlotsUsed = 8,    // active tiles of the current slab (sparse path)
     kSlotsMax = 9,     // largest kSlotsUsed of the job so far
     kStageUsed = 10,   // staged active cells of the current pass (sparse path)
-    kLinearSlots = 11, // slots of the current pass whose program is a left fold
     kStageMax = 12,    // largest kStageUsed of the job so far
     kNumCounters = 16
 };

@@ -611,9 +611,7 @@ __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
     if (valid) P.slot_of_tile[tile] = slot;
     const uint64_t act = __ballot(slot >= 0);  // bit l: tile l of the super-tile has a slot
     if (ln == 0) P.super_act[s] = act;
-    const uint64_t lin = __ballot(slot >= 0 && (lens & kLinearFlag));
-    if (lin && ln == (uint32_t)__builtin_ctzll(lin))
-        atomicAdd((unsigned long long*)&P.counters[kLinearSlots], (unsigned long long)__builtin_popcountll(lin));
+
 }
 
 // Per-axis corner tables of a tile's 8 cells: box-relative lattice index and coordinate
@@ -942,8 +940,7 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
     __shared__ ContourLds L;
     __shared__ uint32_t sh_slot, sh_next;
     __shared__ uint8_t lb[64];  // the live bricks of the slot (contour_tile)
-    const uint32_t nslots = active_slots(P), nlinear = (uint32_t)P.counters[kLinearSlots];
-    if (LINEAR ? nlinear == 0 : nlinear >= nslots) return;  // no slot of this kind
+    const uint32_t nslots = active_slots(P);
     constexpr uint32_t kQueue = LINEAR ? 0u : 1u;
     if (thread_id() == 0) L.left = 0;  // published by the first slot's barriers
     uint64_t tiles_done = 0, points_done = 0, flops_done = 0;  // thread 0: this block's share
@@ -1562,7 +1559,6 @@ __global__ void k_sp_pass_begin(uint64_t* counters, uint32_t* queues, uint32_t* 
         counters[kSlotsUsed] = 0;
         if (counters[kStageUsed] > counters[kStageMax]) counters[kStageMax] = counters[kStageUsed];
         counters[kStageUsed] = 0;
-        counters[kLinearSlots] = 0;
         counters[kVtxPrev] = counters[kVtxTotal];  // the next pass's first vertex and triangle
         counters[kTriPrev] = counters[kTriTotal];
     }
