@@ -127,7 +127,7 @@ struct s2m_ctx {
     DevBuf vtx, keys, signs, tri, corners;
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_slot_rows, sp_boundary;
-    DevBuf sp_stage, sp_slot_stage, sp_slot_live, sp_slot_hdr, sp_slot_rec;
+    DevBuf sp_stage, sp_slot_stage, sp_slot_live, sp_slot_hdr, sp_slot_rec, sp_block_stats;
     DevBuf sp_queues, sp_super_iv, sp_super_lists, sp_super_lens, sp_super_act, sp_rows;
     uint64_t slot_cap = 0;
     uint64_t stage_cap = 0;  // staged-cell entries of sp_stage
@@ -666,8 +666,10 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         b.boffset = ctx->boffset.as<uint32_t>();
 
         uint64_t* job_out = b.counters + kNumCounters;
+        HIP_TRY(ctx->sp_block_stats.ensure(tile_blocks * 3 * sizeof(uint64_t)));
         HIP_TRY(launch_sparse_job_begin(ctx->stream, b.counters, kNumCounters + n * kJobOut,
-                                        ctx->sp_queues.as<uint32_t>(), b.bcount, 2 * nsb_max));
+                                        ctx->sp_queues.as<uint32_t>(), b.bcount, 2 * nsb_max,
+                                        ctx->sp_block_stats.as<uint64_t>(), (uint32_t)tile_blocks));
         if (n == 1 && npass > 1)
             HIP_TRY(hipMemsetAsync(ctx->sp_boundary.p, 0xff, plane_max * sizeof(uint32_t), ctx->stream));
 
@@ -717,6 +719,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.slot_live = ctx->sp_slot_live.as<uint64_t>();
             P.slot_hdr = ctx->sp_slot_hdr.as<SlotHdr>();
             P.slot_rec = ctx->sp_slot_rec.as<uint4>();
+            P.block_stats = ctx->sp_block_stats.as<uint64_t>();
             P.slot_rows = ctx->sp_slot_rows.as<uint32_t>();
             P.row_sum = ctx->sp_rows.as<uint32_t>();
             P.row_off = P.row_sum + 2 * P.nrows;
@@ -747,7 +750,8 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         HIP_TRY(ctx->host_back.ensure(nback * sizeof(uint64_t)));
         void* back_dev = nullptr;
         HIP_TRY(hipHostGetDevicePointer(&back_dev, ctx->host_back.p, 0));
-        HIP_TRY(launch_sparse_job_end(ctx->stream, b.counters, nback, static_cast<uint64_t*>(back_dev)));
+        HIP_TRY(launch_sparse_job_end(ctx->stream, b.counters, nback, static_cast<uint64_t*>(back_dev),
+                                      ctx->sp_block_stats.as<uint64_t>(), (uint32_t)tile_blocks));
         if (ctx->profiling) {
             ev_end = ctx->next_event();
             HIP_TRY(hipEventRecord(ev_end, ctx->stream));
@@ -1133,7 +1137,7 @@ void s2m_destroy(s2m_ctx* ctx)
                       &ctx->sp_job_of_slot, &ctx->sp_cellbits, &ctx->sp_slot_rows,
                       &ctx->sp_boundary, &ctx->sp_queues, &ctx->sp_super_iv,
                       &ctx->sp_super_lists, &ctx->sp_super_lens, &ctx->sp_super_act, &ctx->sp_rows,
-                      &ctx->sp_stage, &ctx->sp_slot_stage, &ctx->sp_slot_live, &ctx->sp_slot_hdr, &ctx->sp_slot_rec})
+                      &ctx->sp_stage, &ctx->sp_slot_stage, &ctx->sp_slot_live, &ctx->sp_slot_hdr, &ctx->sp_slot_rec, &ctx->sp_block_stats})
         b->release();
     ctx->host_img.release();
     ctx->host_back.release();

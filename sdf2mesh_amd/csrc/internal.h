@@ -150,6 +150,7 @@ struct SpPass {
     uint64_t* slot_live;       // [slot] bricks (2x2x2 cells) that may hold an active cell
     SlotHdr* slot_hdr;         // [slot]
     uint4* slot_rec;           // [slot] kTileRecBytes: k_sp_tiles<1>'s staged record
+    uint64_t* block_stats;     // [k_sp_tiles block][3]: tiles, points, flops of the job so far
     uint32_t* slot_rows;       // [4][slot_cap][64] per row of each slot's tile: vertices,
                                // triangles, first vertex / triangle within the row
     uint32_t* row_sum;         // [2][nrows] vertices, triangles per row
@@ -187,11 +188,14 @@ hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t*
                                     uint32_t nbsum);
 // After a job's last pass: slot and staging counters folded into their maxima, then
 // counters[0 .. n) copied to `out` (a device address of mapped host memory).
-hipError_t launch_sparse_job_end(hipStream_t s, uint64_t* counters, uint32_t n, uint64_t* out);
+// The tile blocks' statistics (nstats entries of tiles, points, flops) summed into
+// kTilesActive, kEvalPoints and kEvalFlops first.
+hipError_t launch_sparse_job_end(hipStream_t s, uint64_t* counters, uint32_t n, uint64_t* out,
+                                 const uint64_t* stats, uint32_t nstats);
 // A job's first pass: counters[0 .. n) (the counters and the job output rows), the work
 // queues and bsum[0 .. nbsum) cleared.
 hipError_t launch_sparse_job_begin(hipStream_t s, uint64_t* counters, uint32_t n, uint32_t* queues,
-                                   uint32_t* bsum, uint32_t nbsum);
+                                   uint32_t* bsum, uint32_t nbsum, uint64_t* stats, uint32_t nstats);
 // cus: compute units of the device; tile_blocks: most blocks of a k_sp_tiles launch (each
 // holds a staging chunk); general_tiles: some tile's pruned program may not be a left fold
 // (false when every job's whole program is one: its pruned programs are then left folds).

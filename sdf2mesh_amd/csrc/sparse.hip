@@ -1112,10 +1112,11 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
             flops_done += (uint64_t)evaluated * P.flops[tile];
         }
     }
-    if (thread_id() == 0 && tiles_done) {  // one update per block, not per tile
-        atomicAdd((unsigned long long*)&P.counters[kTilesActive], (unsigned long long)tiles_done);
-        atomicAdd((unsigned long long*)&P.counters[kEvalPoints], (unsigned long long)points_done);
-        atomicAdd((unsigned long long*)&P.counters[kEvalFlops], (unsigned long long)flops_done);
+    if (thread_id() == 0 && tiles_done) {  // the block's own entry (no shared address): k_sp_job_end sums them
+        uint64_t* st = P.block_stats + 3u * blockIdx.x;
+        st[0] += tiles_done;
+        st[1] += points_done;
+        st[2] += flops_done;
     }
 }
 
@@ -1567,29 +1568,46 @@ __global__ void k_sp_pass_begin(uint64_t* counters, uint32_t* queues, uint32_t* 
 }
 
 __global__ __launch_bounds__(256) void k_sp_job_begin(uint64_t* counters, uint32_t n, uint32_t* queues,
-                                                      uint32_t* bsum, uint32_t nbsum)
+                                                      uint32_t* bsum, uint32_t nbsum, uint64_t* stats,
+                                                      uint32_t nstats)
 {
     for (uint32_t i = thread_id(); i < n; i += blockDim.x) counters[i] = 0;
+    for (uint32_t i = thread_id(); i < 3u * nstats; i += blockDim.x) stats[i] = 0;
     for (uint32_t i = thread_id(); i < kQueues * kQueueGroups; i += blockDim.x) queues[i * 32u] = 0;
     for (uint32_t i = thread_id(); i < nbsum; i += blockDim.x) bsum[i] = 0;
 }
 
 // After a job's last pass: the pass folded into the maxima as k_sp_pass_begin does, then
 // counters[0 .. n) copied to `out` (host memory mapped for the device).
-__global__ __launch_bounds__(256) void k_sp_job_end(uint64_t* counters, uint32_t n, uint64_t* out)
+__global__ __launch_bounds__(256) void k_sp_job_end(uint64_t* counters, uint32_t n, uint64_t* out,
+                                                    const uint64_t* stats, uint32_t nstats)
 {
+    __shared__ uint64_t part[4][3];
+    uint64_t sum[3] = {0, 0, 0};  // the tile blocks' tiles, points and flops
+    for (uint32_t i = thread_id(); i < nstats; i += blockDim.x)
+        for (int k = 0; k < 3; ++k) sum[k] += stats[3u * i + k];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+        for (int d = 32; d >= 1; d >>= 1) sum[k] += __shfl_xor(sum[k], d, 64);
+    if (lane() == 0)
+        for (int k = 0; k < 3; ++k) part[thread_id() >> 6][k] = sum[k];
+    __syncthreads();
     if (thread_id() == 0) {
         if (counters[kSlotsUsed] > counters[kSlotsMax]) counters[kSlotsMax] = counters[kSlotsUsed];
         if (counters[kStageUsed] > counters[kStageMax]) counters[kStageMax] = counters[kStageUsed];
+        counters[kTilesActive] = part[0][0] + part[1][0] + part[2][0] + part[3][0];
+        counters[kEvalPoints] = part[0][1] + part[1][1] + part[2][1] + part[3][1];
+        counters[kEvalFlops] = part[0][2] + part[1][2] + part[2][2] + part[3][2];
     }
     __syncthreads();
     for (uint32_t i = thread_id(); i < n; i += blockDim.x) out[i] = counters[i];
 }
 
 // ---------------------------------------------------------------------------------
-hipError_t launch_sparse_job_end(hipStream_t s, uint64_t* counters, uint32_t n, uint64_t* out)
+hipError_t launch_sparse_job_end(hipStream_t s, uint64_t* counters, uint32_t n, uint64_t* out,
+                                 const uint64_t* stats, uint32_t nstats)
 {
-    hipLaunchKernelGGL(k_sp_job_end, dim3(1), dim3(256), 0, s, counters, n, out);
+    hipLaunchKernelGGL(k_sp_job_end, dim3(1), dim3(256), 0, s, counters, n, out, stats, nstats);
     return hipGetLastError();
 }
 
@@ -1601,9 +1619,10 @@ hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t*
 }
 
 hipError_t launch_sparse_job_begin(hipStream_t s, uint64_t* counters, uint32_t n, uint32_t* queues,
-                                   uint32_t* bsum, uint32_t nbsum)
+                                   uint32_t* bsum, uint32_t nbsum, uint64_t* stats, uint32_t nstats)
 {
-    hipLaunchKernelGGL(k_sp_job_begin, dim3(1), dim3(256), 0, s, counters, n, queues, bsum, nbsum);
+    hipLaunchKernelGGL(k_sp_job_begin, dim3(1), dim3(256), 0, s, counters, n, queues, bsum, nbsum, stats,
+                       nstats);
     return hipGetLastError();
 }
 
