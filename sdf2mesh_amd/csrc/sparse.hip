@@ -691,12 +691,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
     __shared__ StagedFold T;
     __shared__ TileAxes A;
     const uint32_t nslots = active_slots(P);
-    for (uint32_t slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
+    // The slots' tile, job and list length are read ahead (tile and job two slots ahead,
+    // the length one slot ahead) with per-lane loads, which barriers do not wait for.
+    const uint32_t z = thread_id() & 0u;  // a per-lane zero: vector loads
+    const uint32_t G = gridDim.x;
+    uint32_t t1 = 0, j1 = 0, l1 = 0, t2 = 0, j2 = 0;
+    if (blockIdx.x < nslots) {
+        t1 = P.tile_of_slot[blockIdx.x + z];
+        j1 = P.job_of_slot[blockIdx.x + z];
+        l1 = P.lens[__builtin_amdgcn_readfirstlane(t1) + z];
+    }
+    if (blockIdx.x + G < nslots) {
+        t2 = P.tile_of_slot[blockIdx.x + G + z];
+        j2 = P.job_of_slot[blockIdx.x + G + z];
+    }
+    for (uint32_t slot = blockIdx.x; slot < nslots; slot += G) {
         const uint32_t b = thread_id();
-        const uint64_t tile = __builtin_amdgcn_readfirstlane(P.tile_of_slot[slot]);
-        const uint32_t lens = __builtin_amdgcn_readfirstlane(P.lens[tile]);
+        const uint64_t tile = __builtin_amdgcn_readfirstlane(t1);
+        const uint32_t lens = __builtin_amdgcn_readfirstlane(l1);
         const uint32_t n = lens & ~kLinearFlag;
-        const uint32_t jid = __builtin_amdgcn_readfirstlane(P.job_of_slot[slot]);
+        const uint32_t jid = __builtin_amdgcn_readfirstlane(j1);
+        t1 = t2;
+        j1 = j2;
+        if (slot + G < nslots) l1 = P.lens[__builtin_amdgcn_readfirstlane(t1) + z];
+        if (slot + 2u * G < nslots) {
+            t2 = P.tile_of_slot[slot + 2u * G + z];
+            j2 = P.job_of_slot[slot + 2u * G + z];
+        }
         const SpJob& J = P.jobs[jid];
         uint32_t tx, ty, tzl;
         tile_xyz(J, tile, tx, ty, tzl);
