@@ -613,8 +613,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     HIP_TRY(ctx->sp_slot_of_tile.ensure(tiles_pass * sizeof(int32_t)));
     HIP_TRY(ctx->sp_rows.ensure(std::max<uint64_t>(nrows_pass, 1) * 4 * sizeof(uint32_t)));
     const uint32_t nsb_max = (uint32_t)((nrows_pass + kRowChunk - 1) / kRowChunk) + 1;  // vertex and triangle sums
-    HIP_TRY(ctx->bcount.ensure((size_t)2 * nsb_max * sizeof(uint32_t)));
-    HIP_TRY(ctx->boffset.ensure((size_t)2 * nsb_max * sizeof(uint32_t)));
+    HIP_TRY(ctx->bcount.ensure((size_t)2 * nsb_max * kChunkSumStride * sizeof(uint32_t)));
     // the counters, then per job kJobOut words (first vertex, first triangle, invalid quads)
     HIP_TRY(ctx->counters.ensure((kNumCounters + (size_t)n * kJobOut) * sizeof(uint64_t)));
     HIP_TRY(ctx->sp_queues.ensure((size_t)kQueues * kQueueGroups * 32 * sizeof(uint32_t)));

@@ -68,8 +68,9 @@ constexpr uint32_t kSuper = 4;  // tiles per super-tile edge (first pruning leve
 constexpr uint32_t kSingleSign = 0xffffffffu;  // super_lens: the SDF keeps one sign there
 constexpr uint32_t kStageChunkHost = 1024;  // staged-cell allocation chunk (sparse.hip)
 constexpr uint32_t kTileBlocksPerCU = 8;    // most k_sp_tiles blocks per compute unit
-constexpr uint32_t kTileRecBytes = 2720;
-constexpr uint32_t kRowChunk = 4096;        // rows per chunk of the row offset scan (sparse.hip)    // per slot: k_sp_tiles' staged record (sparse.hip)
+constexpr uint32_t kTileRecBytes = 2720;    // per slot: k_sp_tiles' staged record (sparse.hip)
+constexpr uint32_t kRowChunk = 4096;        // rows per chunk of the row offset scan (sparse.hip)
+constexpr uint32_t kChunkSumStride = 32;    // words between chunk sums (one 128-byte line each)
 constexpr uint32_t kQueues = 2;        // work queues of a pass (k_sp_tiles<1>, k_sp_tiles<0>)
 constexpr uint32_t kQueueGroups = 64;  // counters per queue, 32 u32 (128 bytes) apart
 // Profiling marks of one sparse pass: prune (tiles, bricks) | eval+contour | quad count |
@@ -183,7 +184,7 @@ hipError_t launch_normals(hipStream_t s, const s2m_op* ops, uint32_t nops, float
 hipError_t launch_quads(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int emit);
 // Between passes of a job: slot and staging counters folded into their maxima, the
 // previous pass's totals made the next pass's first indices, work queues and the
-// offset scan's chunk sums bsum[0 .. nbsum) cleared.
+// offset scan's nbsum chunk sums (bsum, kChunkSumStride words apart) cleared.
 hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t* queues, uint32_t* bsum,
                                     uint32_t nbsum);
 // After a job's last pass: slot and staging counters folded into their maxima, then
@@ -193,7 +194,7 @@ hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t*
 hipError_t launch_sparse_job_end(hipStream_t s, uint64_t* counters, uint32_t n, uint64_t* out,
                                  const uint64_t* stats, uint32_t nstats);
 // A job's first pass: counters[0 .. n) (the counters and the job output rows), the work
-// queues and bsum[0 .. nbsum) cleared.
+// queues and the nbsum chunk sums (bsum, kChunkSumStride words apart) cleared.
 hipError_t launch_sparse_job_begin(hipStream_t s, uint64_t* counters, uint32_t n, uint32_t* queues,
                                    uint32_t* bsum, uint32_t nbsum, uint64_t* stats, uint32_t nstats);
 // cus: compute units of the device; tile_blocks: most blocks of a k_sp_tiles launch (each

@@ -1460,9 +1460,8 @@ __global__ __launch_bounds__(64) void k_sp_rowsum(SpPass P, uint32_t* __restrict
         P.row_sum[row] = sv;
         P.row_sum[P.nrows + row] = st;
     }
-    // the rows' totals into their chunks' sums (bsum: vertices, then triangles), one atomic
-    // per chunk the wave's rows fall in
-    const uint32_t nsb = (uint32_t)((P.nrows + kScanChunk - 1) / kScanChunk);
+    // the rows' totals into their chunks' sums (bsum: per chunk a vertex and a triangle sum,
+    // kChunkSumStride words apart), one atomic per chunk the wave's rows fall in
     const uint32_t chunk = (uint32_t)(row / kScanChunk);
     uint64_t rem = __ballot(row_ok && (sv | st));
     while (rem) {
@@ -1475,8 +1474,8 @@ __global__ __launch_bounds__(64) void k_sp_rowsum(SpPass P, uint32_t* __restrict
             t += __shfl_xor(t, d, 64);
         }
         if (lane() == (uint32_t)__builtin_ctzll(rem)) {
-            if (v) atomicAdd(&bsum[c], v);
-            if (t) atomicAdd(&bsum[nsb + c], t);
+            if (v) atomicAdd(&bsum[(2u * c) * kChunkSumStride], v);
+            if (t) atomicAdd(&bsum[(2u * c + 1u) * kChunkSumStride], t);
         }
         rem &= ~__ballot(mine);
     }
@@ -1514,8 +1513,8 @@ __global__ __launch_bounds__(256) void k_sp_rowdown(SpPass P, const uint32_t* __
     {  // the chunks before this block
         uint32_t cv = 0, ct = 0;
         for (uint32_t c = thread_id(); c < blockIdx.x; c += blockDim.x) {
-            cv += bsum[c];
-            ct += bsum[gridDim.x + c];
+            cv += bsum[(2u * c) * kChunkSumStride];
+            ct += bsum[(2u * c + 1u) * kChunkSumStride];
         }
 #pragma unroll
         for (int d = 32; d >= 1; d >>= 1) {
@@ -1585,7 +1584,7 @@ __global__ void k_sp_pass_begin(uint64_t* counters, uint32_t* queues, uint32_t* 
         counters[kTriPrev] = counters[kTriTotal];
     }
     for (uint32_t i = thread_id(); i < kQueues * kQueueGroups; i += blockDim.x) queues[i * 32u] = 0;
-    for (uint32_t i = thread_id(); i < nbsum; i += blockDim.x) bsum[i] = 0;
+    for (uint32_t i = thread_id(); i < nbsum; i += blockDim.x) bsum[i * kChunkSumStride] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_sp_job_begin(uint64_t* counters, uint32_t n, uint32_t* queues,
@@ -1595,7 +1594,7 @@ __global__ __launch_bounds__(256) void k_sp_job_begin(uint64_t* counters, uint32
     for (uint32_t i = thread_id(); i < n; i += blockDim.x) counters[i] = 0;
     for (uint32_t i = thread_id(); i < 3u * nstats; i += blockDim.x) stats[i] = 0;
     for (uint32_t i = thread_id(); i < kQueues * kQueueGroups; i += blockDim.x) queues[i * 32u] = 0;
-    for (uint32_t i = thread_id(); i < nbsum; i += blockDim.x) bsum[i] = 0;
+    for (uint32_t i = thread_id(); i < nbsum; i += blockDim.x) bsum[i * kChunkSumStride] = 0;
 }
 
 // After a job's last pass: the pass folded into the maxima as k_sp_pass_begin does, then
