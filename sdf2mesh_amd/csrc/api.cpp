@@ -127,7 +127,7 @@ struct s2m_ctx {
     DevBuf vtx, keys, signs, tri, corners;
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_slot_rows, sp_boundary;
-    DevBuf sp_stage, sp_slot_stage, sp_slot_live, sp_slot_hdr, sp_slot_rec, sp_block_stats;
+    DevBuf sp_stage, sp_slot_nb, sp_slot_stage, sp_slot_live, sp_slot_hdr, sp_slot_rec, sp_block_stats;
     DevBuf sp_queues, sp_super_iv, sp_super_lists, sp_super_lens, sp_super_act, sp_rows;
     uint64_t slot_cap = 0;
     uint64_t stage_cap = 0;  // staged-cell entries of sp_stage
@@ -648,6 +648,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         HIP_TRY(ctx->sp_slot_stage.ensure(want_s * 2 * sizeof(uint32_t)));
         HIP_TRY(ctx->sp_slot_live.ensure(want_s * sizeof(uint64_t)));
         HIP_TRY(ctx->sp_slot_hdr.ensure(want_s * sizeof(SlotHdr)));
+        HIP_TRY(ctx->sp_slot_nb.ensure(want_s * 8 * sizeof(int32_t)));
         HIP_TRY(ctx->sp_slot_rec.ensure(want_s * kTileRecBytes));
         ctx->vcap = std::max(ctx->vcap, want_v);
         ctx->tcap = std::max(ctx->tcap, want_t);
@@ -717,6 +718,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             P.slot_stage = ctx->sp_slot_stage.as<uint32_t>();
             P.slot_live = ctx->sp_slot_live.as<uint64_t>();
             P.slot_hdr = ctx->sp_slot_hdr.as<SlotHdr>();
+            P.slot_nb = ctx->sp_slot_nb.as<int32_t>();
             P.slot_rec = ctx->sp_slot_rec.as<uint4>();
             P.block_stats = ctx->sp_block_stats.as<uint64_t>();
             P.slot_rows = ctx->sp_slot_rows.as<uint32_t>();
@@ -1136,7 +1138,7 @@ void s2m_destroy(s2m_ctx* ctx)
                       &ctx->sp_job_of_slot, &ctx->sp_cellbits, &ctx->sp_slot_rows,
                       &ctx->sp_boundary, &ctx->sp_queues, &ctx->sp_super_iv,
                       &ctx->sp_super_lists, &ctx->sp_super_lens, &ctx->sp_super_act, &ctx->sp_rows,
-                      &ctx->sp_stage, &ctx->sp_slot_stage, &ctx->sp_slot_live, &ctx->sp_slot_hdr, &ctx->sp_slot_rec, &ctx->sp_block_stats})
+                      &ctx->sp_stage, &ctx->sp_slot_nb, &ctx->sp_slot_stage, &ctx->sp_slot_live, &ctx->sp_slot_hdr, &ctx->sp_slot_rec, &ctx->sp_block_stats})
         b->release();
     ctx->host_img.release();
     ctx->host_back.release();

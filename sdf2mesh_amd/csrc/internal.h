@@ -150,6 +150,7 @@ struct SpPass {
     uint32_t* slot_stage;      // [slot][2] first staged cell, staged cells
     uint64_t* slot_live;       // [slot] bricks (2x2x2 cells) that may hold an active cell
     SlotHdr* slot_hdr;         // [slot]
+    int32_t* slot_nb;          // [slot][8] slots of the tile (tx - dx, ty - dy, tz - dz) at (dz * 2 + dy) * 2 + dx
     uint4* slot_rec;           // [slot] kTileRecBytes: k_sp_tiles<1>'s staged record
     uint64_t* block_stats;     // [k_sp_tiles block][3]: tiles, points, flops of the job so far
     uint32_t* slot_rows;       // [4][slot_cap][64] per row of each slot's tile: vertices,

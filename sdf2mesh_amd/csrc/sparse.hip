@@ -117,8 +117,11 @@ __device__ __forceinline__ uint32_t slot_row_bits(const SpPass& P, int32_t slot,
 // cell has a vertex.  INDEX: idx[i] = its pass-level vertex index, kNone if none (the
 // first vertex of the row segment plus the cell's rank in it).  Plane z0 - 1 comes from
 // the previous slab's boundary map.
+// ns: the slots of the tile (tx, ty, tz) and of its look-back neighbours, ns[(dz * 2 + dy) * 2
+// + dx] for tile (tx - dx, ty - dy, tz - dz) (k_sp_bricks), -1 without one.
 template <bool INDEX>
-__device__ uint32_t nb_row(const SpPass& P, const SpJob& J, uint32_t tx, int y, int z, uint32_t* idx)
+__device__ uint32_t nb_row(const SpPass& P, const SpJob& J, uint32_t tx, uint32_t ty, uint32_t tz,
+                           const int32_t (&ns)[8], int y, int z, uint32_t* idx)
 {
     uint32_t bits = 0;
     if (INDEX)
@@ -135,13 +138,10 @@ __device__ uint32_t nb_row(const SpPass& P, const SpJob& J, uint32_t tx, int y, 
         return bits;
     }
     const uint32_t zl = (uint32_t)z - J.z0;
-    const uint64_t trow =
-        J.tile_base + ((uint64_t)(((uint32_t)z >> 3) - J.tz0) * J.nty + ((uint32_t)y >> 3)) * J.ntx;
+    const uint32_t nsel = ((tz - ((uint32_t)z >> 3)) * 2u + (ty - ((uint32_t)y >> 3))) * 2u;
     const uint32_t r = ((uint32_t)z & 7u) * 8u + ((uint32_t)y & 7u);
-    for (int c = 0; c < 2; ++c) {  // c = 0: the tile column's own cells, 1: the cell before it
-        if (c == 1 && tx == 0) break;
-        const uint32_t t = c ? tx - 1 : tx;
-        const int32_t slot = P.slot_of_tile[trow + t];
+    for (uint32_t c = 0; c < 2; ++c) {  // c = 0: the tile column's own cells, 1: the cell before it
+        const int32_t slot = ns[nsel + c];
         if (slot < 0) continue;
         const uint32_t b8 = slot_row_bits(P, slot, r);
         if (c ? !(b8 & 0x80u) : !b8) continue;
@@ -750,6 +750,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
                 }
             }
         }
+        if (b >= 48 && b < 56) {  // the slot's look-back neighbours (nb_row): (tx - dx, ty - dy, tzl - dz)
+            const uint32_t i = b - 48u, dx = i & 1u, dy = (i >> 1) & 1u, dz = i >> 2;
+            int32_t ns = -1;
+            if (tx >= dx && ty >= dy && tzl >= dz)
+                ns = P.slot_of_tile[J.tile_base + ((uint64_t)(tzl - dz) * J.nty + (ty - dy)) * J.ntx + (tx - dx)];
+            P.slot_nb[(size_t)slot * 8u + i] = ns;
+        }
         if (!(lens & kLinearFlag) || n > kStageOps) {  // every brick is kept
             if (b == 0) P.slot_live[slot] = RG.live = ~0ull;
             continue;
@@ -1210,9 +1217,12 @@ __global__ __launch_bounds__(64) void k_sp_quadcount(SpPass P, MeshBufs b)
         const uint32_t txy = __builtin_amdgcn_readfirstlane(H.txy);
         const uint32_t tx = txy & 0xffffu, ty = txy >> 16, tzl = __builtin_amdgcn_readfirstlane(H.tzl);
         __syncthreads();  // the previous slot's readers of ex / rowt are done
+        int32_t ns[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) ns[i] = P.slot_nb[(size_t)slot * 8u + i];
         for (uint32_t t = thread_id(); t < 81u; t += blockDim.x)
-            ex[t] = nb_row<false>(P, J, tx, (int)(ty * 8u + t % 9u) - 1, (int)(J.z0 + tzl * 8u + t / 9u) - 1,
-                                  nullptr);
+            ex[t] = nb_row<false>(P, J, tx, ty, J.tz0 + tzl, ns, (int)(ty * 8u + t % 9u) - 1,
+                                  (int)(J.z0 + tzl * 8u + t / 9u) - 1, nullptr);
         rowt[thread_id()] = 0;
         __syncthreads();
         uint32_t invalid = 0;
@@ -1280,9 +1290,12 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
         // left folds: the program as staged in the slot's record, read in place
         const ScalarFold TS =
             ScalarFold::at(reinterpret_cast<const char*>(P.slot_rec + (size_t)slot * kRecVec) + offsetof(TileRec, T));
+        int32_t ns[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) ns[i] = P.slot_nb[(size_t)slot * 8u + i];
         for (uint32_t t = thread_id(); t < 81u; t += blockDim.x)
-            (void)nb_row<true>(P, J, tx, (int)(ty * 8u + t % 9u) - 1, (int)(J.z0 + tzl * 8u + t / 9u) - 1,
-                               nb + 9u * t);
+            (void)nb_row<true>(P, J, tx, ty, J.tz0 + tzl, ns, (int)(ty * 8u + t % 9u) - 1,
+                               (int)(J.z0 + tzl * 8u + t / 9u) - 1, nb + 9u * t);
         for (uint32_t w = thread_id(); w < 128u; w += blockDim.x) tc[w] = 0;
         __syncthreads();
         // vertices, and each cell's triangle count
