@@ -1298,7 +1298,10 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
                                (int)(J.z0 + tzl * 8u + t / 9u) - 1, nb + 9u * t);
         for (uint32_t w = thread_id(); w < 128u; w += blockDim.x) tc[w] = 0;
         __syncthreads();
-        // vertices, and each cell's triangle count
+        // vertices, and each cell's triangle count (the first round's cell, quads and row
+        // triangle offset are kept for the triangle loop)
+        uint32_t meta0 = 0, nq0 = 0;
+        uint64_t trow0 = 0;
         for (uint32_t k = thread_id(); k < n; k += blockDim.x) {
             if ((uint64_t)base + k >= P.stage_cap) break;
             const float4 e = P.stage[(uint64_t)base + k];
@@ -1311,6 +1314,13 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
             const uint32_t nq =
                 cell_quads(sg, tx * 8u + cxl, ty * 8u + cyl, J.z0 + tzl * 8u + czl, lag, exists, invalid);
             if (nq) atomicOr(&tc[c >> 2], (2u * (uint32_t)__builtin_popcount(nq)) << ((c & 3u) * 8u));
+            if (k == thread_id()) {
+                meta0 = meta;
+                nq0 = nq;
+                if (nq)
+                    trow0 = (uint64_t)slot_row(P, kRowTriOff, slot)[c >> 3] +
+                            P.row_off[P.nrows + row_index(J, tzl * 8u + czl, ty * 8u + cyl)];
+            }
             const uint64_t vi = nb[at];
             if (vi >= b.vcap) continue;
             float* o = b.vtx + 6 * vi;
@@ -1351,7 +1361,8 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
         // triangles, in vertex order: each quad (q2, q1, q0), (q0, q3, q2)
         for (uint32_t k = thread_id(); k < n; k += blockDim.x) {
             if ((uint64_t)base + k >= P.stage_cap) break;
-            const uint32_t meta = __float_as_uint(P.stage[(uint64_t)base + k].w);
+            const bool first = k == thread_id();
+            const uint32_t meta = first ? meta0 : __float_as_uint(P.stage[(uint64_t)base + k].w);
             const uint32_t c = meta & 511u, sg = meta >> 12;
             const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
             const uint32_t at = ((czl + 1) * 9u + (cyl + 1)) * 9u + (cxl + 1);
@@ -1359,10 +1370,11 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
             auto exists = [&](int dx, int dy, int dz) { return nbv(dx, dy, dz) != kNone; };
             uint32_t invalid = 0;
             const uint32_t y = ty * 8u + cyl, zl = tzl * 8u + czl;
-            const uint32_t nq = cell_quads(sg, tx * 8u + cxl, y, J.z0 + zl, lag, exists, invalid);
+            const uint32_t nq = first ? nq0 : cell_quads(sg, tx * 8u + cxl, y, J.z0 + zl, lag, exists, invalid);
             if (!nq) continue;
             const uint64_t row2 = ((uint64_t)tc[(c >> 3) * 2u + 1] << 32) | tc[(c >> 3) * 2u];  // 8 counts
-            uint64_t t = (uint64_t)slot_row(P, kRowTriOff, slot)[c >> 3] + P.row_off[P.nrows + row_index(J, zl, y)];
+            uint64_t t = first ? trow0
+                               : (uint64_t)slot_row(P, kRowTriOff, slot)[c >> 3] + P.row_off[P.nrows + row_index(J, zl, y)];
             for (uint32_t i = 0; i < cxl; ++i) t += (row2 >> (8u * i)) & 0xffu;
             const uint32_t v = nbv(0, 0, 0);
             const bool c0 = sg & 1u, c1 = (sg >> 1) & 1u, c2 = (sg >> 2) & 1u;
