@@ -572,6 +572,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         nsuper_pass = n == 1 ? std::max(nsuper_pass, nsup) : super_base;
         J.nops = progs[i]->n;
         J.union_fold = union_fold_program(progs[i]->ops) ? 1u : 0u;
+        J.left_fold = left_fold_program(progs[i]->ops) ? 1u : 0u;
         J.flags = ps[i]->flags;
         J.bx = T.bmax[0];
         J.by = T.bmax[1];

@@ -88,6 +88,7 @@ struct SpJob {
     uint32_t tz0;              // tile layer of plane z0
     uint32_t nops, flags;
     uint32_t union_fold;       // the program is a left fold of unions / smooth unions (k > 0)
+    uint32_t left_fold;        // the program is a left fold of library primitives
     uint32_t bx, by;           // lattice box dims of the job's tiles (slot box indexing)
     uint32_t out_index;        // row of the pass's job_out table
     float eps;

@@ -315,6 +315,58 @@ struct PruneWalk {
     __device__ __forceinline__ bool one_sign() const { return top.known && (top.lo > 0.0 || top.hi < 0.0); }
 };
 
+// PruneWalk for a left fold (P0 P1 op1 P2 op2 ...): the same decisions, list and sums with
+// the accumulated value and the latest primitive in registers instead of the LDS stack
+// (every operator's left operand is the accumulated value, which starts the list).
+struct FoldWalk {
+    Iv top = iv_unknown();
+    uint32_t len = 0;
+    uint32_t top_single = kComposite;
+    uint32_t top_sum = 0;
+    Iv b = iv_unknown();  // the latest primitive (the right operand of the next operator)
+    uint32_t b_i = 0, b_sum = 0, b_start = 0;
+
+    __device__ __forceinline__ void prim(uint32_t q, uint32_t i, const Iv& v, uint16_t* out, const PruneStack&,
+                                         uint32_t, uint32_t sum = 0)
+    {
+        if (q == 0) {
+            top = v;
+            top_single = i;
+            top_sum = sum;
+        } else {
+            b = v;
+            b_i = i;
+            b_sum = sum;
+            b_start = len;
+        }
+        out[len++] = (uint16_t)i;
+    }
+
+    __device__ __forceinline__ void op(uint32_t i, uint32_t code, float k, uint16_t* out, const PruneStack&,
+                                       uint32_t, uint32_t fl = 0)
+    {
+        int keep;
+        const Iv r = iv_op(code, top, b, k, keep);
+        if (keep == kKeepBoth) {
+            out[len++] = (uint16_t)i;
+            const bool fold = (top_sum & kFoldBit) && (b_sum & kFoldBit);
+            top_sum = ((top_sum & ~kFoldBit) + (b_sum & ~kFoldBit) + fl) | (fold ? kFoldBit : 0u);
+            top = r;
+            top_single = kComposite;
+        } else if (keep == kKeepA) {
+            len = b_start;  // drop the primitive and the operator
+        } else {            // drop the accumulated value and the operator: the list is the primitive
+            out[0] = (uint16_t)b_i;
+            len = 1;
+            top = b;
+            top_single = b_i;
+            top_sum = b_sum;
+        }
+    }
+
+    __device__ __forceinline__ bool one_sign() const { return top.known && (top.lo > 0.0 || top.hi < 0.0); }
+};
+
 __device__ __forceinline__ uint32_t job_of(const SpPass& P, uint64_t index, bool super)
 {
     uint32_t lo = 0, hi = P.njobs - 1;  // last job whose first tile (super-tile) <= index
@@ -572,27 +624,35 @@ __global__ __launch_bounds__(64) void k_sp_prune(SpPass P)
         double bl[3], bh[3];
         tiles_box(J, tt, tt, bl, bh);
         uint16_t* L = P.lists + (size_t)tile * P.list_stride;
-        PruneWalk w;
-        if (staged) {  // every lane walks the same list: wave-uniform dispatch
-            for (uint32_t q = 0; q < sl; ++q) {
-                const uint32_t code = uniform_code(T, q), i = idx[q], sum = sums[q];
-                if (code < S2M_OP_UNION) w.prim(q, i, iv_primitive(code, T.f[q], bl, bh), L, S, ln, sum);
-                else w.op(i, code, T.f[q][0], L, S, ln, sum);
+        auto walk = [&](auto& w) {
+            if (staged) {  // every lane walks the same list: wave-uniform dispatch
+                for (uint32_t q = 0; q < sl; ++q) {
+                    const uint32_t code = uniform_code(T, q), i = idx[q], sum = sums[q];
+                    if (code < S2M_OP_UNION) w.prim(q, i, iv_primitive(code, T.f[q], bl, bh), L, S, ln, sum);
+                    else w.op(i, code, T.f[q][0], L, S, ln, sum);
+                }
+            } else {
+                for (uint32_t q = 0; q < sl; ++q) {
+                    const uint32_t i = __builtin_amdgcn_readfirstlane(in[q]);
+                    const uint32_t code = ops[i].code, sum = op_sum(i);
+                    if (code < S2M_OP_UNION) w.prim(q, i, iv_primitive(code, ops[i].f, bl, bh), L, S, ln, sum);
+                    else w.op(i, code, ops[i].f[0], L, S, ln, sum);
+                }
             }
+            surface = !w.one_sign();
+            // the pruned program is one subtree: its flops and whether it is a left fold
+            const bool fold = (w.top_sum & kFoldBit) && w.len <= kStageOps;
+            lens = w.len | (surface && fold ? kLinearFlag : 0u);
+            P.lens[tile] = lens;
+            P.flops[tile] = w.top_sum & ~kFoldBit;
+        };
+        if (J.left_fold) {  // the super-tile's list is a left fold too
+            FoldWalk w;
+            walk(w);
         } else {
-            for (uint32_t q = 0; q < sl; ++q) {
-                const uint32_t i = __builtin_amdgcn_readfirstlane(in[q]);
-                const uint32_t code = ops[i].code, sum = op_sum(i);
-                if (code < S2M_OP_UNION) w.prim(q, i, iv_primitive(code, ops[i].f, bl, bh), L, S, ln, sum);
-                else w.op(i, code, ops[i].f[0], L, S, ln, sum);
-            }
+            PruneWalk w;
+            walk(w);
         }
-        surface = !w.one_sign();
-        // the pruned program is one subtree: its flops and whether it is a left fold
-        const bool fold = (w.top_sum & kFoldBit) && w.len <= kStageOps;
-        lens = w.len | (surface && fold ? kLinearFlag : 0u);
-        P.lens[tile] = lens;
-        P.flops[tile] = w.top_sum & ~kFoldBit;
     }
     // slots for the surviving tiles: one counter update per wave
     const uint64_t m = __ballot(surface);
