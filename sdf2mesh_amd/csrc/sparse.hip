@@ -24,10 +24,10 @@
 //                   slot's cell bits and each row segment's vertex count
 //   k_sp_quadcount  per slot: valid quads of the staged cells (neighbour cells' bits),
 //                   two triangles each, per row segment; invalid quads counted
-//   k_sp_rowsum / k_sp_rowdown   offsets of the row segments in the
-//                   reference's VertexList order (job after job, then z, y, tile x):
-//                   per row, the segments of its active tiles; then a scan over rows
-//   k_sp_ranges     each job's first vertex and triangle index
+//   k_sp_rowsum / k_sp_rowdown   offsets of the row segments in the reference's
+//                   VertexList order (job after job, then z, y, tile x): per row, the
+//                   segments of its active tiles; then a scan over rows, which also gives
+//                   each job's first vertex and triangle index
 //   k_sp_emit<1/0>  per slot: the staged vertices at their ordered index with their
 //                   4-tap normals (the tile's pruned program), then the triangles of
 //                   their quads, vertex indices relative to the job's first vertex
@@ -872,8 +872,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
 // slot's kRowVtx row.  Contains barriers.
 // ---------------------------------------------------------------------------------
 // Staged cells are allocated in chunks of kStageChunk per block (one counter update per
-// chunk); a slot's cells are contiguous, so a chunk with fewer than the slot's cells left
-// is abandoned.  Hence at most 2 x (active cells) + blocks x kStageChunk entries are used.
+// chunk).  Each slot reserves 8 entries per live brick (its cells) when its tile starts,
+// so a slot's cells are contiguous; a chunk with fewer entries left than a slot reserves
+// is abandoned.  Hence at most 2 x (reserved entries) + blocks x kStageChunk are used; the
+// host grows the buffer from the device count if that exceeds it.
 constexpr uint32_t kStageChunk = kStageChunkHost;
 constexpr uint32_t kTileThreads = 256;  // threads of a k_sp_tiles block (one tile at a time)
 constexpr int kLinearPoints = 2;        // points per lane sharing each op's decode (left folds)
@@ -1629,21 +1631,17 @@ __global__ __launch_bounds__(256) void k_sp_rowdown(SpPass P, const uint32_t* __
         rv += v[k];
         rt += t[k];
     }
-}
-
-// First vertex and first triangle index of every job whose first slab this pass holds:
-// the exclusive offsets of its first row.
-__global__ void k_sp_ranges(SpPass P)
-{
-    const uint32_t j = blockIdx.x * blockDim.x + thread_id();
-    if (j >= P.njobs) return;
-    const SpJob& J = P.jobs[j];
-    if (J.z0 != 0) return;
-    // a job with no row (empty grid) starts where the pass's output starts
-    const bool empty = (uint64_t)J.nz * J.ry == 0;
-    uint64_t* o = P.job_out + (size_t)J.out_index * kJobOut;
-    o[0] = empty ? P.counters[kVtxPrev] : (uint64_t)P.row_off[J.row_base];
-    o[1] = empty ? P.counters[kTriPrev] : (uint64_t)P.row_off[P.nrows + J.row_base];
+    // the first vertex and triangle index of every job whose first slab this pass holds and
+    // whose first row is this block's: the exclusive offsets of that row
+    __syncthreads();
+    const uint64_t lo = (uint64_t)blockIdx.x * kScanChunk, hi = lo + kScanChunk;
+    for (uint32_t j = thread_id(); j < P.njobs; j += blockDim.x) {
+        const SpJob& J = P.jobs[j];
+        if (J.z0 != 0 || (uint64_t)J.nz * J.ry == 0 || J.row_base < lo || J.row_base >= hi) continue;
+        uint64_t* o = P.job_out + (size_t)J.out_index * kJobOut;
+        o[0] = (uint64_t)P.row_off[J.row_base];
+        o[1] = (uint64_t)P.row_off[P.nrows + J.row_base];
+    }
 }
 
 // Cell -> vertex map of the slab's last plane, for the next slab (single job).
@@ -1769,7 +1767,6 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
     const uint64_t nsb64 = (P.nrows + kScanChunk - 1) / kScanChunk;
     if (nsb64 > kMaxScanBlocks) return hipErrorInvalidValue;
     const uint32_t nsb = (uint32_t)nsb64;
-    const uint32_t jb = (P.njobs + 63) / 64;
     const uint32_t big = 1u << 20;
     auto mark = [&](int i) {
         if (marks && marks[i]) (void)hipEventRecord(marks[i], s);
@@ -1799,7 +1796,6 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
     // b.bcount: the chunk sums (2 x nsb, cleared by the pass's begin kernel)
     if (P.ntrows) hipLaunchKernelGGL(k_sp_rowsum, dim3((uint32_t)P.ntrows), dim3(64), 0, s, P, b.bcount);
     if (nsb) hipLaunchKernelGGL(k_sp_rowdown, dim3(nsb), dim3(256), 0, s, P, b.bcount);
-    hipLaunchKernelGGL(k_sp_ranges, dim3(jb), dim3(64), 0, s, P);
     mark(4);
     hipLaunchKernelGGL((k_sp_emit<1>), dim3(resident_blocks(k_sp_emit<1>, 64, 0, cus, big)), dim3(64), 0, s, P, b);
     if (general_tiles)
