@@ -89,7 +89,23 @@ def test_pruned_equals_dense_full_size(ctx, res, flags):
     assert np.array_equal(a.vertices[:, :3].view(np.uint32), b.vertices[:, :3].view(np.uint32))
     assert np.array_equal(a.triangles, b.triangles)
     assert a.invalid_quads == b.invalid_quads
-    np.testing.assert_allclose(a.vertices[:, 3:], b.vertices[:, 3:], atol=1e-6, rtol=0)
+    # pruning is exact where the normal taps fall too: the normals agree bit for bit
+    assert np.array_equal(a.vertices[:, 3:].view(np.uint32), b.vertices[:, 3:].view(np.uint32))
+
+
+@pytest.mark.parametrize("dense", [False, True])
+def test_no_normals(ctx, dense):
+    # S2M_NO_NORMALS: the same mesh with zero normals
+    print(f"csg64 no normals dense={dense}", flush=True)
+    prog = ctx.program(tape.csg64())
+    flags = s2m.REFERENCE_COMPAT | (s2m.DENSE_EVAL if dense else 0)
+    a = ctx.mesh(prog, s2m.Params.reference(64, 2.0, flags=flags))
+    b = ctx.mesh(prog, s2m.Params.reference(64, 2.0, flags=flags | s2m.NO_NORMALS))
+    assert len(a.vertices) > 0
+    assert np.array_equal(a.vertices[:, :3].view(np.uint32), b.vertices[:, :3].view(np.uint32))
+    assert np.array_equal(a.triangles, b.triangles)
+    assert np.array_equal(a.cell_keys, b.cell_keys)
+    assert not np.any(b.vertices[:, 3:])
 
 
 @pytest.mark.parametrize("slab", [1, 3, 7])
