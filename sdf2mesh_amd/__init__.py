@@ -197,6 +197,11 @@ class Params:
         return cls(tuple(p.bb_min), tuple(p.bb_max), tuple(p.res), p.eps, flags, slab)
 
     def c(self) -> _Params:
+        """The s2m_params struct (built again only when a field has changed)."""
+        key = (tuple(self.bb_min), tuple(self.bb_max), tuple(self.res), self.eps, self.flags, self.slab)
+        cached = self.__dict__.get("_c")
+        if cached is not None and cached[0] == key:
+            return cached[1]
         p = _Params()
         p.bb_min[:] = [float(np.float32(v)) for v in self.bb_min]
         p.bb_max[:] = [float(np.float32(v)) for v in self.bb_max]
@@ -204,7 +209,12 @@ class Params:
         p.eps = self.eps
         p.flags = self.flags
         p.slab = self.slab
+        self.__dict__["_c"] = (key, p)
         return p
+
+
+def timing_dict(t: "_Timing") -> dict:
+    return {k: getattr(t, k) for k, _ in _Timing._fields_}
 
 
 def round_resolution(res: int) -> int:
@@ -376,9 +386,13 @@ class Context:
         _check(lib().s2m_set_profiling(C.c_void_p(self.handle), int(level)))
 
     def last_timing(self) -> dict:
+        return timing_dict(self.last_timing_raw())
+
+    def last_timing_raw(self) -> "_Timing":
+        """s2m_last_timing as the C struct (timing_dict converts it)."""
         t = _Timing()
         _check(lib().s2m_last_timing(C.c_void_p(self.handle), C.byref(t)))
-        return {k: getattr(t, k) for k, _ in _Timing._fields_}
+        return t
 
     def program(self, src) -> Program:
         """Upload a tape (numpy TAPE_DTYPE array) or a tape.Node expression."""
