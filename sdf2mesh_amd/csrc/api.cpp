@@ -663,8 +663,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         b.vcap = want_v;
         b.tcap = want_t;
         b.counters = ctx->counters.as<uint64_t>();
-        b.bcount = ctx->bcount.as<uint32_t>();
-        b.boffset = ctx->boffset.as<uint32_t>();
+        b.bcount = ctx->bcount.as<uint32_t>();  // the offset scan's chunk sums
 
         uint64_t* job_out = b.counters + kNumCounters;
         HIP_TRY(ctx->sp_block_stats.ensure(tile_blocks * 3 * sizeof(uint64_t)));
