@@ -230,7 +230,6 @@ def run(args) -> dict:
     node, params, cfg = workload(args.workload)
     if args.dense:
         params.flags |= s2m.DENSE_EVAL
-    cfg["eval"] = "dense" if args.dense else "tile-pruned (exact)"
     voxels = int(params.res[0]) * int(params.res[1]) * int(params.res[2])
     ctx = s2m.Context(0)
     prog = ctx.program(node)
@@ -269,6 +268,8 @@ def run(args) -> dict:
     flops_per_launch = agg["eval_flops"] / launches   # executed algorithmic flops
     achieved = flops_per_launch / (eval_avg_ms / 1e3) / 1e12
     tiled = agg["tiles_total"] > 0
+    cfg["eval"] = ("tile-pruned (exact)" if tiled else
+                   "dense" if args.dense else "dense (the program is not interval-prunable)")
     out = {
         "metric": "Mvoxels/s meshed at 512^3 fp32 (CSG-64, reference-exact corners)"
         if args.workload == "csg64-512" else f"Mvoxels/s meshed ({args.workload})",
