@@ -123,7 +123,7 @@ struct s2m_ctx {
     hipStream_t stream = nullptr;
     int profiling = 0;  // 0 off, 1 total + evaluation kernels, 2 every stage
     s2m_timing timing{};
-    DevBuf lattice, vidx, bcount, boffset, counters, tables;
+    DevBuf lattice, vidx, bcount, boffset, actbits, counters, tables;
     DevBuf vtx, keys, signs, tri, corners;
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_slot_rows, sp_boundary;
@@ -872,6 +872,7 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
     HIP_TRY(ctx->vidx.ensure((size_t)(T + 1) * plane * sizeof(uint32_t)));
     HIP_TRY(ctx->bcount.ensure((size_t)max_blocks * sizeof(uint32_t)));
     HIP_TRY(ctx->boffset.ensure((size_t)max_blocks * sizeof(uint32_t)));
+    HIP_TRY(ctx->actbits.ensure((size_t)max_blocks * (kCellsPerBlock / 64) * sizeof(uint64_t)));
     HIP_TRY(ctx->counters.ensure(kNumCounters * sizeof(uint64_t)));
     if (h_corners) {
         HIP_TRY(ctx->corners.ensure((size_t)total_cells * 8 * sizeof(float)));
@@ -904,6 +905,7 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
         b.counters = ctx->counters.as<uint64_t>();
         b.bcount = ctx->bcount.as<uint32_t>();
         b.boffset = ctx->boffset.as<uint32_t>();
+        b.actbits = ctx->actbits.as<uint64_t>();
 
         HIP_TRY(hipMemsetAsync(b.counters, 0, kNumCounters * sizeof(uint64_t), ctx->stream));
         HIP_TRY(hipMemsetAsync(ctx->vidx.p, 0xff, (size_t)plane * sizeof(uint32_t), ctx->stream));
@@ -1131,7 +1133,7 @@ void s2m_destroy(s2m_ctx* ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (DevBuf* b : {&ctx->lattice, &ctx->vidx, &ctx->bcount, &ctx->boffset, &ctx->counters,
+    for (DevBuf* b : {&ctx->lattice, &ctx->vidx, &ctx->bcount, &ctx->boffset, &ctx->actbits, &ctx->counters,
                       &ctx->tables, &ctx->vtx, &ctx->keys, &ctx->signs, &ctx->tri, &ctx->corners,
                       &ctx->tile_tab, &ctx->tile_lists, &ctx->tile_lens,
                       &ctx->tile_flops, &ctx->sp_slot_of_tile, &ctx->sp_tile_of_slot,

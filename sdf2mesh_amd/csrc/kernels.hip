@@ -150,8 +150,11 @@ __device__ __forceinline__ void cell_coords(const SlabArgs& a, uint32_t i, uint3
 }
 
 // ---------------------------------------------------------------------------------
-// Cell pass.  EMIT = 0: block active counts.  EMIT = 1: ordered vertex write and the
-// cell -> vertex index map.  FROM_CORNERS selects the per-cell corner dump source.
+// Cell pass.  EMIT = 0: every cell's vertex rule; block active counts and each cell's
+// activity bit (b.actbits).  EMIT = 1: ordered vertex write and the cell -> vertex index
+// map, the ranks from the activity bits, so only cells with a vertex are gathered and
+// contoured again (the same rule on the same corners gives the same vertex).
+// FROM_CORNERS selects the per-cell corner dump source.
 // ---------------------------------------------------------------------------------
 template <int EMIT, int FROM_CORNERS>
 __global__ __launch_bounds__(256) void k_cells(SlabArgs a, MeshBufs b, const float* __restrict__ corners,
@@ -166,7 +169,23 @@ __global__ __launch_bounds__(256) void k_cells(SlabArgs a, MeshBufs b, const flo
         bool act = false;
         float pos[3], c[8], lo[3], hi[3];
         uint32_t x = 0, y = 0, zl = 0;
-        if (i < a.ncells) {
+        uint32_t rank = 0, total = 0;
+        if (EMIT) {  // the round's 4 activity words (64 cells each)
+            const uint32_t i0 = base + j * kBlock, wv = threadIdx.x >> 6, ln = lane_id();
+            uint64_t wd[kBlock / 64];
+#pragma unroll
+            for (uint32_t k = 0; k < kBlock / 64; ++k) wd[k] = i0 + 64u * k < a.ncells ? b.actbits[(i0 >> 6) + k] : 0ull;
+            const uint64_t mine = wd[wv];
+            act = (mine >> ln) & 1ull;
+#pragma unroll
+            for (uint32_t k = 0; k < kBlock / 64; ++k) {
+                const uint32_t pc = (uint32_t)__builtin_popcountll(wd[k]);
+                total += pc;
+                if (k < wv) rank += pc;
+            }
+            rank += (uint32_t)__builtin_popcountll(mine & ((1ull << ln) - 1ull));
+        }
+        if (i < a.ncells && (!EMIT || act)) {
             cell_coords(a, i, x, y, zl);
             if (FROM_CORNERS) {
                 const float* src = corners + 8 * (size_t)i;
@@ -184,9 +203,13 @@ __global__ __launch_bounds__(256) void k_cells(SlabArgs a, MeshBufs b, const flo
             }
             act = contour_cell(c, lo, hi, pos);
         }
-        uint32_t total;
-        const uint32_t rank = block_exclusive_256(act ? 1u : 0u, lds, &total);
+        if (!EMIT) {
+            const uint64_t w = __ballot(act);
+            if (lane_id() == 0 && i < a.ncells) b.actbits[i >> 6] = w;
+            rank = block_exclusive_256(act ? 1u : 0u, lds, &total);
+        }
         if (EMIT && i < a.ncells) {
+            if (!act) cell_coords(a, i, x, y, zl);
             const uint64_t v = running + rank;
             const size_t m = ((size_t)(zl + 1) * a.ry + y) * a.rx + x;
             a.vidx[m] = act ? (uint32_t)v : kNone;
