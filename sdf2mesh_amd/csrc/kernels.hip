@@ -151,69 +151,96 @@ __device__ __forceinline__ void cell_coords(const SlabArgs& a, uint32_t i, uint3
 
 // ---------------------------------------------------------------------------------
 // Cell pass.  EMIT = 0: every cell's vertex rule; block active counts and each cell's
-// activity bit (b.actbits).  EMIT = 1: ordered vertex write and the cell -> vertex index
-// map, the ranks from the activity bits, so only cells with a vertex are gathered and
-// contoured again (the same rule on the same corners gives the same vertex).
+// activity bit (b.actbits, one ballot word per 64 cells).  EMIT = 1: the cell -> vertex
+// index map for every cell, its ranks from the activity bits, then the block's active
+// cells, listed in LDS in vertex order, are gathered and contoured again on full waves
+// (the same rule on the same corners gives the same vertex) and their vertices written.
 // FROM_CORNERS selects the per-cell corner dump source.
 // ---------------------------------------------------------------------------------
+template <int FROM_CORNERS>
+__device__ __forceinline__ bool cell_vertex(const SlabArgs& a, const float* __restrict__ corners, uint32_t i,
+                                            uint32_t x, uint32_t y, uint32_t zl, float (&c)[8], float (&pos)[3])
+{
+    float lo[3], hi[3];
+    if (FROM_CORNERS) {
+        const float* src = corners + 8 * (size_t)i;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c[k] = src[k];
+        const uint32_t zg = a.z0 + zl;
+        lo[0] = a.cx[a.lox[x]];
+        hi[0] = a.cx[a.hix[x]];
+        lo[1] = a.cy[a.loy[y]];
+        hi[1] = a.cy[a.hiy[y]];
+        lo[2] = a.cz[a.loz[zg]];
+        hi[2] = a.cz[a.hiz[zg]];
+    } else {
+        gather_lattice(a, x, y, zl, c, lo, hi);
+    }
+    return contour_cell(c, lo, hi, pos);
+}
+
 template <int EMIT, int FROM_CORNERS>
 __global__ __launch_bounds__(256) void k_cells(SlabArgs a, MeshBufs b, const float* __restrict__ corners,
                                                int zero_normals)
 {
-    __shared__ uint32_t lds[kBlock / 64];
     const uint32_t base = blockIdx.x * kCellsPerBlock;
-    uint64_t running = EMIT ? (uint64_t)b.boffset[blockIdx.x] : 0;
-    uint32_t count = 0;
-    for (uint32_t j = 0; j < kRounds; ++j) {
-        const uint32_t i = base + j * kBlock + threadIdx.x;
-        bool act = false;
-        float pos[3], c[8], lo[3], hi[3];
-        uint32_t x = 0, y = 0, zl = 0;
-        uint32_t rank = 0, total = 0;
-        if (EMIT) {  // the round's 4 activity words (64 cells each)
-            const uint32_t i0 = base + j * kBlock, wv = threadIdx.x >> 6, ln = lane_id();
+    if constexpr (!EMIT) {
+        __shared__ uint32_t lds[kBlock / 64];
+        uint32_t count = 0;
+        for (uint32_t j = 0; j < kRounds; ++j) {
+            const uint32_t i = base + j * kBlock + threadIdx.x;
+            bool act = false;
+            if (i < a.ncells) {
+                uint32_t x, y, zl;
+                float c[8], pos[3];
+                cell_coords(a, i, x, y, zl);
+                act = cell_vertex<FROM_CORNERS>(a, corners, i, x, y, zl, c, pos);
+            }
+            const uint64_t w = __ballot(act);
+            if (lane_id() == 0 && i < a.ncells) b.actbits[i >> 6] = w;
+            uint32_t total;
+            block_exclusive_256(act ? 1u : 0u, lds, &total);
+            count += total;
+        }
+        if (threadIdx.x == 0) b.bcount[blockIdx.x] = count;
+    } else {
+        __shared__ uint32_t list[kCellsPerBlock];  // the block's active cells, in vertex order
+        const uint32_t first = b.boffset[blockIdx.x];
+        uint32_t before = 0;  // active cells of earlier rounds
+        for (uint32_t j = 0; j < kRounds; ++j) {
+            const uint32_t i0 = base + j * kBlock, i = i0 + threadIdx.x;
+            const uint32_t wv = threadIdx.x >> 6, ln = lane_id();
             uint64_t wd[kBlock / 64];
 #pragma unroll
             for (uint32_t k = 0; k < kBlock / 64; ++k) wd[k] = i0 + 64u * k < a.ncells ? b.actbits[(i0 >> 6) + k] : 0ull;
-            const uint64_t mine = wd[wv];
-            act = (mine >> ln) & 1ull;
+            uint32_t rank = before, total = 0;
 #pragma unroll
             for (uint32_t k = 0; k < kBlock / 64; ++k) {
                 const uint32_t pc = (uint32_t)__builtin_popcountll(wd[k]);
                 total += pc;
                 if (k < wv) rank += pc;
             }
+            const uint64_t mine = wd[wv];
+            const bool act = (mine >> ln) & 1ull;
             rank += (uint32_t)__builtin_popcountll(mine & ((1ull << ln) - 1ull));
-        }
-        if (i < a.ncells && (!EMIT || act)) {
-            cell_coords(a, i, x, y, zl);
-            if (FROM_CORNERS) {
-                const float* src = corners + 8 * (size_t)i;
-#pragma unroll
-                for (int k = 0; k < 8; ++k) c[k] = src[k];
-                const uint32_t zg = a.z0 + zl;
-                lo[0] = a.cx[a.lox[x]];
-                hi[0] = a.cx[a.hix[x]];
-                lo[1] = a.cy[a.loy[y]];
-                hi[1] = a.cy[a.hiy[y]];
-                lo[2] = a.cz[a.loz[zg]];
-                hi[2] = a.cz[a.hiz[zg]];
-            } else {
-                gather_lattice(a, x, y, zl, c, lo, hi);
+            if (i < a.ncells) {
+                uint32_t x, y, zl;
+                cell_coords(a, i, x, y, zl);
+                const size_t m = ((size_t)(zl + 1) * a.ry + y) * a.rx + x;
+                a.vidx[m] = act ? first + rank : kNone;
+                if (act) list[rank] = i;
             }
-            act = contour_cell(c, lo, hi, pos);
+            before += total;
         }
-        if (!EMIT) {
-            const uint64_t w = __ballot(act);
-            if (lane_id() == 0 && i < a.ncells) b.actbits[i >> 6] = w;
-            rank = block_exclusive_256(act ? 1u : 0u, lds, &total);
-        }
-        if (EMIT && i < a.ncells) {
-            if (!act) cell_coords(a, i, x, y, zl);
-            const uint64_t v = running + rank;
-            const size_t m = ((size_t)(zl + 1) * a.ry + y) * a.rx + x;
-            a.vidx[m] = act ? (uint32_t)v : kNone;
-            if (act && v < b.vcap) {
+        __syncthreads();
+        for (uint32_t q = threadIdx.x; q < before; q += kBlock) {
+            const uint32_t i = list[q];
+            const uint64_t v = (uint64_t)first + q;
+            uint32_t x, y, zl;
+            float c[8], pos[3];
+            cell_coords(a, i, x, y, zl);
+            cell_vertex<FROM_CORNERS>(a, corners, i, x, y, zl, c, pos);
+            if (v < b.vcap) {
                 float* o = b.vtx + 6 * v;
                 o[0] = pos[0];
                 o[1] = pos[1];
@@ -227,10 +254,7 @@ __global__ __launch_bounds__(256) void k_cells(SlabArgs a, MeshBufs b, const flo
                 b.signs[v] = (uint8_t)sign_bits(c);
             }
         }
-        running += total;
-        count += total;
     }
-    if (!EMIT && threadIdx.x == 0) b.bcount[blockIdx.x] = count;
 }
 
 // ---------------------------------------------------------------------------------
@@ -241,11 +265,33 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ bcount, ui
                                                uint32_t* __restrict__ boffset,
                                                uint64_t* __restrict__ counters, uint32_t which)
 {
+    // Thread t owns the run [t * per, t * per + per) (per a multiple of 4, at most
+    // kScanPer), read as independent 16-byte vectors twice: summed, scanned across the
+    // block, then read again (from L2) and written as offsets.
+    constexpr uint32_t kScanPer = kMaxScanBlocks / kScanThreads;
+    static_assert(kScanPer % 4 == 0, "runs are whole 16-byte vectors");
     __shared__ uint64_t wsum[kScanThreads / 64];
-    const uint32_t per = (nb + kScanThreads - 1) / kScanThreads;
-    const uint32_t beg = min(threadIdx.x * per, nb), end = min(beg + per, nb);
+    const uint32_t per = ((nb + kScanThreads - 1) / kScanThreads + 3u) & ~3u;
+    const uint32_t beg = threadIdx.x * per;
+    auto load4 = [&](uint32_t i) {
+        uint4 q = make_uint4(0u, 0u, 0u, 0u);
+        if (i + 3 < nb) {
+            q = *reinterpret_cast<const uint4*>(bcount + i);
+        } else if (i < nb) {
+            q.x = bcount[i];
+            q.y = i + 1 < nb ? bcount[i + 1] : 0u;
+            q.z = i + 2 < nb ? bcount[i + 2] : 0u;
+        }
+        return q;
+    };
     uint64_t s = 0;
-    for (uint32_t i = beg; i < end; ++i) s += bcount[i];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; k += 4) {
+        if (k < per) {
+            const uint4 q = load4(beg + k);
+            s += (uint64_t)q.x + q.y + q.z + q.w;
+        }
+    }
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     uint64_t inc = s;
 #pragma unroll
@@ -263,9 +309,28 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ bcount, ui
         total += wsum[k];
     }
     uint64_t run = base + woff + inc - s;
-    for (uint32_t i = beg; i < end; ++i) {
-        boffset[i] = (uint32_t)run;
-        run += bcount[i];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanPer; k += 4) {
+        const uint32_t i = beg + k;
+        if (k < per && i < nb) {
+            const uint4 q = load4(i);
+            uint4 o;
+            o.x = (uint32_t)run;
+            run += q.x;
+            o.y = (uint32_t)run;
+            run += q.y;
+            o.z = (uint32_t)run;
+            run += q.z;
+            o.w = (uint32_t)run;
+            run += q.w;
+            if (i + 3 < nb) {
+                *reinterpret_cast<uint4*>(boffset + i) = o;
+            } else {
+                boffset[i] = o.x;
+                if (i + 1 < nb) boffset[i + 1] = o.y;
+                if (i + 2 < nb) boffset[i + 2] = o.z;
+            }
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -425,6 +490,7 @@ hipError_t launch_cells_from_corners(hipStream_t s, const SlabArgs& a, const flo
 
 hipError_t launch_scan(hipStream_t s, const MeshBufs& b, uint32_t nblocks, uint32_t which)
 {
+    if (nblocks > kMaxScanBlocks) return hipErrorInvalidValue;  // k_scan holds at most this many
     hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, s, b.bcount, nblocks, b.boffset,
                        b.counters, which);
     return hipGetLastError();
