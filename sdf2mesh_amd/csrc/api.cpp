@@ -911,6 +911,9 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
 
         HIP_TRY(hipMemsetAsync(b.counters, 0, kNumCounters * sizeof(uint64_t), ctx->stream));
         HIP_TRY(hipMemsetAsync(ctx->vidx.p, 0xff, (size_t)plane * sizeof(uint32_t), ctx->stream));
+        // the count pass ORs into zeroed words; each slab's emit pass clears what it read
+        HIP_TRY(hipMemsetAsync(b.actbits, 0, (size_t)max_blocks * (kCellsPerBlock / 64) * sizeof(uint64_t),
+                               ctx->stream));
 
         ctx->events_used = 0;
         std::vector<hipEvent_t> ev;  // per slab: 5 marks (eval, cells, normals, quads, end)
@@ -1025,6 +1028,9 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             ctx->last_invalid = cnt[kInvalid];
             return S2M_OK;
         }
+        // vertices past the capacity form no quads, so after a vertex overflow the triangle
+        // count is low: size triangles from the vertices as at the start
+        if (cnt[kVtxTotal] > want_v) want_t = std::max<uint64_t>(want_t, 3 * (cnt[kVtxTotal] + cnt[kVtxTotal] / 8));
         want_v = std::max<uint64_t>(want_v, cnt[kVtxTotal] + cnt[kVtxTotal] / 8 + 1024);
         want_t = std::max<uint64_t>(want_t, cnt[kTriTotal] + cnt[kTriTotal] / 8 + 2048);
     }

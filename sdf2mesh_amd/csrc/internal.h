@@ -172,7 +172,7 @@ struct MeshBufs {
     uint64_t* counters;    // Counter
     uint32_t* bcount;      // per-block counts
     uint32_t* boffset;     // per-block exclusive offsets
-    uint64_t* actbits;     // dense slab: bit c % 64 of word c / 64 = cell c has a vertex (k_cells)
+    uint64_t* actbits;     // dense slab: bit c % 64 of word c / 64 = cell c has a vertex (zero between slabs)
 };
 
 // ---- launchers (kernels.hip) ----

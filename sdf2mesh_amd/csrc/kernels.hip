@@ -150,8 +150,9 @@ __device__ __forceinline__ void cell_coords(const SlabArgs& a, uint32_t i, uint3
 }
 
 // ---------------------------------------------------------------------------------
-// Cell pass.  EMIT = 0: every cell's vertex rule; block active counts and each cell's
-// activity bit (b.actbits, one ballot word per 64 cells).  EMIT = 1: the cell -> vertex
+// Cell pass.  EMIT = 0 (per-cell corner dumps; the lattice has k_count_cols): every
+// cell's vertex rule; block active counts and each cell's activity bit (b.actbits, one
+// ballot word per 64 cells).  EMIT = 1: the cell -> vertex
 // index map for every cell, its ranks from the activity bits, then the block's active
 // cells, listed in LDS in vertex order, are gathered and contoured again on full waves
 // (the same rule on the same corners gives the same vertex) and their vertices written.
@@ -233,6 +234,9 @@ __global__ __launch_bounds__(256) void k_cells(SlabArgs a, MeshBufs b, const flo
             before += total;
         }
         __syncthreads();
+        // every read of the block's words is done: clear them for the next slab's count
+        if (threadIdx.x < kCellsPerBlock / 64 && base + 64u * threadIdx.x < a.ncells)
+            b.actbits[(base >> 6) + threadIdx.x] = 0ull;
         for (uint32_t q = threadIdx.x; q < before; q += kBlock) {
             const uint32_t i = list[q];
             const uint64_t v = (uint64_t)first + q;
@@ -255,6 +259,80 @@ __global__ __launch_bounds__(256) void k_cells(SlabArgs a, MeshBufs b, const flo
             }
         }
     }
+}
+
+// ---------------------------------------------------------------------------------
+// Count pass over the slab lattice, one z-column of cells per thread: block (bx, y),
+// thread x = 256 bx + t walks zl = 0 .. nzl - 1.  A cell's upper corner face is the next
+// cell's lower face whenever hiz[z] == loz[z + 1] (the same lattice plane), so it is kept
+// in registers.  The vertex rule's result (contour_cell's, which does not depend on the
+// cell bounds) sets bit i % 64 of word i / 64 of `bits` (i = the cell's slab index); a
+// wave's 64 cells of a row span at most two words, ORed in with atomics only where a bit
+// is set.  `bits` is zero on entry (k_cells<1> clears the words it reads).
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_count_cols(SlabArgs a, uint32_t nzl, uint64_t* __restrict__ bits)
+{
+    const uint32_t x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
+    const uint32_t x0 = blockIdx.x * kBlock + (threadIdx.x & ~63u);  // the wave's first cell
+    if (x0 >= a.rx) return;  // whole wave past the row (wave-uniform)
+    const bool in = x < a.rx;
+    const uint32_t xi0 = in ? a.lox[x] : 0u, xi1 = in ? a.hix[x] : 0u;
+    const size_t pl = (size_t)a.lat_ny * a.pitch;
+    const float* r0 = a.lat + (size_t)a.loy[y] * a.pitch;  // lattice rows of the cell's y lo / hi
+    const float* r1 = a.lat + (size_t)a.hiy[y] * a.pitch;
+    float u[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    uint32_t uz = 0xffffffffu;  // lattice plane (slab-relative) held in u
+    const float zero3[3] = {0.0f, 0.0f, 0.0f};
+    for (uint32_t zl = 0; zl < nzl; ++zl) {
+        const uint32_t zg = a.z0 + zl;
+        const uint32_t zlo = a.loz[zg] - a.lz0, zhi = a.hiz[zg] - a.lz0;
+        float c[8];
+        if (zlo == uz) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) c[k] = u[k];
+        } else {
+            const size_t o = (size_t)zlo * pl;
+            c[0] = r0[o + xi0];
+            c[1] = r0[o + xi1];
+            c[2] = r1[o + xi0];
+            c[3] = r1[o + xi1];
+        }
+        const size_t o = (size_t)zhi * pl;
+        c[4] = r0[o + xi0];
+        c[5] = r0[o + xi1];
+        c[6] = r1[o + xi0];
+        c[7] = r1[o + xi1];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) u[k] = c[4 + k];
+        uz = zhi;
+        float pos[3];
+        const bool act = in && contour_cell(c, zero3, zero3, pos);
+        const uint64_t w = __ballot(act);
+        if (w != 0ull && lane_id() == 0) {
+            const uint64_t i0 = x0 + (uint64_t)a.rx * (y + (uint64_t)a.ry * zl);
+            const uint32_t sh = (uint32_t)(i0 & 63u);
+            unsigned long long* wp = reinterpret_cast<unsigned long long*>(bits + (i0 >> 6));
+            atomicOr(wp, (unsigned long long)(w << sh));
+            if (sh != 0u && (w >> (64u - sh)) != 0ull) atomicOr(wp + 1, (unsigned long long)(w >> (64u - sh)));
+        }
+    }
+}
+
+// Active cells per 1024-cell block: the popcount of its 16 bit words (words past the
+// slab's last cell are zero).
+__global__ __launch_bounds__(256) void k_popc(const uint64_t* __restrict__ bits, uint32_t nb,
+                                              uint32_t* __restrict__ bcount)
+{
+    const uint32_t blk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blk >= nb) return;
+    const uint4* w = reinterpret_cast<const uint4*>(bits + (size_t)blk * (kCellsPerBlock / 64));
+    uint32_t n = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kCellsPerBlock / 128; ++k) {
+        const uint4 q = w[k];
+        n += __builtin_popcount(q.x) + __builtin_popcount(q.y) + __builtin_popcount(q.z) + __builtin_popcount(q.w);
+    }
+    bcount[blk] = n;
 }
 
 // ---------------------------------------------------------------------------------
@@ -358,30 +436,40 @@ __global__ __launch_bounds__(256) void k_normals(const s2m_op* __restrict__ ops,
 // ---------------------------------------------------------------------------------
 // Quad pass (mesh.rs:267-324).  Per cell with a vertex: the x-, y- and z-edge quads,
 // neighbour vertices from the map planes of this and the previous cell plane.  A quad
-// with a missing vertex is skipped and counted (mesh.rs:270-278).  EMIT = 0 counts
-// triangles per block, EMIT = 1 writes them in order.
+// with a missing vertex is skipped and counted (mesh.rs:270-278).  The cells with a
+// vertex are the slab's vertices [kVtxPrev, kVtxTotal), in cell order, each with its
+// cell key and sign bits, so block b walks vertices kVtxPrev + 1024 b ... (a grid sized
+// for a vertex in every cell; blocks past the slab's vertices count zero).  EMIT = 0
+// counts triangles per block, EMIT = 1 writes them in order.  A vertex beyond the
+// output capacity has no key: it forms no quads (the job repeats with a larger capacity).
 // ---------------------------------------------------------------------------------
 template <int EMIT>
 __global__ __launch_bounds__(256) void k_quads(SlabArgs a, MeshBufs b)
 {
     __shared__ uint32_t lds[kBlock / 64];
-    const uint32_t base = blockIdx.x * kCellsPerBlock;
+    const uint64_t vbeg = b.counters[kVtxPrev] + (uint64_t)blockIdx.x * kCellsPerBlock;
+    const uint64_t vend = min(b.counters[kVtxTotal], b.vcap);
+    if (vbeg >= vend) {  // block-uniform
+        if (!EMIT && threadIdx.x == 0) b.bcount[blockIdx.x] = 0;
+        return;
+    }
     uint64_t running = EMIT ? (uint64_t)b.boffset[blockIdx.x] : 0;
     uint32_t count = 0, invalid = 0;
     const size_t plane = (size_t)a.rx * a.ry;
     const bool lag = (a.flags & S2M_COMPAT_DROP_LAST_SLICE) != 0;
     for (uint32_t j = 0; j < kRounds; ++j) {
-        const uint32_t i = base + j * kBlock + threadIdx.x;
+        const uint64_t vv = vbeg + j * kBlock + threadIdx.x;
         uint32_t q[3][4];
         uint32_t nq = 0;  // bit e set: quad e is valid
-        if (i < a.ncells) {
-            uint32_t x, y, zl;
-            cell_coords(a, i, x, y, zl);
+        if (vv < vend) {
+            const uint32_t v = (uint32_t)vv;
+            const uint64_t key = b.keys[v];
+            const uint32_t x = (uint32_t)(key & 0xffffu), y = (uint32_t)((key >> 16) & 0xffffu);
+            const uint32_t zl = (uint32_t)(key >> 32) - a.z0;
             const uint32_t* cur = a.vidx + (size_t)(zl + 1) * plane;
             const uint32_t* prv = a.vidx + (size_t)zl * plane;
-            const uint32_t v = cur[(size_t)y * a.rx + x];
-            if (v != kNone) {
-                const uint32_t s = (v < b.vcap) ? b.signs[v] : 0u;
+            {
+                const uint32_t s = b.signs[v];
                 const bool c0 = s & 1, c1 = (s >> 1) & 1, c2 = (s >> 2) & 1, c3 = (s >> 3) & 1;
                 const uint32_t zg = a.z0 + zl;
                 const size_t r0 = (size_t)y * a.rx, rm = (size_t)(y - 1) * a.rx;
@@ -471,8 +559,12 @@ hipError_t launch_cells(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int
     if (emit)
         hipLaunchKernelGGL((k_cells<1, 0>), dim3(nb), dim3(kBlock), 0, s, a, b, nullptr,
                            write_zero_normals);
-    else
-        hipLaunchKernelGGL((k_cells<0, 0>), dim3(nb), dim3(kBlock), 0, s, a, b, nullptr, 0);
+    else {
+        const uint32_t nzl = a.ncells / (a.rx * a.ry);
+        hipLaunchKernelGGL(k_count_cols, dim3((a.rx + kBlock - 1) / kBlock, a.ry), dim3(kBlock), 0, s, a, nzl,
+                           b.actbits);
+        hipLaunchKernelGGL(k_popc, dim3((nb + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b.actbits, nb, b.bcount);
+    }
     return hipGetLastError();
 }
 

@@ -85,6 +85,32 @@ def test_glsl_mandelbulb_equals_tape_op(ctx):
     same_mesh(ctx.mesh(c, p), ctx.mesh(t, p))
 
 
+GYROID = """use sdf3d::*;
+
+fn sdf3d(p: vec3f) -> f32 {
+    let q = p * 20.0;
+    return sin(q.x) * cos(q.y) + sin(q.y) * cos(q.z) + sin(q.z) * cos(q.x);
+}
+"""
+
+
+def test_capacity_regrow():
+    # a fresh context's first vertex capacity at 64^3 is 65536; this gyroid has more
+    # vertices, so the first job overflows and repeats with exact sizes (on the dense
+    # path, vertices past the capacity form no quads until then).  It must equal the same
+    # job run again in the grown context.
+    print("capacity regrow", flush=True)
+    with s2m.Context(0) as fresh:
+        prog = fresh.program_from_source(GYROID)
+        p = s2m.Params.reference(64, 2.0)
+        first = fresh.mesh(prog, p)
+        again = fresh.mesh(prog, p)
+        prog.close()
+    assert len(first.vertices) > 65536
+    same_mesh(first, again)
+    assert len(first.triangles) > len(first.vertices)
+
+
 def test_batch_mixes_compiled_and_tape(ctx, engraved):
     print("mixed batch", flush=True)
     progs = [ctx.program(tape.torus_example()), engraved, ctx.program(tape.random_csg(5))]
