@@ -424,13 +424,34 @@ __global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ bcount, ui
 // ---------------------------------------------------------------------------------
 static_assert(kDenseVtxTotal == kVtxTotal && kDenseVtxPrev == kVtxPrev, "counter layout");
 
+// library_normal over the vertices [counters[prev], min(counters[total], vcap)) with one
+// lane per tap: lanes 4j .. 4j + 3 of a wave evaluate taps d1 .. d4 of one vertex (the
+// same points as library_normal), lane 4j gathers them and writes the unit gradient.
 __global__ __launch_bounds__(256) void k_normals(const s2m_op* __restrict__ ops, uint32_t nops,
                                                  float eps, float* __restrict__ vtx,
                                                  const uint64_t* __restrict__ counters, uint64_t vcap)
 {
-    const auto sdf = [=](float x, float y, float z) { return tape_eval(ops, nops, x, y, z); };
-    normals_range([=](float px, float py, float pz, float e, float* o) { library_normal(sdf, px, py, pz, e, o); },
-                  eps, vtx, counters, vcap);
+    const uint64_t beg = counters[kVtxPrev];
+    const uint64_t end = counters[kVtxTotal] < vcap ? counters[kVtxTotal] : vcap;
+    const uint32_t tap = threadIdx.x & 3u, l0 = lane_id() & ~3u;
+    // taps v1 = (1,-1,-1), v2 = (-1,-1,1), v3 = (-1,1,-1), v4 = (1,1,1) times eps
+    const float ex = (tap == 0u || tap == 3u) ? eps : -eps;
+    const float ey = tap >= 2u ? eps : -eps;
+    const float ez = (tap == 1u || tap == 3u) ? eps : -eps;
+    // the 4 lanes of a vertex share the loop condition
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; beg + (i >> 2) < end;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        float* o = vtx + 6 * (beg + (i >> 2));
+        const float d = tape_eval(ops, nops, o[0] + ex, o[1] + ey, o[2] + ez);
+        const float d1 = __shfl(d, (int)l0, 64), d2 = __shfl(d, (int)l0 + 1, 64);
+        const float d3 = __shfl(d, (int)l0 + 2, 64), d4 = __shfl(d, (int)l0 + 3, 64);
+        if (tap == 0u) {
+            const float gx = ((d1 + (-d2)) + (-d3)) + d4;
+            const float gy = (((-d1) + (-d2)) + d3) + d4;
+            const float gz = (((-d1) + d2) + (-d3)) + d4;
+            write_unit(o, gx, gy, gz);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -591,7 +612,7 @@ hipError_t launch_scan(hipStream_t s, const MeshBufs& b, uint32_t nblocks, uint3
 hipError_t launch_normals(hipStream_t s, const s2m_op* ops, uint32_t nops, float eps,
                           const MeshBufs& b)
 {
-    hipLaunchKernelGGL(k_normals, dim3(1024), dim3(256), 0, s, ops, nops, eps, b.vtx, b.counters,
+    hipLaunchKernelGGL(k_normals, dim3(4096), dim3(256), 0, s, ops, nops, eps, b.vtx, b.counters,
                        b.vcap);
     return hipGetLastError();
 }
