@@ -280,31 +280,33 @@ __global__ __launch_bounds__(256) void k_count_cols(SlabArgs a, uint32_t nzl, ui
     const size_t pl = (size_t)a.lat_ny * a.pitch;
     const float* r0 = a.lat + (size_t)a.loy[y] * a.pitch;  // lattice rows of the cell's y lo / hi
     const float* r1 = a.lat + (size_t)a.hiy[y] * a.pitch;
-    float u[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    uint32_t uz = 0xffffffffu;  // lattice plane (slab-relative) held in u
     const float zero3[3] = {0.0f, 0.0f, 0.0f};
+    auto face = [&](uint32_t zi, float* f) {  // the 4 corners of lattice plane zi
+        const size_t o = (size_t)zi * pl;
+        f[0] = r0[o + xi0];
+        f[1] = r0[o + xi1];
+        f[2] = r1[o + xi0];
+        f[3] = r1[o + xi1];
+    };
+    // software pipelined: the next cell's corner loads are issued before this cell's rule
+    float c[8];
+    uint32_t zhi = a.hiz[a.z0] - a.lz0;
+    face(a.loz[a.z0] - a.lz0, c);
+    face(zhi, c + 4);
     for (uint32_t zl = 0; zl < nzl; ++zl) {
-        const uint32_t zg = a.z0 + zl;
-        const uint32_t zlo = a.loz[zg] - a.lz0, zhi = a.hiz[zg] - a.lz0;
-        float c[8];
-        if (zlo == uz) {
+        float n[8];
+        uint32_t nhi = zhi;
+        if (zl + 1 < nzl) {
+            const uint32_t nlo = a.loz[a.z0 + zl + 1] - a.lz0;
+            nhi = a.hiz[a.z0 + zl + 1] - a.lz0;
+            if (nlo == zhi) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) c[k] = u[k];
-        } else {
-            const size_t o = (size_t)zlo * pl;
-            c[0] = r0[o + xi0];
-            c[1] = r0[o + xi1];
-            c[2] = r1[o + xi0];
-            c[3] = r1[o + xi1];
+                for (int k = 0; k < 4; ++k) n[k] = c[4 + k];
+            } else {
+                face(nlo, n);
+            }
+            face(nhi, n + 4);
         }
-        const size_t o = (size_t)zhi * pl;
-        c[4] = r0[o + xi0];
-        c[5] = r0[o + xi1];
-        c[6] = r1[o + xi0];
-        c[7] = r1[o + xi1];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) u[k] = c[4 + k];
-        uz = zhi;
         float pos[3];
         const bool act = in && contour_cell(c, zero3, zero3, pos);
         const uint64_t w = __ballot(act);
@@ -314,6 +316,11 @@ __global__ __launch_bounds__(256) void k_count_cols(SlabArgs a, uint32_t nzl, ui
             unsigned long long* wp = reinterpret_cast<unsigned long long*>(bits + (i0 >> 6));
             atomicOr(wp, (unsigned long long)(w << sh));
             if (sh != 0u && (w >> (64u - sh)) != 0ull) atomicOr(wp + 1, (unsigned long long)(w >> (64u - sh)));
+        }
+        if (zl + 1 < nzl) {
+#pragma unroll
+            for (int k = 0; k < 8; ++k) c[k] = n[k];
+            zhi = nhi;
         }
     }
 }
