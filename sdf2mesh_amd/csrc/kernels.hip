@@ -150,9 +150,9 @@ __device__ __forceinline__ void cell_coords(const SlabArgs& a, uint32_t i, uint3
 }
 
 // ---------------------------------------------------------------------------------
-// Cell pass.  EMIT = 0 (per-cell corner dumps; the lattice has k_count_cols): every
-// cell's vertex rule; block active counts and each cell's activity bit (b.actbits, one
-// ballot word per 64 cells).  EMIT = 1: the cell -> vertex
+// Cell passes over per-cell corner dumps (FROM_CORNERS; the slab lattice has k_count_cols,
+// k_cells_map and k_cell_vertices).  EMIT = 0: every cell's vertex rule; block active
+// counts and each cell's activity bit (b.actbits, one ballot word per 64 cells).  EMIT = 1: the cell -> vertex
 // index map for every cell, its ranks from the activity bits, then the block's active
 // cells, listed in LDS in vertex order, are gathered and contoured again on full waves
 // (the same rule on the same corners gives the same vertex) and their vertices written.
@@ -258,6 +258,74 @@ __global__ __launch_bounds__(256) void k_cells(SlabArgs a, MeshBufs b, const flo
                 b.signs[v] = (uint8_t)sign_bits(c);
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Emit pass over the slab lattice, in two kernels.  k_cells_map: the cell -> vertex map
+// for every cell, the ranks from the activity bits (as k_cells<1>), and each vertex's
+// cell key; it clears the bit words it read.  k_cell_vertices: a grid-stride walk over
+// the slab's vertices [kVtxPrev, min(kVtxTotal, vcap)) that gathers each vertex's cell
+// from its key and writes the vertex (contour_cell, as the count pass decided) and its
+// sign bits.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_cells_map(SlabArgs a, MeshBufs b)
+{
+    const uint32_t base = blockIdx.x * kCellsPerBlock;
+    const uint32_t first = b.boffset[blockIdx.x];
+    uint32_t before = 0;  // active cells of earlier rounds
+    for (uint32_t j = 0; j < kRounds; ++j) {
+        const uint32_t i0 = base + j * kBlock, i = i0 + threadIdx.x;
+        const uint32_t wv = threadIdx.x >> 6, ln = lane_id();
+        uint64_t wd[kBlock / 64];
+#pragma unroll
+        for (uint32_t k = 0; k < kBlock / 64; ++k) wd[k] = i0 + 64u * k < a.ncells ? b.actbits[(i0 >> 6) + k] : 0ull;
+        uint32_t rank = before, total = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kBlock / 64; ++k) {
+            const uint32_t pc = (uint32_t)__builtin_popcountll(wd[k]);
+            total += pc;
+            if (k < wv) rank += pc;
+        }
+        const uint64_t mine = wd[wv];
+        const bool act = (mine >> ln) & 1ull;
+        rank += (uint32_t)__builtin_popcountll(mine & ((1ull << ln) - 1ull));
+        if (i < a.ncells) {
+            uint32_t x, y, zl;
+            cell_coords(a, i, x, y, zl);
+            const size_t m = ((size_t)(zl + 1) * a.ry + y) * a.rx + x;
+            a.vidx[m] = act ? first + rank : kNone;
+            const uint64_t v = (uint64_t)first + rank;
+            if (act && v < b.vcap) b.keys[v] = (uint64_t)x | ((uint64_t)y << 16) | ((uint64_t)(a.z0 + zl) << 32);
+        }
+        before += total;
+    }
+    __syncthreads();  // every read of the block's words is done
+    if (threadIdx.x < kCellsPerBlock / 64 && base + 64u * threadIdx.x < a.ncells)
+        b.actbits[(base >> 6) + threadIdx.x] = 0ull;
+}
+
+__global__ __launch_bounds__(256) void k_cell_vertices(SlabArgs a, MeshBufs b, int zero_normals)
+{
+    const uint64_t beg = b.counters[kVtxPrev];
+    const uint64_t end = b.counters[kVtxTotal] < b.vcap ? b.counters[kVtxTotal] : b.vcap;
+    for (uint64_t v = beg + (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < end;
+         v += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t key = b.keys[v];
+        const uint32_t x = (uint32_t)(key & 0xffffu), y = (uint32_t)((key >> 16) & 0xffffu);
+        const uint32_t zl = (uint32_t)(key >> 32) - a.z0;
+        float c[8], pos[3];
+        cell_vertex<0>(a, nullptr, 0, x, y, zl, c, pos);
+        float* o = b.vtx + 6 * v;
+        o[0] = pos[0];
+        o[1] = pos[1];
+        o[2] = pos[2];
+        if (zero_normals) {
+            o[3] = 0.0f;
+            o[4] = 0.0f;
+            o[5] = 0.0f;
+        }
+        b.signs[v] = (uint8_t)sign_bits(c);
     }
 }
 
@@ -584,10 +652,10 @@ hipError_t launch_cells(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int
 {
     const uint32_t nb = (a.ncells + kCellsPerBlock - 1) / kCellsPerBlock;
     if (nb == 0) return hipSuccess;
-    if (emit)
-        hipLaunchKernelGGL((k_cells<1, 0>), dim3(nb), dim3(kBlock), 0, s, a, b, nullptr,
-                           write_zero_normals);
-    else {
+    if (emit) {
+        hipLaunchKernelGGL(k_cells_map, dim3(nb), dim3(kBlock), 0, s, a, b);
+        hipLaunchKernelGGL(k_cell_vertices, dim3(1024), dim3(kBlock), 0, s, a, b, write_zero_normals);
+    } else {
         const uint32_t nzl = a.ncells / (a.rx * a.ry);
         hipLaunchKernelGGL(k_count_cols, dim3((a.rx + kBlock - 1) / kBlock, a.ry), dim3(kBlock), 0, s, a, nzl,
                            b.actbits);
