@@ -822,9 +822,9 @@ hipError_t eval_lattice_prog(hipStream_t s, const s2m_program* prog, const float
     if (!prog->module)
         return launch_eval_lattice(s, prog->d_ops, prog->n, cx, cy, cz, nx, ny, lz0, nzl, pitch, lat);
     if (nx == 0 || ny == 0 || nzl == 0) return hipSuccess;
-    void* args[] = {&cx, &cy, &cz, &nx, &ny, &lz0, &pitch, &lat};
-    return hipModuleLaunchKernel(prog->k_eval, (nx + 63) / 64, (ny + 3) / 4, nzl, 256, 1, 1, 0, s, args,
-                                 nullptr);
+    void* args[] = {&cx, &cy, &cz, &nx, &ny, &lz0, &nzl, &pitch, &lat};
+    return hipModuleLaunchKernel(prog->k_eval, (nx + 15) / 16, (ny + 3) / 4, (nzl + 3) / 4, 256, 1, 1, 0, s,
+                                 args, nullptr);
 }
 
 hipError_t normals_prog(hipStream_t s, const s2m_program* prog, float eps, const MeshBufs& b)

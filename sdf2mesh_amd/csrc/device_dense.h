@@ -14,19 +14,21 @@ namespace s2m {
 constexpr uint32_t kDenseVtxTotal = 0;  // vertices emitted so far
 constexpr uint32_t kDenseVtxPrev = 1;   // vertex total before the current slab
 
-// One lattice point per thread: 256-thread blocks, 64 consecutive x per wave (each wave
-// stores 256 contiguous bytes), grid (ceil(nx / 64), ceil(ny / 4), planes).
+// One lattice point per thread: 256-thread blocks of 16 x 4 x 4 points, each wave a
+// 4 x 4 x 4 block (lanes close together, so data-dependent loops diverge less than along
+// a row), grid (ceil(nx / 16), ceil(ny / 4), ceil(planes / 4)).
 template <class F>
 __device__ __forceinline__ void eval_lattice_point(F sdf, const float* __restrict__ cx,
                                                    const float* __restrict__ cy,
                                                    const float* __restrict__ cz, uint32_t nx,
-                                                   uint32_t ny, uint32_t lz0, uint32_t pitch,
+                                                   uint32_t ny, uint32_t lz0, uint32_t nzl, uint32_t pitch,
                                                    float* __restrict__ lat)
 {
-    const uint32_t ix = blockIdx.x * 64u + (threadIdx.x & 63u);
-    const uint32_t iy = blockIdx.y * 4u + (threadIdx.x >> 6);
-    const uint32_t iz = blockIdx.z;
-    if (ix >= nx || iy >= ny) return;
+    const uint32_t l = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t ix = blockIdx.x * 16u + w * 4u + (l & 3u);
+    const uint32_t iy = blockIdx.y * 4u + ((l >> 2) & 3u);
+    const uint32_t iz = blockIdx.z * 4u + (l >> 4);
+    if (ix >= nx || iy >= ny || iz >= nzl) return;
     lat[((size_t)iz * ny + iy) * pitch + ix] = sdf(cx[ix], cy[iy], cz[lz0 + iz]);
 }
 

@@ -801,8 +801,8 @@ bool emit_hip(const Module& m, Lang lang, const std::string& entry, std::string&
          "};\n"
          "extern \"C\" __global__ __launch_bounds__(256) void s2m_c_eval_lattice(const float* __restrict__ cx,\n"
          "    const float* __restrict__ cy, const float* __restrict__ cz, uint32_t nx, uint32_t ny, uint32_t lz0,\n"
-         "    uint32_t pitch, float* __restrict__ lat)\n{\n"
-         "    s2m::eval_lattice_point(s2m_user_fn{}, cx, cy, cz, nx, ny, lz0, pitch, lat);\n}\n"
+         "    uint32_t nzl, uint32_t pitch, float* __restrict__ lat)\n{\n"
+         "    s2m::eval_lattice_point(s2m_user_fn{}, cx, cy, cz, nx, ny, lz0, nzl, pitch, lat);\n}\n"
          "struct s2m_user_normal {\n"
          "    __device__ void operator()(float px, float py, float pz, float eps, float* o) const\n    {\n";
     if (user_normal)
