@@ -570,43 +570,41 @@ __global__ __launch_bounds__(256) void k_quads(SlabArgs a, MeshBufs b)
             const uint32_t zl = (uint32_t)(key >> 32) - a.z0;
             const uint32_t* cur = a.vidx + (size_t)(zl + 1) * plane;
             const uint32_t* prv = a.vidx + (size_t)zl * plane;
-            {
-                const uint32_t s = b.signs[v];
-                const bool c0 = s & 1, c1 = (s >> 1) & 1, c2 = (s >> 2) & 1, c3 = (s >> 3) & 1;
-                const uint32_t zg = a.z0 + zl;
-                const size_t r0 = (size_t)y * a.rx, rm = (size_t)(y - 1) * a.rx;
-                // x-edge: Quad(v(x,y-1,z-1), v(x,y,z-1), v(x,y,z), v(x,y-1,z)).swap(c.0)
-                if (c0 != c3 && y > 0) {
-                    if (zg > 0) {
-                        uint32_t t[4] = {prv[rm + x], prv[r0 + x], v, cur[rm + x]};
-                        if (c0) { q[0][0] = t[3]; q[0][1] = t[2]; q[0][2] = t[1]; q[0][3] = t[0]; }
-                        else { q[0][0] = t[0]; q[0][1] = t[1]; q[0][2] = t[2]; q[0][3] = t[3]; }
-                        if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 1;
-                        else ++invalid;
-                    } else if (lag) {
-                        ++invalid;  // the lagged reference looks up label z-1 and misses
-                    }
-                }
-                // y-edge: Quad(v(x-1,y,z-1), v(x,y,z-1), v(x,y,z), v(x-1,y,z)).swap(!c.1)
-                if (c1 != c3 && x > 0) {
-                    if (zg > 0) {
-                        uint32_t t[4] = {prv[r0 + x - 1], prv[r0 + x], v, cur[r0 + x - 1]};
-                        if (!c1) { q[1][0] = t[3]; q[1][1] = t[2]; q[1][2] = t[1]; q[1][3] = t[0]; }
-                        else { q[1][0] = t[0]; q[1][1] = t[1]; q[1][2] = t[2]; q[1][3] = t[3]; }
-                        if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 2;
-                        else ++invalid;
-                    } else if (lag) {
-                        ++invalid;
-                    }
-                }
-                // z-edge: Quad(v(x-1,y-1,z), v(x,y-1,z), v(x,y,z), v(x-1,y,z)).swap(c.2)
-                if (c2 != c3 && x > 0 && y > 0) {
-                    uint32_t t[4] = {cur[rm + x - 1], cur[rm + x], v, cur[r0 + x - 1]};
-                    if (c2) { q[2][0] = t[3]; q[2][1] = t[2]; q[2][2] = t[1]; q[2][3] = t[0]; }
-                    else { q[2][0] = t[0]; q[2][1] = t[1]; q[2][2] = t[2]; q[2][3] = t[3]; }
-                    if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 4;
+            const uint32_t s = b.signs[v];
+            const bool c0 = s & 1, c1 = (s >> 1) & 1, c2 = (s >> 2) & 1, c3 = (s >> 3) & 1;
+            const uint32_t zg = a.z0 + zl;
+            const size_t r0 = (size_t)y * a.rx, rm = (size_t)(y - 1) * a.rx;
+            // x-edge: Quad(v(x,y-1,z-1), v(x,y,z-1), v(x,y,z), v(x,y-1,z)).swap(c.0)
+            if (c0 != c3 && y > 0) {
+                if (zg > 0) {
+                    uint32_t t[4] = {prv[rm + x], prv[r0 + x], v, cur[rm + x]};
+                    if (c0) { q[0][0] = t[3]; q[0][1] = t[2]; q[0][2] = t[1]; q[0][3] = t[0]; }
+                    else { q[0][0] = t[0]; q[0][1] = t[1]; q[0][2] = t[2]; q[0][3] = t[3]; }
+                    if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 1;
                     else ++invalid;
+                } else if (lag) {
+                    ++invalid;  // the lagged reference looks up label z-1 and misses
                 }
+            }
+            // y-edge: Quad(v(x-1,y,z-1), v(x,y,z-1), v(x,y,z), v(x-1,y,z)).swap(!c.1)
+            if (c1 != c3 && x > 0) {
+                if (zg > 0) {
+                    uint32_t t[4] = {prv[r0 + x - 1], prv[r0 + x], v, cur[r0 + x - 1]};
+                    if (!c1) { q[1][0] = t[3]; q[1][1] = t[2]; q[1][2] = t[1]; q[1][3] = t[0]; }
+                    else { q[1][0] = t[0]; q[1][1] = t[1]; q[1][2] = t[2]; q[1][3] = t[3]; }
+                    if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 2;
+                    else ++invalid;
+                } else if (lag) {
+                    ++invalid;
+                }
+            }
+            // z-edge: Quad(v(x-1,y-1,z), v(x,y-1,z), v(x,y,z), v(x-1,y,z)).swap(c.2)
+            if (c2 != c3 && x > 0 && y > 0) {
+                uint32_t t[4] = {cur[rm + x - 1], cur[rm + x], v, cur[r0 + x - 1]};
+                if (c2) { q[2][0] = t[3]; q[2][1] = t[2]; q[2][2] = t[1]; q[2][3] = t[0]; }
+                else { q[2][0] = t[0]; q[2][1] = t[1]; q[2][2] = t[2]; q[2][3] = t[3]; }
+                if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 4;
+                else ++invalid;
             }
         }
         const uint32_t ntri = 2u * (uint32_t)__builtin_popcount(nq);
