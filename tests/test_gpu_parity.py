@@ -108,16 +108,24 @@ def test_no_normals(ctx, dense):
     assert not np.any(b.vertices[:, 3:])
 
 
-@pytest.mark.parametrize("slab", [1, 3, 7])
+@pytest.mark.parametrize("slab", [0, 1, 3, 7])
 def test_slab_invariance(ctx, slab):
-    print(f"slab={slab}", flush=True)
+    # the dense path in z-slabs of `slab` cell planes (0: automatic): the column count
+    # pass, the activity bits cleared between slabs and the map plane carried from one slab
+    # to the next must give the tile-pruned path's mesh, also on a size that is not a
+    # multiple of 64 (rows that end inside a wave, bit words shared by two rows)
     node = tape.csg64()
     prog = ctx.program(node)
-    a = ctx.mesh(prog, s2m.Params.reference(64, 2.0))
-    b = ctx.mesh(prog, s2m.Params.reference(64, 2.0, slab=slab))
-    assert np.array_equal(a.vertices.view(np.uint32), b.vertices.view(np.uint32))
-    assert np.array_equal(a.triangles, b.triangles)
-    assert a.invalid_quads == b.invalid_quads
+    for res in (64, 61):
+        print(f"slab={slab} res={res}", flush=True)
+        a = ctx.mesh(prog, s2m.Params.reference(res, 2.0))
+        b = ctx.mesh(prog, s2m.Params.reference(res, 2.0, flags=s2m.REFERENCE_COMPAT | s2m.DENSE_EVAL, slab=slab))
+        assert len(a.vertices) > 0
+        assert np.array_equal(a.vertices.view(np.uint32), b.vertices.view(np.uint32))
+        assert np.array_equal(a.cell_keys, b.cell_keys)
+        assert np.array_equal(a.signs, b.signs)
+        assert np.array_equal(a.triangles, b.triangles)
+        assert a.invalid_quads == b.invalid_quads
 
 
 def test_lattice_values_bit_exact(ctx):
