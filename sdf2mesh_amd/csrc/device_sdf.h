@@ -116,7 +116,11 @@ __device__ __forceinline__ float sd_mandelbulb(float px, float py, float pz, flo
         r = powf(r, power);
         theta = theta * power;
         phi = phi * power;
-        const float ct = cosf(theta), cp = cosf(phi), st = sinf(theta), sp = sinf(phi);
+        // sincosf gives sinf's and cosf's bits (checked over every float input) and reduces
+        // the argument once
+        float st, ct, sp, cp;
+        sincosf(theta, &st, &ct);
+        sincosf(phi, &sp, &cp);
         zx = r * (ct * cp) + px;
         zy = r * (st * cp) + py;
         zz = r * sp + pz;
