@@ -280,38 +280,47 @@ __global__ __launch_bounds__(256) void k_cells(SlabArgs a, MeshBufs b, const flo
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_cells_map(SlabArgs a, MeshBufs b)
 {
-    const uint32_t base = blockIdx.x * kCellsPerBlock;
+    // thread t: cells base + 4t .. base + 4t + 3, bits 4t % 64 .. of word t / 16; the map
+    // entry of slab cell i is vidx[plane + i] (plane 0 holds the previous slab's last plane)
+    constexpr uint32_t kWords = kCellsPerBlock / 64;
+    const uint32_t base = blockIdx.x * kCellsPerBlock, t = threadIdx.x;
     const uint32_t first = b.boffset[blockIdx.x];
-    uint32_t before = 0;  // active cells of earlier rounds
-    for (uint32_t j = 0; j < kRounds; ++j) {
-        const uint32_t i0 = base + j * kBlock, i = i0 + threadIdx.x;
-        const uint32_t wv = threadIdx.x >> 6, ln = lane_id();
-        uint64_t wd[kBlock / 64];
+    uint64_t wd[kWords];
 #pragma unroll
-        for (uint32_t k = 0; k < kBlock / 64; ++k) wd[k] = i0 + 64u * k < a.ncells ? b.actbits[(i0 >> 6) + k] : 0ull;
-        uint32_t rank = before, total = 0;
+    for (uint32_t k = 0; k < kWords; ++k) wd[k] = base + 64u * k < a.ncells ? b.actbits[(base >> 6) + k] : 0ull;
+    const uint32_t w = t >> 4, sh = (4u * t) & 63u;
+    uint32_t rank = first;
+    uint64_t mine = 0;
 #pragma unroll
-        for (uint32_t k = 0; k < kBlock / 64; ++k) {
-            const uint32_t pc = (uint32_t)__builtin_popcountll(wd[k]);
-            total += pc;
-            if (k < wv) rank += pc;
-        }
-        const uint64_t mine = wd[wv];
-        const bool act = (mine >> ln) & 1ull;
-        rank += (uint32_t)__builtin_popcountll(mine & ((1ull << ln) - 1ull));
-        if (i < a.ncells) {
+    for (uint32_t k = 0; k < kWords; ++k) {
+        if (k < w) rank += (uint32_t)__builtin_popcountll(wd[k]);
+        if (k == w) mine = wd[k];
+    }
+    rank += (uint32_t)__builtin_popcountll(mine & ((1ull << sh) - 1ull));
+    const uint32_t i0 = base + 4u * t;
+    const size_t plane = (size_t)a.rx * a.ry;
+    uint32_t v[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4u; ++u) {
+        const bool act = (mine >> (sh + u)) & 1ull;
+        v[u] = act ? rank : kNone;
+        if (act && i0 + u < a.ncells && rank < b.vcap) {
             uint32_t x, y, zl;
-            cell_coords(a, i, x, y, zl);
-            const size_t m = ((size_t)(zl + 1) * a.ry + y) * a.rx + x;
-            a.vidx[m] = act ? first + rank : kNone;
-            const uint64_t v = (uint64_t)first + rank;
-            if (act && v < b.vcap) b.keys[v] = (uint64_t)x | ((uint64_t)y << 16) | ((uint64_t)(a.z0 + zl) << 32);
+            cell_coords(a, i0 + u, x, y, zl);
+            b.keys[rank] = (uint64_t)x | ((uint64_t)y << 16) | ((uint64_t)(a.z0 + zl) << 32);
         }
-        before += total;
+        rank += act ? 1u : 0u;
+    }
+    uint32_t* m = a.vidx + plane + i0;
+    if (i0 + 3u < a.ncells && (plane & 3u) == 0u) {
+        *reinterpret_cast<uint4*>(m) = make_uint4(v[0], v[1], v[2], v[3]);
+    } else {
+#pragma unroll
+        for (uint32_t u = 0; u < 4u; ++u)
+            if (i0 + u < a.ncells) m[u] = v[u];
     }
     __syncthreads();  // every read of the block's words is done
-    if (threadIdx.x < kCellsPerBlock / 64 && base + 64u * threadIdx.x < a.ncells)
-        b.actbits[(base >> 6) + threadIdx.x] = 0ull;
+    if (t < kWords && base + 64u * t < a.ncells) b.actbits[(base >> 6) + t] = 0ull;
 }
 
 __global__ __launch_bounds__(256) void k_cell_vertices(SlabArgs a, MeshBufs b, int zero_normals)
