@@ -351,6 +351,13 @@ int s2m_eval_lattice(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* pa
 int s2m_contour_cells(s2m_ctx* ctx, const float* corners, const uint32_t res[3],
                       const float* cell_lo, const float* cell_hi, uint32_t flags,
                       s2m_mesh_out* out);
+/* No reference counterpart: checks a device math identity this build relies on.  The
+ * Mandelbulb's escape step takes the sine and cosine of an angle with one sincosf,
+ * where the reference's shader calls sin and cos.  Over every float bit pattern,
+ * counts[0] / counts[2] receive the sine / cosine results of sincosf that differ from
+ * sinf / cosf, counts[1] / counts[3] those of them where both are NaN.  A fraction of a
+ * second on one GPU. */
+int s2m_check_sincos(s2m_ctx* ctx, uint64_t counts[4]);
 
 #ifdef __cplusplus
 }

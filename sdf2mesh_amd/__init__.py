@@ -144,6 +144,7 @@ _SIGNATURES = {
     "s2m_write_ply": (C.c_int, [C.POINTER(_MeshOut), C.c_char_p]),
     "s2m_write_to_file": (C.c_int, [C.POINTER(_MeshOut), C.c_char_p]),
     "s2m_format_f32": (C.c_int, [C.c_float, C.c_char_p]),
+    "s2m_check_sincos": (C.c_int, [C.c_void_p, C.c_void_p]),
     "s2m_axis_table": (C.c_uint32, [C.c_float, C.c_float, C.c_uint32, C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_void_p]),
     "s2m_eval_lattice": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_Params), C.c_void_p,
@@ -457,6 +458,13 @@ class Context:
                                      C.byref(p), C.byref(dm)))
         return {"nvtx": int(dm.nvtx), "ntri": int(dm.ntri), "invalid_quads": int(dm.invalid_quads),
                 "vtx_ptr": dm.vtx, "tri_ptr": dm.tri}
+
+    def check_sincos(self) -> tuple:
+        """s2m_check_sincos: (sine mismatches, of them NaN pairs, cosine mismatches, of
+        them NaN pairs) of sincosf against sinf / cosf over every float."""
+        counts = (C.c_uint64 * 4)()
+        _check(lib().s2m_check_sincos(C.c_void_p(self.handle), counts))
+        return tuple(int(c) for c in counts)
 
     def eval_lattice(self, prog: Program, params: Params) -> np.ndarray:
         """SDF values on the corner lattice, shape (nz, ny, nx)."""

@@ -123,7 +123,7 @@ struct s2m_ctx {
     hipStream_t stream = nullptr;
     int profiling = 0;  // 0 off, 1 total + evaluation kernels, 2 every stage
     s2m_timing timing{};
-    DevBuf lattice, vidx, bcount, boffset, actbits, counters, tables;
+    DevBuf lattice, vidx, bcount, boffset, actbits, counters, tables, check;
     DevBuf vtx, keys, signs, tri, corners;
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_slot_rows, sp_boundary;
@@ -1141,7 +1141,7 @@ void s2m_destroy(s2m_ctx* ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (DevBuf* b : {&ctx->lattice, &ctx->vidx, &ctx->bcount, &ctx->boffset, &ctx->actbits, &ctx->counters,
+    for (DevBuf* b : {&ctx->lattice, &ctx->vidx, &ctx->bcount, &ctx->boffset, &ctx->actbits, &ctx->check, &ctx->counters,
                       &ctx->tables, &ctx->vtx, &ctx->keys, &ctx->signs, &ctx->tri, &ctx->corners,
                       &ctx->tile_tab, &ctx->tile_lists, &ctx->tile_lens,
                       &ctx->tile_flops, &ctx->sp_slot_of_tile, &ctx->sp_tile_of_slot,
@@ -1478,6 +1478,18 @@ uint32_t s2m_axis_table(float bmin, float bmax, uint32_t res, int exact_corners,
     if (lo) std::copy(t.lo.begin(), t.lo.end(), lo);
     if (hi) std::copy(t.hi.begin(), t.hi.end(), hi);
     return (uint32_t)t.coord.size();
+}
+
+int s2m_check_sincos(s2m_ctx* ctx, uint64_t counts[4])
+{
+    if (!ctx || !counts) return fail(S2M_E_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(ctx->check.ensure(4 * sizeof(uint64_t)));
+    HIP_TRY(hipMemsetAsync(ctx->check.p, 0, 4 * sizeof(uint64_t), ctx->stream));
+    HIP_TRY(launch_sincos_check(ctx->stream, ctx->check.as<uint64_t>()));
+    HIP_TRY(hipMemcpyAsync(counts, ctx->check.p, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return S2M_OK;
 }
 
 int s2m_eval_lattice(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, float* out,

@@ -650,6 +650,24 @@ __global__ __launch_bounds__(256) void k_quads(SlabArgs a, MeshBufs b)
 }
 
 // ---------------------------------------------------------------------------------
+// Every float bit pattern x: sincosf(x) against sinf(x) and cosf(x), bitwise.  cnt[0] /
+// cnt[2]: sine / cosine results that differ, cnt[1] / cnt[3]: those of them where both
+// are NaN.  (The Mandelbulb's escape step takes sine and cosine with one sincosf.)
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sincos_check(unsigned long long* __restrict__ cnt)
+{
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (1ull << 32);
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const float x = __uint_as_float((uint32_t)i);
+        const float s1 = sinf(x), c1 = cosf(x);
+        float s2, c2;
+        sincosf(x, &s2, &c2);
+        if (__float_as_uint(s1) != __float_as_uint(s2)) atomicAdd(&cnt[(s1 != s1 && s2 != s2) ? 1 : 0], 1ull);
+        if (__float_as_uint(c1) != __float_as_uint(c2)) atomicAdd(&cnt[(c1 != c1 && c2 != c2) ? 3 : 2], 1ull);
+    }
+}
+
+// ---------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------
 hipError_t launch_eval_lattice(hipStream_t s, const s2m_op* ops, uint32_t nops, const float* cx,
@@ -689,6 +707,12 @@ hipError_t launch_cells_from_corners(hipStream_t s, const SlabArgs& a, const flo
         hipLaunchKernelGGL((k_cells<1, 1>), dim3(nb), dim3(kBlock), 0, s, a, b, corners, 1);
     else
         hipLaunchKernelGGL((k_cells<0, 1>), dim3(nb), dim3(kBlock), 0, s, a, b, corners, 1);
+    return hipGetLastError();
+}
+
+hipError_t launch_sincos_check(hipStream_t s, uint64_t* cnt)
+{
+    hipLaunchKernelGGL(k_sincos_check, dim3(8192), dim3(256), 0, s, reinterpret_cast<unsigned long long*>(cnt));
     return hipGetLastError();
 }
 

@@ -128,6 +128,14 @@ def test_slab_invariance(ctx, slab):
         assert a.invalid_quads == b.invalid_quads
 
 
+def test_sincos_matches_sin_cos(ctx):
+    # the Mandelbulb's escape step takes sine and cosine with one sincosf; over every
+    # float input its bits must be sinf's and cosf's (the calls the reference's shader
+    # makes), NaN payloads included
+    print("sincos over every float", flush=True)
+    assert ctx.check_sincos() == (0, 0, 0, 0)
+
+
 def test_lattice_values_bit_exact(ctx):
     node = tape.csg64()
     p = s2m.Params.reference(64, 2.0)
