@@ -109,7 +109,9 @@ __device__ __forceinline__ float sd_mandelbulb(float px, float py, float pz, flo
     float r = 0.0f, dr = 1.0f;
     for (uint32_t i = 0; i < iters; ++i) {
         r = len3(zx, zy, zz);
-        if (r > bailout) continue;
+        // past the bailout z no longer changes, so every later step would recompute this
+        // r and skip the rest: leaving the loop gives the same r and dr
+        if (r > bailout) break;
         float theta = atanf(zy / zx);
         float phi = asinf(zz / r);
         dr = (powf(r, power - 1.0f) * dr) * power + 1.0f;
