@@ -1,30 +1,44 @@
 #!/usr/bin/env python3
 """Throughput of the meshing path on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--workload csg64-512] [--gpus 1]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--workload NAME] [--only]
 
-A step meshes one job (SDF lattice evaluation, contouring, normals, quads) with the
-program and parameters already on the device; the mesh stays in HBM.  Mvoxels/s =
-res^3 / 1e6 / seconds per step.  Prints one JSON line with:
+Headline (``value``): CSG-64 at 512^3, bounds 2, reference-exact corners.  A step is one
+meshing job, timed from the call with the program and parameters already on the device
+until the vertex (position + normal) and triangle index buffers are in host memory
+(s2m_mesh: the library copies them into page-locked host memory it recycles), the span
+of the reference's own t0 -> t1 (main.rs:292-357, which includes the per-slice readback).
+Mvoxels/s = res^3 / 1e6 / median step time.  ``device_resident`` gives the same job with
+the mesh left in HBM (s2m_mesh_device).
 
-  roofline      the SDF evaluation kernel (k_sp_tiles on the tile-pruned path: the
-                pruned program at the tile's lattice points fused with contouring, one
-                launch per pass for left-fold programs; k_eval_lattice on the dense path,
-                one per slab): algorithmic fp32 flops executed per launch (static op
-                count of the program each tile runs x the lattice points it evaluates) /
-                mean launch time from HIP events recorded around the tile launches on the
-                library's stream, against the fp32 peak (vector ALU: the path has no
-                matrix work).  traffic: HBM bytes per launch from the rocprofv3
-                FETCH_SIZE / WRITE_SIZE summary in profiles/hbm_traffic.json
+The same JSON line carries the other configurations under ``workloads``: torus-128,
+mandelbulb-1024 (host landing, median step), batch256 (256 random CSG jobs per step,
+s2m_mesh_batch, against the same jobs one at a time with s2m_mesh) and serve (Poisson
+request streams through the continuous-batching server at 10-2000 requests/s, p50/p99
+latency, against one-at-a-time meshing of the same stream).  ``--workload NAME --only``
+runs one of them alone (profiling runs).
+
+  roofline      the SDF evaluation kernel (k_sp_tiles on the tile-pruned path: the pruned
+                program at each live tile's lattice points fused with contouring;
+                k_eval_lattice on the dense path): executed pruned-program fp32 flops per
+                launch (static op count of the program each tile runs x the lattice points
+                it evaluates, counted on the device) / mean launch time from HIP events
+                recorded around the launches on the library's stream during the timed
+                steps.  Priced against the fp32 peak (157.3 TFLOP/s) and, as
+                frac_of_issue_ceiling, against 39.3 T lane-ops/s (one non-FMA, non-packed
+                fp32 op per lane per clock: the ceiling of code compiled without
+                contraction or packed fp32).  traffic: HBM bytes per launch from the
+                rocprofv3 FETCH_SIZE / WRITE_SIZE summary in profiles/hbm_traffic.json.
   cpu_baseline  the oracle (CPU restatement of the reference algorithm: 8 corner
-                evaluations per cell, OpenMP) on a bounded sample of z-slices of the
-                same job
+                evaluations per cell, OpenMP) on a bounded sample of z-slices of the same
+                job, median rate over 10 chunks
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 from pathlib import Path
@@ -34,8 +48,11 @@ sys.path.insert(0, str(ROOT))
 
 import numpy as np  # noqa: E402
 
-PEAK_FP32_TFLOPS = 157.3   # MI355X fp32 (vector and matrix) peak
+PEAK_FP32_TFLOPS = 157.3   # MI355X fp32 peak (FMA, packed)
+ISSUE_CEILING_TOPS = 39.3  # 256 CUs x 64 lanes x 2.4 GHz: one plain fp32 op per lane per clock
 PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
+
+SINGLE = ("csg64-512", "csg64-512-lattice", "torus-128", "mandelbulb-1024")
 
 
 def workload(name: str):
@@ -52,7 +69,7 @@ def workload(name: str):
             "corners": "shared lattice"}
     if name == "torus-128":
         return tape.torus_example(), s2m.Params.reference(128, 2.0), {
-            "workload": "torus-128", "res": 128, "bounds": 2.0}
+            "workload": "torus-128", "sdf": "examples/torus.sdf3d", "res": 128, "bounds": 2.0}
     if name == "mandelbulb-1024":
         return tape.mandelbulb(8), s2m.Params.reference(1024, 2.5), {
             "workload": "mandelbulb-1024", "res": 1024, "bounds": 2.5,
@@ -60,119 +77,262 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
+def cpu_info() -> dict:
+    model = platform.processor() or ""
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": allowed}
+
+
 def cpu_baseline(node, params, seconds: float) -> dict:
+    """The oracle on z-slices through the middle of the job: ~`seconds` of CPU work cut
+    into 10 equal chunks, median chunk rate.  Threads: OMP_NUM_THREADS when set (the
+    CPU share of a one-GPU box), else every CPU this process may run on."""
     import oracle
 
-    threads = min(16, os.cpu_count() or 1)
-    os.environ.setdefault("OMP_NUM_THREADS", str(threads))
+    info = cpu_info()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or info["affinity_cpus"]
+    os.environ["OMP_NUM_THREADS"] = str(threads)
     res = [int(r) for r in params.res]
     mid = res[2] // 2
     t0 = time.perf_counter()
-    cells, _ = oracle.bench_slices(node, params.bb_min, params.bb_max, res, mid, mid + 1,
-                                   params.eps)
+    oracle.bench_slices(node, params.bb_min, params.bb_max, res, mid, mid + 1, params.eps)
     one = time.perf_counter() - t0
-    n = max(1, min(res[2] - mid, int(seconds / max(one, 1e-6))))
-    t0 = time.perf_counter()
-    cells, active = oracle.bench_slices(node, params.bb_min, params.bb_max, res, mid, mid + n,
-                                        params.eps)
-    dt = time.perf_counter() - t0
-    return {"value": cells / dt / 1e6, "unit": "Mvoxels/s",
-            "cores": int(os.environ.get("OMP_NUM_THREADS", threads)), "kind": "port",
-            "sample": f"{n} z-slices ({cells} cells, {active} active) from z={mid} of the "
-                      f"{res[0]}^3 job, reference algorithm (8 corner evaluations + contour + "
-                      f"normal per cell), {dt:.1f} s"}
+    per_chunk = max(1, min((res[2] - mid) // 10, int(seconds / 10 / max(one, 1e-6))))
+    rates, total_cells, total_active, total_s = [], 0, 0, 0.0
+    for k in range(10):
+        z0 = mid + k * per_chunk
+        t0 = time.perf_counter()
+        cells, active = oracle.bench_slices(node, params.bb_min, params.bb_max, res, z0,
+                                            z0 + per_chunk, params.eps)
+        dt = time.perf_counter() - t0
+        rates.append(cells / dt / 1e6)
+        total_cells += cells
+        total_active += active
+        total_s += dt
+    return {"value": float(np.median(rates)), "unit": "Mvoxels/s", "cores": threads,
+            "kind": "port", **info,
+            "sample": f"10 chunks of {per_chunk} z-slices ({total_cells} cells, {total_active} "
+                      f"active) from z={mid} of the {res[0]}^3 job, reference algorithm (8 corner "
+                      f"evaluations + contour + normal per cell), {total_s:.1f} s in all; "
+                      f"median chunk rate, {threads} OpenMP threads"}
+
+
+def _median_ms(xs) -> float:
+    return float(np.median(np.asarray(xs))) * 1e3
+
+
+def single_job(name: str, args, ctx, headline: bool) -> dict:
+    """Host-landing and device-resident step times of one single-job workload, with the
+    evaluation kernel's roofline from events recorded during the host-landing steps."""
+    import sdf2mesh_amd as s2m
+
+    node, params, cfg = workload(name)
+    if args.dense:
+        params.flags |= s2m.DENSE_EVAL
+    voxels = int(params.res[0]) * int(params.res[1]) * int(params.res[2])
+    prog = ctx.program(node)
+    for _ in range(args.warmup):
+        ctx.mesh_to_host(prog, params).free()
+        ctx.mesh_device(prog, params)
+    ctx.set_profiling(1)  # events around the whole call and the evaluation launches only
+    host_s, raw = [], []
+    last = None
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        hm = ctx.mesh_to_host(prog, params)
+        host_s.append(time.perf_counter() - t0)
+        raw.append(ctx.last_timing_raw())  # converted after the timed steps
+        if last is not None:
+            last.free()
+        last = hm
+    # the last result stays readable: a checksum proves the data landed
+    vsum = float(np.asarray(last.vertices()[:, :3], np.float64).sum()) if last.nvtx else 0.0
+    tsum = int(np.asarray(last.triangles(), np.uint64).sum()) if last.ntri else 0
+    nvtx, ntri, invalid = last.nvtx, last.ntri, last.invalid_quads
+    last.free()
+    ctx.set_profiling(0)
+    dev_s = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        ctx.mesh_device(prog, params)
+        dev_s.append(time.perf_counter() - t0)
+    agg = {k: 0.0 for k in ("eval_ms", "total_ms", "d2h_ms", "eval_launches", "lattice_points",
+                            "slabs", "tiles_total", "tiles_active", "eval_flops")}
+    for r in raw:
+        t = s2m.timing_dict(r)
+        for k in agg:
+            agg[k] += t[k]
+    ms = _median_ms(host_s)
+    dev_ms = _median_ms(dev_s)
+    launches = max(1, agg["eval_launches"])
+    eval_avg_ms = agg["eval_ms"] / launches
+    flops_per_launch = agg["eval_flops"] / launches
+    achieved = flops_per_launch / (eval_avg_ms / 1e3) / 1e12 if eval_avg_ms > 0 else 0.0
+    tiled = agg["tiles_total"] > 0
+    kernel = "k_sp_tiles" if tiled else "k_eval_lattice"
+    d2h_bytes = nvtx * 24 + ntri * 12
+    d2h_ms = agg["d2h_ms"] / args.steps
+    cfg = {**cfg, "eval": "tile-pruned (exact)" if tiled else
+           ("dense" if args.dense else "dense (the program is not interval-prunable)"),
+           "nvtx": nvtx, "ntri": ntri, "invalid_quads": invalid,
+           "flops_per_point": node.flops(),
+           "lattice_points_per_step": int(agg["lattice_points"] // args.steps),
+           "tiles_active_per_step": int(agg["tiles_active"] // args.steps),
+           "tiles_per_step": int(agg["tiles_total"] // args.steps),
+           "slabs": int(agg["slabs"] // args.steps)}
+    out = {
+        "value": round(voxels / 1e6 / (ms / 1e3), 1),
+        "unit": "Mvoxels/s",
+        "ms_per_step": round(ms, 3),
+        "step_ms": {"median": round(ms, 3), "mean": round(float(np.mean(host_s)) * 1e3, 3),
+                    "min": round(float(np.min(host_s)) * 1e3, 3),
+                    "max": round(float(np.max(host_s)) * 1e3, 3)},
+        "config": cfg,
+        "mesh_in": "host memory (vertices pos+normal f32x6, triangles u32x3)",
+        "d2h": {"bytes_per_step": int(d2h_bytes), "ms_per_step": round(d2h_ms, 3),
+                "GB_per_s": round(d2h_bytes / (d2h_ms / 1e3) / 1e9, 1) if d2h_ms > 0 else None},
+        "device_resident": {"value": round(voxels / 1e6 / (dev_ms / 1e3), 1), "unit": "Mvoxels/s",
+                            "ms_per_step": round(dev_ms, 3),
+                            "note": "same job, mesh left in HBM (s2m_mesh_device)"},
+        "device_ms_per_step": round(agg["total_ms"] / args.steps, 3),
+        "checksum": {"vertex_xyz_sum": vsum, "index_sum": tsum},
+        "roofline": {
+            "bound": "valu",
+            "kernel": kernel,
+            "achieved": round(achieved, 2),
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "frac_of_issue_ceiling": round(achieved / ISSUE_CEILING_TOPS, 4),
+            "traffic": measured_traffic(name + ("-dense" if args.dense else ""), kernel),
+            "launch_ms": round(eval_avg_ms, 4),
+            "pruned_program_flops_per_launch" if tiled else "flops_per_launch": flops_per_launch,
+            "points_per_launch": agg["lattice_points"] / launches,
+        },
+    }
+    if headline:
+        # stage split (every stage bracketed by events), outside the timed steps
+        ctx.set_profiling(2)
+        stages = {k: 0.0 for k in ("prune_ms", "eval_ms", "contour_ms", "normal_ms", "quad_ms",
+                                   "total_ms")}
+        n_stage = 3
+        for _ in range(n_stage):
+            ctx.mesh_device(prog, params)
+            t = ctx.last_timing()
+            for k in stages:
+                stages[k] += t[k]
+        ctx.set_profiling(0)
+        out["stages_ms_per_step"] = {k: round(v / n_stage, 3) for k, v in stages.items()}
+        out["roofline"]["reference_equivalent_tflops"] = round(
+            node.flops() * (int(params.res[0]) + 1) ** 3 / (ms / 1e3) / 1e12, 1)
+    prog.close()
+    return out
 
 
 def run_batch(args) -> dict:
-    """Batching workload: 256 seeded random CSG jobs of mixed resolution.  A step meshes
-    the whole batch in shared device passes (s2m_mesh_batch_device); the comparison is
-    the same 256 jobs meshed one at a time on the plain GPU path (s2m_mesh_device)."""
+    """256 seeded random CSG jobs of mixed resolution per step.  Batched: s2m_mesh_batch
+    (shared device passes, every mesh copied to host memory); one at a time: s2m_mesh per
+    job.  Device-resident variants of both are reported beside them."""
     import sdf2mesh_amd as s2m
     from sdf2mesh_amd.workloads import batch_jobs
 
     n = 256
     nodes, params = batch_jobs(n)
-    voxels = sum(int(p.res[0]) * int(p.res[1]) * int(p.res[2]) for p in
-                 [s2m.Params(q.bb_min, q.bb_max, tuple(s2m.round_resolution(r) for r in q.res))
-                  for q in params])
+    voxels = sum(np.prod([s2m.round_resolution(int(r)) for r in q.res]) for q in params)
     ctx = s2m.Context(0)
     progs = [ctx.program(x) for x in nodes]
-    for _ in range(args.warmup):
+    steps = max(3, min(args.steps, 10))
+    for _ in range(max(1, args.warmup)):
+        ctx.mesh_batch(progs, params, want_keys=False)
         ctx.mesh_batch_device(progs, params)
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        out = ctx.mesh_batch_device(progs, params)
-    dt_b = (time.perf_counter() - t0) / args.steps
+    bt = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        out = ctx.mesh_batch(progs, params, want_keys=False)
+        bt.append(time.perf_counter() - t0)
+    bdev = []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        ctx.mesh_batch_device(progs, params)
+        bdev.append(time.perf_counter() - t0)
     for q, p in zip(progs, params):  # warm the single-job path
-        ctx.mesh_device(q, p)
-    solo_lat = []
-    t0 = time.perf_counter()
+        ctx.mesh_to_host(q, p).free()
+    solo, solo_dev = [], []
     for q, p in zip(progs, params):
-        t1 = time.perf_counter()
+        t0 = time.perf_counter()
+        hm = ctx.mesh_to_host(q, p)
+        solo.append(time.perf_counter() - t0)
+        hm.free()
+    for q, p in zip(progs, params):
+        t0 = time.perf_counter()
         ctx.mesh_device(q, p)
-        solo_lat.append(time.perf_counter() - t1)
-    dt_s = time.perf_counter() - t0
+        solo_dev.append(time.perf_counter() - t0)
     ctx.close()
-    jobs_b, jobs_s = n / dt_b, n / dt_s
-    lat = np.sort(np.asarray(solo_lat)) * 1e3
+    dt_b = float(np.median(bt))
+    jobs_b, jobs_s = n / dt_b, n / float(np.sum(solo))
+    lat = np.sort(np.asarray(solo)) * 1e3
     return {
-        "metric": "batched jobs/s (256 random CSG jobs, res 32-256)",
-        "value": round(jobs_b, 1),
-        "unit": "jobs/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(dt_b * 1e3, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded CSG programs)",
-        "config": {"workload": "batch256", "jobs": n, "voxels_per_batch": voxels,
-                   "vertices": int(sum(o["nvtx"] for o in out)),
-                   "triangles": int(sum(o["ntri"] for o in out))},
+        "value": round(jobs_b, 1), "unit": "jobs/s", "ms_per_step": round(dt_b * 1e3, 3),
+        "steps": steps,
+        "config": {"workload": "batch256", "jobs": n, "voxels_per_batch": int(voxels),
+                   "vertices": int(sum(len(m.vertices) for m in out)),
+                   "triangles": int(sum(len(m.triangles) for m in out)),
+                   "mesh_in": "host memory"},
         "one_at_a_time_jobs_per_s": round(jobs_s, 1),
         "speedup_vs_one_at_a_time": round(jobs_b / jobs_s, 2),
-        "one_at_a_time_latency_ms": {"p50": round(float(lat[len(lat) // 2]), 3),
-                                     "p99": round(float(lat[int(len(lat) * 0.99)]), 3)},
+        "one_at_a_time_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                                     "p99": round(float(np.percentile(lat, 99)), 3)},
         "batched_mvoxels_per_s": round(voxels / 1e6 / dt_b, 1),
+        "device_resident": {"jobs_per_s": round(n / float(np.median(bdev)), 1),
+                            "one_at_a_time_jobs_per_s": round(n / float(np.sum(solo_dev)), 1)},
     }
 
 
 def run_serve(args) -> dict:
-    """Serving workload: Poisson request streams (rates 10-2000 jobs/s) of the seeded
-    request mix (resolution log-uniform in [32, 512], 80 % random CSG, 20 % Mandelbulb
-    with 4-8 iterations) through the continuous-batching server.  The comparison is the
-    plain GPU path serving the same stream one request at a time (FIFO): its latencies
-    are replayed from each request's measured service time on one context."""
+    """Poisson request streams (10-2000 requests/s) of the seeded request mix (resolution
+    log-uniform in [32, 512], 80 % random CSG, 20 % Mandelbulb with 4-8 iterations)
+    through the continuous-batching server; meshes land in host memory.  The comparison
+    is the plain GPU path serving the same stream one request at a time (FIFO): each
+    request's measured s2m_mesh time, replayed on the same arrivals."""
     import sdf2mesh_amd as s2m
     from sdf2mesh_amd.workloads import poisson_arrivals, serve_jobs
 
-    n = args.serve_requests
-    nodes, params = serve_jobs(n)
+    n_max = args.serve_requests
+    nodes, params = serve_jobs(n_max)
     tapes = [x.tape() for x in nodes]
-    # one at a time: measured service time of every request (mesh to host memory)
     ctx = s2m.Context(0)
     progs = [ctx.program(t) for t in tapes]
     for q, p in zip(progs[:8], params[:8]):
-        ctx.mesh(q, p)
+        ctx.mesh_to_host(q, p).free()
     service = []
     for q, p in zip(progs, params):
         t0 = time.perf_counter()
-        ctx.mesh(q, p)
+        hm = ctx.mesh_to_host(q, p)
         service.append(time.perf_counter() - t0)
+        hm.free()
     for q in progs:
         q.close()
     ctx.close()
     service = np.asarray(service)
-    solo_jobs_s = n / service.sum()
 
     rates = [float(r) for r in args.serve_rates.split(",")]
     per_rate = []
-    with s2m.Server(0, workers=args.serve_workers, max_batch=args.serve_batch) as srv:
+    with s2m.Server(0, workers=args.serve_workers, max_batch=args.serve_batch,
+                    want_keys=False) as srv:
         for t, p in zip(tapes[:8], params[:8]):  # warm every worker's buffers
             srv.take(srv.submit(t, p))
         for rate in rates:
+            n = min(n_max, max(60, int(rate * 4)))  # ~4 s of arrivals at low rates
             arr = np.asarray(poisson_arrivals(n, rate, seed=int(rate)))
             tickets = []
             t0 = time.perf_counter()
@@ -185,14 +345,13 @@ def run_serve(args) -> dict:
             elapsed = time.perf_counter() - t0
             lat = np.asarray([st["latency_ms"] for _, st in res])
             bs = np.asarray([st["batch_size"] for _, st in res])
-            # FIFO replay of the one-at-a-time server on the same arrivals
             done, solo_lat = 0.0, []
-            for a, sv in zip(arr, service):
+            for a, sv in zip(arr, service[:n]):
                 done = max(done, a) + sv
                 solo_lat.append((done - a) * 1e3)
             solo_lat = np.asarray(solo_lat)
             per_rate.append({
-                "rate": rate, "jobs_per_s": round(n / elapsed, 1),
+                "rate": rate, "requests": n, "jobs_per_s": round(n / elapsed, 1),
                 "p50_ms": round(float(np.percentile(lat, 50)), 2),
                 "p99_ms": round(float(np.percentile(lat, 99)), 2),
                 "mean_batch": round(float(bs.mean()), 2),
@@ -201,117 +360,18 @@ def run_serve(args) -> dict:
                 "one_at_a_time_p99_ms": round(float(np.percentile(solo_lat, 99)), 2),
             })
     top = per_rate[-1]
+    solo_jobs_s = top["requests"] / float(service[: top["requests"]].sum())
     return {
-        "metric": "served jobs/s at the highest offered rate, with p50/p99 latency",
-        "value": top["jobs_per_s"],
-        "unit": "jobs/s",
-        "n_gpus": 1,
-        "steps": n,
-        "warmup": 8,
+        "value": top["jobs_per_s"], "unit": "jobs/s", "rate": top["rate"],
         "ms_per_step": round(1e3 / top["jobs_per_s"], 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded request stream)",
-        "config": {"workload": "serve", "requests_per_rate": n, "workers": args.serve_workers,
-                   "max_batch": args.serve_batch},
-        "p50_ms": top["p50_ms"],
-        "p99_ms": top["p99_ms"],
+        "config": {"workload": "serve", "requests_per_rate": "min(%d, max(60, 4 x rate))" % n_max,
+                   "workers": args.serve_workers, "max_batch": args.serve_batch,
+                   "mesh_in": "host memory"},
+        "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
         "one_at_a_time_jobs_per_s": round(solo_jobs_s, 1),
         "speedup_vs_one_at_a_time": round(top["jobs_per_s"] / solo_jobs_s, 2),
         "per_rate": per_rate,
     }
-
-
-def run(args) -> dict:
-    import sdf2mesh_amd as s2m
-
-    node, params, cfg = workload(args.workload)
-    if args.dense:
-        params.flags |= s2m.DENSE_EVAL
-    voxels = int(params.res[0]) * int(params.res[1]) * int(params.res[2])
-    ctx = s2m.Context(0)
-    prog = ctx.program(node)
-    for _ in range(args.warmup):
-        ctx.mesh_device(prog, params)
-    ctx.set_profiling(1)  # events around the whole call and the evaluation launches only
-    agg = {"prune_ms": 0.0, "eval_ms": 0.0, "contour_ms": 0.0, "normal_ms": 0.0, "quad_ms": 0.0,
-           "total_ms": 0.0, "eval_launches": 0, "lattice_points": 0, "slabs": 0,
-           "tiles_total": 0, "tiles_active": 0, "eval_flops": 0}
-    raw = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        dm = ctx.mesh_device(prog, params)
-        raw.append(ctx.last_timing_raw())  # converted after the timed region
-    dt = time.perf_counter() - t0
-    for r in raw:
-        t = s2m.timing_dict(r)
-        for k in agg:
-            agg[k] += t[k]
-    ms = dt / args.steps * 1e3
-    value = voxels / 1e6 / (ms / 1e3)
-    # stage split (every stage bracketed by events), outside the timed region
-    ctx.set_profiling(2)
-    stages = {k: 0.0 for k in ("prune_ms", "eval_ms", "contour_ms", "normal_ms", "quad_ms", "total_ms")}
-    n_stage = 3
-    for _ in range(n_stage):
-        ctx.mesh_device(prog, params)
-        t = ctx.last_timing()
-        for k in stages:
-            stages[k] += t[k]
-
-    flops = node.flops()
-    launches = max(1, agg["eval_launches"])
-    eval_avg_ms = agg["eval_ms"] / launches
-    pts_per_launch = agg["lattice_points"] / launches
-    flops_per_launch = agg["eval_flops"] / launches   # executed algorithmic flops
-    achieved = flops_per_launch / (eval_avg_ms / 1e3) / 1e12
-    tiled = agg["tiles_total"] > 0
-    cfg["eval"] = ("tile-pruned (exact)" if tiled else
-                   "dense" if args.dense else "dense (the program is not interval-prunable)")
-    out = {
-        "metric": "Mvoxels/s meshed at 512^3 fp32 (CSG-64, reference-exact corners)"
-        if args.workload == "csg64-512" else f"Mvoxels/s meshed ({args.workload})",
-        "value": round(value, 1),
-        "unit": "Mvoxels/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms, 3),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded SDF program)",
-        "config": {**cfg, "nvtx": dm["nvtx"], "ntri": dm["ntri"],
-                   "invalid_quads": dm["invalid_quads"], "flops_per_point": flops,
-                   "lattice_points_per_step": agg["lattice_points"] // args.steps,
-                   "tiles_active_per_step": agg["tiles_active"] // args.steps,
-                   "tiles_per_step": agg["tiles_total"] // args.steps,
-                   "slabs": agg["slabs"] // args.steps},
-        "roofline": {
-            "bound": "valu",
-            "kernel": "k_sp_tiles" if tiled else "k_eval_lattice",
-            "achieved": round(achieved, 2),
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": measured_traffic(args.workload + ("-dense" if args.dense else ""),
-                                        "k_sp_tiles" if tiled else "k_eval_lattice"),
-            "launch_ms": round(eval_avg_ms, 4),
-            "flops_per_launch": flops_per_launch,
-            "points_per_launch": pts_per_launch,
-        },
-        "device_ms_per_step": round(agg["total_ms"] / args.steps, 3),
-        "stages_ms_per_step": {k: round(v / n_stage, 3) for k, v in stages.items()},
-    }
-    ctx.close()
-    if not args.no_cpu_baseline:
-        cb = cpu_baseline(node, params, args.cpu_seconds)
-        out["cpu_baseline"] = cb
-        out["speedup_vs_cpu_baseline"] = round(value / cb["value"], 1)
-    return out
 
 
 def run_copy_calibration(args) -> dict:
@@ -342,13 +402,20 @@ def measured_traffic(workload: str, kernel: str):
     return entry.get("bytes_per_launch")
 
 
+def common(args, metric: str, unit: str, value, ms: float, data: str) -> dict:
+    return {"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": data}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="csg64-512")
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--only", action="store_true", help="run --workload alone")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rates", default="10,30,100,300,1000,2000")
@@ -358,17 +425,50 @@ def main() -> int:
                     help="evaluate every lattice point with the full program (no tile pruning)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; jobs shard by job or by z-slab "
-                                   "across GPUs only by design, --gpus must be 1"}))
+        print(json.dumps({"error": "this build runs on one GPU (a box holds one); jobs would "
+                                   "shard by z-slab or by job across GPUs by design only, so "
+                                   "--gpus must be 1"}))
         return 2
     if args.workload == "copy-calibration":
-        out = run_copy_calibration(args)
-    elif args.workload == "batch256":
-        out = run_batch(args)
-    elif args.workload == "serve":
-        out = run_serve(args)
+        print(json.dumps(run_copy_calibration(args)), flush=True)
+        return 0
+    import sdf2mesh_amd as s2m
+
+    if args.only and args.workload == "batch256":
+        r = run_batch(args)
+        out = {**common(args, "batched jobs/s (256 random CSG jobs, res 32-256)", "jobs/s",
+                        r["value"], r["ms_per_step"], "synthetic (seeded CSG programs)"), **r}
+    elif args.only and args.workload == "serve":
+        r = run_serve(args)
+        out = {**common(args, "served jobs/s at the highest offered rate, with p50/p99 latency",
+                        "jobs/s", r["value"], r["ms_per_step"], "synthetic (seeded request stream)"),
+               **r}
+    elif args.workload in SINGLE:
+        ctx = s2m.Context(0)
+        head = single_job(args.workload, args, ctx, headline=True)
+        metric = ("Mvoxels/s meshed at 512^3 fp32 (CSG-64, reference-exact corners), mesh in host "
+                  "memory" if args.workload == "csg64-512" else
+                  f"Mvoxels/s meshed ({args.workload}), mesh in host memory")
+        out = {**common(args, metric, "Mvoxels/s", head.pop("value"), head.pop("ms_per_step"),
+                        "synthetic (seeded SDF program)"), **head}
+        if not args.only:
+            others = {}
+            for w in ("torus-128", "mandelbulb-1024"):
+                if w != args.workload:
+                    others[w] = single_job(w, args, ctx, headline=False)
+            ctx.close()
+            others["batch256"] = run_batch(args)
+            others["serve"] = run_serve(args)
+            out["workloads"] = others
+        else:
+            ctx.close()
+        if not args.no_cpu_baseline:
+            node, params, _ = workload(args.workload)
+            cb = cpu_baseline(node, params, args.cpu_seconds)
+            out["cpu_baseline"] = cb
+            out["speedup_vs_cpu_baseline"] = round(out["value"] / cb["value"], 1)
     else:
-        out = run(args)
+        raise SystemExit(f"unknown workload {args.workload}")
     print(json.dumps(out), flush=True)
     return 0
 

@@ -344,6 +344,14 @@ uint32_t s2m_axis_table(float bmin, float bmax, uint32_t res, int exact_corners,
  * `out` (capacity cap floats) and report the lattice extent in dims[3]. */
 int s2m_eval_lattice(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params,
                      float* out, uint64_t cap, uint32_t dims[3]);
+/* Evaluate the program on the lattice planes that hold the corners of cell planes
+ * [z0, z1) (after the resolution rule of params): lattice planes lz0 .. lz0+dims[2]-1 of
+ * the corner lattice, written to `out` x fastest.  dims receives the window's extent and
+ * *lz0 its first plane.  Lets a test contour one z-window of a grid too large to copy
+ * whole.  No reference counterpart (the reference never exposes corner values). */
+int s2m_eval_lattice_range(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params,
+                           uint32_t z0, uint32_t z1, float* out, uint64_t cap, uint32_t dims[3],
+                           uint32_t* lz0);
 /* Contour given per-cell corner values (8 per cell in the order 000,100,010,110,001,
  * 101,011,111; cells in (z,y,x) order) with cell bounds from per-axis tables
  * (cell_lo/cell_hi hold res_x, then res_y, then res_z floats).  No SDF is involved,

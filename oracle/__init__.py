@@ -60,6 +60,7 @@ def lib() -> C.CDLL:
             "ora_tape_check": (C.c_int, [vp, u32]),
             "ora_eval_f32": (None, [vp, u32, vp, u64, vp]),
             "ora_eval_f64": (None, [vp, u32, vp, u64, vp]),
+            "ora_mandel_cond": (None, [vp, u32, vp, u64, vp]),
             "ora_cell_bounds": (None, [f32, f32, u32, u32, C.c_int, C.POINTER(f32), C.POINTER(f32)]),
             "ora_cell_signs": (u32, [vp]),
             "ora_cell_vertex": (C.c_int, [vp, vp, vp, vp]),
@@ -126,6 +127,18 @@ def eval_f64(t, pts: np.ndarray) -> np.ndarray:
     pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
     out = np.empty(len(pts), np.float64)
     lib().ora_eval_f64(_p(t), len(t), _p(pts), len(pts), _p(out))
+    return out
+
+
+def mandel_cond(t, pts: np.ndarray) -> np.ndarray:
+    """(N, 3) conditioning of the tape's Mandelbulb at each point (fp64 trajectory, see
+    ora_mandel_cond): [:, 0] the smallest relative distance |r - bailout| / bailout of any
+    bailout test, [:, 1] the smallest rho / r of any escape step, [:, 2] the first-order
+    error scale (summed effect of one 2^-24 relative rounding of each intermediate)."""
+    t = _tape(t)
+    pts = np.ascontiguousarray(pts, np.float32).reshape(-1, 3)
+    out = np.empty((len(pts), 3), np.float64)
+    lib().ora_mandel_cond(_p(t), len(t), _p(pts), len(pts), _p(out))
     return out
 
 

@@ -146,6 +146,80 @@ ORA_API void ora_eval_f64(const s2m_op* ops, uint32_t n, const float* pts, uint6
                                (double)pts[3 * i + 2]);
 }
 
+/* The fp64 Mandelbulb of mandelbulb_f64 (examples/mandelmesh.frag:3-28) with one
+ * intermediate value multiplied by (1 + u): site 8k + j perturbs, in escape step k,
+ * j = 0..2 the components of z entering the step, 3 its r = length(z), 4 theta * power,
+ * 5 phi * power, 6 the new dr, 7 the new r = pow(r, power); sites 8 * iters + 0..2 the
+ * final 0.5 * log(r), its product with r and the quotient by dr.  site < 0 perturbs
+ * nothing and records the margins: *bail the smallest |r - bailout| / bailout of any
+ * bailout test, *axis the smallest rho / r (rho = |z.xy|) of any escape step. */
+static double mandel_site(const s2m_op* m, double px, double py, double pz, int site, double u,
+                          double* bail, double* axis)
+{
+    const double power = m->f[0], bailout = m->f[1], offset = m->f[2];
+    const uint32_t iters = m->iarg & 0xffffu;
+    if (m->iarg & S2M_MANDEL_SWAP_YZ) {
+        const double t = py;
+        py = pz;
+        pz = t;
+    }
+#define PERTURB(v, s) ((site == (int)(s)) ? (v) * (1.0 + u) : (v))
+    double zx = px, zy = py, zz = pz, r = 0.0, dr = 1.0;
+    for (uint32_t k = 0; k < iters; ++k) {
+        zx = PERTURB(zx, 8 * k + 0);
+        zy = PERTURB(zy, 8 * k + 1);
+        zz = PERTURB(zz, 8 * k + 2);
+        r = PERTURB(sqrt((zx * zx + zy * zy) + zz * zz), 8 * k + 3);
+        if (site < 0) *bail = fmin(*bail, fabs(r - bailout) / bailout);
+        if (r > bailout) continue;
+        if (site < 0) *axis = fmin(*axis, r > 0.0 ? sqrt(zx * zx + zy * zy) / r : 0.0);
+        const double theta = PERTURB(atan(zy / zx) * power, 8 * k + 4);
+        const double phi = PERTURB(asin(zz / r) * power, 8 * k + 5);
+        dr = PERTURB((pow(r, power - 1.0) * dr) * power + 1.0, 8 * k + 6);
+        r = PERTURB(pow(r, power), 8 * k + 7);
+        zx = r * (cos(theta) * cos(phi)) + px;
+        zy = r * (sin(theta) * cos(phi)) + py;
+        zz = r * sin(phi) + pz;
+    }
+    const double l = PERTURB(0.5 * log(r), 8 * iters + 0);
+    const double lr = PERTURB(l * r, 8 * iters + 1);
+    return PERTURB(lr / dr, 8 * iters + 2) - offset;
+#undef PERTURB
+}
+
+/* Conditioning of the first Mandelbulb op of the tape at each point, three values:
+ *   out[3i]     min over the executed bailout tests of |r - bailout| / bailout: a rounding
+ *               of r by about this relative amount takes the other branch of
+ *               `if (r > bailout) continue;` and changes the result discontinuously;
+ *   out[3i + 1] min over the escape steps of rho / r: the derivatives of atan(zy / zx)
+ *               and asin(zz / r) grow as r / rho (0 at r = 0);
+ *   out[3i + 2] first-order error scale of an fp32 evaluation: the sum over the sites of
+ *               mandel_site of |f(site perturbed by u = 2^-24) - f|, i.e. how much one
+ *               relative rounding of each intermediate moves the fp64 result.
+ * A tape without a Mandelbulb op gives +inf, +inf, 0. */
+ORA_API void ora_mandel_cond(const s2m_op* ops, uint32_t n, const float* pts, uint64_t npts,
+                             double* out)
+{
+    const s2m_op* m = NULL;
+    for (uint32_t k = 0; k < n && !m; ++k)
+        if (ops[k].code == S2M_OP_MANDELBULB) m = &ops[k];
+    const double u = ldexp(1.0, -24);
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < (int64_t)npts; ++i) {
+        double bail = INFINITY, axis = INFINITY, scale = 0.0;
+        if (m) {
+            const double px = pts[3 * i], py = pts[3 * i + 1], pz = pts[3 * i + 2];
+            const double f0 = mandel_site(m, px, py, pz, -1, u, &bail, &axis);
+            const int nsite = 8 * (int)(m->iarg & 0xffffu) + 3;
+            for (int s = 0; s < nsite; ++s)
+                scale += fabs(mandel_site(m, px, py, pz, s, u, &bail, &axis) - f0);
+        }
+        out[3 * i] = bail;
+        out[3 * i + 1] = axis;
+        out[3 * i + 2] = scale;
+    }
+}
+
 /* ---- cell geometry -------------------------------------------------------------- */
 /* dualcontour.wgsl:22-27 for one axis: size = (max-min)/f32(res-1),
  * cmin = min + size*f32(i), cmax = cmin + size.  With lattice != 0 the max corner is

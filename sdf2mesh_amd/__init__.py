@@ -149,6 +149,9 @@ _SIGNATURES = {
                                     C.c_void_p, C.c_void_p]),
     "s2m_eval_lattice": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_Params), C.c_void_p,
                                    C.c_uint64, C.POINTER(C.c_uint32)]),
+    "s2m_eval_lattice_range": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_Params), C.c_uint32,
+                                         C.c_uint32, C.c_void_p, C.c_uint64, C.POINTER(C.c_uint32),
+                                         C.POINTER(C.c_uint32)]),
     "s2m_contour_cells": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint32),
                                     C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(_MeshOut)]),
 }
@@ -266,6 +269,40 @@ def _take_mesh(out: _MeshOut, want_keys: bool) -> TriangleMesh:
         return TriangleMesh(v, t, int(out.invalid_quads), keys, signs)
     finally:
         lib().s2m_mesh_free(C.byref(out))
+
+
+class HostMesh:
+    """A mesh in library-owned host memory (s2m_mesh_out); views, no copies."""
+
+    def __init__(self, out: _MeshOut):
+        self._out = out
+        self.nvtx, self.ntri = int(out.nvtx), int(out.ntri)
+        self.invalid_quads = int(out.invalid_quads)
+
+    def vertices(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self._out.vtx, (max(self.nvtx, 1) * 6,))[: self.nvtx * 6] \
+            .reshape(self.nvtx, 6)
+
+    def triangles(self) -> np.ndarray:
+        return np.ctypeslib.as_array(self._out.tri, (max(self.ntri, 1) * 3,))[: self.ntri * 3] \
+            .reshape(self.ntri, 3)
+
+    def free(self) -> None:
+        if self._out is not None:
+            lib().s2m_mesh_free(C.byref(self._out))
+            self._out = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.free()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def format_f32(x: float) -> str:
@@ -427,6 +464,18 @@ class Context:
                               int(want_keys), C.byref(out)))
         return _take_mesh(out, want_keys)
 
+    def mesh_to_host(self, prog: Program, params: Params, want_keys: bool = False) -> "HostMesh":
+        """s2m_mesh without the copy into numpy arrays: the result stays in the library's
+        host memory (page-locked, recycled) until HostMesh.free()."""
+        p = params.c()
+        out = _MeshOut()
+        rc = lib().s2m_mesh(C.c_void_p(self.handle), C.c_void_p(prog.handle), C.byref(p),
+                            int(want_keys), C.byref(out))
+        if rc != OK:
+            lib().s2m_mesh_free(C.byref(out))
+            _check(rc)
+        return HostMesh(out)
+
     def mesh_batch(self, progs, params, want_keys: bool = True) -> list:
         """Mesh many jobs in shared device passes; one TriangleMesh per job."""
         n = len(progs)
@@ -480,6 +529,21 @@ class Context:
         _check(lib().s2m_eval_lattice(C.c_void_p(self.handle), C.c_void_p(prog.handle), C.byref(p),
                                       _ptr(out), n, dims))
         return out.reshape(int(dims[2]), int(dims[1]), int(dims[0]))
+
+    def eval_lattice_range(self, prog: Program, params: Params, z0: int, z1: int):
+        """(lattice, lz0): SDF values on the lattice planes holding the corners of cell
+        planes [z0, z1), shape (planes, ny, nx); lz0 is the first plane's index in the
+        whole corner lattice."""
+        p = params.c()
+        dims = (C.c_uint32 * 3)()
+        lz0 = C.c_uint32()
+        lib().s2m_eval_lattice_range(C.c_void_p(self.handle), C.c_void_p(prog.handle), C.byref(p),
+                                     z0, z1, None, 0, dims, C.byref(lz0))
+        n = int(dims[0]) * int(dims[1]) * int(dims[2])
+        out = np.empty(max(n, 1), np.float32)
+        _check(lib().s2m_eval_lattice_range(C.c_void_p(self.handle), C.c_void_p(prog.handle),
+                                            C.byref(p), z0, z1, _ptr(out), n, dims, C.byref(lz0)))
+        return out[:n].reshape(int(dims[2]), int(dims[1]), int(dims[0])), int(lz0.value)
 
     def contour_cells(self, corners: np.ndarray, cell_lo, cell_hi, flags: int = 0) -> TriangleMesh:
         """Contour per-cell corner values, corners shape (rz, ry, rx, 8); cell_lo/cell_hi

@@ -1044,11 +1044,11 @@ int copy_out(s2m_ctx* ctx, int want_keys, s2m_mesh_out* out, double* d2h_ms)
     out->nvtx = nv;
     out->ntri = nt;
     out->invalid_quads = ctx->last_invalid;
-    out->vtx = (float*)std::malloc(std::max<uint64_t>(nv, 1) * 6 * sizeof(float));
-    out->tri = (uint32_t*)std::malloc(std::max<uint64_t>(nt, 1) * 3 * sizeof(uint32_t));
+    out->vtx = (float*)host_alloc(std::max<uint64_t>(nv, 1) * 6 * sizeof(float));
+    out->tri = (uint32_t*)host_alloc(std::max<uint64_t>(nt, 1) * 3 * sizeof(uint32_t));
     if (want_keys) {
-        out->cell_keys = (uint64_t*)std::malloc(std::max<uint64_t>(nv, 1) * sizeof(uint64_t));
-        out->signs = (uint8_t*)std::malloc(std::max<uint64_t>(nv, 1));
+        out->cell_keys = (uint64_t*)host_alloc(std::max<uint64_t>(nv, 1) * sizeof(uint64_t));
+        out->signs = (uint8_t*)host_alloc(std::max<uint64_t>(nv, 1));
     }
     if (!out->vtx || !out->tri || (want_keys && (!out->cell_keys || !out->signs))) {
         s2m_mesh_free(out);
@@ -1294,11 +1294,11 @@ int copy_ranges(s2m_ctx* ctx, const SpRange* r, uint32_t n, int want_keys, s2m_m
         o->nvtx = r[i].nv;
         o->ntri = r[i].nt;
         o->invalid_quads = r[i].invalid;
-        o->vtx = (float*)std::malloc(std::max<uint64_t>(r[i].nv, 1) * 6 * sizeof(float));
-        o->tri = (uint32_t*)std::malloc(std::max<uint64_t>(r[i].nt, 1) * 3 * sizeof(uint32_t));
+        o->vtx = (float*)host_alloc(std::max<uint64_t>(r[i].nv, 1) * 6 * sizeof(float));
+        o->tri = (uint32_t*)host_alloc(std::max<uint64_t>(r[i].nt, 1) * 3 * sizeof(uint32_t));
         if (want_keys) {
-            o->cell_keys = (uint64_t*)std::malloc(std::max<uint64_t>(r[i].nv, 1) * sizeof(uint64_t));
-            o->signs = (uint8_t*)std::malloc(std::max<uint64_t>(r[i].nv, 1));
+            o->cell_keys = (uint64_t*)host_alloc(std::max<uint64_t>(r[i].nv, 1) * sizeof(uint64_t));
+            o->signs = (uint8_t*)host_alloc(std::max<uint64_t>(r[i].nv, 1));
         }
         if (!o->vtx || !o->tri || (want_keys && (!o->cell_keys || !o->signs)))
             return fail(S2M_E_OOM, "host allocation failed");
@@ -1462,10 +1462,10 @@ int s2m_mesh_batch_device(s2m_ctx* ctx, uint32_t n, const s2m_program* const* pr
 void s2m_mesh_free(s2m_mesh_out* out)
 {
     if (!out) return;
-    std::free(out->vtx);
-    std::free(out->tri);
-    std::free(out->cell_keys);
-    std::free(out->signs);
+    host_release(out->vtx);
+    host_release(out->tri);
+    host_release(out->cell_keys);
+    host_release(out->signs);
     std::memset(out, 0, sizeof *out);
 }
 
@@ -1492,17 +1492,25 @@ int s2m_check_sincos(s2m_ctx* ctx, uint64_t counts[4])
     return S2M_OK;
 }
 
-int s2m_eval_lattice(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, float* out,
-                     uint64_t cap, uint32_t dims[3])
+int s2m_eval_lattice_range(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, uint32_t z0,
+                           uint32_t z1, float* out, uint64_t cap, uint32_t dims[3], uint32_t* lz0)
 {
-    if (!ctx || !prog || !params || !dims) return fail(S2M_E_ARG, "NULL argument");
+    if (!ctx || !prog || !params || !dims || !lz0) return fail(S2M_E_ARG, "NULL argument");
     HIP_TRY(hipSetDevice(ctx->device));
     Job j;
     int rc = prepare_job(params, j);
     if (rc) return rc;
-    for (int a = 0; a < 3; ++a) dims[a] = (uint32_t)j.ax[a].coord.size();
+    dims[0] = (uint32_t)j.ax[0].coord.size();
+    dims[1] = (uint32_t)j.ax[1].coord.size();
+    dims[2] = 0;
+    *lz0 = 0;
+    if (z1 > j.res[2] || z0 > z1)
+        return fail(S2M_E_ARG, "cell planes [%u, %u) outside the grid of %u planes", z0, z1, j.res[2]);
+    if (z0 == z1 || dims[0] == 0 || dims[1] == 0) return S2M_OK;
+    // the lattice planes holding the corners of cell planes z0 .. z1-1
+    *lz0 = j.ax[2].lo[z0];
+    dims[2] = j.ax[2].hi[z1 - 1] - *lz0 + 1;
     const uint64_t n = (uint64_t)dims[0] * dims[1] * dims[2];
-    if (n == 0) return S2M_OK;
     if (!out || cap < n) return fail(S2M_E_ARG, "output holds %llu floats, lattice has %llu",
                                      (unsigned long long)cap, (unsigned long long)n);
     const float* dc[3];
@@ -1511,12 +1519,26 @@ int s2m_eval_lattice(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* pa
     rc = upload_tables(ctx, j, dc, dlo, dhi);
     if (rc) return rc;
     HIP_TRY(ctx->lattice.ensure((size_t)n * sizeof(float)));
-    HIP_TRY(eval_lattice_prog(ctx->stream, prog, dc[0], dc[1], dc[2], dims[0], dims[1], 0, dims[2], dims[0],
+    HIP_TRY(eval_lattice_prog(ctx->stream, prog, dc[0], dc[1], dc[2], dims[0], dims[1], *lz0, dims[2], dims[0],
                               ctx->lattice.as<float>()));
     HIP_TRY(hipMemcpyAsync(out, ctx->lattice.p, (size_t)n * sizeof(float), hipMemcpyDeviceToHost,
                            ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return S2M_OK;
+}
+
+int s2m_eval_lattice(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, float* out,
+                     uint64_t cap, uint32_t dims[3])
+{
+    if (!ctx || !prog || !params || !dims) return fail(S2M_E_ARG, "NULL argument");
+    Job j;
+    int rc = prepare_job(params, j);
+    if (rc) return rc;
+    // every cell plane: lattice planes 0 .. hi[res-1], the whole table
+    uint32_t lz0 = 0;
+    rc = s2m_eval_lattice_range(ctx, prog, params, 0, j.res[2], out, cap, dims, &lz0);
+    if (rc == S2M_OK && j.res[2] == 0) dims[2] = (uint32_t)j.ax[2].coord.size();
+    return rc;
 }
 
 int s2m_contour_cells(s2m_ctx* ctx, const float* corners, const uint32_t res[3],

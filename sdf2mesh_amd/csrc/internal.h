@@ -215,6 +215,12 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
 hipError_t launch_cells_from_corners(hipStream_t s, const SlabArgs& a, const float* corners,
                                      const MeshBufs& b, int emit);
 
+// ---- host memory for results (hostmem.cpp) ----
+// Host memory for a result array: page-locked and recycled when large, malloc otherwise.
+// Returned to the pool (or freed) by host_release; any other pointer is passed to free().
+void* host_alloc(size_t bytes);
+void host_release(void* p);
+
 // ---- host helpers (api.cpp) for the server ----
 // A program from a gfx950 code object holding s2m_c_eval_lattice and s2m_c_normals.
 int program_from_code_object(s2m_ctx* ctx, const void* code, size_t size, s2m_program** out);

@@ -654,14 +654,21 @@ __global__ __launch_bounds__(256) void k_quads(SlabArgs a, MeshBufs b)
 // cnt[2]: sine / cosine results that differ, cnt[1] / cnt[3]: those of them where both
 // are NaN.  (The Mandelbulb's escape step takes sine and cosine with one sincosf.)
 // ---------------------------------------------------------------------------------
+// Each side in a function of its own that is never inlined: a sinf and a cosf of the
+// same argument in one function may be merged into one sincos call by the compiler,
+// and the check would then compare sincosf with itself.
+__device__ __attribute__((noinline)) float check_sin(float x) { return sinf(x); }
+__device__ __attribute__((noinline)) float check_cos(float x) { return cosf(x); }
+__device__ __attribute__((noinline)) void check_sincos(float x, float* s, float* c) { sincosf(x, s, c); }
+
 __global__ __launch_bounds__(256) void k_sincos_check(unsigned long long* __restrict__ cnt)
 {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < (1ull << 32);
          i += (uint64_t)gridDim.x * blockDim.x) {
         const float x = __uint_as_float((uint32_t)i);
-        const float s1 = sinf(x), c1 = cosf(x);
+        const float s1 = check_sin(x), c1 = check_cos(x);
         float s2, c2;
-        sincosf(x, &s2, &c2);
+        check_sincos(x, &s2, &c2);
         if (__float_as_uint(s1) != __float_as_uint(s2)) atomicAdd(&cnt[(s1 != s1 && s2 != s2) ? 1 : 0], 1ull);
         if (__float_as_uint(c1) != __float_as_uint(c2)) atomicAdd(&cnt[(c1 != c1 && c2 != c2) ? 3 : 2], 1ull);
     }

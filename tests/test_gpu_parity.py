@@ -199,43 +199,46 @@ def test_mandelbulb_contour_parity(ctx, iters, res, flags):
     assert_same_mesh(m, r, normal_atol=None)
 
 
-def test_mandelbulb_lattice_vs_fp64(ctx):
-    # GPU fp32 (ocml transcendentals) against the fp64 oracle.  The estimator iterates a
-    # chaotic map, so besides an absolute cap the bound is stated relative to the input
-    # conditioning s (largest change of the fp64 value when one coordinate moves by one
-    # fp32 ulp): every point within 1e-5; 99.9 % of points within 64 ulp(f64) + 64 s; and
-    # the GPU's max and 99th-percentile errors at most 4x those of the fp32 host
-    # restatement (glibc) on the same points.
-    node = tape.mandelbulb(8)
-    res = 32
-    p = s2m.Params.reference(res, 2.5, flags=s2m.COMPAT_DROP_LAST_SLICE)
+MANDEL_ULP_FACTOR = 32    # C in |f32 - f64| <= C * (ulp(f) + s)
+MANDEL_BAIL_MARGIN = 1e-5  # points whose bailout tests all clear the threshold by this
+
+
+@pytest.mark.parametrize("iters,res,flags", [(8, 32, s2m.COMPAT_DROP_LAST_SLICE),
+                                             (8, 48, s2m.REFERENCE_COMPAT),
+                                             (5, 40, s2m.REFERENCE_COMPAT)])
+def test_mandelbulb_lattice_vs_fp64(ctx, iters, res, flags):
+    # GPU fp32 (ocml transcendentals) against the fp64 oracle, at every lattice point.
+    # Bound: |f32 - f64| <= C * (ulp_f32(f) + s) with C = 32, where s is the first-order
+    # error scale of the evaluation (oracle.mandel_cond: the summed effect on the fp64
+    # result of one relative rounding 2^-24 of each intermediate - z per step, r, theta,
+    # phi, r^8, dr, the final log, product and quotient).  It holds at every point whose
+    # bailout tests all clear |r - 2| / 2 >= 1e-5; the rest (a rounding there can take the
+    # other branch of `if (r > 2.0) continue;`, a jump no error bound covers) are counted
+    # and must stay under 0.1 % of the points.
+    node = tape.mandelbulb(iters)
+    p = s2m.Params.reference(res, 2.5, flags=flags)
+    exact = bool(flags & s2m.CORNERS_EXACT)
+    res = s2m.round_resolution(res) if flags & s2m.COMPAT_POW2 else res
     lat = ctx.eval_lattice(ctx.program(node), p)
-    cs = [s2m.axis_table(p.bb_min[a], p.bb_max[a], res, False)[0] for a in range(3)]
+    cs = [s2m.axis_table(p.bb_min[a], p.bb_max[a], res, exact)[0] for a in range(3)]
     Z, Y, X = np.meshgrid(cs[2], cs[1], cs[0], indexing="ij")
     pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.float32)
     g = lat.ravel().astype(np.float64)
     r64 = oracle.eval_f64(node, pts)
-    r32 = oracle.eval_f32(node, pts).astype(np.float64)
-    sens = np.zeros_like(r64)
-    for a in range(3):
-        for d in (np.inf, -np.inf):
-            q = pts.copy()
-            q[:, a] = np.nextafter(q[:, a], np.float32(d))
-            sens = np.maximum(sens, np.abs(oracle.eval_f64(node, q) - r64))
+    cond = oracle.mandel_cond(node, pts)
     fin = np.isfinite(r64)
     assert np.array_equal(np.isfinite(g), fin)
-    ulp = np.spacing(np.abs(r64[fin]).astype(np.float32)).astype(np.float64)
-    bound = 64 * ulp + 64 * sens[fin]
-    err_g = np.abs(g[fin] - r64[fin])
-    err_c = np.abs(r32[fin] - r64[fin])
-    print("mandelbulb fp32-vs-fp64: gpu max", err_g.max(), "p99", np.quantile(err_g, 0.99),
-          "cpu max", err_c.max(), "p99", np.quantile(err_c, 0.99),
-          "gpu within conditioned bound", np.mean(err_g <= bound),
-          "gpu==cpu32", np.mean(g == r32), flush=True)
-    assert err_g.max() <= 1e-5
-    assert np.mean(err_g <= bound) >= 0.999
-    assert err_g.max() <= 4 * err_c.max()
-    assert np.quantile(err_g, 0.99) <= 4 * np.quantile(err_c, 0.99) + 1e-12
+    keep = fin & (cond[:, 0] >= MANDEL_BAIL_MARGIN)
+    ulp = np.spacing(np.abs(r64).astype(np.float32)).astype(np.float64)
+    bound = MANDEL_ULP_FACTOR * (ulp + cond[:, 2])
+    err = np.abs(g - r64)
+    ratio = err[keep] / (ulp + cond[:, 2])[keep]
+    print(f"mandelbulb iters={iters} res={res}: {keep.sum()} of {fin.sum()} points checked, "
+          f"{fin.sum() - keep.sum()} within {MANDEL_BAIL_MARGIN} of a bailout test; "
+          f"max err / (ulp + s) = {ratio.max():.3g}, max err {err[fin].max():.3g}", flush=True)
+    assert fin.sum() - keep.sum() <= 0.001 * fin.sum()
+    bad = np.nonzero(keep & (err > bound))[0]
+    assert len(bad) == 0, (pts[bad[:5]], g[bad[:5]], r64[bad[:5]], cond[bad[:5]])
 
 
 @pytest.mark.parametrize("res,flags", [(37, 0), (61, s2m.CORNERS_EXACT), (99, s2m.CORNERS_EXACT),

@@ -39,6 +39,51 @@ def test_served_equals_alone(ctx):
         assert st["latency_ms"] >= st["queue_ms"] >= 0.0
 
 
+@pytest.mark.timeout(600)
+def test_served_config5_mix(ctx):
+    # the serving workload's request mix at full size: resolution log-uniform in [32, 512]
+    # (rounded up to a power of two), 80 % random CSG trees, 20 % Mandelbulbs with 4-8
+    # iterations; 120 requests submitted on a seeded Poisson schedule at 400 requests/s
+    # so that arrivals overlap running passes.  Every served mesh must equal the request
+    # meshed alone, and the smallest CSG requests the oracle's meshes.
+    import oracle
+    from sdf2mesh_amd.workloads import poisson_arrivals
+
+    from test_gpu_parity import assert_same_mesh, reference_mesh
+
+    n = 120
+    nodes, params = serve_jobs(n, seed=5)
+    res = [s2m.round_resolution(int(p.res[0])) for p in params]
+    mandel = [nd.tape()[0]["code"] == tape.MANDELBULB for nd in nodes]
+    print(f"config-5 mix: {sum(mandel)} Mandelbulbs, resolutions {sorted(set(res))}", flush=True)
+    assert max(res) == 512 and min(res) <= 64 and sum(mandel) >= 12
+    arrivals = poisson_arrivals(n, 400.0, seed=5)
+    with s2m.Server(0, workers=2, max_batch=16) as srv:
+        tickets = []
+        t0 = time.perf_counter()
+        for a, nd, p in zip(arrivals, nodes, params):
+            now = time.perf_counter() - t0
+            if a > now:
+                time.sleep(a - now)
+            tickets.append(srv.submit(nd, p))
+        results = [srv.take(t) for t in tickets]
+    batches = [st["batch_size"] for _, st in results]
+    lat = sorted(st["latency_ms"] for _, st in results)
+    print(f"served {n}: batch sizes {sorted(set(batches))}, latency p50 {lat[n // 2]:.1f} ms "
+          f"p99 {lat[int(n * 0.99)]:.1f} ms", flush=True)
+    assert max(batches) > 1  # requests shared passes
+    for i, ((m, st), nd, p) in enumerate(zip(results, nodes, params)):
+        solo = ctx.mesh(ctx.program(nd), p)
+        same(m, solo)
+        assert st["latency_ms"] >= st["queue_ms"] >= 0.0
+    # the oracle on the CSG requests of the smallest grids (and the largest at most 128^3)
+    small = [i for i in range(n) if not mandel[i] and res[i] <= 64][:8]
+    mid = [i for i in range(n) if not mandel[i] and res[i] == 128][:1]
+    for i in small + mid:
+        print(f"request {i}: res {res[i]} against the oracle", flush=True)
+        assert_same_mesh(results[i][0], reference_mesh(nodes[i], params[i]))
+
+
 def test_budget_rejects_oversized_request():
     with s2m.Server(0, hbm_budget=1 << 20, workers=1) as srv:
         small = s2m.Params.reference(16, 2.0)

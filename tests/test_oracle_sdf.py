@@ -141,6 +141,55 @@ def test_mandelbulb_immediate_bailout_and_one_step():
                                                          rel=1e-12)
 
 
+U = 2.0 ** -24
+
+
+def test_mandel_cond_immediate_bailout_closed_form():
+    # p = (3, 0, 0): every step tests r = 3 > 2 and continues, so f = 0.5 r ln r - off
+    # with r = zx.  One relative rounding u of zx entering any of the 8 steps persists
+    # to the end (8 sites), of r only at the last step (1 site); each moves f by
+    # f'(r) r u = 0.5 (ln r + 1) r u.  The final 0.5 ln r, its product with r and the
+    # quotient by dr = 1 each move f by 0.5 r ln r u.  Escape-step sites never run.
+    c = oracle.mandel_cond(T.mandelbulb(8), np.asarray([[3.0, 0.0, 0.0]], np.float32))[0]
+    assert c[0] == pytest.approx(0.5, rel=1e-12)      # |3 - 2| / 2
+    assert c[1] == math.inf                          # no escape step
+    want = 9 * 0.5 * (math.log(3) + 1) * 3 * U + 3 * 0.5 * 3 * math.log(3) * U
+    assert c[2] == pytest.approx(want, rel=1e-6)
+
+
+def test_mandel_cond_margins():
+    pts = np.asarray([[0.0, 0.5, 0.0],    # on the axis of the swapped frame: rho = 0
+                      [0.6, 0.0, 0.0],    # on the x axis: theta = phi = 0, rho / r = 1
+                      [2.0, 0.0, 0.0]],   # r = bailout at the first test
+                     np.float32)
+    c = oracle.mandel_cond(T.mandelbulb(8), pts)
+    assert c[0, 1] == 0.0
+    assert c[1, 1] == 1.0
+    assert c[2, 0] == 0.0
+    # no Mandelbulb in the tape: nothing to condition
+    c = oracle.mandel_cond(T.csg64(), pts)
+    assert np.all(np.isinf(c[:, :2])) and np.all(c[:, 2] == 0.0)
+
+
+@pytest.mark.parametrize("iters,res", [(8, 24), (5, 20)])
+def test_mandel_cond_bounds_fp32_error(iters, res):
+    # the first-order error scale must account for the fp32 restatement's actual error
+    # (glibc transcendentals) at every lattice point away from a bailout discontinuity,
+    # and be nonzero wherever the result depends on a rounding
+    node = T.mandelbulb(iters)
+    g = np.linspace(-1.25, 1.25, res + 1, dtype=np.float32)
+    Z, Y, X = np.meshgrid(g, g, g, indexing="ij")
+    pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    r64 = oracle.eval_f64(node, pts)
+    r32 = oracle.eval_f32(node, pts).astype(np.float64)
+    c = oracle.mandel_cond(node, pts)
+    keep = np.isfinite(r64) & (c[:, 0] >= 1e-5)
+    assert keep.mean() > 0.99
+    ulp = np.spacing(np.abs(r64).astype(np.float32)).astype(np.float64)
+    assert np.all(np.abs(r32 - r64)[keep] <= 16 * (ulp + c[:, 2])[keep])
+    assert np.all(c[keep, 2] > 0)
+
+
 def test_mandelbulb_swizzle():
     # p.xyz = p.xzy: the estimator at (x, y, z) without the swap equals the one at
     # (x, z, y) with it
