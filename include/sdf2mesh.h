@@ -267,6 +267,33 @@ int s2m_mesh_device(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* par
                     s2m_device_mesh* out);
 void s2m_mesh_free(s2m_mesh_out* out);
 
+/* ---- meshing without device allocation ------------------------------------------ */
+/* No counterpart in the reference.  For callers that cannot allocate device memory
+ * (serving loops that must not call hipMalloc, harnesses that forbid it): the mesh of
+ * s2m_mesh_device, computed by the dense-path kernels, written into the caller's device
+ * buffers.  Every working buffer (coordinate tables, lattice slab, cell -> vertex planes,
+ * block counts, activity bits, counters) is carved from one static device array of
+ * S2M_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until the process
+ * ends; slabs shrink until they fit it.  The call performs no hipMalloc, hipFree or
+ * other allocating HIP call.  Errors: S2M_E_ARG when one cell plane per slab does not fit
+ * the scratch; S2M_E_OOM when the mesh exceeds vcap / tcap, with out->nvtx / out->ntri set
+ * to the sizes it needs (the triangle count is a lower bound when the vertices
+ * overflowed).  All calls on one GPU share the array: calls that overlapped would race,
+ * so the library runs them one at a time per device (a lock held until the call has
+ * synchronised the context's stream).  Not for use under stream capture: it
+ * synchronises the stream. */
+#define S2M_NOALLOC_SCRATCH_BYTES (64u << 20)
+typedef struct s2m_mesh_buffers {
+    float* vtx;          /* device: 6 floats per vertex, vcap vertices                    */
+    uint64_t* cell_keys; /* device: vcap keys (working storage of the quad pass too)      */
+    uint8_t* signs;      /* device: vcap sign bytes (read by the quad pass)               */
+    uint64_t vcap;
+    uint32_t* tri;       /* device: 3 u32 per triangle, tcap triangles                    */
+    uint64_t tcap;
+} s2m_mesh_buffers;
+int s2m_mesh_device_noalloc(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params,
+                            const s2m_mesh_buffers* bufs, s2m_device_mesh* out);
+
 /* ---- batching ------------------------------------------------------------------- */
 /* Mesh n independent jobs together: the tiles of all jobs go through one pass of the
  * tile-sparse kernels (one set of launches for the batch), so small jobs share the

@@ -84,6 +84,11 @@ class _DeviceMesh(C.Structure):
                 ("invalid_quads", C.c_uint64)]
 
 
+class _MeshBuffers(C.Structure):
+    _fields_ = [("vtx", C.c_void_p), ("cell_keys", C.c_void_p), ("signs", C.c_void_p),
+                ("vcap", C.c_uint64), ("tri", C.c_void_p), ("tcap", C.c_uint64)]
+
+
 class _ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget", C.c_uint64), ("max_batch", C.c_uint32),
                 ("workers", C.c_uint32), ("want_keys", C.c_int)]
@@ -129,6 +134,8 @@ _SIGNATURES = {
     "s2m_mesh_device": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_Params),
                                   C.POINTER(_DeviceMesh)]),
     "s2m_mesh_free": (None, [C.POINTER(_MeshOut)]),
+    "s2m_mesh_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_Params),
+                                          C.POINTER(_MeshBuffers), C.POINTER(_DeviceMesh)]),
     "s2m_mesh_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(_Params),
                                  C.c_int, C.POINTER(_MeshOut)]),
     "s2m_mesh_batch_device": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p),
@@ -507,6 +514,22 @@ class Context:
                                      C.byref(p), C.byref(dm)))
         return {"nvtx": int(dm.nvtx), "ntri": int(dm.ntri), "invalid_quads": int(dm.invalid_quads),
                 "vtx_ptr": dm.vtx, "tri_ptr": dm.tri}
+
+    def mesh_device_noalloc(self, prog: Program, params: Params, vtx: int, cell_keys: int,
+                            signs: int, vcap: int, tri: int, tcap: int) -> dict:
+        """s2m_mesh_device_noalloc into caller-owned device buffers (addresses): the
+        library allocates nothing.  Returns the counts; raises S2MError (E_OOM with
+        .needed = (vertices, triangles) when the buffers are too small)."""
+        p = params.c()
+        bufs = _MeshBuffers(vtx, cell_keys, signs, vcap, tri, tcap)
+        dm = _DeviceMesh()
+        rc = lib().s2m_mesh_device_noalloc(C.c_void_p(self.handle), C.c_void_p(prog.handle),
+                                           C.byref(p), C.byref(bufs), C.byref(dm))
+        if rc != OK:
+            err = S2MError(rc, lib().s2m_last_error().decode(errors="replace"))
+            err.needed = (int(dm.nvtx), int(dm.ntri))
+            raise err
+        return {"nvtx": int(dm.nvtx), "ntri": int(dm.ntri), "invalid_quads": int(dm.invalid_quads)}
 
     def check_sincos(self) -> tuple:
         """s2m_check_sincos: (sine mismatches, of them NaN pairs, cosine mismatches, of

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,7 @@ using namespace s2m;
 namespace {
 
 thread_local std::string g_err;
+std::mutex g_noalloc_mu[s2m::kMaxDevices];  // s2m_mesh_device_noalloc: one call per device at a time
 
 int fail(int code, const char* fmt, ...)
 {
@@ -314,16 +316,26 @@ uint32_t pick_slab(const s2m_params* p, const Job& j)
     return (uint32_t)t;
 }
 
-// Upload the three axis tables: [cx cy cz | lox hix loy hiy loz hiz] as one buffer.
-int upload_tables(s2m_ctx* ctx, const Job& j, const float* (&c)[3], const uint32_t* (&lo)[3],
-                  const uint32_t* (&hi)[3])
+// Bytes of the three axis tables as upload_tables lays them out.
+size_t tables_bytes(const Job& j)
 {
     size_t nf = 0, nu = 0;
     for (int a = 0; a < 3; ++a) {
         nf += j.ax[a].coord.size();
         nu += 2 * j.ax[a].lo.size();
     }
-    std::vector<uint8_t> host(nf * 4 + nu * 4 + 64);
+    return nf * 4 + nu * 4 + 64;
+}
+
+// Upload the three axis tables: [cx cy cz | lox hix loy hiy loz hiz] as one buffer, into
+// `dst` (tables_bytes(j) bytes of device memory) or, when dst is NULL, the context's
+// table buffer.
+int upload_tables(s2m_ctx* ctx, const Job& j, const float* (&c)[3], const uint32_t* (&lo)[3],
+                  const uint32_t* (&hi)[3], void* dst = nullptr)
+{
+    size_t nf = 0;
+    for (int a = 0; a < 3; ++a) nf += j.ax[a].coord.size();
+    std::vector<uint8_t> host(tables_bytes(j));
     float* fp = reinterpret_cast<float*>(host.data());
     uint32_t* up = reinterpret_cast<uint32_t*>(host.data() + nf * 4);
     size_t fo[3], lo_o[3], hi_o[3], f = 0, u = 0;
@@ -338,13 +350,15 @@ int upload_tables(s2m_ctx* ctx, const Job& j, const float* (&c)[3], const uint32
         std::copy(j.ax[a].hi.begin(), j.ax[a].hi.end(), up + u);
         u += j.ax[a].hi.size();
     }
-    HIP_TRY(ctx->tables.ensure(host.size()));
-    HIP_TRY(hipMemcpyAsync(ctx->tables.p, host.data(), host.size(), hipMemcpyHostToDevice,
-                           ctx->stream));
+    if (!dst) {
+        HIP_TRY(ctx->tables.ensure(host.size()));
+        dst = ctx->tables.p;
+    }
+    HIP_TRY(hipMemcpyAsync(dst, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
     // the host vector must outlive the async copy
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    const float* dfp = ctx->tables.as<float>();
-    const uint32_t* dup = reinterpret_cast<const uint32_t*>(ctx->tables.as<uint8_t>() + nf * 4);
+    const float* dfp = static_cast<const float*>(dst);
+    const uint32_t* dup = reinterpret_cast<const uint32_t*>(static_cast<const uint8_t*>(dst) + nf * 4);
     for (int a = 0; a < 3; ++a) {
         c[a] = dfp + fo[a];
         lo[a] = dup + lo_o[a];
@@ -837,10 +851,60 @@ hipError_t normals_prog(hipStream_t s, const s2m_program* prog, float eps, const
     return hipModuleLaunchKernel(prog->k_normals, 1024, 1, 1, 256, 1, 1, 0, s, args, nullptr);
 }
 
+// Buffers of a dense job that must not allocate (s2m_mesh_device_noalloc): every working
+// buffer is carved from `scratch` (device memory of `scratch_bytes`), the mesh goes to
+// the caller's buffers.
+struct FixedBufs {
+    uint8_t* scratch;
+    size_t scratch_bytes;
+    float* vtx;
+    uint64_t* keys;
+    uint8_t* signs;
+    uint32_t* tri;
+    uint64_t vcap, tcap;
+};
+
+// Working-buffer sizes of a dense job streamed in slabs of T cell planes.
+struct DenseLayout {
+    uint32_t T, nx, ny, pitch, max_nzl, max_blocks;
+    size_t tables, lattice, vidx, blocks, actbits, counters;
+    size_t carved() const  // bytes when carved from one region, each part 256-byte aligned
+    {
+        size_t b = 0;
+        for (size_t n : {tables, lattice, vidx, blocks, blocks, actbits, counters}) b += (n + 255) & ~size_t(255);
+        return b;
+    }
+};
+
+DenseLayout dense_layout(const Job& j, uint32_t T, bool lattice)
+{
+    DenseLayout L{};
+    const uint64_t plane = (uint64_t)j.res[0] * j.res[1];
+    L.T = T;
+    L.nx = (uint32_t)j.ax[0].coord.size();
+    L.ny = (uint32_t)j.ax[1].coord.size();
+    L.pitch = (L.nx + 63u) & ~63u;
+    for (uint32_t z0 = 0; z0 < j.nz_cells; z0 += T) {  // largest lattice z extent of any slab
+        const uint32_t z1 = std::min(z0 + T, j.nz_cells);
+        L.max_nzl = std::max(L.max_nzl, j.ax[2].hi[z1 - 1] - j.ax[2].lo[z0] + 1);
+    }
+    L.max_blocks = (uint32_t)((plane * T + kCellsPerBlock - 1) / kCellsPerBlock);
+    L.tables = tables_bytes(j);
+    L.lattice = lattice ? (size_t)L.pitch * L.ny * L.max_nzl * sizeof(float) : 0;
+    L.vidx = (size_t)(T + 1) * plane * sizeof(uint32_t);
+    L.blocks = (size_t)L.max_blocks * sizeof(uint32_t);
+    L.actbits = (size_t)L.max_blocks * (kCellsPerBlock / 64) * sizeof(uint64_t);
+    L.counters = kNumCounters * sizeof(uint64_t);
+    return L;
+}
+
 // Mesh one job on the device.  corners != NULL: contour the given per-cell corner
-// dump (test hook) instead of evaluating a program.
+// dump (test hook) instead of evaluating a program.  fx != NULL: allocate nothing (the
+// working buffers come from fx->scratch, the slab shrinks until they fit, and a mesh
+// larger than the caller's buffers fails with S2M_E_OOM and its size in last_nvtx /
+// last_ntri instead of being repeated with larger buffers).
 int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Job& j,
-            const float* h_corners)
+            const float* h_corners, const FixedBufs* fx = nullptr)
 {
     int rc = S2M_OK;
     ctx->timing = s2m_timing{};
@@ -851,31 +915,57 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
     if (total_cells == 0) return S2M_OK;
     if (total_cells > 0xffffffffull * 4ull)
         return fail(S2M_E_ARG, "grid of %llu cells is too large", (unsigned long long)total_cells);
+    if (fx && h_corners) return fail(S2M_E_ARG, "corner dumps are not contoured without allocation");
+    const bool prof = ctx->profiling && !fx;  // events are created on first use
 
+    DenseLayout L = dense_layout(j, pick_slab(p, j), !h_corners);
+    if (fx) {  // the largest slab (at most the usual one) whose buffers fit the scratch
+        while (L.T > 1 && L.carved() > fx->scratch_bytes) L = dense_layout(j, L.T - 1, true);
+        if (L.carved() > fx->scratch_bytes)
+            return fail(S2M_E_ARG, "a %u x %u grid needs %llu bytes of working memory per cell plane; "
+                                   "the no-allocation scratch holds %llu",
+                        j.res[0], j.res[1], (unsigned long long)L.carved(),
+                        (unsigned long long)fx->scratch_bytes);
+    }
+    const uint32_t T = L.T, nx = L.nx, ny = L.ny, pitch = L.pitch, max_blocks = L.max_blocks;
+
+    float* lat = nullptr;
+    uint32_t *vidx = nullptr, *bcount = nullptr, *boffset = nullptr;
+    uint64_t *actbits = nullptr, *counters = nullptr;
+    void* tables_at = nullptr;
+    if (fx) {
+        uint8_t* at = fx->scratch;
+        auto take = [&at](size_t n) {
+            uint8_t* q = at;
+            at += (n + 255) & ~size_t(255);
+            return q;
+        };
+        tables_at = take(L.tables);
+        lat = reinterpret_cast<float*>(take(L.lattice));
+        vidx = reinterpret_cast<uint32_t*>(take(L.vidx));
+        bcount = reinterpret_cast<uint32_t*>(take(L.blocks));
+        boffset = reinterpret_cast<uint32_t*>(take(L.blocks));
+        actbits = reinterpret_cast<uint64_t*>(take(L.actbits));
+        counters = reinterpret_cast<uint64_t*>(take(L.counters));
+    } else {
+        if (!h_corners) HIP_TRY(ctx->lattice.ensure(L.lattice));
+        HIP_TRY(ctx->vidx.ensure(L.vidx));
+        HIP_TRY(ctx->bcount.ensure(L.blocks));
+        HIP_TRY(ctx->boffset.ensure(L.blocks));
+        HIP_TRY(ctx->actbits.ensure(L.actbits));
+        HIP_TRY(ctx->counters.ensure(L.counters));
+        lat = ctx->lattice.as<float>();
+        vidx = ctx->vidx.as<uint32_t>();
+        bcount = ctx->bcount.as<uint32_t>();
+        boffset = ctx->boffset.as<uint32_t>();
+        actbits = ctx->actbits.as<uint64_t>();
+        counters = ctx->counters.as<uint64_t>();
+    }
     const float* dc[3];
     const uint32_t* dlo[3];
     const uint32_t* dhi[3];
-    rc = upload_tables(ctx, j, dc, dlo, dhi);
+    rc = upload_tables(ctx, j, dc, dlo, dhi, tables_at);
     if (rc) return rc;
-
-    const uint32_t T = pick_slab(p, j);
-    const uint32_t nx = (uint32_t)j.ax[0].coord.size(), ny = (uint32_t)j.ax[1].coord.size();
-    const uint32_t pitch = (nx + 63u) & ~63u;
-    // largest lattice z extent of any slab
-    uint32_t max_nzl = 0;
-    for (uint32_t z0 = 0; z0 < j.nz_cells; z0 += T) {
-        const uint32_t z1 = std::min(z0 + T, j.nz_cells);
-        max_nzl = std::max(max_nzl, j.ax[2].hi[z1 - 1] - j.ax[2].lo[z0] + 1);
-    }
-    const uint64_t slab_cells = plane * T;
-    const uint32_t max_blocks = (uint32_t)((slab_cells + kCellsPerBlock - 1) / kCellsPerBlock);
-
-    if (!h_corners) HIP_TRY(ctx->lattice.ensure((size_t)pitch * ny * max_nzl * sizeof(float)));
-    HIP_TRY(ctx->vidx.ensure((size_t)(T + 1) * plane * sizeof(uint32_t)));
-    HIP_TRY(ctx->bcount.ensure((size_t)max_blocks * sizeof(uint32_t)));
-    HIP_TRY(ctx->boffset.ensure((size_t)max_blocks * sizeof(uint32_t)));
-    HIP_TRY(ctx->actbits.ensure((size_t)max_blocks * (kCellsPerBlock / 64) * sizeof(uint64_t)));
-    HIP_TRY(ctx->counters.ensure(kNumCounters * sizeof(uint64_t)));
     if (h_corners) {
         HIP_TRY(ctx->corners.ensure((size_t)total_cells * 8 * sizeof(float)));
         HIP_TRY(hipMemcpyAsync(ctx->corners.p, h_corners, (size_t)total_cells * 8 * sizeof(float),
@@ -887,30 +977,40 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
     uint64_t want_v = std::min<uint64_t>(total_cells, std::max<uint64_t>(65536, 12 * plane));
     want_v = std::max<uint64_t>(want_v, ctx->vcap);
     uint64_t want_t = std::max<uint64_t>(3 * want_v, ctx->tcap);
+    if (fx) {
+        want_v = fx->vcap;
+        want_t = fx->tcap;
+    }
     const bool normals = prog && !(p->flags & S2M_NO_NORMALS);
 
-    for (int attempt = 0; attempt < 3; ++attempt) {
-        HIP_TRY(ctx->vtx.ensure(want_v * 6 * sizeof(float)));
-        HIP_TRY(ctx->keys.ensure(want_v * sizeof(uint64_t)));
-        HIP_TRY(ctx->signs.ensure(want_v));
-        HIP_TRY(ctx->tri.ensure(want_t * 3 * sizeof(uint32_t)));
-        ctx->vcap = want_v;
-        ctx->tcap = want_t;
-
+    for (int attempt = 0; attempt < (fx ? 1 : 3); ++attempt) {
         MeshBufs b{};
-        b.vtx = ctx->vtx.as<float>();
-        b.keys = ctx->keys.as<uint64_t>();
-        b.signs = ctx->signs.as<uint8_t>();
-        b.tri = ctx->tri.as<uint32_t>();
+        if (fx) {
+            b.vtx = fx->vtx;
+            b.keys = fx->keys;
+            b.signs = fx->signs;
+            b.tri = fx->tri;
+        } else {
+            HIP_TRY(ctx->vtx.ensure(want_v * 6 * sizeof(float)));
+            HIP_TRY(ctx->keys.ensure(want_v * sizeof(uint64_t)));
+            HIP_TRY(ctx->signs.ensure(want_v));
+            HIP_TRY(ctx->tri.ensure(want_t * 3 * sizeof(uint32_t)));
+            ctx->vcap = want_v;
+            ctx->tcap = want_t;
+            b.vtx = ctx->vtx.as<float>();
+            b.keys = ctx->keys.as<uint64_t>();
+            b.signs = ctx->signs.as<uint8_t>();
+            b.tri = ctx->tri.as<uint32_t>();
+        }
         b.vcap = want_v;
         b.tcap = want_t;
-        b.counters = ctx->counters.as<uint64_t>();
-        b.bcount = ctx->bcount.as<uint32_t>();
-        b.boffset = ctx->boffset.as<uint32_t>();
-        b.actbits = ctx->actbits.as<uint64_t>();
+        b.counters = counters;
+        b.bcount = bcount;
+        b.boffset = boffset;
+        b.actbits = actbits;
 
         HIP_TRY(hipMemsetAsync(b.counters, 0, kNumCounters * sizeof(uint64_t), ctx->stream));
-        HIP_TRY(hipMemsetAsync(ctx->vidx.p, 0xff, (size_t)plane * sizeof(uint32_t), ctx->stream));
+        HIP_TRY(hipMemsetAsync(vidx, 0xff, (size_t)plane * sizeof(uint32_t), ctx->stream));
         // the count pass ORs into zeroed words; each slab's emit pass clears what it read
         HIP_TRY(hipMemsetAsync(b.actbits, 0, (size_t)max_blocks * (kCellsPerBlock / 64) * sizeof(uint64_t),
                                ctx->stream));
@@ -918,7 +1018,7 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
         ctx->events_used = 0;
         std::vector<hipEvent_t> ev;  // per slab: 5 marks (eval, cells, normals, quads, end)
         hipEvent_t ev_start = nullptr, ev_end = nullptr;
-        if (ctx->profiling) {
+        if (prof) {
             ev_start = ctx->next_event();
             HIP_TRY(hipEventRecord(ev_start, ctx->stream));
         }
@@ -930,7 +1030,7 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             const uint32_t nzl = j.ax[2].hi[z1 - 1] - lz0 + 1;
 
             SlabArgs a{};
-            a.lat = ctx->lattice.as<float>();
+            a.lat = lat;
             a.pitch = pitch;
             a.lat_ny = ny;
             a.lz0 = lz0;
@@ -949,17 +1049,16 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             a.ncells = (uint32_t)(plane * nzc);
             a.div_rx = make_fastdiv(j.res[0]);
             a.div_rxy = make_fastdiv((uint32_t)plane);
-            a.vidx = ctx->vidx.as<uint32_t>();
+            a.vidx = vidx;
             a.flags = p->flags;
             const uint32_t nb = (uint32_t)(((uint64_t)a.ncells + kCellsPerBlock - 1) / kCellsPerBlock);
 
             hipEvent_t m[5] = {};
-            if (ctx->profiling)
+            if (prof)
                 for (auto& e : m) e = ctx->next_event();
             if (m[0]) HIP_TRY(hipEventRecord(m[0], ctx->stream));
             if (!h_corners) {
-                HIP_TRY(eval_lattice_prog(ctx->stream, prog, dc[0], dc[1], dc[2], nx, ny, lz0, nzl, pitch,
-                                          ctx->lattice.as<float>()));
+                HIP_TRY(eval_lattice_prog(ctx->stream, prog, dc[0], dc[1], dc[2], nx, ny, lz0, nzl, pitch, lat));
                 lattice_points += (uint64_t)nx * ny * nzl;
             }
             if (m[1]) HIP_TRY(hipEventRecord(m[1], ctx->stream));
@@ -981,21 +1080,20 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             HIP_TRY(launch_quads(ctx->stream, a, b, 1));
             if (m[4]) HIP_TRY(hipEventRecord(m[4], ctx->stream));
             if (z1 < j.nz_cells) {  // last map plane becomes the previous plane
-                uint32_t* vp = ctx->vidx.as<uint32_t>();
-                HIP_TRY(hipMemcpyAsync(vp, vp + (size_t)nzc * plane, plane * sizeof(uint32_t),
+                HIP_TRY(hipMemcpyAsync(vidx, vidx + (size_t)nzc * plane, plane * sizeof(uint32_t),
                                        hipMemcpyDeviceToDevice, ctx->stream));
             }
-            if (ctx->profiling) ev.insert(ev.end(), m, m + 5);
+            if (prof) ev.insert(ev.end(), m, m + 5);
             ++slabs;
         }
-        if (ctx->profiling) {
+        if (prof) {
             ev_end = ctx->next_event();
             HIP_TRY(hipEventRecord(ev_end, ctx->stream));
         }
         uint64_t cnt[kNumCounters];
         HIP_TRY(hipMemcpyAsync(cnt, b.counters, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (ctx->profiling) {
+        if (prof) {
             float ms = 0;
             HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_end));
             ctx->timing.total_ms = ms;
@@ -1027,6 +1125,15 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             ctx->last_ntri = cnt[kTriTotal];
             ctx->last_invalid = cnt[kInvalid];
             return S2M_OK;
+        }
+        if (fx) {  // no second attempt: report the sizes the mesh needs
+            ctx->last_nvtx = cnt[kVtxTotal];
+            ctx->last_ntri = cnt[kTriTotal];
+            ctx->last_invalid = cnt[kInvalid];
+            return fail(S2M_E_OOM, "the mesh needs %llu vertices and %llu%s triangles; the buffers hold %llu and %llu",
+                        (unsigned long long)cnt[kVtxTotal], (unsigned long long)cnt[kTriTotal],
+                        cnt[kVtxTotal] > want_v ? " or more" : "", (unsigned long long)want_v,
+                        (unsigned long long)want_t);
         }
         // vertices past the capacity form no quads, so after a vertex overflow the triangle
         // count is low: size triangles from the vertices as at the start
@@ -1133,6 +1240,11 @@ s2m_ctx* s2m_create(int device)
         delete ctx;
         return nullptr;
     }
+    // set-up of the no-allocation entry point: the scratch array's address on this
+    // device, looked up once per device while it is current (a failure is kept and
+    // reported by s2m_mesh_device_noalloc)
+    void* scratch_at = nullptr;
+    (void)noalloc_scratch(device, &scratch_at);
     return ctx;
 }
 
@@ -1457,6 +1569,39 @@ int s2m_mesh_batch_device(s2m_ctx* ctx, uint32_t n, const s2m_program* const* pr
         o.invalid_quads = r[q].invalid;
     }
     return S2M_OK;
+}
+
+int s2m_mesh_device_noalloc(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params,
+                            const s2m_mesh_buffers* bufs, s2m_device_mesh* out)
+{
+    if (!ctx || !prog || !params || !bufs || !out) return fail(S2M_E_ARG, "NULL argument");
+    std::memset(out, 0, sizeof *out);
+    if (prog->device != ctx->device) return fail(S2M_E_ARG, "program built for another device");
+    if (!bufs->vtx || !bufs->cell_keys || !bufs->signs || !bufs->tri)
+        return fail(S2M_E_ARG, "an output buffer is NULL");
+    if (ctx->device < 0 || ctx->device >= (int)kMaxDevices)
+        return fail(S2M_E_ARG, "device %d has no scratch slot", ctx->device);
+    HIP_TRY(hipSetDevice(ctx->device));
+    void* scratch_at = nullptr;
+    HIP_TRY(noalloc_scratch(ctx->device, &scratch_at));
+    Job j;
+    int rc = prepare_job(params, j);
+    if (rc) return rc;
+    const FixedBufs fx{static_cast<uint8_t*>(scratch_at), kNoAllocScratchBytes, bufs->vtx, bufs->cell_keys,
+                       bufs->signs, bufs->tri, bufs->vcap, bufs->tcap};
+    {
+        // one call at a time per device: every call on the device uses the same array
+        std::lock_guard<std::mutex> g(g_noalloc_mu[ctx->device]);
+        rc = run_job(ctx, prog, params, j, nullptr, &fx);
+    }
+    out->vtx = bufs->vtx;
+    out->tri = bufs->tri;
+    out->cell_keys = bufs->cell_keys;
+    out->signs = bufs->signs;
+    out->nvtx = ctx->last_nvtx;
+    out->ntri = ctx->last_ntri;
+    out->invalid_quads = ctx->last_invalid;
+    return rc;
 }
 
 void s2m_mesh_free(s2m_mesh_out* out)

@@ -215,6 +215,14 @@ hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b,
 hipError_t launch_cells_from_corners(hipStream_t s, const SlabArgs& a, const float* corners,
                                      const MeshBufs& b, int emit);
 
+// ---- no-allocation scratch (kernels.hip) ----
+constexpr size_t kNoAllocScratchBytes = (size_t)S2M_NOALLOC_SCRATCH_BYTES;
+constexpr uint32_t kMaxDevices = 64;
+// Device address of the scratch array on `device`, which must be the current device.
+// Looked up once per device (hipGetSymbolAddress); later calls return the kept address
+// and the first lookup's status.
+hipError_t noalloc_scratch(int device, void** ptr);
+
 // ---- host memory for results (hostmem.cpp) ----
 // Host memory for a result array: page-locked and recycled when large, malloc otherwise.
 // Returned to the pool (or freed) by host_release; any other pointer is passed to free().

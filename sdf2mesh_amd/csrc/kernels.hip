@@ -10,6 +10,8 @@
 // Cells are numbered in (z, y, x) order inside the slab and every block owns 1024
 // consecutive cells, so block offsets from the scan give each vertex and triangle its
 // place in the reference's output order (VertexList order, mesh.rs:280-321).
+#include <mutex>
+
 #include "device_dense.h"
 #include "device_sdf.h"
 #include "internal.h"
@@ -746,6 +748,42 @@ hipError_t launch_quads(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int
     if (emit) hipLaunchKernelGGL((k_quads<1>), dim3(nb), dim3(kBlock), 0, s, a, b);
     else hipLaunchKernelGGL((k_quads<0>), dim3(nb), dim3(kBlock), 0, s, a, b);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------------
+// Scratch of the no-allocation entry point (s2m_mesh_device_noalloc): one static device
+// array per GPU that holds every working buffer of a dense job (coordinate tables, the
+// slab lattice, cell -> vertex planes, block counts and offsets, activity bits,
+// counters).  Host code never passes the array's name to a kernel or a copy (on the
+// host it names a stand-in, not the device memory): it uses the address
+// hipGetSymbolAddress returns for the device, taken once per device by noalloc_scratch.
+// ---------------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+bool g_scratch_done[kMaxDevices] = {};
+}  // namespace
+
+hipError_t noalloc_scratch(int device, void** ptr)
+{
+    *ptr = nullptr;
+    if (device < 0 || device >= (int)kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e == hipSuccess && cur != device) e = hipErrorInvalidDevice;  // caller makes it current
+        void* a = nullptr;
+        if (e == hipSuccess) e = hipGetSymbolAddress(&a, HIP_SYMBOL(scratch));
+        g_scratch_addr[device] = e == hipSuccess ? a : nullptr;
+        g_scratch_err[device] = e;
+        g_scratch_done[device] = true;
+    }
+    *ptr = g_scratch_addr[device];
+    return g_scratch_err[device];
 }
 
 }  // namespace s2m
