@@ -141,6 +141,7 @@ def single_job(name: str, args, ctx, headline: bool) -> dict:
         params.flags |= s2m.DENSE_EVAL
     voxels = int(params.res[0]) * int(params.res[1]) * int(params.res[2])
     prog = ctx.program(node)
+    tuned = ctx.tune(prog, params) if not args.no_tune else None  # before the timed steps
     for _ in range(args.warmup):
         ctx.mesh_to_host(prog, params).free()
         ctx.mesh_device(prog, params)
@@ -189,7 +190,9 @@ def single_job(name: str, args, ctx, headline: bool) -> dict:
            "lattice_points_per_step": int(agg["lattice_points"] // args.steps),
            "tiles_active_per_step": int(agg["tiles_active"] // args.steps),
            "tiles_per_step": int(agg["tiles_total"] // args.steps),
-           "slabs": int(agg["slabs"] // args.steps)}
+           "slabs": int(agg["slabs"] // args.steps),
+           "autotuned": None if tuned is None else {
+               k: tuned[k] for k in ("stream_passes", "dense_slab", "ms", "measured")}}
     out = {
         "value": round(voxels / 1e6 / (ms / 1e3), 1),
         "unit": "Mvoxels/s",
@@ -252,14 +255,19 @@ def run_batch(args) -> dict:
     ctx = s2m.Context(0)
     progs = [ctx.program(x) for x in nodes]
     steps = max(3, min(args.steps, 10))
+    tuned = ctx.tune_batch(progs, params) if not args.no_tune else None
     for _ in range(max(1, args.warmup)):
-        ctx.mesh_batch(progs, params, want_keys=False)
+        for m in ctx.mesh_batch_to_host(progs, params):
+            m.free()
         ctx.mesh_batch_device(progs, params)
-    bt = []
+    bt, nv, nt = [], 0, 0
     for _ in range(steps):
         t0 = time.perf_counter()
-        out = ctx.mesh_batch(progs, params, want_keys=False)
+        out = ctx.mesh_batch_to_host(progs, params)
         bt.append(time.perf_counter() - t0)
+        nv, nt = sum(m.nvtx for m in out), sum(m.ntri for m in out)
+        for m in out:
+            m.free()
     bdev = []
     for _ in range(steps):
         t0 = time.perf_counter()
@@ -285,9 +293,10 @@ def run_batch(args) -> dict:
         "value": round(jobs_b, 1), "unit": "jobs/s", "ms_per_step": round(dt_b * 1e3, 3),
         "steps": steps,
         "config": {"workload": "batch256", "jobs": n, "voxels_per_batch": int(voxels),
-                   "vertices": int(sum(len(m.vertices) for m in out)),
-                   "triangles": int(sum(len(m.triangles) for m in out)),
-                   "mesh_in": "host memory"},
+                   "vertices": int(nv), "triangles": int(nt),
+                   "mesh_in": "host memory",
+                   "autotuned": None if tuned is None else {
+                       k: tuned[k] for k in ("batch_cells", "ms", "measured")}},
         "one_at_a_time_jobs_per_s": round(jobs_s, 1),
         "speedup_vs_one_at_a_time": round(jobs_b / jobs_s, 2),
         "one_at_a_time_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
@@ -417,6 +426,7 @@ def main() -> int:
     ap.add_argument("--only", action="store_true", help="run --workload alone")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="use the library's rules, not the autotuner")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rates", default="10,30,100,300,1000,2000")
     ap.add_argument("--serve-workers", type=int, default=2)

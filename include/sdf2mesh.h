@@ -309,6 +309,41 @@ int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
 int s2m_mesh_batch_device(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
                           const s2m_params* params, s2m_device_mesh* outs);
 
+/* ---- autotuning ---------------------------------------------------------------- */
+/* No reference counterpart (the reference dispatches one z-slice at a time, main.rs:298-356).
+ * The library picks three shapes by rule unless the autotuner has chosen one for the job's
+ * key (device, program class, grid, flags; batches: device, job count, total cells):
+ *   stream_passes  z-passes of a single tile-sparse job meshed by s2m_mesh (the copy of one
+ *                  pass to host memory overlaps the next pass's work)
+ *   dense_slab     cell planes per slab of the dense path (Mandelbulb, compiled programs)
+ *   batch_cells    most cells per tile-sparse pass of s2m_mesh_batch
+ * The tile edge is fixed at 8 cells: the tile kernels' LDS layouts are built for it.
+ * s2m_tune times each candidate of the knob that applies to (prog, params) with s2m_mesh
+ * (one untimed run, then the median of `reps` runs, at least 3), keeps the first candidate
+ * unless a later one is more than 2 % faster (s2m_tune_pick), stores the choice in a
+ * process-wide table that every later meshing call of that key uses, and reports it.
+ * A key already in the table is returned without timing (measured = 0).  s2m_params.slab
+ * != 0 overrides any choice. */
+typedef struct s2m_tuning {
+    uint32_t stream_passes; /* 0: not this job's knob */
+    uint32_t dense_slab;
+    uint64_t batch_cells;
+    double ms;              /* median time of the chosen shape when it was timed */
+    uint32_t measured;      /* candidates timed by this call */
+} s2m_tuning;
+int s2m_tune(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, uint32_t reps,
+             s2m_tuning* out);
+int s2m_tune_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
+                   const s2m_params* params, uint32_t reps, s2m_tuning* out);
+/* The table as a JSON file ({"entries": [{"key": ..., "stream_passes": ..., "dense_slab": ...,
+ * "batch_cells": ..., "ms": ...}, ...]}): save writes every entry, load merges a file's
+ * entries into the table (S2M_E_IO, S2M_E_PARSE on failure), clear empties it. */
+int s2m_tuning_save(const char* path);
+int s2m_tuning_load(const char* path);
+void s2m_tuning_clear(void);
+/* The choice rule on its own: index of the chosen one of n timings (0 for n = 0). */
+uint32_t s2m_tune_pick(const double* ms, uint32_t n);
+
 /* ---- serving ------------------------------------------------------------------- */
 /* A request loop with continuous batching.  `workers` threads each own a context (HIP
  * stream + resident device buffers) on `device`.  Submitted requests wait in a FIFO
@@ -362,6 +397,10 @@ int s2m_write_to_file(const s2m_mesh_out* mesh, const char* path);
 int s2m_format_f32(float x, char* buf);
 
 /* ---- test hooks ----------------------------------------------------------------- */
+/* The page-locked host memory results are copied into: stats[0] blocks page-locked so far
+ * (hipHostMalloc calls), [1] page-locked bytes held, [2] idle blocks kept for reuse,
+ * [3] their bytes.  A repeated meshing call of the same size adds no block. */
+void s2m_host_pool_stats(uint64_t stats[4]);
 /* Per-axis coordinate tables used for corner evaluation.  coord receives the
  * distinct coordinates (at most 2*res+1), lo/hi (res entries each) the index of each
  * cell's min and max corner.  Returns the number of coordinates, or 0 on error. */

@@ -89,6 +89,11 @@ class _MeshBuffers(C.Structure):
                 ("vcap", C.c_uint64), ("tri", C.c_void_p), ("tcap", C.c_uint64)]
 
 
+class _Tuning(C.Structure):
+    _fields_ = [("stream_passes", C.c_uint32), ("dense_slab", C.c_uint32), ("batch_cells", C.c_uint64),
+                ("ms", C.c_double), ("measured", C.c_uint32)]
+
+
 class _ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget", C.c_uint64), ("max_batch", C.c_uint32),
                 ("workers", C.c_uint32), ("want_keys", C.c_int)]
@@ -140,6 +145,13 @@ _SIGNATURES = {
                                  C.c_int, C.POINTER(_MeshOut)]),
     "s2m_mesh_batch_device": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p),
                                         C.POINTER(_Params), C.POINTER(_DeviceMesh)]),
+    "s2m_tune": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_Params), C.c_uint32, C.POINTER(_Tuning)]),
+    "s2m_tune_batch": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_void_p), C.POINTER(_Params),
+                                 C.c_uint32, C.POINTER(_Tuning)]),
+    "s2m_tuning_save": (C.c_int, [C.c_char_p]),
+    "s2m_tuning_load": (C.c_int, [C.c_char_p]),
+    "s2m_tuning_clear": (None, []),
+    "s2m_tune_pick": (C.c_uint32, [C.POINTER(C.c_double), C.c_uint32]),
     "s2m_server_create": (C.c_void_p, [C.POINTER(_ServerConfig)]),
     "s2m_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(_Params),
                                     C.POINTER(C.c_uint64)]),
@@ -152,6 +164,7 @@ _SIGNATURES = {
     "s2m_write_to_file": (C.c_int, [C.POINTER(_MeshOut), C.c_char_p]),
     "s2m_format_f32": (C.c_int, [C.c_float, C.c_char_p]),
     "s2m_check_sincos": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "s2m_host_pool_stats": (None, [C.c_void_p]),
     "s2m_axis_table": (C.c_uint32, [C.c_float, C.c_float, C.c_uint32, C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_void_p]),
     "s2m_eval_lattice": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_Params), C.c_void_p,
@@ -310,6 +323,39 @@ class HostMesh:
             self.free()
         except Exception:
             pass
+
+
+def tuning_save(path) -> None:
+    """Write the autotuner's table as JSON (s2m_tuning_save)."""
+    _check(lib().s2m_tuning_save(str(path).encode()))
+
+
+def tuning_load(path) -> None:
+    """Merge a JSON file's entries into the autotuner's table (s2m_tuning_load)."""
+    _check(lib().s2m_tuning_load(str(path).encode()))
+
+
+def tuning_clear() -> None:
+    lib().s2m_tuning_clear()
+
+
+def host_pool_stats() -> dict:
+    """s2m_host_pool_stats: page-locked blocks allocated so far, bytes held, idle blocks
+    and idle bytes of the result memory pool."""
+    st = (C.c_uint64 * 4)()
+    lib().s2m_host_pool_stats(st)
+    return {"pinned_allocs": int(st[0]), "pinned_bytes": int(st[1]), "idle_blocks": int(st[2]),
+            "idle_bytes": int(st[3])}
+
+
+def tune_pick(ms) -> int:
+    """The autotuner's choice rule: index of the chosen timing (s2m_tune_pick)."""
+    a = (C.c_double * len(ms))(*[float(x) for x in ms])
+    return int(lib().s2m_tune_pick(a, len(ms)))
+
+
+def _tuning_dict(t: "_Tuning") -> dict:
+    return {k: getattr(t, k) for k, _ in _Tuning._fields_}
 
 
 def format_f32(x: float) -> str:
@@ -496,6 +542,20 @@ class Context:
             _check(rc)
         return [_take_mesh(outs[i], want_keys) for i in range(n)]
 
+    def mesh_batch_to_host(self, progs, params, want_keys: bool = False) -> list:
+        """s2m_mesh_batch without copies into numpy arrays: one HostMesh per job, in the
+        library's host memory until each is freed."""
+        n = len(progs)
+        hp = (C.c_void_p * n)(*[p.handle for p in progs])
+        pa = (_Params * n)(*[q.c() for q in params])
+        outs = (_MeshOut * n)()
+        rc = lib().s2m_mesh_batch(C.c_void_p(self.handle), n, hp, pa, int(want_keys), outs)
+        if rc != OK:
+            for o in outs:
+                lib().s2m_mesh_free(C.byref(o))
+            _check(rc)
+        return [HostMesh(outs[i]) for i in range(n)]
+
     def mesh_batch_device(self, progs, params) -> list:
         """One shared pass; results stay in device memory owned by the context."""
         n = len(progs)
@@ -530,6 +590,22 @@ class Context:
             err.needed = (int(dm.nvtx), int(dm.ntri))
             raise err
         return {"nvtx": int(dm.nvtx), "ntri": int(dm.ntri), "invalid_quads": int(dm.invalid_quads)}
+
+    def tune(self, prog: Program, params: Params, reps: int = 5) -> dict:
+        """s2m_tune: time the candidate shapes of this job, keep the fastest for its key."""
+        t = _Tuning()
+        _check(lib().s2m_tune(C.c_void_p(self.handle), C.c_void_p(prog.handle), C.byref(params.c()),
+                              reps, C.byref(t)))
+        return _tuning_dict(t)
+
+    def tune_batch(self, progs, params, reps: int = 5) -> dict:
+        """s2m_tune_batch: the cells per pass of this batch."""
+        n = len(progs)
+        hp = (C.c_void_p * n)(*[p.handle for p in progs])
+        pa = (_Params * n)(*[q.c() for q in params])
+        t = _Tuning()
+        _check(lib().s2m_tune_batch(C.c_void_p(self.handle), n, hp, pa, reps, C.byref(t)))
+        return _tuning_dict(t)
 
     def check_sincos(self) -> tuple:
         """s2m_check_sincos: (sine mismatches, of them NaN pairs, cosine mismatches, of

@@ -34,7 +34,7 @@ DEVICE = [
     "-fno-gpu-flush-denormals-to-zero",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-SOURCES = ["kernels.hip", "sparse.hip", "api.cpp", "hostmem.cpp", "server.cpp", "writer.cpp", "compile.cpp",
+SOURCES = ["kernels.hip", "sparse.hip", "api.cpp", "hostmem.cpp", "tune.cpp", "server.cpp", "writer.cpp", "compile.cpp",
            "frontend_parse.cpp", "frontend_check.cpp", "frontend_emit.cpp"]
 
 # Device headers a program compiled at run time includes, in order; written into one

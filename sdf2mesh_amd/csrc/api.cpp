@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -134,6 +135,7 @@ struct s2m_ctx {
     uint64_t slot_cap = 0;
     uint64_t stage_cap = 0;  // staged-cell entries of sp_stage
     int cus = 256;
+    std::string arch = "gfx";  // the device's architecture name (autotuner keys)
     PinnedBuf host_img;   // staging of the last pass's tables (kept until its sync)
     std::vector<uint8_t> dev_img;  // the tables image in tile_tab (an identical image is not re-sent)
     const void* dev_img_at = nullptr;  // ... at this device address
@@ -142,6 +144,21 @@ struct s2m_ctx {
     uint64_t last_nvtx = 0, last_ntri = 0, last_invalid = 0;
     std::vector<hipEvent_t> events;
     size_t events_used = 0;
+    // streamed readback (s2m_mesh): device-to-host copies of finished passes run on
+    // copy_stream while later passes compute on `stream`
+    hipStream_t copy_stream = nullptr;
+    PinnedBuf host_snap;                 // per pass: k_snapshot's totals and capacity use
+    std::vector<hipEvent_t> pass_done;   // per pass: recorded after its snapshot
+
+    hipEvent_t pass_event(size_t k)
+    {
+        while (pass_done.size() <= k) {
+            hipEvent_t e;
+            if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+            pass_done.push_back(e);
+        }
+        return pass_done[k];
+    }
 
     hipEvent_t next_event()
     {
@@ -439,6 +456,166 @@ struct SpRange {
     uint64_t v0, nv, t0, nt, invalid;
 };
 
+// Host arrays one job's mesh is copied into while the job runs (s2m_mesh).  After each
+// pass of the sparse pipeline (or slab of the dense one) the compute stream records a
+// snapshot of the totals (k_snapshot, into mapped host memory) and an event;
+// readback_passes waits for each event in turn and queues the copy of that pass's
+// vertices and triangles on the copy stream, so the copy of pass k overlaps the
+// computation of pass k + 1.  Vertices and triangles are appended in pass order and never
+// rewritten, so a finished pass's range is final.
+struct Readback {
+    int want_keys = 0;
+    float* vtx = nullptr;
+    uint32_t* tri = nullptr;
+    uint64_t* keys = nullptr;
+    uint8_t* signs = nullptr;
+    uint64_t vcap = 0, tcap = 0;      // entries of the host arrays
+    uint64_t v_done = 0, t_done = 0;  // copies queued so far
+    bool stopped = false;             // a pass overflowed a capacity: the attempt repeats
+    hipEvent_t first = nullptr, last = nullptr;  // profiling: around the copies
+
+    void release()
+    {
+        host_release(vtx);
+        host_release(tri);
+        host_release(keys);
+        host_release(signs);
+        vtx = nullptr;
+        tri = nullptr;
+        keys = nullptr;
+        signs = nullptr;
+        vcap = tcap = 0;
+    }
+};
+
+// Before an attempt of the job: host arrays of at least vcap vertices and tcap triangles
+// (kept from an earlier call when large enough), no copy queued, no earlier attempt's copy
+// still reading the device buffers the attempt will write.
+int readback_begin(s2m_ctx* ctx, Readback& rb, uint64_t vcap, uint64_t tcap)
+{
+    HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
+    if (rb.vcap < vcap || !rb.vtx) {
+        host_release(rb.vtx);
+        host_release(rb.keys);
+        host_release(rb.signs);
+        rb.keys = nullptr;
+        rb.signs = nullptr;
+        rb.vtx = static_cast<float*>(host_alloc(std::max<uint64_t>(vcap, 1) * 6 * sizeof(float)));
+        if (rb.want_keys) {
+            rb.keys = static_cast<uint64_t*>(host_alloc(std::max<uint64_t>(vcap, 1) * sizeof(uint64_t)));
+            rb.signs = static_cast<uint8_t*>(host_alloc(std::max<uint64_t>(vcap, 1)));
+        }
+        rb.vcap = vcap;
+    }
+    if (rb.tcap < tcap || !rb.tri) {
+        host_release(rb.tri);
+        rb.tri = static_cast<uint32_t*>(host_alloc(std::max<uint64_t>(tcap, 1) * 3 * sizeof(uint32_t)));
+        rb.tcap = tcap;
+    }
+    if (!rb.vtx || !rb.tri || (rb.want_keys && (!rb.keys || !rb.signs)))
+        return fail(S2M_E_OOM, "host allocation of %llu vertices / %llu triangles failed",
+                    (unsigned long long)vcap, (unsigned long long)tcap);
+    rb.v_done = rb.t_done = 0;
+    rb.stopped = false;
+    rb.first = rb.last = nullptr;
+    return S2M_OK;
+}
+
+// After pass (or slab) k's kernels: its snapshot and event.
+int readback_mark(s2m_ctx* ctx, const uint64_t* counters, uint32_t k)
+{
+    void* dev = nullptr;
+    HIP_TRY(hipHostGetDevicePointer(&dev, ctx->host_snap.p, 0));
+    HIP_TRY(launch_snapshot(ctx->stream, counters, static_cast<uint64_t*>(dev) + 4 * (size_t)k));
+    hipEvent_t e = ctx->pass_event(k);
+    if (!e) return fail(S2M_E_HIP, "event creation failed");
+    HIP_TRY(hipEventRecord(e, ctx->stream));
+    return S2M_OK;
+}
+
+// Queue the copies of [rb.v_done, nv) vertices and [rb.t_done, nt) triangles.
+int readback_copy(s2m_ctx* ctx, Readback& rb, uint64_t nv, uint64_t nt)
+{
+    hipStream_t s = ctx->copy_stream;
+    if (ctx->profiling && !rb.first) {
+        rb.first = ctx->next_event();
+        if (rb.first) HIP_TRY(hipEventRecord(rb.first, s));
+    }
+    if (nv > rb.v_done) {
+        const uint64_t v0 = rb.v_done, m = nv - v0;
+        HIP_TRY(hipMemcpyAsync(rb.vtx + 6 * v0, ctx->vtx.as<float>() + 6 * v0, m * 6 * sizeof(float),
+                               hipMemcpyDeviceToHost, s));
+        if (rb.want_keys) {
+            HIP_TRY(hipMemcpyAsync(rb.keys + v0, ctx->keys.as<uint64_t>() + v0, m * sizeof(uint64_t),
+                                   hipMemcpyDeviceToHost, s));
+            HIP_TRY(hipMemcpyAsync(rb.signs + v0, ctx->signs.as<uint8_t>() + v0, m, hipMemcpyDeviceToHost, s));
+        }
+        rb.v_done = nv;
+    }
+    if (nt > rb.t_done) {
+        const uint64_t t0 = rb.t_done;
+        HIP_TRY(hipMemcpyAsync(rb.tri + 3 * t0, ctx->tri.as<uint32_t>() + 3 * t0, (nt - t0) * 3 * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, s));
+        rb.t_done = nt;
+    }
+    return S2M_OK;
+}
+
+// Passes 0 .. npass-1 are queued, each followed by readback_mark: as each one finishes,
+// queue the copies of its vertices and triangles.  A pass whose totals or capacity use
+// exceed the attempt's capacities stops the copies (the attempt will repeat).
+int readback_passes(s2m_ctx* ctx, Readback& rb, uint32_t npass, uint64_t vcap, uint64_t tcap, uint64_t slot_cap,
+                    uint64_t stage_cap)
+{
+    const volatile uint64_t* snap = ctx->host_snap.as<uint64_t>();
+    for (uint32_t k = 0; k < npass && !rb.stopped; ++k) {
+        HIP_TRY(hipEventSynchronize(ctx->pass_done[k]));
+        const uint64_t nv = snap[4 * k], nt = snap[4 * k + 1], slots = snap[4 * k + 2], staged = snap[4 * k + 3];
+        if (nv > vcap || nt > tcap || nv > rb.vcap || nt > rb.tcap || slots > slot_cap || staged > stage_cap ||
+            nv < rb.v_done || nt < rb.t_done) {
+            rb.stopped = true;
+            break;
+        }
+        HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, ctx->pass_done[k], 0));
+        int rc = readback_copy(ctx, rb, nv, nt);
+        if (rc) return rc;
+    }
+    return S2M_OK;
+}
+
+// After a successful attempt with nv vertices and nt triangles: the rest copied (when a
+// pass stopped the streaming) and every copy finished.
+int readback_finish(s2m_ctx* ctx, Readback& rb, uint64_t nv, uint64_t nt)
+{
+    if (!rb.vtx || nv > rb.vcap || nt > rb.tcap) {  // not streamed, or larger than its arrays
+        HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
+        rb.release();
+        const int brc = readback_begin(ctx, rb, std::max<uint64_t>(nv, 1), std::max<uint64_t>(nt, 1));
+        if (brc) return brc;
+    }
+    int rc = readback_copy(ctx, rb, nv, nt);
+    if (rc) return rc;
+    if (rb.first) {
+        rb.last = ctx->next_event();
+        if (rb.last) HIP_TRY(hipEventRecord(rb.last, ctx->copy_stream));
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
+    if (rb.first && rb.last) {
+        float ms = 0;
+        HIP_TRY(hipEventElapsedTime(&ms, rb.first, rb.last));
+        ctx->timing.d2h_ms = ms;
+    }
+    return S2M_OK;
+}
+
+// Passes of a single sparse job whose mesh is streamed to host memory: enough that the
+// copy of one pass overlaps the next pass's work (copying a CSG-64 512^3 mesh over PCIe
+// takes about twice its device time), few enough that every pass keeps the device busy.
+uint32_t stream_passes(uint32_t tile_layers)
+{
+    return std::max<uint32_t>(1, std::min<uint32_t>(4, tile_layers / 16));
+}
+
 // Is the program a left fold of library primitives (P0 P1 op1 P2 op2 ...) short enough to
 // be staged (sparse.hip, kStageOps = 128)?  Every pruned sub-program of one is one too:
 // pruning keeps either the accumulated value or the right operand, a single primitive.
@@ -470,11 +647,37 @@ bool left_fold_program(const std::vector<s2m_op>& ops)
     return true;
 }
 
+// Autotuner key of a single job: device, program class, grid and the flags that change
+// the work (corner mode, lag, normals, dense evaluation).
+std::string tune_key(const s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Job& j)
+{
+    const char* cls = prog->module ? "compiled"
+                      : ((p->flags & S2M_DENSE_EVAL) || !tape_prunable(prog->ops)) ? "dense"
+                      : union_fold_program(prog->ops)                             ? "fold"
+                      : left_fold_program(prog->ops)                              ? "linear"
+                                                                                  : "tape";
+    char buf[192];
+    std::snprintf(buf, sizeof buf, "%s:%d|%s|%ux%ux%u|%u", ctx->arch.c_str(), ctx->cus, cls, j.res[0], j.res[1],
+                  j.res[2],
+                  p->flags & (S2M_CORNERS_EXACT | S2M_COMPAT_DROP_LAST_SLICE | S2M_NO_NORMALS | S2M_DENSE_EVAL));
+    return buf;
+}
+
+// Autotuner key of a batch: device, job count and total cells (to a power of two).
+std::string tune_batch_key(const s2m_ctx* ctx, uint32_t n, uint64_t cells)
+{
+    uint32_t lg = 0;
+    while (lg < 63 && (1ull << (lg + 1)) <= cells) ++lg;
+    char buf[128];
+    std::snprintf(buf, sizeof buf, "%s:%d|batch|%u|2^%u", ctx->arch.c_str(), ctx->cus, n, lg);
+    return buf;
+}
+
 // Mesh `n` jobs with the sparse pipeline.  n == 1 may be cut into z-slabs; a batch runs
 // as one pass with every job whole.  Capacities (slots, vertices, triangles) start from
 // estimates and grow if the device counters report more.
 int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const s2m_params* const* ps,
-               const Job* jobs, std::vector<SpRange>& out)
+               const Job* jobs, std::vector<SpRange>& out, Readback* rb = nullptr)
 {
     out.assign(n, SpRange{});
     std::vector<SpTables> tabs(n);
@@ -516,6 +719,14 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     uint32_t layers = tabs[0].nt[2];
     if (n == 1) {
         if (ps[0]->slab) layers = std::max<uint32_t>(1, (ps[0]->slab + kTile - 1) / kTile);
+        else if (rb) {  // streamed to host memory: the autotuner's passes, else the rule's
+            TuneChoice tc;
+            uint32_t passes = stream_passes(tabs[0].nt[2]);
+            if (tune_lookup(tune_key(ctx, progs[0], ps[0], jobs[0]), &tc) && tc.stream_passes)
+                passes = tc.stream_passes;
+            passes = std::min(passes, std::max<uint32_t>(tabs[0].nt[2], 1));
+            layers = (tabs[0].nt[2] + passes - 1) / passes;
+        }
         layers = std::min(layers, std::max<uint32_t>(tabs[0].nt[2], 1));
         auto pass_bytes = [&](uint64_t L) {
             const uint64_t t = (uint64_t)tabs[0].nt[0] * tabs[0].nt[1] * L;
@@ -524,6 +735,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         while (layers > 1 && pass_bytes(layers) > (1ull << 30)) layers = (layers + 1) / 2;
     }
     const uint32_t npass = n == 1 ? (tabs[0].nt[2] + layers - 1) / std::max(layers, 1u) : 1;
+    if (npass == 1) rb = nullptr;  // nothing to overlap: the caller copies the finished mesh
     const uint64_t tiles_pass = n == 1 ? (uint64_t)tabs[0].nt[0] * tabs[0].nt[1] * layers : ntiles_all;
     const uint64_t rows_pass = n == 1 ? (uint64_t)layers * kTile * jobs[0].res[1] : nrows_all;
     if (tiles_pass > 0x7fffffffull || rows_pass > 0xffffffffull ||
@@ -641,12 +853,13 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     HIP_TRY(ctx->sp_super_act.ensure(nsup_alloc * sizeof(uint64_t)));
 
     // a closed surface crosses O(res^2) cells: start from 6 per row-plane cell
-    uint64_t want_v = std::min<uint64_t>(cells_all, std::max<uint64_t>(65536, 6 * planes_sum));
-    want_v = std::max<uint64_t>(want_v, ctx->vcap);
+    const uint64_t est_v = std::min<uint64_t>(cells_all, std::max<uint64_t>(65536, 6 * planes_sum));
+    uint64_t want_v = std::max<uint64_t>(est_v, ctx->vcap);
     uint64_t want_t = std::max<uint64_t>(3 * want_v, ctx->tcap);
     uint64_t want_s = std::min<uint64_t>(tiles_pass, std::max<uint64_t>({4096, tiles_pass / 4, ctx->slot_cap}));
     const uint64_t tile_blocks = (uint64_t)std::max(1, ctx->cus) * kTileBlocksPerCU;
     uint64_t stage_cap = ctx->stage_cap;
+    if (rb) HIP_TRY(ctx->host_snap.ensure((size_t)npass * 4 * sizeof(uint64_t)));
 
     for (int attempt = 0; attempt < 4; ++attempt) {
         HIP_TRY(ctx->vtx.ensure(want_v * 6 * sizeof(float)));
@@ -670,6 +883,10 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         ctx->vcap = std::max(ctx->vcap, want_v);
         ctx->tcap = std::max(ctx->tcap, want_t);
         ctx->slot_cap = std::max(ctx->slot_cap, want_s);
+        if (rb) {  // host arrays for this job's own estimate, not the context's largest job
+            const int rrc = readback_begin(ctx, *rb, std::min(want_v, est_v), std::min(want_t, 3 * est_v));
+            if (rrc) return rrc;
+        }
 
         MeshBufs b{};
         b.vtx = ctx->vtx.as<float>();
@@ -761,6 +978,10 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
                                          ctx->profiling ? m : nullptr));
             if (ctx->profiling) ev.insert(ev.end(), m, m + kSparseMarks);
             tiles_total += P.ntiles;
+            if (rb) {
+                const int rrc = readback_mark(ctx, b.counters, pass);
+                if (rrc) return rrc;
+            }
         }
         // the last pass folded and the counters and job rows written to mapped host memory
         const uint32_t nback = kNumCounters + n * kJobOut;
@@ -772,6 +993,10 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         if (ctx->profiling) {
             ev_end = ctx->next_event();
             HIP_TRY(hipEventRecord(ev_end, ctx->stream));
+        }
+        if (rb) {  // copies of each pass's mesh while the later passes run
+            const int rrc = readback_passes(ctx, *rb, npass, want_v, want_t, want_s, stage_cap);
+            if (rrc) return rrc;
         }
         const uint64_t* cnt = ctx->host_back.as<uint64_t>();
         const uint64_t* jo = cnt + kNumCounters;
@@ -904,7 +1129,7 @@ DenseLayout dense_layout(const Job& j, uint32_t T, bool lattice)
 // larger than the caller's buffers fails with S2M_E_OOM and its size in last_nvtx /
 // last_ntri instead of being repeated with larger buffers).
 int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Job& j,
-            const float* h_corners, const FixedBufs* fx = nullptr)
+            const float* h_corners, const FixedBufs* fx = nullptr, Readback* rb = nullptr)
 {
     int rc = S2M_OK;
     ctx->timing = s2m_timing{};
@@ -918,7 +1143,16 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
     if (fx && h_corners) return fail(S2M_E_ARG, "corner dumps are not contoured without allocation");
     const bool prof = ctx->profiling && !fx;  // events are created on first use
 
-    DenseLayout L = dense_layout(j, pick_slab(p, j), !h_corners);
+    uint32_t slab = pick_slab(p, j);
+    if (prog && !fx && p->slab == 0) {  // the autotuner's slab for this job, if it chose one
+        TuneChoice tc;
+        if (tune_lookup(tune_key(ctx, prog, p, j), &tc) && tc.dense_slab) {
+            s2m_params q = *p;
+            q.slab = tc.dense_slab;
+            slab = pick_slab(&q, j);
+        }
+    }
+    DenseLayout L = dense_layout(j, slab, !h_corners);
     if (fx) {  // the largest slab (at most the usual one) whose buffers fit the scratch
         while (L.T > 1 && L.carved() > fx->scratch_bytes) L = dense_layout(j, L.T - 1, true);
         if (L.carved() > fx->scratch_bytes)
@@ -974,14 +1208,17 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
 
     // Output capacity: a closed surface crosses O(res^2) cells; start from 12 per
     // row-plane cell and grow once if the counters report more.
-    uint64_t want_v = std::min<uint64_t>(total_cells, std::max<uint64_t>(65536, 12 * plane));
-    want_v = std::max<uint64_t>(want_v, ctx->vcap);
+    const uint64_t est_v = std::min<uint64_t>(total_cells, std::max<uint64_t>(65536, 12 * plane));
+    uint64_t want_v = std::max<uint64_t>(est_v, ctx->vcap);
     uint64_t want_t = std::max<uint64_t>(3 * want_v, ctx->tcap);
     if (fx) {
         want_v = fx->vcap;
         want_t = fx->tcap;
     }
     const bool normals = prog && !(p->flags & S2M_NO_NORMALS);
+    const uint32_t nslabs = (j.nz_cells + T - 1) / T;
+    if (nslabs == 1) rb = nullptr;  // nothing to overlap: the caller copies the finished mesh
+    if (rb) HIP_TRY(ctx->host_snap.ensure((size_t)nslabs * 4 * sizeof(uint64_t)));
 
     for (int attempt = 0; attempt < (fx ? 1 : 3); ++attempt) {
         MeshBufs b{};
@@ -1008,6 +1245,10 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
         b.bcount = bcount;
         b.boffset = boffset;
         b.actbits = actbits;
+        if (rb) {  // host arrays for this job's own estimate, not the context's largest job
+            const int rrc = readback_begin(ctx, *rb, std::min(want_v, est_v), std::min(want_t, 3 * est_v));
+            if (rrc) return rrc;
+        }
 
         HIP_TRY(hipMemsetAsync(b.counters, 0, kNumCounters * sizeof(uint64_t), ctx->stream));
         HIP_TRY(hipMemsetAsync(vidx, 0xff, (size_t)plane * sizeof(uint32_t), ctx->stream));
@@ -1079,6 +1320,10 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             HIP_TRY(launch_scan(ctx->stream, b, nb, 1));
             HIP_TRY(launch_quads(ctx->stream, a, b, 1));
             if (m[4]) HIP_TRY(hipEventRecord(m[4], ctx->stream));
+            if (rb) {  // the slab's vertices and triangles are final
+                const int rrc = readback_mark(ctx, b.counters, (uint32_t)slabs);
+                if (rrc) return rrc;
+            }
             if (z1 < j.nz_cells) {  // last map plane becomes the previous plane
                 HIP_TRY(hipMemcpyAsync(vidx, vidx + (size_t)nzc * plane, plane * sizeof(uint32_t),
                                        hipMemcpyDeviceToDevice, ctx->stream));
@@ -1089,6 +1334,10 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
         if (prof) {
             ev_end = ctx->next_event();
             HIP_TRY(hipEventRecord(ev_end, ctx->stream));
+        }
+        if (rb) {  // copies of each slab's mesh while the later slabs run
+            const int rrc = readback_passes(ctx, *rb, (uint32_t)slabs, want_v, want_t, ~0ull, ~0ull);
+            if (rrc) return rrc;
         }
         uint64_t cnt[kNumCounters];
         HIP_TRY(hipMemcpyAsync(cnt, b.counters, sizeof cnt, hipMemcpyDeviceToHost, ctx->stream));
@@ -1232,11 +1481,16 @@ s2m_ctx* s2m_create(int device)
     s2m_ctx* ctx = new s2m_ctx();
     ctx->device = device;
     hipDeviceProp_t prop;
-    if (hipGetDeviceProperties(&prop, device) == hipSuccess && prop.multiProcessorCount > 0)
-        ctx->cus = prop.multiProcessorCount;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
+        if (prop.multiProcessorCount > 0) ctx->cus = prop.multiProcessorCount;
+        const char* colon = std::strchr(prop.gcnArchName, ':');
+        ctx->arch.assign(prop.gcnArchName, colon ? (size_t)(colon - prop.gcnArchName) : std::strlen(prop.gcnArchName));
+    }
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         fail(S2M_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return nullptr;
     }
@@ -1253,6 +1507,7 @@ void s2m_destroy(s2m_ctx* ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->copy_stream);
     for (DevBuf* b : {&ctx->lattice, &ctx->vidx, &ctx->bcount, &ctx->boffset, &ctx->actbits, &ctx->check, &ctx->counters,
                       &ctx->tables, &ctx->vtx, &ctx->keys, &ctx->signs, &ctx->tri, &ctx->corners,
                       &ctx->tile_tab, &ctx->tile_lists, &ctx->tile_lens,
@@ -1264,7 +1519,10 @@ void s2m_destroy(s2m_ctx* ctx)
         b->release();
     ctx->host_img.release();
     ctx->host_back.release();
+    ctx->host_snap.release();
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+    for (hipEvent_t e : ctx->pass_done) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->copy_stream);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1351,10 +1609,15 @@ uint32_t s2m_program_tape(const s2m_program* prog, s2m_op* ops, uint32_t cap)
     return prog->n;
 }
 
-int s2m_mesh_device(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params,
-                    s2m_device_mesh* out)
+}  // extern "C"
+
+namespace {
+
+// Mesh one job: the tile-sparse pipeline for prunable tapes, the dense slabs otherwise.
+// With rb, the mesh is also streamed into rb's host arrays pass by pass.
+int mesh_one(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, Readback* rb)
 {
-    if (!ctx || !prog || !params || !out) return fail(S2M_E_ARG, "NULL argument");
+    if (!ctx || !prog || !params) return fail(S2M_E_ARG, "NULL argument");
     if (prog->device != ctx->device) return fail(S2M_E_ARG, "program built for another device");
     HIP_TRY(hipSetDevice(ctx->device));
     Job j;
@@ -1366,10 +1629,20 @@ int s2m_mesh_device(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* par
     const uint64_t cells = (uint64_t)j.res[0] * j.res[1] * j.nz_cells;
     if (j.tiled && cells > 0) {
         std::vector<SpRange> r;
-        rc = run_sparse(ctx, 1, &prog, &params, &j, r);
-    } else {
-        rc = run_job(ctx, prog, params, j, nullptr);
+        return run_sparse(ctx, 1, &prog, &params, &j, r, rb);
     }
+    return run_job(ctx, prog, params, j, nullptr, nullptr, rb);
+}
+
+}  // namespace
+
+extern "C" {
+
+int s2m_mesh_device(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params,
+                    s2m_device_mesh* out)
+{
+    if (!out) return fail(S2M_E_ARG, "NULL argument");
+    int rc = mesh_one(ctx, prog, params, nullptr);
     if (rc) return rc;
     out->vtx = ctx->vtx.as<float>();
     out->tri = ctx->tri.as<uint32_t>();
@@ -1386,13 +1659,24 @@ int s2m_mesh(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, in
 {
     if (!out) return fail(S2M_E_ARG, "out is NULL");
     std::memset(out, 0, sizeof *out);
-    s2m_device_mesh dm;
-    int rc = s2m_mesh_device(ctx, prog, params, &dm);
-    if (rc) return rc;
-    double d2h = 0;
-    rc = copy_out(ctx, want_keys, out, &d2h);
-    ctx->timing.d2h_ms = d2h;
-    return rc;
+    // the mesh is copied to host memory pass by pass while the job runs
+    Readback rb;
+    rb.want_keys = want_keys;
+    int rc = mesh_one(ctx, prog, params, &rb);
+    if (rc == S2M_OK) rc = readback_finish(ctx, rb, ctx->last_nvtx, ctx->last_ntri);
+    if (rc) {
+        (void)hipStreamSynchronize(ctx->copy_stream);
+        rb.release();
+        return rc;
+    }
+    out->vtx = rb.vtx;
+    out->tri = rb.tri;
+    out->cell_keys = rb.keys;
+    out->signs = rb.signs;
+    out->nvtx = ctx->last_nvtx;
+    out->ntri = ctx->last_ntri;
+    out->invalid_quads = ctx->last_invalid;
+    return S2M_OK;
 }
 
 namespace {
@@ -1400,36 +1684,89 @@ namespace {
 // Copy job ranges of the context's output buffers to fresh host meshes.
 int copy_ranges(s2m_ctx* ctx, const SpRange* r, uint32_t n, int want_keys, s2m_mesh_out* outs)
 {
+    // One page-locked block per array for the whole pass (one DMA copy each): every job's
+    // array points inside it, and the block is recycled after the last job's mesh is freed.
+    // Jobs without vertices (or triangles) get arrays of their own.
+    uint64_t v_lo = ~0ull, v_hi = 0, t_lo = ~0ull, t_hi = 0;
+    uint32_t v_refs = 0, t_refs = 0;
     for (uint32_t i = 0; i < n; ++i) {
+        std::memset(&outs[i], 0, sizeof outs[i]);
+        if (r[i].nv) {
+            v_lo = std::min(v_lo, r[i].v0);
+            v_hi = std::max(v_hi, r[i].v0 + r[i].nv);
+            ++v_refs;
+        }
+        if (r[i].nt) {
+            t_lo = std::min(t_lo, r[i].t0);
+            t_hi = std::max(t_hi, r[i].t0 + r[i].nt);
+            ++t_refs;
+        }
+    }
+    float* V = nullptr;
+    uint32_t* Tr = nullptr;
+    uint64_t* K = nullptr;
+    uint8_t* S = nullptr;
+    if (v_refs) {
+        V = static_cast<float*>(host_alloc_shared((v_hi - v_lo) * 6 * sizeof(float), v_refs));
+        if (want_keys) {
+            K = static_cast<uint64_t*>(host_alloc_shared((v_hi - v_lo) * sizeof(uint64_t), v_refs));
+            S = static_cast<uint8_t*>(host_alloc_shared(v_hi - v_lo, v_refs));
+        }
+    }
+    if (t_refs) Tr = static_cast<uint32_t*>(host_alloc_shared((t_hi - t_lo) * 3 * sizeof(uint32_t), t_refs));
+    bool ok = (!v_refs || (V && (!want_keys || (K && S)))) && (!t_refs || Tr);
+    uint32_t v_given = 0, t_given = 0;  // references of the blocks handed to jobs
+    for (uint32_t i = 0; i < n && ok; ++i) {
         s2m_mesh_out* o = &outs[i];
-        std::memset(o, 0, sizeof *o);
+        v_given += r[i].nv ? 1u : 0u;
+        t_given += r[i].nt ? 1u : 0u;
         o->nvtx = r[i].nv;
         o->ntri = r[i].nt;
         o->invalid_quads = r[i].invalid;
-        o->vtx = (float*)host_alloc(std::max<uint64_t>(r[i].nv, 1) * 6 * sizeof(float));
-        o->tri = (uint32_t*)host_alloc(std::max<uint64_t>(r[i].nt, 1) * 3 * sizeof(uint32_t));
+        o->vtx = r[i].nv ? V + 6 * (r[i].v0 - v_lo) : static_cast<float*>(host_alloc(6 * sizeof(float)));
+        o->tri = r[i].nt ? Tr + 3 * (r[i].t0 - t_lo) : static_cast<uint32_t*>(host_alloc(3 * sizeof(uint32_t)));
         if (want_keys) {
-            o->cell_keys = (uint64_t*)host_alloc(std::max<uint64_t>(r[i].nv, 1) * sizeof(uint64_t));
-            o->signs = (uint8_t*)host_alloc(std::max<uint64_t>(r[i].nv, 1));
+            o->cell_keys = r[i].nv ? K + (r[i].v0 - v_lo) : static_cast<uint64_t*>(host_alloc(sizeof(uint64_t)));
+            o->signs = r[i].nv ? S + (r[i].v0 - v_lo) : static_cast<uint8_t*>(host_alloc(1));
         }
-        if (!o->vtx || !o->tri || (want_keys && (!o->cell_keys || !o->signs)))
-            return fail(S2M_E_OOM, "host allocation failed");
-        if (r[i].nv) {
-            HIP_TRY(hipMemcpyAsync(o->vtx, ctx->vtx.as<float>() + 6 * r[i].v0, r[i].nv * 24,
-                                   hipMemcpyDeviceToHost, ctx->stream));
-            if (want_keys) {
-                HIP_TRY(hipMemcpyAsync(o->cell_keys, ctx->keys.as<uint64_t>() + r[i].v0, r[i].nv * 8,
-                                       hipMemcpyDeviceToHost, ctx->stream));
-                HIP_TRY(hipMemcpyAsync(o->signs, ctx->signs.as<uint8_t>() + r[i].v0, r[i].nv,
-                                       hipMemcpyDeviceToHost, ctx->stream));
-            }
-        }
-        if (r[i].nt)
-            HIP_TRY(hipMemcpyAsync(o->tri, ctx->tri.as<uint32_t>() + 3 * r[i].t0, r[i].nt * 12,
-                                   hipMemcpyDeviceToHost, ctx->stream));
+        ok = o->vtx && o->tri && (!want_keys || (o->cell_keys && o->signs));
     }
+    if (!ok) {  // release every reference the jobs hold, then the blocks' unused ones
+        for (uint32_t i = 0; i < n; ++i) s2m_mesh_free(&outs[i]);
+        for (; v_given < v_refs; ++v_given) {
+            host_release(V);
+            host_release(K);
+            host_release(S);
+        }
+        for (; t_given < t_refs; ++t_given) host_release(Tr);
+        return fail(S2M_E_OOM, "host allocation failed");
+    }
+    if (v_refs) {
+        HIP_TRY(hipMemcpyAsync(V, ctx->vtx.as<float>() + 6 * v_lo, (v_hi - v_lo) * 6 * sizeof(float),
+                               hipMemcpyDeviceToHost, ctx->stream));
+        if (want_keys) {
+            HIP_TRY(hipMemcpyAsync(K, ctx->keys.as<uint64_t>() + v_lo, (v_hi - v_lo) * sizeof(uint64_t),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(S, ctx->signs.as<uint8_t>() + v_lo, v_hi - v_lo, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        }
+    }
+    if (t_refs)
+        HIP_TRY(hipMemcpyAsync(Tr, ctx->tri.as<uint32_t>() + 3 * t_lo, (t_hi - t_lo) * 3 * sizeof(uint32_t),
+                               hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return S2M_OK;
+}
+
+// A job of a batch meshed on its own with the dense slabs rather than in the batch's tile
+// pass: compiled programs (no tape for the pass), S2M_DENSE_EVAL, and programs interval
+// pruning cannot cut (the Mandelbulb) on grids above 128^3 cells, where the dense slabs
+// evaluate each lattice point once instead of every point of every tile's box.
+bool dense_job(const s2m_program* prog, const s2m_params& p, const Job& j)
+{
+    if (prog->module || (p.flags & S2M_DENSE_EVAL)) return true;
+    const uint64_t cells = (uint64_t)j.res[0] * j.res[1] * j.nz_cells;
+    return !tape_prunable(prog->ops) && cells > (1ull << 21);
 }
 
 int prepare_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const s2m_params* params,
@@ -1458,8 +1795,8 @@ int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
     if (rc) return rc;
     HIP_TRY(hipSetDevice(ctx->device));
     ctx->timing = s2m_timing{};
-    // consecutive passes of at most ~64M cells and 2M tiles (empty grids cost nothing);
-    // compiled programs have no tape for the sparse pass and are meshed one at a time
+    // consecutive passes of at most pass_cells cells and 2M tiles (empty grids cost
+    // nothing); jobs for the dense path are meshed one at a time (dense_job)
     uint32_t i = 0;
     s2m_timing sum{};
     auto add_timing = [&sum](const s2m_timing& t) {
@@ -1477,8 +1814,17 @@ int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
         sum.tiles_active += t.tiles_active;
         sum.eval_flops += t.eval_flops;
     };
+    uint64_t pass_cells = 64ull << 20;  // the rule; the autotuner may have chosen for this batch
+    {
+        uint64_t all = 0;
+        for (uint32_t q = 0; q < n; ++q)
+            if (!dense_job(progs[q], params[q], jobs[q]))
+                all += (uint64_t)jobs[q].res[0] * jobs[q].res[1] * jobs[q].nz_cells;
+        TuneChoice tc;
+        if (tune_lookup(tune_batch_key(ctx, n, all), &tc) && tc.batch_cells) pass_cells = tc.batch_cells;
+    }
     for (uint32_t q = 0; q < n; ++q)
-        if (progs[q]->module) {
+        if (dense_job(progs[q], params[q], jobs[q])) {
             rc = s2m_mesh(ctx, progs[q], &params[q], want_keys, &outs[q]);
             if (rc) return rc;
             add_timing(ctx->timing);
@@ -1494,15 +1840,16 @@ int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
             const Job& j = jobs[k];
             const uint64_t c = (uint64_t)j.res[0] * j.res[1] * j.nz_cells;
             const uint64_t t = ((j.res[0] + 7) / 8ull) * ((j.res[1] + 7) / 8ull) * ((j.nz_cells + 7) / 8ull);
-            if (!idx.empty() && (cells + c > (64ull << 20) || tiles + t > (2ull << 20))) break;
-            if (c && !progs[k]->module) {
+            const bool batched = c && !dense_job(progs[k], params[k], j);  // else: meshed above
+            if (batched && !idx.empty() && (cells + c > pass_cells || tiles + t > (2ull << 20))) break;
+            if (batched) {
                 pp.push_back(progs[k]);
                 qq.push_back(&params[k]);
                 jj.push_back(j);
                 idx.push_back(k);
+                cells += c;
+                tiles += t;
             }
-            cells += c;
-            tiles += t;
             ++k;
         }
         if (!idx.empty()) {
@@ -1612,6 +1959,126 @@ void s2m_mesh_free(s2m_mesh_out* out)
     host_release(out->cell_keys);
     host_release(out->signs);
     std::memset(out, 0, sizeof *out);
+}
+
+}  // extern "C"
+
+namespace {
+
+using TuneClock = std::chrono::steady_clock;
+
+// Median wall time (ms) of `reps` runs of `run` after one untimed run; run returns a status.
+template <class F>
+int time_median(uint32_t reps, F run, double* ms)
+{
+    int rc = run();
+    std::vector<double> t;
+    for (uint32_t r = 0; r < reps && rc == S2M_OK; ++r) {
+        const TuneClock::time_point t0 = TuneClock::now();
+        rc = run();
+        t.push_back(std::chrono::duration<double, std::milli>(TuneClock::now() - t0).count());
+    }
+    if (rc) return rc;
+    std::sort(t.begin(), t.end());
+    *ms = t[t.size() / 2];
+    return S2M_OK;
+}
+
+// Time each candidate with `run` under the calling thread's override, pick one with
+// tune_pick, store it under `key`.
+template <class F>
+int tune_candidates(const std::string& key, const std::vector<TuneChoice>& cand, uint32_t reps, F run,
+                    s2m_tuning* out)
+{
+    std::vector<double> med(cand.size(), 0.0);
+    int rc = S2M_OK;
+    for (size_t k = 0; k < cand.size() && rc == S2M_OK; ++k) {
+        tune_override(&cand[k]);
+        rc = time_median(reps, run, &med[k]);
+        tune_override(nullptr);
+    }
+    if (rc) return rc;
+    const uint32_t best = tune_pick(med.data(), (uint32_t)med.size());
+    tune_store(key, cand[best], med[best]);
+    out->stream_passes = cand[best].stream_passes;
+    out->dense_slab = cand[best].dense_slab;
+    out->batch_cells = cand[best].batch_cells;
+    out->ms = med[best];
+    out->measured = (uint32_t)cand.size();
+    return S2M_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int s2m_tune(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, uint32_t reps, s2m_tuning* out)
+{
+    if (!ctx || !prog || !params || !out) return fail(S2M_E_ARG, "NULL argument");
+    std::memset(out, 0, sizeof *out);
+    Job j;
+    int rc = prepare_job(params, j);
+    if (rc) return rc;
+    s2m_params p = *params;
+    p.slab = 0;  // an explicit slab would bypass the choice being timed
+    const std::string key = tune_key(ctx, prog, &p, j);
+    TuneChoice tc;
+    double ms = 0;
+    if (tune_cached(key, &tc, &ms)) {
+        out->stream_passes = tc.stream_passes;
+        out->dense_slab = tc.dense_slab;
+        out->batch_cells = tc.batch_cells;
+        out->ms = ms;
+        return S2M_OK;
+    }
+    std::vector<TuneChoice> cand;
+    const bool tiled = !prog->module && !(p.flags & S2M_DENSE_EVAL) && tape_prunable(prog->ops);
+    if (tiled) {  // z-passes of the streamed readback
+        const uint32_t layers = std::max<uint32_t>(1, (j.nz_cells + kTile - 1) / kTile);
+        for (uint32_t c : {1u, 2u, 3u, 4u, 6u, 8u})
+            if (c <= layers) cand.push_back(TuneChoice{c, 0, 0});
+    } else {  // cell planes per dense slab
+        for (uint32_t c : {4u, 8u, 16u, 32u, 64u, 128u})
+            if (c <= std::max<uint32_t>(j.nz_cells, 4)) cand.push_back(TuneChoice{0, c, 0});
+    }
+    auto run = [&]() {
+        s2m_mesh_out m;
+        const int r = s2m_mesh(ctx, prog, &p, 0, &m);
+        s2m_mesh_free(&m);
+        return r;
+    };
+    return tune_candidates(key, cand, std::max<uint32_t>(reps, 3), run, out);
+}
+
+int s2m_tune_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const s2m_params* params,
+                   uint32_t reps, s2m_tuning* out)
+{
+    if (!out) return fail(S2M_E_ARG, "NULL argument");
+    std::memset(out, 0, sizeof *out);
+    std::vector<Job> jobs;
+    int rc = prepare_batch(ctx, n, progs, params, jobs);
+    if (rc) return rc;
+    uint64_t all = 0;
+    for (uint32_t q = 0; q < n; ++q)
+        if (!dense_job(progs[q], params[q], jobs[q])) all += (uint64_t)jobs[q].res[0] * jobs[q].res[1] * jobs[q].nz_cells;
+    const std::string key = tune_batch_key(ctx, n, all);
+    TuneChoice tc;
+    double ms = 0;
+    if (tune_cached(key, &tc, &ms)) {
+        out->batch_cells = tc.batch_cells;
+        out->ms = ms;
+        return S2M_OK;
+    }
+    std::vector<TuneChoice> cand;
+    for (uint64_t c : {16ull << 20, 32ull << 20, 64ull << 20, 128ull << 20, 256ull << 20})
+        cand.push_back(TuneChoice{0, 0, c});
+    std::vector<s2m_mesh_out> outs(n);
+    auto run = [&]() {
+        const int r = s2m_mesh_batch(ctx, n, progs, params, 0, outs.data());
+        for (s2m_mesh_out& m : outs) s2m_mesh_free(&m);
+        return r;
+    };
+    return tune_candidates(key, cand, std::max<uint32_t>(reps, 3), run, out);
 }
 
 uint32_t s2m_axis_table(float bmin, float bmax, uint32_t res, int exact_corners, float* coord,

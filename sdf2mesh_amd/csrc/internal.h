@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <string>
+
 #include "../../include/sdf2mesh.h"
 
 namespace s2m {
@@ -210,6 +212,9 @@ hipError_t launch_sparse_job_begin(hipStream_t s, uint64_t* counters, uint32_t n
 hipError_t launch_sparse_pass(hipStream_t s, const SpPass& P, const MeshBufs& b, int cus,
                               uint32_t tile_blocks, bool general_tiles, bool fold_jobs, bool other_jobs,
                               int boundary_plane, hipEvent_t* marks);
+// out[0..4) (a device address of mapped host memory) = the vertex and triangle totals so
+// far and the current pass's slot and staging use, after the kernels queued before it.
+hipError_t launch_snapshot(hipStream_t s, const uint64_t* counters, uint64_t* out);
 // Contour corner values given per cell (test hook): cells of the slab read their 8
 // corners from `corners` (8 floats per cell, cells in (z,y,x) order from plane z0).
 hipError_t launch_cells_from_corners(hipStream_t s, const SlabArgs& a, const float* corners,
@@ -227,7 +232,27 @@ hipError_t noalloc_scratch(int device, void** ptr);
 // Host memory for a result array: page-locked and recycled when large, malloc otherwise.
 // Returned to the pool (or freed) by host_release; any other pointer is passed to free().
 void* host_alloc(size_t bytes);
+// A page-locked block shared by `refs` results: each points somewhere inside it and is
+// released with host_release; the block returns to the pool after the last release.
+void* host_alloc_shared(size_t bytes, uint32_t refs);
 void host_release(void* p);
+
+// ---- autotuner table (tune.cpp) ----
+// Shapes the meshing calls use instead of their rules (0 = no choice for that knob).
+struct TuneChoice {
+    uint32_t stream_passes = 0;  // z-passes of a single tile-sparse job streamed to host memory
+    uint32_t dense_slab = 0;     // cell planes per slab of the dense path
+    uint64_t batch_cells = 0;    // most cells per pass of a batch
+};
+// The calling thread's override while s2m_tune times a candidate, else the table's entry.
+bool tune_lookup(const std::string& key, TuneChoice* out);
+// The table's entry only, with the time it was chosen for.
+bool tune_cached(const std::string& key, TuneChoice* out, double* ms);
+void tune_store(const std::string& key, const TuneChoice& c, double ms);
+void tune_override(const TuneChoice* c);  // NULL: back to the table
+// Index of the chosen candidate: the first one, replaced by any later one faster by more
+// than 2 % (so a choice depends only on the timings and their order).
+uint32_t tune_pick(const double* ms, uint32_t n);
 
 // ---- host helpers (api.cpp) for the server ----
 // A program from a gfx950 code object holding s2m_c_eval_lattice and s2m_c_normals.

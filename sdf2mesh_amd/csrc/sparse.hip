@@ -1670,6 +1670,19 @@ __global__ void k_sp_pass_begin(uint64_t* counters, uint32_t* queues, uint32_t* 
     for (uint32_t i = thread_id(); i < nbsum; i += blockDim.x) bsum[i * kChunkSumStride] = 0;
 }
 
+// After a pass (or a dense slab): the totals so far and the pass's use of the slot and
+// staging capacities, written to `out` (host memory mapped for the device) so that the
+// host can copy the pass's finished vertices and triangles while later passes run.
+__global__ void k_snapshot(const uint64_t* counters, uint64_t* out)
+{
+    if (thread_id() == 0 && blockIdx.x == 0) {
+        out[0] = counters[kVtxTotal];
+        out[1] = counters[kTriTotal];
+        out[2] = counters[kSlotsUsed];
+        out[3] = counters[kStageUsed];
+    }
+}
+
 __global__ __launch_bounds__(256) void k_sp_job_begin(uint64_t* counters, uint32_t n, uint32_t* queues,
                                                       uint32_t* bsum, uint32_t nbsum, uint64_t* stats,
                                                       uint32_t nstats)
@@ -1711,6 +1724,12 @@ hipError_t launch_sparse_job_end(hipStream_t s, uint64_t* counters, uint32_t n, 
                                  const uint64_t* stats, uint32_t nstats)
 {
     hipLaunchKernelGGL(k_sp_job_end, dim3(1), dim3(256), 0, s, counters, n, out, stats, nstats);
+    return hipGetLastError();
+}
+
+hipError_t launch_snapshot(hipStream_t s, const uint64_t* counters, uint64_t* out)
+{
+    hipLaunchKernelGGL(k_snapshot, dim3(1), dim3(64), 0, s, counters, out);
     return hipGetLastError();
 }
 
