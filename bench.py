@@ -39,7 +39,9 @@ import argparse
 import json
 import os
 import platform
+import queue
 import sys
+import threading
 import time
 from pathlib import Path
 
@@ -336,24 +338,34 @@ def run_serve(args) -> dict:
 
     rates = [float(r) for r in args.serve_rates.split(",")]
     per_rate = []
-    with s2m.Server(0, workers=args.serve_workers, max_batch=args.serve_batch,
-                    want_keys=False) as srv:
+    with s2m.Server(0, workers=args.serve_workers, max_batch=args.serve_batch, want_keys=False,
+                    dense_workers=args.serve_dense_workers) as srv:
         for t, p in zip(tapes[:8], params[:8]):  # warm every worker's buffers
-            srv.take(srv.submit(t, p))
+            srv.take(srv.submit(t, p), host=True)[0].free()
         for rate in rates:
             n = min(n_max, max(60, int(rate * 4)))  # ~4 s of arrivals at low rates
             arr = np.asarray(poisson_arrivals(n, rate, seed=int(rate)))
-            tickets = []
+            tickets = queue.Queue()
+            stats = []
+
+            def consume():  # the client: takes each result as it finishes, in order
+                for _ in range(n):
+                    hm, st = srv.take(tickets.get(), host=True)
+                    stats.append(st)
+                    hm.free()
+
+            taker = threading.Thread(target=consume)
             t0 = time.perf_counter()
+            taker.start()
             for a, t, p in zip(arr, tapes, params):
                 now = time.perf_counter() - t0
                 if a > now:
                     time.sleep(a - now)
-                tickets.append(srv.submit(t, p))
-            res = [srv.take(k) for k in tickets]
+                tickets.put(srv.submit(t, p))
+            taker.join()
             elapsed = time.perf_counter() - t0
-            lat = np.asarray([st["latency_ms"] for _, st in res])
-            bs = np.asarray([st["batch_size"] for _, st in res])
+            lat = np.asarray([st["latency_ms"] for st in stats])
+            bs = np.asarray([st["batch_size"] for st in stats])
             done, solo_lat = 0.0, []
             for a, sv in zip(arr, service[:n]):
                 done = max(done, a) + sv
@@ -374,7 +386,8 @@ def run_serve(args) -> dict:
         "value": top["jobs_per_s"], "unit": "jobs/s", "rate": top["rate"],
         "ms_per_step": round(1e3 / top["jobs_per_s"], 3),
         "config": {"workload": "serve", "requests_per_rate": "min(%d, max(60, 4 x rate))" % n_max,
-                   "workers": args.serve_workers, "max_batch": args.serve_batch,
+                   "workers": args.serve_workers, "dense_workers": args.serve_dense_workers,
+                   "max_batch": args.serve_batch,
                    "mesh_in": "host memory"},
         "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
         "one_at_a_time_jobs_per_s": round(solo_jobs_s, 1),
@@ -430,6 +443,7 @@ def main() -> int:
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rates", default="10,30,100,300,1000,2000")
     ap.add_argument("--serve-workers", type=int, default=2)
+    ap.add_argument("--serve-dense-workers", type=int, default=1)
     ap.add_argument("--serve-batch", type=int, default=64)
     ap.add_argument("--dense", action="store_true",
                     help="evaluate every lattice point with the full program (no tile pruning)")

@@ -69,6 +69,7 @@ enum s2m_status {
  *   CYLINDER  f[0..2]=c, f[3]=h, f[4]=r    sdf3d_cylinder(p - c, h, r) primitives.wgsl:13-17
  *   CAPSULE   f[0..2]=a, f[3..5]=b, f[6]=r sdf3d_capsule(p, a, b, r)   primitives.wgsl:19-25
  *   MANDELBULB f[0]=power, f[1]=bailout, f[2]=offset, iarg = iterations | (swap_yz<<16)
+ *             | (trig<<17)
  *             the distance estimator of examples/mandelmesh.frag:3-28
  *   UNION min(d1,d2)  INTERSECTION max(d1,d2)  SUBTRACTION max(-d1,d2)
  *   SMOOTH_UNION / SMOOTH_INTERSECTION / SMOOTH_SUBTRACTION  f[0]=k,  sdf_op.wgsl:7-23
@@ -98,6 +99,11 @@ typedef struct s2m_op {
 #define S2M_MAX_TAPE 4096u  /* longest accepted tape                  */
 #define S2M_MAX_STACK 16u   /* deepest accepted operand stack         */
 #define S2M_MANDEL_SWAP_YZ (1u << 16) /* apply p.xyz = p.xzy first (mandelmesh.frag:4) */
+#define S2M_MANDEL_TRIG (1u << 17)    /* evaluate the escape step as written, with atan, asin,
+                                         pow, sin and cos; by default a power-8 step takes
+                                         the same values by complex squaring, without
+                                         transcendentals (fewer roundings, several times
+                                         fewer instructions) */
 
 /* ------------------------------------------------------------------------------------
  * Meshing parameters: the AppState uniform (main.rs:28-33) as plain fields.
@@ -187,6 +193,7 @@ typedef struct s2m_timing {
     uint64_t tiles_active;   /* tiles whose lattice points were evaluated            */
     uint64_t eval_flops;     /* algorithmic flops executed on the lattice            */
     double prune_ms;         /* interval pruning of tiles and bricks (pruned path)   */
+    uint64_t graph_launches; /* 1 when the job's kernels ran as a captured graph      */
 } s2m_timing;
 
 typedef struct s2m_ctx s2m_ctx;
@@ -345,22 +352,28 @@ void s2m_tuning_clear(void);
 uint32_t s2m_tune_pick(const double* ms, uint32_t n);
 
 /* ---- serving ------------------------------------------------------------------- */
-/* A request loop with continuous batching.  `workers` threads each own a context (HIP
- * stream + resident device buffers) on `device`.  Submitted requests wait in a FIFO
- * queue; an idle worker admits requests from the head of the queue while their
- * estimated device memory fits in what is left of `hbm_budget` (and at most
- * `max_batch` of them), meshes them in one batched pass, then returns their memory to
- * the budget.  Requests arriving while others are on the device are admitted as soon as
- * a worker and memory free up, and passes of different workers overlap on the GPU.
- * Every result is identical to meshing the request alone with s2m_mesh. */
+/* A request loop with continuous batching.  `workers` + `dense_workers` threads each own
+ * a context (HIP streams + resident device buffers) on `device`.  A submitted request
+ * waits in one of two FIFO lanes: the dense lane (compiled programs, S2M_DENSE_EVAL, and
+ * Mandelbulbs above 128^3 cells, meshed alone on the dense path with each slab's mesh
+ * copied to host memory while the next one computes) or the pass lane (everything else).
+ * An idle worker admits requests from the head of its lane (from the other lane when its
+ * own is empty) while their estimated device memory fits in what is left of
+ * `hbm_budget`: up to `max_batch` pass-lane requests meshed in one batched pass, or one
+ * dense request; then it returns their memory to the budget.  Requests arriving while
+ * others are on the device are admitted as soon as a worker and memory free up, and the
+ * passes and dense jobs of different workers overlap on the GPU, so a large dense job
+ * does not hold back the small requests behind it.  Every result is identical to meshing
+ * the request alone with s2m_mesh. */
 typedef struct s2m_server s2m_server;
 
 typedef struct s2m_server_config {
     int device;
     uint64_t hbm_budget; /* bytes of device memory admitted requests may use (0: 16 GiB) */
     uint32_t max_batch;  /* requests per pass (0: 64)                                  */
-    uint32_t workers;    /* concurrent passes, one HIP stream each (0: 2)              */
+    uint32_t workers;    /* workers preferring the pass lane (0: 2)                    */
     int want_keys;       /* also return cell keys and sign bits                         */
+    uint32_t dense_workers; /* workers preferring the dense lane (0: 1)                 */
 } s2m_server_config;
 
 typedef struct s2m_request_stats {
@@ -375,6 +388,11 @@ s2m_server* s2m_server_create(const s2m_server_config* cfg);
  * memory exceeds the whole budget, S2M_E_VALIDATION for an invalid tape. */
 int s2m_server_submit(s2m_server* srv, const s2m_op* ops, uint32_t n_ops, const s2m_params* params,
                       uint64_t* ticket);
+/* Queue a request for any program: a tape program's tape is copied as s2m_server_submit
+ * does; a compiled program is meshed as it is and must stay valid until its request has
+ * been taken.  S2M_E_ARG for a program of another device. */
+int s2m_server_submit_program(s2m_server* srv, const s2m_program* prog, const s2m_params* params,
+                              uint64_t* ticket);
 /* Take a finished request's mesh: S2M_E_BUSY while it is queued or running; with
  * wait != 0 blocks until it is done.  Each ticket can be taken once. */
 int s2m_server_take(s2m_server* srv, uint64_t ticket, int wait, s2m_mesh_out* out,

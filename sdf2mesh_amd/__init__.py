@@ -96,7 +96,7 @@ class _Tuning(C.Structure):
 
 class _ServerConfig(C.Structure):
     _fields_ = [("device", C.c_int), ("hbm_budget", C.c_uint64), ("max_batch", C.c_uint32),
-                ("workers", C.c_uint32), ("want_keys", C.c_int)]
+                ("workers", C.c_uint32), ("want_keys", C.c_int), ("dense_workers", C.c_uint32)]
 
 
 class _RequestStats(C.Structure):
@@ -109,7 +109,7 @@ class _Timing(C.Structure):
                 ("normal_ms", C.c_double), ("quad_ms", C.c_double), ("d2h_ms", C.c_double),
                 ("eval_launches", C.c_uint64), ("lattice_points", C.c_uint64),
                 ("slabs", C.c_uint64), ("tiles_total", C.c_uint64), ("tiles_active", C.c_uint64),
-                ("eval_flops", C.c_uint64), ("prune_ms", C.c_double)]
+                ("eval_flops", C.c_uint64), ("prune_ms", C.c_double), ("graph_launches", C.c_uint64)]
 
 
 _lib_handle: Optional[C.CDLL] = None
@@ -155,6 +155,8 @@ _SIGNATURES = {
     "s2m_server_create": (C.c_void_p, [C.POINTER(_ServerConfig)]),
     "s2m_server_submit": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.POINTER(_Params),
                                     C.POINTER(C.c_uint64)]),
+    "s2m_server_submit_program": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_Params),
+                                            C.POINTER(C.c_uint64)]),
     "s2m_server_take": (C.c_int, [C.c_void_p, C.c_uint64, C.c_int, C.POINTER(_MeshOut),
                                   C.POINTER(_RequestStats)]),
     "s2m_server_request_bytes": (C.c_uint64, [C.c_void_p, C.c_uint32, C.POINTER(_Params)]),
@@ -573,7 +575,7 @@ class Context:
         _check(lib().s2m_mesh_device(C.c_void_p(self.handle), C.c_void_p(prog.handle),
                                      C.byref(p), C.byref(dm)))
         return {"nvtx": int(dm.nvtx), "ntri": int(dm.ntri), "invalid_quads": int(dm.invalid_quads),
-                "vtx_ptr": dm.vtx, "tri_ptr": dm.tri}
+                "vtx_ptr": dm.vtx, "tri_ptr": dm.tri, "keys_ptr": dm.cell_keys, "signs_ptr": dm.signs}
 
     def mesh_device_noalloc(self, prog: Program, params: Params, vtx: int, cell_keys: int,
                             signs: int, vcap: int, tri: int, tcap: int) -> dict:
@@ -662,8 +664,8 @@ class Server:
     """Continuous-batching request loop (s2m_server_*): submit tapes, take meshes."""
 
     def __init__(self, device: int = 0, hbm_budget: int = 0, max_batch: int = 0, workers: int = 0,
-                 want_keys: bool = True):
-        cfg = _ServerConfig(device, hbm_budget, max_batch, workers, int(want_keys))
+                 want_keys: bool = True, dense_workers: int = 0):
+        cfg = _ServerConfig(device, hbm_budget, max_batch, workers, int(want_keys), dense_workers)
         h = lib().s2m_server_create(C.byref(cfg))
         if not h:
             raise S2MError(E_HIP, lib().s2m_last_error().decode(errors="replace"))
@@ -671,15 +673,23 @@ class Server:
         self.want_keys = want_keys
 
     def submit(self, program, params: Params) -> int:
-        t = program.tape() if isinstance(program, tape.Node) else np.ascontiguousarray(program, TAPE_DTYPE)
+        """Queue a tape (tape.Node or TAPE_DTYPE array) or a Program (a compiled one must
+        stay open until its request has been taken)."""
         p = params.c()
         ticket = C.c_uint64()
+        if isinstance(program, Program):
+            _check(lib().s2m_server_submit_program(C.c_void_p(self.handle), C.c_void_p(program.handle),
+                                                   C.byref(p), C.byref(ticket)))
+            return int(ticket.value)
+        t = program.tape() if isinstance(program, tape.Node) else np.ascontiguousarray(program, TAPE_DTYPE)
         _check(lib().s2m_server_submit(C.c_void_p(self.handle), _ptr(t), len(t), C.byref(p),
                                        C.byref(ticket)))
         return int(ticket.value)
 
-    def take(self, ticket: int, wait: bool = True):
-        """(TriangleMesh, stats dict), or None if not finished and wait is False."""
+    def take(self, ticket: int, wait: bool = True, host: bool = False):
+        """(TriangleMesh, stats dict), or None if not finished and wait is False.  With
+        host=True the mesh is a HostMesh: views of the library's host memory, no copies
+        (free() hands the memory back for later results)."""
         out = _MeshOut()
         st = _RequestStats()
         rc = lib().s2m_server_take(C.c_void_p(self.handle), ticket, int(wait), C.byref(out),
@@ -690,7 +700,7 @@ class Server:
             lib().s2m_mesh_free(C.byref(out))
             _check(rc)
         stats = {k: getattr(st, k) for k, _ in _RequestStats._fields_}
-        return _take_mesh(out, self.want_keys), stats
+        return (HostMesh(out) if host else _take_mesh(out, self.want_keys)), stats
 
     @staticmethod
     def request_bytes(program, params: Params) -> int:

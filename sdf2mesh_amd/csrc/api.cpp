@@ -148,6 +148,11 @@ struct s2m_ctx {
     // copy_stream while later passes compute on `stream`
     hipStream_t copy_stream = nullptr;
     PinnedBuf host_snap;                 // per pass: k_snapshot's totals and capacity use
+    // captured graphs of repeated one-pass jobs, most recent first, and the keys of jobs
+    // seen once (run_graph)
+    static constexpr size_t kGraphs = 4, kSeen = 8;
+    std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
+    std::vector<std::string> graph_seen;
     std::vector<hipEvent_t> pass_done;   // per pass: recorded after its snapshot
 
     hipEvent_t pass_event(size_t k)
@@ -647,6 +652,52 @@ bool left_fold_program(const std::vector<s2m_op>& ops)
     return true;
 }
 
+// Run `enqueue` (which queues a job's kernels on the context's stream) as a graph: replay
+// the context's graph captured for `key`; capture it and replay it when `key` was seen
+// before (a repeated job); else queue the kernels as usual and remember `key`.  The
+// context keeps the kGraphs graphs and kSeen keys used last.  A graph holds kernels only
+// (every clear of the path is a kernel).
+template <class F>
+int run_graph(s2m_ctx* ctx, const std::string& key, F enqueue)
+{
+    auto& G = ctx->graphs;
+    for (size_t i = 0; i < G.size(); ++i)
+        if (G[i].first == key) {
+            std::rotate(G.begin(), G.begin() + i, G.begin() + i + 1);  // most recent first
+            HIP_TRY(hipGraphLaunch(G[0].second, ctx->stream));
+            ctx->timing.graph_launches = 1;
+            return S2M_OK;
+        }
+    auto& S = ctx->graph_seen;
+    auto it = std::find(S.begin(), S.end(), key);
+    if (it == S.end()) {
+        S.insert(S.begin(), key);
+        if (S.size() > s2m_ctx::kSeen) S.pop_back();
+        return enqueue();
+    }
+    S.erase(it);
+    HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue();
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    if (rc || e != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : fail(S2M_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ie = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) return fail(S2M_E_HIP, "graph instantiation failed: %s", hipGetErrorString(ie));
+    if (G.size() == s2m_ctx::kGraphs) {
+        (void)hipGraphExecDestroy(G.back().second);
+        G.pop_back();
+    }
+    G.insert(G.begin(), {key, x});
+    HIP_TRY(hipGraphLaunch(x, ctx->stream));
+    ctx->timing.graph_launches = 1;
+    return S2M_OK;
+}
+
 // Autotuner key of a single job: device, program class, grid and the flags that change
 // the work (corner mode, lag, normals, dense evaluation).
 std::string tune_key(const s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Job& j)
@@ -900,20 +951,26 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
 
         uint64_t* job_out = b.counters + kNumCounters;
         HIP_TRY(ctx->sp_block_stats.ensure(tile_blocks * 3 * sizeof(uint64_t)));
+        // the last pass folded and the counters and job rows written to mapped host memory
+        const uint32_t nback = kNumCounters + n * kJobOut;
+        HIP_TRY(ctx->host_back.ensure(nback * sizeof(uint64_t)));
+        void* back_dev = nullptr;
+        HIP_TRY(hipHostGetDevicePointer(&back_dev, ctx->host_back.p, 0));
+
+        ctx->events_used = 0;
+        std::vector<hipEvent_t> ev;
+        hipEvent_t ev_start = nullptr, ev_end = nullptr;
+        uint64_t tiles_total = 0;
+        auto enqueue = [&]() -> int {  // the job's kernels, in stream order
         HIP_TRY(launch_sparse_job_begin(ctx->stream, b.counters, kNumCounters + n * kJobOut,
                                         ctx->sp_queues.as<uint32_t>(), b.bcount, 2 * nsb_max,
                                         ctx->sp_block_stats.as<uint64_t>(), (uint32_t)tile_blocks));
         if (n == 1 && npass > 1)
             HIP_TRY(hipMemsetAsync(ctx->sp_boundary.p, 0xff, plane_max * sizeof(uint32_t), ctx->stream));
-
-        ctx->events_used = 0;
-        std::vector<hipEvent_t> ev;
-        hipEvent_t ev_start = nullptr, ev_end = nullptr;
         if (ctx->profiling) {
             ev_start = ctx->next_event();
             HIP_TRY(hipEventRecord(ev_start, ctx->stream));
         }
-        uint64_t tiles_total = 0;
         for (uint32_t pass = 0; pass < npass; ++pass) {
             SpPass P{};
             P.jobs = n == 1 ? ddesc + pass : ddesc;
@@ -983,16 +1040,41 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
                 if (rrc) return rrc;
             }
         }
-        // the last pass folded and the counters and job rows written to mapped host memory
-        const uint32_t nback = kNumCounters + n * kJobOut;
-        HIP_TRY(ctx->host_back.ensure(nback * sizeof(uint64_t)));
-        void* back_dev = nullptr;
-        HIP_TRY(hipHostGetDevicePointer(&back_dev, ctx->host_back.p, 0));
         HIP_TRY(launch_sparse_job_end(ctx->stream, b.counters, nback, static_cast<uint64_t*>(back_dev),
                                       ctx->sp_block_stats.as<uint64_t>(), (uint32_t)tile_blocks));
         if (ctx->profiling) {
             ev_end = ctx->next_event();
             HIP_TRY(hipEventRecord(ev_end, ctx->stream));
+        }
+        return S2M_OK;
+        };
+        // A one-pass job without streamed readback or profiling events (its kernels and
+        // their arguments fixed by the tables image, the buffers and the capacities) is
+        // captured into a graph the second time the same job comes, and the graph replayed
+        // from then on: one launch instead of a dozen.
+        ctx->timing.graph_launches = 0;
+        if (!rb && !ctx->profiling && npass == 1) {
+            std::string key(reinterpret_cast<const char*>(img), img_bytes);
+            auto put = [&key](const void* q, size_t m) { key.append(static_cast<const char*>(q), m); };
+            for (const DevBuf* d : {&ctx->vtx, &ctx->keys, &ctx->signs, &ctx->tri, &ctx->counters, &ctx->bcount,
+                                    &ctx->sp_queues, &ctx->sp_block_stats, &ctx->tile_tab, &ctx->tile_lists,
+                                    &ctx->tile_lens, &ctx->tile_flops, &ctx->sp_slot_of_tile, &ctx->sp_tile_of_slot,
+                                    &ctx->sp_job_of_slot, &ctx->sp_cellbits, &ctx->sp_stage, &ctx->sp_slot_stage,
+                                    &ctx->sp_slot_live, &ctx->sp_slot_hdr, &ctx->sp_slot_nb, &ctx->sp_slot_rec,
+                                    &ctx->sp_slot_rows, &ctx->sp_rows, &ctx->sp_super_iv, &ctx->sp_super_lists,
+                                    &ctx->sp_super_lens, &ctx->sp_super_act, &ctx->sp_boundary})
+                put(&d->p, sizeof d->p);
+            const uint64_t scal[] = {want_v, want_t, want_s, stage_cap, tile_blocks, nsb_max, list_stride,
+                                     prune_depth, box_stride, (uint64_t)general_tiles, (uint64_t)fold_jobs,
+                                     (uint64_t)other_jobs, n, (uint64_t)(uintptr_t)back_dev};
+            put(scal, sizeof scal);
+            const int grc = run_graph(ctx, key, enqueue);
+            if (grc) return grc;
+            tiles_total = n == 1 ? (uint64_t)desc[0].ntx * desc[0].nty * ((desc[0].nz + kTile - 1) / kTile)
+                                 : ntiles_all;
+        } else {
+            const int erc = enqueue();
+            if (erc) return erc;
         }
         if (rb) {  // copies of each pass's mesh while the later passes run
             const int rrc = readback_passes(ctx, *rb, npass, want_v, want_t, want_s, stage_cap);
@@ -1522,6 +1604,7 @@ void s2m_destroy(s2m_ctx* ctx)
     ctx->host_snap.release();
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->pass_done) (void)hipEventDestroy(e);
+    for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
     (void)hipStreamDestroy(ctx->copy_stream);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -1762,11 +1845,16 @@ int copy_ranges(s2m_ctx* ctx, const SpRange* r, uint32_t n, int want_keys, s2m_m
 // pass: compiled programs (no tape for the pass), S2M_DENSE_EVAL, and programs interval
 // pruning cannot cut (the Mandelbulb) on grids above 128^3 cells, where the dense slabs
 // evaluate each lattice point once instead of every point of every tile's box.
+bool dense_rule(bool compiled, bool prunable, uint32_t flags, const Job& j)
+{
+    if (compiled || (flags & S2M_DENSE_EVAL)) return true;
+    const uint64_t cells = (uint64_t)j.res[0] * j.res[1] * j.nz_cells;
+    return !prunable && cells > (1ull << 21);
+}
+
 bool dense_job(const s2m_program* prog, const s2m_params& p, const Job& j)
 {
-    if (prog->module || (p.flags & S2M_DENSE_EVAL)) return true;
-    const uint64_t cells = (uint64_t)j.res[0] * j.res[1] * j.nz_cells;
-    return !tape_prunable(prog->ops) && cells > (1ull << 21);
+    return dense_rule(prog->module != nullptr, tape_prunable(prog->ops), p.flags, j);
 }
 
 int prepare_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const s2m_params* params,
@@ -1813,6 +1901,7 @@ int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
         sum.tiles_total += t.tiles_total;
         sum.tiles_active += t.tiles_active;
         sum.eval_flops += t.eval_flops;
+        sum.graph_launches += t.graph_launches;
     };
     uint64_t pass_cells = 64ull << 20;  // the rule; the autotuner may have chosen for this batch
     {
@@ -2240,5 +2329,27 @@ int job_geometry(const s2m_params* p, uint32_t res[3], uint32_t* nz_cells)
 }
 
 int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
+bool dense_request(const s2m_program* prog, const s2m_op* ops, uint32_t n, const s2m_params* p)
+{
+    Job j;
+    if (!p || prepare_job(p, j) != S2M_OK) return false;
+    bool prunable = true;
+    for (uint32_t i = 0; ops && i < n; ++i)
+        if (ops[i].code == S2M_OP_MANDELBULB) prunable = false;
+    return dense_rule(prog && prog->module, prunable, p->flags, j);
+}
+
+uint64_t dense_request_bytes(const s2m_params* p)
+{
+    Job j;
+    if (!p || prepare_job(p, j) != S2M_OK) return 0;
+    const uint64_t plane = (uint64_t)j.res[0] * j.res[1];
+    const uint64_t est_v = std::min<uint64_t>(plane * j.nz_cells, std::max<uint64_t>(65536, 12 * plane));
+    const DenseLayout L = dense_layout(j, pick_slab(p, j), true);
+    return L.carved() + est_v * (24 + 8 + 1) + 3 * est_v * 12;
+}
+
+int program_device(const s2m_program* prog) { return prog->device; }
 
 }  // namespace s2m

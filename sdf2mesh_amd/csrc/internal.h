@@ -262,5 +262,11 @@ hipStream_t context_stream(s2m_ctx* ctx);
 bool tape_valid(const s2m_op* ops, uint32_t n);
 int job_geometry(const s2m_params* p, uint32_t res[3], uint32_t* nz_cells);
 int set_error(int code, const char* msg);
+// Whether a request of the program (or tape `ops`) meshes alone on the dense path, by the
+// rule s2m_mesh_batch uses (compiled, S2M_DENSE_EVAL, or a Mandelbulb above 128^3).
+bool dense_request(const s2m_program* prog, const s2m_op* ops, uint32_t n, const s2m_params* p);
+// Estimated device bytes of a dense job (working buffers of its slabs + outputs).
+uint64_t dense_request_bytes(const s2m_params* p);
+int program_device(const s2m_program* prog);
 
 }  // namespace s2m

@@ -360,8 +360,18 @@ __global__ __launch_bounds__(256) void k_cell_vertices(SlabArgs a, MeshBufs b, i
 // ---------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_count_cols(SlabArgs a, uint32_t nzl, uint64_t* __restrict__ bits)
 {
-    const uint32_t x = blockIdx.x * kBlock + threadIdx.x, y = blockIdx.y;
-    const uint32_t x0 = blockIdx.x * kBlock + (threadIdx.x & ~63u);  // the wave's first cell
+    // Workgroups go to the device's 8 XCDs in turn (block b to XCD b % 8), and each XCD
+    // has its own L2.  Cell rows y and y + 1 share a lattice row (the upper face of one,
+    // the lower face of the other), so each XCD walks one contiguous band of rows: block b
+    // takes row-chunk (b % 8) * per + b / 8.  Neighbouring rows then run on one XCD and
+    // the shared lattice row is fetched from HBM once instead of once per XCD.
+    const uint32_t nxc = (a.rx + kBlock - 1) / kBlock, nblk = nxc * a.ry;
+    const uint32_t per = (nblk + 7u) / 8u;
+    const uint32_t lin = (blockIdx.x % 8u) * per + blockIdx.x / 8u;
+    if (lin >= nblk) return;  // the launch rounds up to a multiple of 8 blocks
+    const uint32_t xc = lin % nxc, y = lin / nxc;
+    const uint32_t x = xc * kBlock + threadIdx.x;
+    const uint32_t x0 = xc * kBlock + (threadIdx.x & ~63u);  // the wave's first cell
     if (x0 >= a.rx) return;  // whole wave past the row (wave-uniform)
     const bool in = x < a.rx;
     const uint32_t xi0 = in ? a.lox[x] : 0u, xi1 = in ? a.hix[x] : 0u;
@@ -700,8 +710,8 @@ hipError_t launch_cells(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int
         hipLaunchKernelGGL(k_cell_vertices, dim3(1024), dim3(kBlock), 0, s, a, b, write_zero_normals);
     } else {
         const uint32_t nzl = a.ncells / (a.rx * a.ry);
-        hipLaunchKernelGGL(k_count_cols, dim3((a.rx + kBlock - 1) / kBlock, a.ry), dim3(kBlock), 0, s, a, nzl,
-                           b.actbits);
+        const uint32_t nblk = (a.rx + kBlock - 1) / kBlock * a.ry;  // one block per 256 cells of a row
+        hipLaunchKernelGGL(k_count_cols, dim3((nblk + 7u) / 8u * 8u), dim3(kBlock), 0, s, a, nzl, b.actbits);
         hipLaunchKernelGGL(k_popc, dim3((nb + kBlock - 1) / kBlock), dim3(kBlock), 0, s, b.actbits, nb, b.bcount);
     }
     return hipGetLastError();

@@ -31,6 +31,7 @@ SPHERE, BOX, TORUS, CYLINDER, CAPSULE, MANDELBULB = 1, 2, 3, 4, 5, 6
 UNION, INTERSECTION, SUBTRACTION = 16, 17, 18
 SMOOTH_UNION, SMOOTH_INTERSECTION, SMOOTH_SUBTRACTION = 19, 20, 21
 MANDEL_SWAP_YZ = 1 << 16
+MANDEL_TRIG = 1 << 17  # the escape step as written (atan, asin, pow, sin, cos)
 
 _OP_WGSL = {
     SMOOTH_UNION: "sdf_op_smooth_union",
@@ -216,9 +217,12 @@ def capsule(a, b, r) -> Prim:
 
 
 def mandelbulb(iterations: int = 8, power: float = 8.0, bailout: float = 2.0,
-               offset: float = 0.003, swap_yz: bool = True) -> Prim:
+               offset: float = 0.003, swap_yz: bool = True, trig: bool = False) -> Prim:
+    """The estimator of examples/mandelmesh.frag; trig=True evaluates its escape step as
+    written (else a power-8 step is computed without transcendentals, same function)."""
     return Prim(MANDELBULB, [power, bailout, offset],
-                iarg=(iterations & 0xFFFF) | (MANDEL_SWAP_YZ if swap_yz else 0))
+                iarg=(iterations & 0xFFFF) | (MANDEL_SWAP_YZ if swap_yz else 0)
+                | (MANDEL_TRIG if trig else 0))
 
 
 def union(a, b) -> Op:

@@ -77,10 +77,12 @@ def test_forced_compiled_equals_tape(ctx, name):
 
 
 def test_glsl_mandelbulb_equals_tape_op(ctx):
+    # the compiled GLSL takes the same device math calls as the tape op evaluated as
+    # written (S2M_MANDEL_TRIG): identical meshes
     print("glsl mandelbulb", flush=True)
     c = ctx.program_from_source(GOLDEN / "mandelbulb8.frag", lang="glsl", path=True)
     assert c.kind == s2m.PROGRAM_COMPILED
-    t = ctx.program(tape.mandelbulb(8))
+    t = ctx.program(tape.mandelbulb(8, trig=True))
     p = s2m.Params.reference(64, 2.5)
     same_mesh(ctx.mesh(c, p), ctx.mesh(t, p))
 

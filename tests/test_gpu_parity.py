@@ -203,10 +203,11 @@ MANDEL_ULP_FACTOR = 32    # C in |f32 - f64| <= C * (ulp(f) + s)
 MANDEL_BAIL_MARGIN = 1e-5  # points whose bailout tests all clear the threshold by this
 
 
+@pytest.mark.parametrize("form", ["squaring", "trig", "glsl"])
 @pytest.mark.parametrize("iters,res,flags", [(8, 32, s2m.COMPAT_DROP_LAST_SLICE),
                                              (8, 48, s2m.REFERENCE_COMPAT),
                                              (5, 40, s2m.REFERENCE_COMPAT)])
-def test_mandelbulb_lattice_vs_fp64(ctx, iters, res, flags):
+def test_mandelbulb_lattice_vs_fp64(ctx, iters, res, flags, form):
     # GPU fp32 (ocml transcendentals) against the fp64 oracle, at every lattice point.
     # Bound: |f32 - f64| <= C * (ulp_f32(f) + s) with C = 32, where s is the first-order
     # error scale of the evaluation (oracle.mandel_cond: the summed effect on the fp64
@@ -214,12 +215,19 @@ def test_mandelbulb_lattice_vs_fp64(ctx, iters, res, flags):
     # phi, r^8, dr, the final log, product and quotient).  It holds at every point whose
     # bailout tests all clear |r - 2| / 2 >= 1e-5; the rest (a rounding there can take the
     # other branch of `if (r > 2.0) continue;`, a jump no error bound covers) are counted
-    # and must stay under 0.1 % of the points.
+    # and must stay under 0.1 % of the points.  Three forms of the estimator: the tape op's
+    # power-8 step by complex squaring (the default), the tape op evaluated as written
+    # (S2M_MANDEL_TRIG), and the GLSL source compiled at run time.
     node = tape.mandelbulb(iters)
+    if form == "glsl":
+        prog = ctx.program_from_source(tape.mandelbulb_glsl(iters), lang="glsl")
+        assert prog.kind == s2m.PROGRAM_COMPILED
+    else:
+        prog = ctx.program(tape.mandelbulb(iters, trig=form == "trig"))
     p = s2m.Params.reference(res, 2.5, flags=flags)
     exact = bool(flags & s2m.CORNERS_EXACT)
     res = s2m.round_resolution(res) if flags & s2m.COMPAT_POW2 else res
-    lat = ctx.eval_lattice(ctx.program(node), p)
+    lat = ctx.eval_lattice(prog, p)
     cs = [s2m.axis_table(p.bb_min[a], p.bb_max[a], res, exact)[0] for a in range(3)]
     Z, Y, X = np.meshgrid(cs[2], cs[1], cs[0], indexing="ij")
     pts = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1).astype(np.float32)
@@ -233,7 +241,7 @@ def test_mandelbulb_lattice_vs_fp64(ctx, iters, res, flags):
     bound = MANDEL_ULP_FACTOR * (ulp + cond[:, 2])
     err = np.abs(g - r64)
     ratio = err[keep] / (ulp + cond[:, 2])[keep]
-    print(f"mandelbulb iters={iters} res={res}: {keep.sum()} of {fin.sum()} points checked, "
+    print(f"mandelbulb {form} iters={iters} res={res}: {keep.sum()} of {fin.sum()} points checked, "
           f"{fin.sum() - keep.sum()} within {MANDEL_BAIL_MARGIN} of a bailout test; "
           f"max err / (ulp + s) = {ratio.max():.3g}, max err {err[fin].max():.3g}", flush=True)
     assert fin.sum() - keep.sum() <= 0.001 * fin.sum()

@@ -84,6 +84,28 @@ def test_served_config5_mix(ctx):
         assert_same_mesh(results[i][0], reference_mesh(nodes[i], params[i]))
 
 
+def test_served_compiled_programs(ctx):
+    # compiled programs (hiprtc) go through s2m_server_submit_program to the dense lane,
+    # mixed with tape requests of both lanes; each result equals meshing it alone
+    from pathlib import Path
+
+    golden = Path(__file__).resolve().parent / "golden"
+    eng = ctx.program_from_source(golden / "engraved_cube.sdf3d", path=True)
+    glsl = ctx.program_from_source(golden / "mandelbulb8.frag", lang="glsl", path=True)
+    assert eng.kind == glsl.kind == s2m.PROGRAM_COMPILED
+    jobs = [(eng, s2m.Params.reference(96, 2.0)), (glsl, s2m.Params.reference(200, 2.5)),
+            (tape.csg64(), s2m.Params.reference(64, 2.0)), (tape.mandelbulb(6), s2m.Params.reference(256, 2.5)),
+            (eng, s2m.Params.reference(40, 2.0, flags=0)), (tape.random_csg(9), s2m.Params.reference(128, 2.0)),
+            (glsl, s2m.Params.reference(48, 2.5))]
+    with s2m.Server(0, workers=2, dense_workers=1, max_batch=8) as srv:
+        tickets = [srv.submit(prog, p) for prog, p in jobs]
+        results = [srv.take(t) for t in tickets]
+    for (m, st), (prog, p) in zip(results, jobs):
+        solo = ctx.mesh(prog if isinstance(prog, s2m.Program) else ctx.program(prog), p)
+        assert len(m.vertices) > 0
+        same(m, solo)
+
+
 def test_budget_rejects_oversized_request():
     with s2m.Server(0, hbm_budget=1 << 20, workers=1) as srv:
         small = s2m.Params.reference(16, 2.0)
