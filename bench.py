@@ -342,9 +342,10 @@ def run_serve(args) -> dict:
                     dense_workers=args.serve_dense_workers) as srv:
         for t, p in zip(tapes[:8], params[:8]):  # warm every worker's buffers
             srv.take(srv.submit(t, p), host=True)[0].free()
-        for rate in rates:
-            n = min(n_max, max(60, int(rate * 4)))  # ~4 s of arrivals at low rates
-            arr = np.asarray(poisson_arrivals(n, rate, seed=int(rate)))
+        for rate in rates + [float("inf")]:  # inf: every request at once (the capacity)
+            n = n_max if rate == float("inf") else min(n_max, max(60, int(rate * 4)))  # ~4 s at low rates
+            arr = (np.zeros(n) if rate == float("inf") else
+                   np.asarray(poisson_arrivals(n, rate, seed=int(rate))))
             tickets = queue.Queue()
             stats = []
 
@@ -380,6 +381,7 @@ def run_serve(args) -> dict:
                 "one_at_a_time_p50_ms": round(float(np.percentile(solo_lat, 50)), 2),
                 "one_at_a_time_p99_ms": round(float(np.percentile(solo_lat, 99)), 2),
             })
+    burst = per_rate.pop()
     top = per_rate[-1]
     solo_jobs_s = top["requests"] / float(service[: top["requests"]].sum())
     return {
@@ -393,6 +395,10 @@ def run_serve(args) -> dict:
         "one_at_a_time_jobs_per_s": round(solo_jobs_s, 1),
         "speedup_vs_one_at_a_time": round(top["jobs_per_s"] / solo_jobs_s, 2),
         "per_rate": per_rate,
+        "capacity": {"requests": burst["requests"], "jobs_per_s": burst["jobs_per_s"],
+                     "one_at_a_time_jobs_per_s": burst["one_at_a_time_jobs_per_s"],
+                     "speedup_vs_one_at_a_time": round(burst["jobs_per_s"] / burst["one_at_a_time_jobs_per_s"], 2),
+                     "p50_ms": burst["p50_ms"], "p99_ms": burst["p99_ms"], "mean_batch": burst["mean_batch"]},
     }
 
 

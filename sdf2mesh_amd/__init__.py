@@ -166,6 +166,7 @@ _SIGNATURES = {
     "s2m_write_to_file": (C.c_int, [C.POINTER(_MeshOut), C.c_char_p]),
     "s2m_format_f32": (C.c_int, [C.c_float, C.c_char_p]),
     "s2m_check_sincos": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "s2m_check_bailout": (C.c_int, [C.c_void_p, C.c_void_p]),
     "s2m_host_pool_stats": (None, [C.c_void_p]),
     "s2m_axis_table": (C.c_uint32, [C.c_float, C.c_float, C.c_uint32, C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_void_p]),
@@ -614,6 +615,13 @@ class Context:
         them NaN pairs) of sincosf against sinf / cosf over every float."""
         counts = (C.c_uint64 * 4)()
         _check(lib().s2m_check_sincos(C.c_void_p(self.handle), counts))
+        return tuple(int(c) for c in counts)
+
+    def check_bailout(self) -> tuple:
+        """s2m_check_bailout: (comparisons, disagreements) of the Mandelbulb's squared
+        bailout test r^2 > t against sqrtf(r^2) > bailout."""
+        counts = (C.c_uint64 * 2)()
+        _check(lib().s2m_check_bailout(C.c_void_p(self.handle), counts))
         return tuple(int(c) for c in counts)
 
     def eval_lattice(self, prog: Program, params: Params) -> np.ndarray:

@@ -2193,6 +2193,18 @@ int s2m_check_sincos(s2m_ctx* ctx, uint64_t counts[4])
     return S2M_OK;
 }
 
+int s2m_check_bailout(s2m_ctx* ctx, uint64_t counts[2])
+{
+    if (!ctx || !counts) return fail(S2M_E_ARG, "NULL argument");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(ctx->check.ensure(4 * sizeof(uint64_t)));
+    HIP_TRY(hipMemsetAsync(ctx->check.p, 0, 2 * sizeof(uint64_t), ctx->stream));
+    HIP_TRY(launch_bailout_check(ctx->stream, ctx->check.as<uint64_t>()));
+    HIP_TRY(hipMemcpyAsync(counts, ctx->check.p, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return S2M_OK;
+}
+
 int s2m_eval_lattice_range(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, uint32_t z0,
                            uint32_t z1, float* out, uint64_t cap, uint32_t dims[3], uint32_t* lz0)
 {

@@ -93,6 +93,34 @@ __device__ __forceinline__ float op_smooth_subtraction(float d1, float d2, float
     return wmix(d2, -d1, h) + (k * h) * (1.0f - h);
 }
 
+// 1 / sqrt(x) from the hardware reciprocal square root (about 1 ulp).  Inputs below
+// 2^-96 are scaled by 2^64 first (and the result by 2^32), so that denormal inputs get
+// their true reciprocal root instead of a flushed one; 0 gives +inf.
+__device__ __forceinline__ float rsqrt_full(float x)
+{
+    const bool tiny = x < 0x1p-96f;
+    const float y = __builtin_amdgcn_rsqf(tiny ? x * 0x1p64f : x);
+    return tiny ? y * 0x1p32f : y;
+}
+
+// The largest float t with sqrtf(t) <= b (correctly rounded square root), so that
+// sqrtf(x) > b exactly when x > t: b * b rounded, moved to that boundary by at most a few
+// ulp (the boundary (b + ulp(b) / 2)^2 lies within 1.5 ulp of b^2).  b < 0 gives a
+// negative threshold (every length exceeds b), b = +-0 gives b (every nonzero length
+// exceeds it) and NaN gives NaN (no length exceeds it), as the comparisons of lengths do.
+__device__ __forceinline__ float bailout_threshold(float b)
+{
+    if (!(b > 0.0f)) return b < 0.0f ? -1.0f : b;
+    float t = b * b;
+    for (int k = 0; k < 4 && sqrtf(t) > b; ++k) t = __uint_as_float(__float_as_uint(t) - 1u);
+    for (int k = 0; k < 4; ++k) {
+        const float up = __uint_as_float(__float_as_uint(t) + 1u);
+        if (!(sqrtf(up) <= b)) break;
+        t = up;
+    }
+    return t;
+}
+
 // examples/mandelmesh.frag:3-28 with power, bailout, offset and the iteration count as
 // operands.  A bailed-out point keeps r = length(z): the loop `continue`s, so r is
 // recomputed from the unchanged z in every remaining iteration.
@@ -108,19 +136,30 @@ __device__ __forceinline__ float sd_mandelbulb(float px, float py, float pz, flo
     float zx = px, zy = py, zz = pz;
     float r = 0.0f, dr = 1.0f;
     if (power == 8.0f && !(iarg & S2M_MANDEL_TRIG)) {
-        // The power-8 step without transcendentals.  theta = atan(zy / zx) differs from the
-        // direction angle of (zx, zy) by 0 or pi, which 8 * theta turns into a multiple of
-        // 8 pi: cos / sin of 8 theta are the components of ((zx, zy) / rho)^8 as complex
-        // numbers (rho = |(zx, zy)|).  phi = asin(zz / r) lies in [-pi/2, pi/2], so
-        // (cos phi, sin phi) = (rho / r, zz / r) and (cos 8 phi, sin 8 phi) is its 8th
-        // power.  Three complex squarings each; r^8 and r^7 by products.  The same function
-        // as the written form, with fewer roundings (its error against the fp64 estimator is
-        // bounded as for the written form).  rho = 0 gives NaN, as atan(0 / 0) does.
+        // The power-8 step without transcendentals or divisions.  theta = atan(zy / zx)
+        // differs from the direction angle of (zx, zy) by 0 or pi, which 8 * theta turns
+        // into a multiple of 8 pi: cos / sin of 8 theta are the components of
+        // ((zx, zy) / rho)^8 as complex numbers (rho = |(zx, zy)|).  phi = asin(zz / r) lies
+        // in [-pi/2, pi/2], so (cos phi, sin phi) = (rho / r, zz / r) and (cos 8 phi,
+        // sin 8 phi) is its 8th power.  Three complex squarings each; 1 / r and 1 / rho from
+        // reciprocal square roots; r^8 and r^7 by products of r^2.  The same function as the
+        // written form, with fewer roundings (its error against the fp64 estimator is
+        // bounded as for the written form).  The bailout test compares r^2 with the
+        // threshold whose correctly rounded square root first exceeds the bailout, so every
+        // point leaves the loop where the written form does; the r it leaves with is the
+        // same correctly rounded length.  rho = 0 gives NaN, as atan(0 / 0) does.
+        const float t2 = bailout_threshold(bailout);
+        float r2_out = -1.0f;  // r^2 where the point bailed out (the root is taken once, below)
         for (uint32_t i = 0; i < iters; ++i) {
-            r = len3(zx, zy, zz);
-            if (r > bailout) break;  // as below: z no longer changes past the bailout
-            const float rho = len2(zx, zy);
-            float ct = zx / rho, st = zy / rho, cp = rho / r, sp = zz / r;
+            const float xy2 = zx * zx + zy * zy;
+            const float r2 = xy2 + zz * zz;
+            if (r2 > t2) {  // as below: z no longer changes past the bailout
+                r2_out = r2;
+                break;
+            }
+            const float ir = rsqrt_full(r2), irho = rsqrt_full(xy2);
+            const float rho = xy2 * irho;
+            float ct = zx * irho, st = zy * irho, cp = rho * ir, sp = zz * ir;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
                 const float c2 = ct * ct - st * st, s2 = 2.0f * (ct * st);
@@ -130,13 +169,14 @@ __device__ __forceinline__ float sd_mandelbulb(float px, float py, float pz, flo
                 cp = p2;
                 sp = q2;
             }
-            const float r2 = r * r, r4 = r2 * r2;
-            dr = (((r4 * r2) * r) * dr) * 8.0f + 1.0f;
+            const float r4 = r2 * r2;
             r = r4 * r4;
+            dr = ((r * ir) * dr) * 8.0f + 1.0f;
             zx = r * (ct * cp) + px;
             zy = r * (st * cp) + py;
             zz = r * sp + pz;
         }
+        if (r2_out >= 0.0f) r = sqrtf(r2_out);
         return ((0.5f * logf(r)) * r) / dr - offset;
     }
     for (uint32_t i = 0; i < iters; ++i) {

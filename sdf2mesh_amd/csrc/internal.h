@@ -186,6 +186,8 @@ hipError_t launch_cells(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int
 hipError_t launch_scan(hipStream_t s, const MeshBufs& b, uint32_t nblocks, uint32_t which);
 // cnt[0..4) (zeroed by the caller): k_sincos_check's mismatch counts.
 hipError_t launch_sincos_check(hipStream_t s, uint64_t* cnt);
+// cnt[0..2) (zeroed by the caller): k_bailout_check's comparisons and disagreements.
+hipError_t launch_bailout_check(hipStream_t s, uint64_t* cnt);
 hipError_t launch_normals(hipStream_t s, const s2m_op* ops, uint32_t nops, float eps,
                           const MeshBufs& b);
 hipError_t launch_quads(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int emit);

@@ -123,6 +123,33 @@ __device__ __forceinline__ bool contour_cell(const float c[8], const float lo[3]
     return true;
 }
 
+// contour_cell's verdict alone (whether the cell gets a vertex), without the position.
+// An edge counts when its corners straddle `> 0` and its point has a component > 0.  Nine
+// of the twelve edges have a fixed component 1, so they count whenever their corners
+// straddle; the three edges from corner 000 (z: 0-4, y: 0-2, x: 0-1) have only the edge
+// parameter t = (0 - v0) / (v1 - v0), which must be > 0 as the IEEE quotient (it is 0
+// when the difference overflows or the quotient underflows).  The cell gets a vertex
+// when at least two edges count.
+__device__ __forceinline__ bool cell_active(const float c[8])
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) m |= (uint32_t)(c[k] > 0.0f) << k;
+    if (m == 0u || m == 0xffu) return false;
+    const uint32_t ch = (m ^ (m >> 4)) | ((m ^ (m >> 2)) << 8) | ((m ^ (m >> 1)) << 16);
+    // bits of `ch`: z-edges a-(a+4) at a = 0..3, y-edges a-(a+2) at 8 + {0, 1, 4, 5},
+    // x-edges a-(a+1) at 16 + {0, 2, 4, 6}
+    const uint32_t fixed1 = ch & ((0xeu) | (0x32u << 8) | (0x54u << 16));
+    uint32_t cnt = (uint32_t)__builtin_popcount(fixed1);
+    auto from0 = [&](uint32_t bit, int b) -> uint32_t {
+        if (!((ch >> bit) & 1u)) return 0u;
+        const float t = (0.0f - c[0]) / (c[b] - c[0]);
+        return t > 0.0f ? 1u : 0u;
+    };
+    cnt += from0(0, 4) + from0(8, 2) + from0(16, 1);
+    return cnt >= 2u;
+}
+
 __device__ __forceinline__ uint32_t sign_bits(const float c[8])
 {
     return (uint32_t)(c[1] > 0.0f) | ((uint32_t)(c[2] > 0.0f) << 1) |
@@ -204,9 +231,17 @@ __global__ __launch_bounds__(256) void k_cells(SlabArgs a, MeshBufs b, const flo
             bool act = false;
             if (i < a.ncells) {
                 uint32_t x, y, zl;
-                float c[8], pos[3];
+                float c[8];
                 cell_coords(a, i, x, y, zl);
-                act = cell_vertex<FROM_CORNERS>(a, corners, i, x, y, zl, c, pos);
+                if (FROM_CORNERS) {
+                    const float* src = corners + 8 * (size_t)i;
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) c[k] = src[k];
+                } else {
+                    float lo[3], hi[3];
+                    gather_lattice(a, x, y, zl, c, lo, hi);
+                }
+                act = cell_active(c);
             }
             const uint64_t w = __ballot(act);
             if (lane_id() == 0 && i < a.ncells) b.actbits[i >> 6] = w;
@@ -351,13 +386,16 @@ __global__ __launch_bounds__(256) void k_cell_vertices(SlabArgs a, MeshBufs b, i
 
 // ---------------------------------------------------------------------------------
 // Count pass over the slab lattice, one z-column of cells per thread: block (bx, y),
-// thread x = 256 bx + t walks zl = 0 .. nzl - 1.  A cell's upper corner face is the next
-// cell's lower face whenever hiz[z] == loz[z + 1] (the same lattice plane), so it is kept
-// in registers.  The vertex rule's result (contour_cell's, which does not depend on the
-// cell bounds) sets bit i % 64 of word i / 64 of `bits` (i = the cell's slab index); a
-// wave's 64 cells of a row span at most two words, ORed in with atomics only where a bit
-// is set.  `bits` is zero on entry (k_cells<1> clears the words it reads).
+// thread x = 256 bx + t walks zl = 0 .. nzl - 1.  The walk is latency-bound (a handful
+// of gathers per cell, then the rule), so the corners of kCountDepth cells are loaded
+// before any of them is tested: one memory round trip per kCountDepth cells.  A cell's
+// faces shared with its z-neighbour are loaded twice; the second load hits the cache.
+// The vertex rule's verdict (cell_active) sets bit i % 64 of word i / 64 of `bits` (i = the cell's slab index); a wave's 64
+// cells of a row span at most two words, ORed in with atomics only where a bit is set.
+// `bits` is zero on entry (k_cells<1> clears the words it reads).
 // ---------------------------------------------------------------------------------
+constexpr uint32_t kCountDepth = 8;
+
 __global__ __launch_bounds__(256) void k_count_cols(SlabArgs a, uint32_t nzl, uint64_t* __restrict__ bits)
 {
     // Workgroups go to the device's 8 XCDs in turn (block b to XCD b % 8), and each XCD
@@ -378,7 +416,6 @@ __global__ __launch_bounds__(256) void k_count_cols(SlabArgs a, uint32_t nzl, ui
     const size_t pl = (size_t)a.lat_ny * a.pitch;
     const float* r0 = a.lat + (size_t)a.loy[y] * a.pitch;  // lattice rows of the cell's y lo / hi
     const float* r1 = a.lat + (size_t)a.hiy[y] * a.pitch;
-    const float zero3[3] = {0.0f, 0.0f, 0.0f};
     auto face = [&](uint32_t zi, float* f) {  // the 4 corners of lattice plane zi
         const size_t o = (size_t)zi * pl;
         f[0] = r0[o + xi0];
@@ -386,39 +423,27 @@ __global__ __launch_bounds__(256) void k_count_cols(SlabArgs a, uint32_t nzl, ui
         f[2] = r1[o + xi0];
         f[3] = r1[o + xi1];
     };
-    // software pipelined: the next cell's corner loads are issued before this cell's rule
-    float c[8];
-    uint32_t zhi = a.hiz[a.z0] - a.lz0;
-    face(a.loz[a.z0] - a.lz0, c);
-    face(zhi, c + 4);
-    for (uint32_t zl = 0; zl < nzl; ++zl) {
-        float n[8];
-        uint32_t nhi = zhi;
-        if (zl + 1 < nzl) {
-            const uint32_t nlo = a.loz[a.z0 + zl + 1] - a.lz0;
-            nhi = a.hiz[a.z0 + zl + 1] - a.lz0;
-            if (nlo == zhi) {
+    for (uint32_t zb = 0; zb < nzl; zb += kCountDepth) {
+        float c[kCountDepth][8];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) n[k] = c[4 + k];
-            } else {
-                face(nlo, n);
+        for (uint32_t j = 0; j < kCountDepth; ++j) {
+            if (zb + j < nzl) {  // wave-uniform
+                face(a.loz[a.z0 + zb + j] - a.lz0, c[j]);
+                face(a.hiz[a.z0 + zb + j] - a.lz0, c[j] + 4);
             }
-            face(nhi, n + 4);
         }
-        float pos[3];
-        const bool act = in && contour_cell(c, zero3, zero3, pos);
-        const uint64_t w = __ballot(act);
-        if (w != 0ull && lane_id() == 0) {
-            const uint64_t i0 = x0 + (uint64_t)a.rx * (y + (uint64_t)a.ry * zl);
-            const uint32_t sh = (uint32_t)(i0 & 63u);
-            unsigned long long* wp = reinterpret_cast<unsigned long long*>(bits + (i0 >> 6));
-            atomicOr(wp, (unsigned long long)(w << sh));
-            if (sh != 0u && (w >> (64u - sh)) != 0ull) atomicOr(wp + 1, (unsigned long long)(w >> (64u - sh)));
-        }
-        if (zl + 1 < nzl) {
 #pragma unroll
-            for (int k = 0; k < 8; ++k) c[k] = n[k];
-            zhi = nhi;
+        for (uint32_t j = 0; j < kCountDepth; ++j) {
+            if (zb + j >= nzl) break;
+            const bool act = in && cell_active(c[j]);
+            const uint64_t w = __ballot(act);
+            if (w != 0ull && lane_id() == 0) {
+                const uint64_t i0 = x0 + (uint64_t)a.rx * (y + (uint64_t)a.ry * (zb + j));
+                const uint32_t sh = (uint32_t)(i0 & 63u);
+                unsigned long long* wp = reinterpret_cast<unsigned long long*>(bits + (i0 >> 6));
+                atomicOr(wp, (unsigned long long)(w << sh));
+                if (sh != 0u && (w >> (64u - sh)) != 0ull) atomicOr(wp + 1, (unsigned long long)(w >> (64u - sh)));
+            }
         }
     }
 }
@@ -686,6 +711,39 @@ __global__ __launch_bounds__(256) void k_sincos_check(unsigned long long* __rest
     }
 }
 
+// The Mandelbulb's bailout test r > b on r = sqrtf(r^2) taken as r^2 > t with
+// t = bailout_threshold(b).  Bailouts b: 2^20 bit patterns spread over all 2^32
+// (stride 4099: denormals, normals, infinities, NaNs, both signs) and 2.0; squared
+// lengths x: the 65 floats around b * b and 0, the smallest denormal, 1, +inf and NaN,
+// each x >= 0 or NaN as a sum of squares is.  cnt[0]: comparisons, cnt[1]: those where
+// the two tests disagree.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bailout_check(unsigned long long* __restrict__ cnt)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > (1u << 20)) return;
+    const float b = i == (1u << 20) ? 2.0f : __uint_as_float(i * 4099u);
+    const float t = bailout_threshold(b);
+    const uint32_t u = __float_as_uint(b * b);
+    uint32_t checked = 0, bad = 0;
+    for (int d = -37; d <= 32; ++d) {
+        float x;
+        switch (d) {
+        case -37: x = 0.0f; break;
+        case -36: x = __uint_as_float(1u); break;
+        case -35: x = 1.0f; break;
+        case -34: x = __uint_as_float(0x7f800000u); break;
+        case -33: x = __uint_as_float(0x7fc00000u); break;
+        default: x = __uint_as_float(u + (uint32_t)d); break;
+        }
+        if (x != x ? false : (__float_as_uint(x) >> 31) != 0u) continue;  // negative: no sum of squares
+        ++checked;
+        bad += (sqrtf(x) > b) != (x > t);
+    }
+    atomicAdd(&cnt[0], (unsigned long long)checked);
+    if (bad) atomicAdd(&cnt[1], (unsigned long long)bad);
+}
+
 // ---------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------
@@ -732,6 +790,13 @@ hipError_t launch_cells_from_corners(hipStream_t s, const SlabArgs& a, const flo
 hipError_t launch_sincos_check(hipStream_t s, uint64_t* cnt)
 {
     hipLaunchKernelGGL(k_sincos_check, dim3(8192), dim3(256), 0, s, reinterpret_cast<unsigned long long*>(cnt));
+    return hipGetLastError();
+}
+
+hipError_t launch_bailout_check(hipStream_t s, uint64_t* cnt)
+{
+    hipLaunchKernelGGL(k_bailout_check, dim3(((1u << 20) + 256u) / 256u), dim3(256), 0, s,
+                       reinterpret_cast<unsigned long long*>(cnt));
     return hipGetLastError();
 }
 

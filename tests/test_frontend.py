@@ -46,9 +46,13 @@ _HOST_PREFIX = f"""
 #include <math.h>
 #include <stdint.h>
 #include <stddef.h>
+#include <string.h>
 #define __HIPCC_RTC__ 1
 #define __device__
 #define __forceinline__ inline
+#define __builtin_amdgcn_rsqf(x) (1.0f / sqrtf(x))
+static inline uint32_t __float_as_uint(float f) {{ uint32_t u; memcpy(&u, &f, 4); return u; }}
+static inline float __uint_as_float(uint32_t u) {{ float f; memcpy(&f, &u, 4); return f; }}
 #include "{CSRC / 'device_sdf.h'}"
 #include "{CSRC / 'rtc_prelude.h'}"
 """

@@ -136,6 +136,15 @@ def test_sincos_matches_sin_cos(ctx):
     assert ctx.check_sincos() == (0, 0, 0, 0)
 
 
+def test_bailout_threshold_exact(ctx):
+    # the power-8 step tests the bailout on the squared length; over bailouts spread
+    # across every float bit pattern, each squared-length test must agree with the
+    # correctly rounded root's (the written form's `r > bailout`)
+    checked, bad = ctx.check_bailout()
+    print(f"bailout threshold: {checked} comparisons, {bad} disagreements", flush=True)
+    assert checked > 40 * (1 << 20) and bad == 0
+
+
 def test_lattice_values_bit_exact(ctx):
     node = tape.csg64()
     p = s2m.Params.reference(64, 2.0)
@@ -159,6 +168,31 @@ def test_contour_cells_bit_exact(ctx):
         m = ctx.contour_cells(corners, [blo] * 3, [bhi] * 3,
                               s2m.COMPAT_DROP_LAST_SLICE if lag else 0)
         r = oracle.contour_corners(corners, [blo] * 3, [bhi] * 3, lag=lag)
+        assert_same_mesh(m, r, normal_atol=None)
+
+
+def test_contour_cells_special_values(ctx):
+    # corners drawn half from special values (+-0, +-smallest denormal, +-1e-38, +-1e-30,
+    # +-1, +-2e38, +-3e38) and half from N(0, 1): edge differences that overflow and edge
+    # parameters that underflow to 0 (such an edge from corner 000 does not count toward
+    # the cell's vertex).  The count pass's verdict and the emitted mesh must equal the
+    # oracle's.
+    rng = np.random.default_rng(1)
+    pool = np.array([0.0, -0.0, 1e-45, -1e-45, 1e-38, -1e-38, 3e38, -3e38, 1.0, -1.0, 1e-30, -1e-30,
+                     2e38, -2.5e38], np.float32)
+    res = 24
+    corners = rng.standard_normal((res, res, res, 8)).astype(np.float32)
+    pick = rng.random(corners.shape) < 0.5
+    corners[pick] = pool[rng.integers(0, len(pool), int(pick.sum()))]
+    with np.errstate(all="ignore"):
+        t = (np.float32(0) - corners[..., 0]) / (corners[..., 4] - corners[..., 0])
+    straddle = (corners[..., 0] > 0) != (corners[..., 4] > 0)
+    assert (straddle & ~(t > 0)).sum() > 100  # edges whose parameter is 0
+    blo, bhi = oracle.axis_bounds(np.float32(-1), np.float32(1), res)
+    for lag in (False, True):
+        m = ctx.contour_cells(corners, [blo] * 3, [bhi] * 3, s2m.COMPAT_DROP_LAST_SLICE if lag else 0)
+        r = oracle.contour_corners(corners, [blo] * 3, [bhi] * 3, lag=lag)
+        assert len(r.vertices) > 1000
         assert_same_mesh(m, r, normal_atol=None)
 
 
@@ -208,7 +242,7 @@ MANDEL_BAIL_MARGIN = 1e-5  # points whose bailout tests all clear the threshold 
                                              (8, 48, s2m.REFERENCE_COMPAT),
                                              (5, 40, s2m.REFERENCE_COMPAT)])
 def test_mandelbulb_lattice_vs_fp64(ctx, iters, res, flags, form):
-    # GPU fp32 (ocml transcendentals) against the fp64 oracle, at every lattice point.
+    # GPU fp32 against the fp64 oracle, at every lattice point.
     # Bound: |f32 - f64| <= C * (ulp_f32(f) + s) with C = 32, where s is the first-order
     # error scale of the evaluation (oracle.mandel_cond: the summed effect on the fp64
     # result of one relative rounding 2^-24 of each intermediate - z per step, r, theta,
@@ -216,7 +250,8 @@ def test_mandelbulb_lattice_vs_fp64(ctx, iters, res, flags, form):
     # bailout tests all clear |r - 2| / 2 >= 1e-5; the rest (a rounding there can take the
     # other branch of `if (r > 2.0) continue;`, a jump no error bound covers) are counted
     # and must stay under 0.1 % of the points.  Three forms of the estimator: the tape op's
-    # power-8 step by complex squaring (the default), the tape op evaluated as written
+    # power-8 step by complex squaring with hardware reciprocal square roots (the
+    # default), the tape op evaluated as written with ocml's transcendentals
     # (S2M_MANDEL_TRIG), and the GLSL source compiled at run time.
     node = tape.mandelbulb(iters)
     if form == "glsl":
