@@ -879,6 +879,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8))) void
 constexpr uint32_t kStageChunk = kStageChunkHost;
 constexpr uint32_t kTileThreads = 256;  // threads of a k_sp_tiles block (one tile at a time)
 constexpr int kLinearPoints = 2;        // points per lane sharing each op's decode (left folds)
+static_assert(kRecVec - kRecPre <= kTileThreads, "one thread per record word past the read-ahead");
 
 struct ContourLds {  // block-shared state of contour_tile (and `cand`, 512 entries: the cells
                      // with a sign change among their corners)
@@ -1073,6 +1074,9 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
             continue;
         }
         const uint64_t tile = __builtin_amdgcn_readfirstlane(H.tile);
+        // thread 0's flop count for the block statistics, read now so that the load has
+        // landed when the tile ends
+        const uint32_t tile_flops = thread_id() == 0 ? P.flops[tile] : 0u;
         const uint32_t n = lens & ~kLinearFlag;
         const SpJob& J = P.jobs[__builtin_amdgcn_readfirstlane(H.job)];
         const uint32_t txy = __builtin_amdgcn_readfirstlane(H.txy);
@@ -1098,10 +1102,12 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
                 skip();
                 continue;
             }
-            // the rest of a long program's record (rows past the prefetched part)
+            // the rest of a long program's record (rows past the prefetched part): loaded
+            // now, stored to R once the point list is built (nothing before reads it)
             const uint32_t need = (kRecRowsAt + 32u * ((n + 1u) >> 1) + 15u) >> 4;
-            for (uint32_t k = kRecPre + thread_id(); k < need; k += blockDim.x)
-                reinterpret_cast<uint4*>(&R)[k] = P.slot_rec[(size_t)slot * kRecVec + k];
+            const uint32_t kr = kRecPre + thread_id();
+            uint4 rest = make_uint4(0u, 0u, 0u, 0u);
+            if (kr < need) rest = P.slot_rec[(size_t)slot * kRecVec + kr];
             // The points that some live brick uses.  Per box row (iz, iy): the x-bricks of the
             // live bricks whose cells use it (rm), hence the row's needed x positions (the
             // union of those x-bricks' positions, xb).  One row per thread; each wave reserves
@@ -1130,6 +1136,7 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
             uint32_t wbase = 0;
             if (lane() == 63 && inc) wbase = atomicAdd(&nlist, inc);
             const uint32_t off = __shfl(wbase, 63, 64) + inc - cnt;
+            if (kr < need) reinterpret_cast<uint4*>(&R)[kr] = rest;
             // window 0 is filled at once (every wave has its offsets); the total is read
             // after the barrier that completes it
             uint32_t total = 0;
@@ -1143,12 +1150,12 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
                 __syncthreads();
                 if (w0 == 0) total = nlist;  // reset for the next slot after the box is complete
                 const uint32_t m = min(kList, total - w0);
-                for (uint32_t i0 = thread_id(); i0 < m; i0 += kLinearPoints * blockDim.x) {
+                for (uint32_t i0 = thread_id(); i0 < m; i0 += kLinearPoints * kTileThreads) {
                     float x[kLinearPoints], y[kLinearPoints], z[kLinearPoints], v[kLinearPoints];
                     uint32_t at[kLinearPoints];
 #pragma unroll
                     for (int p = 0; p < kLinearPoints; ++p) {
-                        const uint32_t i = i0 + p * blockDim.x;
+                        const uint32_t i = i0 + p * kTileThreads;
                         const uint32_t e = list[i < m ? i : i0];
                         const uint32_t ix = e & 31u, iy = (e >> 5) & 31u, iz = e >> 10;
                         x[p] = R.pc[0][ix];
@@ -1159,7 +1166,7 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
                     run_linear<kLinearPoints>(R.T, n, x, y, z, v);
 #pragma unroll
                     for (int p = 0; p < kLinearPoints; ++p)
-                        if (i0 + p * blockDim.x < m) box[at[p]] = v[p];
+                        if (i0 + p * kTileThreads < m) box[at[p]] = v[p];
                 }
                 if (w0 + kList >= total) break;
             }
@@ -1179,7 +1186,7 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
                 if (i < (a == 0 ? nx : a == 1 ? ny : nz)) R.pc[a][i] = c[i];
             }
             __syncthreads();
-            for (uint32_t q = thread_id(); q < npts; q += blockDim.x) {
+            for (uint32_t q = thread_id(); q < npts; q += kTileThreads) {
                 const uint32_t iz = div16(q, nxy, mxy), r = q - iz * nxy;
                 const uint32_t iy = div16(r, nx, mx), ix = r - iy * nx;
                 const float x = R.pc[0][ix], y = R.pc[1][iy], z = R.pc[2][iz];
@@ -1199,7 +1206,7 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
         if (thread_id() == 0) {
             tiles_done += 1;
             points_done += evaluated;
-            flops_done += (uint64_t)evaluated * P.flops[tile];
+            flops_done += (uint64_t)evaluated * tile_flops;
         }
     }
     if (thread_id() == 0 && tiles_done) {  // the block's own entry (no shared address): k_sp_job_end sums them
@@ -1282,13 +1289,13 @@ __global__ __launch_bounds__(64) void k_sp_quadcount(SpPass P, MeshBufs b)
         int32_t ns[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) ns[i] = P.slot_nb[(size_t)slot * 8u + i];
-        for (uint32_t t = thread_id(); t < 81u; t += blockDim.x)
+        for (uint32_t t = thread_id(); t < 81u; t += 64u)
             ex[t] = nb_row<false>(P, J, tx, ty, J.tz0 + tzl, ns, (int)(ty * 8u + t % 9u) - 1,
                                   (int)(J.z0 + tzl * 8u + t / 9u) - 1, nullptr);
         rowt[thread_id()] = 0;
         __syncthreads();
         uint32_t invalid = 0;
-        for (uint32_t k = thread_id(); k < n; k += blockDim.x) {
+        for (uint32_t k = thread_id(); k < n; k += 64u) {
             if ((uint64_t)base + k >= P.stage_cap) break;
             const uint32_t meta = __float_as_uint(P.stage[(uint64_t)base + k].w);
             const uint32_t c = meta & 511u, sg = meta >> 12;
@@ -1355,16 +1362,16 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
         int32_t ns[8];
 #pragma unroll
         for (int i = 0; i < 8; ++i) ns[i] = P.slot_nb[(size_t)slot * 8u + i];
-        for (uint32_t t = thread_id(); t < 81u; t += blockDim.x)
+        for (uint32_t t = thread_id(); t < 81u; t += 64u)
             (void)nb_row<true>(P, J, tx, ty, J.tz0 + tzl, ns, (int)(ty * 8u + t % 9u) - 1,
                                (int)(J.z0 + tzl * 8u + t / 9u) - 1, nb + 9u * t);
-        for (uint32_t w = thread_id(); w < 128u; w += blockDim.x) tc[w] = 0;
+        for (uint32_t w = thread_id(); w < 128u; w += 64u) tc[w] = 0;
         __syncthreads();
         // vertices, and each cell's triangle count (the first round's cell, quads and row
         // triangle offset are kept for the triangle loop)
         uint32_t meta0 = 0, nq0 = 0;
         uint64_t trow0 = 0;
-        for (uint32_t k = thread_id(); k < n; k += blockDim.x) {
+        for (uint32_t k = thread_id(); k < n; k += 64u) {
             if ((uint64_t)base + k >= P.stage_cap) break;
             const float4 e = P.stage[(uint64_t)base + k];
             const uint32_t meta = __float_as_uint(e.w);
@@ -1421,7 +1428,7 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
         }
         __syncthreads();  // every cell's triangle count is in tc
         // triangles, in vertex order: each quad (q2, q1, q0), (q0, q3, q2)
-        for (uint32_t k = thread_id(); k < n; k += blockDim.x) {
+        for (uint32_t k = thread_id(); k < n; k += 64u) {
             if ((uint64_t)base + k >= P.stage_cap) break;
             const bool first = k == thread_id();
             const uint32_t meta = first ? meta0 : __float_as_uint(P.stage[(uint64_t)base + k].w);
