@@ -452,7 +452,7 @@ int s2m_contour_cells(s2m_ctx* ctx, const float* corners, const uint32_t res[3],
 int s2m_check_sincos(s2m_ctx* ctx, uint64_t counts[4]);
 
 /* No reference counterpart: checks a device identity this build relies on.  The power-8
- * Mandelbulb step tests the bailout (`if (r > bailout)`, examples/mandelmesh.frag:10-11)
+ * Mandelbulb step tests the bailout (`if(r > 2.0) continue;`, examples/mandelmesh.frag:13-14)
  * as r^2 > t, t the largest float whose correctly rounded square root does not exceed
  * the bailout.  Over 2^20 bailouts spread across every bit pattern and the squared
  * lengths around each bailout's square, counts[0] receives the comparisons made and
