@@ -464,10 +464,12 @@ struct SpRange {
 // Host arrays one job's mesh is copied into while the job runs (s2m_mesh).  After each
 // pass of the sparse pipeline (or slab of the dense one) the compute stream records a
 // snapshot of the totals (k_snapshot, into mapped host memory) and an event;
-// readback_passes waits for each event in turn and queues the copy of that pass's
-// vertices and triangles on the copy stream, so the copy of pass k overlaps the
-// computation of pass k + 1.  Vertices and triangles are appended in pass order and never
-// rewritten, so a finished pass's range is final.
+// readback_pass waits for a pass's event and queues the copy of that pass's vertices and
+// triangles on the copy stream.  The host calls it for pass k right after queueing pass
+// k + 1, so the copy of pass k starts as soon as pass k ends and overlaps the computation
+// of pass k + 1 (waiting until every pass is queued would delay the first copy by the
+// launches of all later passes).  Vertices and triangles are appended in pass order and
+// never rewritten, so a finished pass's range is final.
 struct Readback {
     int want_keys = 0;
     float* vtx = nullptr;
@@ -476,6 +478,7 @@ struct Readback {
     uint8_t* signs = nullptr;
     uint64_t vcap = 0, tcap = 0;      // entries of the host arrays
     uint64_t v_done = 0, t_done = 0;  // copies queued so far
+    uint32_t passes_done = 0;         // passes whose copies are queued (or that stopped them)
     bool stopped = false;             // a pass overflowed a capacity: the attempt repeats
     hipEvent_t first = nullptr, last = nullptr;  // profiling: around the copies
 
@@ -521,6 +524,7 @@ int readback_begin(s2m_ctx* ctx, Readback& rb, uint64_t vcap, uint64_t tcap)
         return fail(S2M_E_OOM, "host allocation of %llu vertices / %llu triangles failed",
                     (unsigned long long)vcap, (unsigned long long)tcap);
     rb.v_done = rb.t_done = 0;
+    rb.passes_done = 0;
     rb.stopped = false;
     rb.first = rb.last = nullptr;
     return S2M_OK;
@@ -566,23 +570,31 @@ int readback_copy(s2m_ctx* ctx, Readback& rb, uint64_t nv, uint64_t nt)
     return S2M_OK;
 }
 
-// Passes 0 .. npass-1 are queued, each followed by readback_mark: as each one finishes,
-// queue the copies of its vertices and triangles.  A pass whose totals or capacity use
-// exceed the attempt's capacities stops the copies (the attempt will repeat).
+// Pass rb.passes_done is queued and followed by readback_mark: once it finishes, queue
+// the copies of its vertices and triangles.  A pass whose totals or capacity use exceed
+// the attempt's capacities stops the copies (the attempt will repeat).
+int readback_pass(s2m_ctx* ctx, Readback& rb, uint64_t vcap, uint64_t tcap, uint64_t slot_cap, uint64_t stage_cap)
+{
+    const uint32_t k = rb.passes_done++;
+    if (rb.stopped) return S2M_OK;
+    const volatile uint64_t* snap = ctx->host_snap.as<uint64_t>();
+    HIP_TRY(hipEventSynchronize(ctx->pass_done[k]));
+    const uint64_t nv = snap[4 * k], nt = snap[4 * k + 1], slots = snap[4 * k + 2], staged = snap[4 * k + 3];
+    if (nv > vcap || nt > tcap || nv > rb.vcap || nt > rb.tcap || slots > slot_cap || staged > stage_cap ||
+        nv < rb.v_done || nt < rb.t_done) {
+        rb.stopped = true;
+        return S2M_OK;
+    }
+    HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, ctx->pass_done[k], 0));
+    return readback_copy(ctx, rb, nv, nt);
+}
+
+// Every pass up to npass-1 whose copies are not queued yet, in order.
 int readback_passes(s2m_ctx* ctx, Readback& rb, uint32_t npass, uint64_t vcap, uint64_t tcap, uint64_t slot_cap,
                     uint64_t stage_cap)
 {
-    const volatile uint64_t* snap = ctx->host_snap.as<uint64_t>();
-    for (uint32_t k = 0; k < npass && !rb.stopped; ++k) {
-        HIP_TRY(hipEventSynchronize(ctx->pass_done[k]));
-        const uint64_t nv = snap[4 * k], nt = snap[4 * k + 1], slots = snap[4 * k + 2], staged = snap[4 * k + 3];
-        if (nv > vcap || nt > tcap || nv > rb.vcap || nt > rb.tcap || slots > slot_cap || staged > stage_cap ||
-            nv < rb.v_done || nt < rb.t_done) {
-            rb.stopped = true;
-            break;
-        }
-        HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, ctx->pass_done[k], 0));
-        int rc = readback_copy(ctx, rb, nv, nt);
+    while (rb.passes_done < npass) {
+        const int rc = readback_pass(ctx, rb, vcap, tcap, slot_cap, stage_cap);
         if (rc) return rc;
     }
     return S2M_OK;
@@ -1035,8 +1047,9 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
                                          ctx->profiling ? m : nullptr));
             if (ctx->profiling) ev.insert(ev.end(), m, m + kSparseMarks);
             tiles_total += P.ntiles;
-            if (rb) {
-                const int rrc = readback_mark(ctx, b.counters, pass);
+            if (rb) {  // this pass's snapshot; the previous pass's copies once it ends
+                int rrc = readback_mark(ctx, b.counters, pass);
+                if (!rrc && pass > 0) rrc = readback_pass(ctx, *rb, want_v, want_t, want_s, stage_cap);
                 if (rrc) return rrc;
             }
         }
@@ -1402,8 +1415,9 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             HIP_TRY(launch_scan(ctx->stream, b, nb, 1));
             HIP_TRY(launch_quads(ctx->stream, a, b, 1));
             if (m[4]) HIP_TRY(hipEventRecord(m[4], ctx->stream));
-            if (rb) {  // the slab's vertices and triangles are final
-                const int rrc = readback_mark(ctx, b.counters, (uint32_t)slabs);
+            if (rb) {  // the slab's vertices and triangles are final; the previous slab's copies
+                int rrc = readback_mark(ctx, b.counters, (uint32_t)slabs);
+                if (!rrc && slabs > 0) rrc = readback_pass(ctx, *rb, want_v, want_t, ~0ull, ~0ull);
                 if (rrc) return rrc;
             }
             if (z1 < j.nz_cells) {  // last map plane becomes the previous plane
