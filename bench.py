@@ -141,6 +141,9 @@ def single_job(name: str, args, ctx, headline: bool) -> dict:
     node, params, cfg = workload(name)
     if args.dense:
         params.flags |= s2m.DENSE_EVAL
+    if args.slab:
+        params.slab = args.slab
+        cfg = {**cfg, "slab": args.slab}
     voxels = int(params.res[0]) * int(params.res[1]) * int(params.res[2])
     prog = ctx.program(node)
     tuned = ctx.tune(prog, params) if not args.no_tune else None  # before the timed steps
@@ -451,6 +454,8 @@ def main() -> int:
     ap.add_argument("--serve-workers", type=int, default=2)
     ap.add_argument("--serve-dense-workers", type=int, default=1)
     ap.add_argument("--serve-batch", type=int, default=64)
+    ap.add_argument("--slab", type=int, default=0,
+                    help="cell planes per dense slab or sparse pass (0: the autotuner's or the library's choice)")
     ap.add_argument("--dense", action="store_true",
                     help="evaluate every lattice point with the full program (no tile pruning)")
     args = ap.parse_args()

@@ -127,6 +127,7 @@ struct s2m_ctx {
     int profiling = 0;  // 0 off, 1 total + evaluation kernels, 2 every stage
     s2m_timing timing{};
     DevBuf lattice, vidx, bcount, boffset, actbits, counters, tables, check;
+    DevBuf lattice2;  // the dense path's second slab lattice (evaluation one slab ahead)
     DevBuf vtx, keys, signs, tri, corners;
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_slot_rows, sp_boundary;
@@ -147,6 +148,9 @@ struct s2m_ctx {
     // streamed readback (s2m_mesh): device-to-host copies of finished passes run on
     // copy_stream while later passes compute on `stream`
     hipStream_t copy_stream = nullptr;
+    // dense slabs: the lattice of slab k + 1 is evaluated on eval_stream while slab k is
+    // contoured on `stream`
+    hipStream_t eval_stream = nullptr;
     PinnedBuf host_snap;                 // per pass: k_snapshot's totals and capacity use
     // captured graphs of repeated one-pass jobs, most recent first, and the keys of jobs
     // seen once (run_graph)
@@ -1214,7 +1218,7 @@ DenseLayout dense_layout(const Job& j, uint32_t T, bool lattice)
     L.vidx = (size_t)(T + 1) * plane * sizeof(uint32_t);
     L.blocks = (size_t)L.max_blocks * sizeof(uint32_t);
     L.actbits = (size_t)L.max_blocks * (kCellsPerBlock / 64) * sizeof(uint64_t);
-    L.counters = kNumCounters * sizeof(uint64_t);
+    L.counters = (kNumCounters + kDenseScanParts) * sizeof(uint64_t);  // the scan's chunk sums after them
     return L;
 }
 
@@ -1314,6 +1318,19 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
     const uint32_t nslabs = (j.nz_cells + T - 1) / T;
     if (nslabs == 1) rb = nullptr;  // nothing to overlap: the caller copies the finished mesh
     if (rb) HIP_TRY(ctx->host_snap.ensure((size_t)nslabs * 4 * sizeof(uint64_t)));
+    // Several slabs: slab k + 1's lattice is evaluated (VALU-bound) on eval_stream into the
+    // other of two lattice buffers while slab k is contoured (memory- and latency-bound) on
+    // the context's stream.  Events order it: a slab's contouring waits for its evaluation,
+    // an evaluation waits until the previous user of its buffer (the slab two before) has
+    // read it for the last time (the vertex pass), and the first evaluation waits for the
+    // tables and clears queued before it.
+    const bool piped = !fx && !h_corners && nslabs > 1;
+    hipStream_t es = piped ? ctx->eval_stream : ctx->stream;
+    float* lat_buf[2] = {lat, lat};
+    if (piped) {
+        HIP_TRY(ctx->lattice2.ensure(L.lattice));
+        lat_buf[1] = ctx->lattice2.as<float>();
+    }
 
     for (int attempt = 0; attempt < (fx ? 1 : 3); ++attempt) {
         MeshBufs b{};
@@ -1352,11 +1369,24 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
                                ctx->stream));
 
         ctx->events_used = 0;
-        std::vector<hipEvent_t> ev;  // per slab: 5 marks (eval, cells, normals, quads, end)
+        // per slab 6 marks: evaluation start and end (on the evaluation's stream), then on
+        // the context's stream the ends of cells, normals and quads, and the cells' start
+        std::vector<hipEvent_t> ev;
         hipEvent_t ev_start = nullptr, ev_end = nullptr;
         if (prof) {
             ev_start = ctx->next_event();
             HIP_TRY(hipEventRecord(ev_start, ctx->stream));
+        }
+        auto mark = [&](hipStream_t st) -> hipEvent_t {  // a new event recorded on st, or null
+            hipEvent_t e = ctx->next_event();
+            if (e && hipEventRecord(e, st) != hipSuccess) e = nullptr;
+            return e;
+        };
+        hipEvent_t lat_free[2] = {nullptr, nullptr};  // per lattice buffer: its last read done
+        if (piped) {
+            hipEvent_t ready = mark(ctx->stream);
+            if (!ready) return fail(S2M_E_HIP, "event record failed");
+            HIP_TRY(hipStreamWaitEvent(es, ready, 0));
         }
         uint64_t lattice_points = 0, slabs = 0;
         for (uint32_t z0 = 0; z0 < j.nz_cells; z0 += T) {
@@ -1365,8 +1395,9 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             const uint32_t lz0 = j.ax[2].lo[z0];
             const uint32_t nzl = j.ax[2].hi[z1 - 1] - lz0 + 1;
 
+            float* const lat_s = lat_buf[slabs & 1];
             SlabArgs a{};
-            a.lat = lat;
+            a.lat = lat_s;
             a.pitch = pitch;
             a.lat_ny = ny;
             a.lz0 = lz0;
@@ -1389,15 +1420,22 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             a.flags = p->flags;
             const uint32_t nb = (uint32_t)(((uint64_t)a.ncells + kCellsPerBlock - 1) / kCellsPerBlock);
 
-            hipEvent_t m[5] = {};
+            hipEvent_t m[6] = {};
             if (prof)
                 for (auto& e : m) e = ctx->next_event();
-            if (m[0]) HIP_TRY(hipEventRecord(m[0], ctx->stream));
+            if (piped && lat_free[slabs & 1]) HIP_TRY(hipStreamWaitEvent(es, lat_free[slabs & 1], 0));
+            if (m[0]) HIP_TRY(hipEventRecord(m[0], es));
             if (!h_corners) {
-                HIP_TRY(eval_lattice_prog(ctx->stream, prog, dc[0], dc[1], dc[2], nx, ny, lz0, nzl, pitch, lat));
+                HIP_TRY(eval_lattice_prog(es, prog, dc[0], dc[1], dc[2], nx, ny, lz0, nzl, pitch, lat_s));
                 lattice_points += (uint64_t)nx * ny * nzl;
             }
-            if (m[1]) HIP_TRY(hipEventRecord(m[1], ctx->stream));
+            if (m[1]) HIP_TRY(hipEventRecord(m[1], es));
+            if (piped) {
+                hipEvent_t evald = mark(es);
+                if (!evald) return fail(S2M_E_HIP, "event record failed");
+                HIP_TRY(hipStreamWaitEvent(ctx->stream, evald, 0));
+            }
+            if (m[5]) HIP_TRY(hipEventRecord(m[5], ctx->stream));
             if (h_corners) {
                 const float* src = ctx->corners.as<float>() + (size_t)z0 * plane * 8;
                 HIP_TRY(launch_cells_from_corners(ctx->stream, a, src, b, 0));
@@ -1407,6 +1445,10 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
                 HIP_TRY(launch_cells(ctx->stream, a, b, 0, 0));
                 HIP_TRY(launch_scan(ctx->stream, b, nb, 0));
                 HIP_TRY(launch_cells(ctx->stream, a, b, 1, normals ? 0 : 1));
+                if (piped) {  // the vertex pass was the slab lattice's last reader
+                    lat_free[slabs & 1] = mark(ctx->stream);
+                    if (!lat_free[slabs & 1]) return fail(S2M_E_HIP, "event record failed");
+                }
             }
             if (m[2]) HIP_TRY(hipEventRecord(m[2], ctx->stream));
             if (normals) HIP_TRY(normals_prog(ctx->stream, prog, p->eps, b));
@@ -1415,16 +1457,17 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             HIP_TRY(launch_scan(ctx->stream, b, nb, 1));
             HIP_TRY(launch_quads(ctx->stream, a, b, 1));
             if (m[4]) HIP_TRY(hipEventRecord(m[4], ctx->stream));
-            if (rb) {  // the slab's vertices and triangles are final; the previous slab's copies
+            if (rb) {  // the slab's vertices and triangles are final; the copies of the slab two
+                       // before (waiting for it leaves this slab and the next evaluation queued)
                 int rrc = readback_mark(ctx, b.counters, (uint32_t)slabs);
-                if (!rrc && slabs > 0) rrc = readback_pass(ctx, *rb, want_v, want_t, ~0ull, ~0ull);
+                if (!rrc && slabs > 1) rrc = readback_pass(ctx, *rb, want_v, want_t, ~0ull, ~0ull);
                 if (rrc) return rrc;
             }
             if (z1 < j.nz_cells) {  // last map plane becomes the previous plane
                 HIP_TRY(hipMemcpyAsync(vidx, vidx + (size_t)nzc * plane, plane * sizeof(uint32_t),
                                        hipMemcpyDeviceToDevice, ctx->stream));
             }
-            if (prof) ev.insert(ev.end(), m, m + 5);
+            if (prof) ev.insert(ev.end(), m, m + 6);
             ++slabs;
         }
         if (prof) {
@@ -1442,10 +1485,10 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             float ms = 0;
             HIP_TRY(hipEventElapsedTime(&ms, ev_start, ev_end));
             ctx->timing.total_ms = ms;
-            for (size_t s = 0; s + 4 < ev.size(); s += 5) {
+            for (size_t s = 0; s + 5 < ev.size(); s += 6) {
                 float t0, t1, t2, t3;
                 HIP_TRY(hipEventElapsedTime(&t0, ev[s], ev[s + 1]));
-                HIP_TRY(hipEventElapsedTime(&t1, ev[s + 1], ev[s + 2]));
+                HIP_TRY(hipEventElapsedTime(&t1, ev[s + 5], ev[s + 2]));
                 HIP_TRY(hipEventElapsedTime(&t2, ev[s + 2], ev[s + 3]));
                 HIP_TRY(hipEventElapsedTime(&t3, ev[s + 3], ev[s + 4]));
                 ctx->timing.eval_ms += t0;
@@ -1584,8 +1627,10 @@ s2m_ctx* s2m_create(int device)
     }
     e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->eval_stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         fail(S2M_E_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+        if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return nullptr;
@@ -1604,7 +1649,8 @@ void s2m_destroy(s2m_ctx* ctx)
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamSynchronize(ctx->copy_stream);
-    for (DevBuf* b : {&ctx->lattice, &ctx->vidx, &ctx->bcount, &ctx->boffset, &ctx->actbits, &ctx->check, &ctx->counters,
+    (void)hipStreamSynchronize(ctx->eval_stream);
+    for (DevBuf* b : {&ctx->lattice, &ctx->lattice2, &ctx->vidx, &ctx->bcount, &ctx->boffset, &ctx->actbits, &ctx->check, &ctx->counters,
                       &ctx->tables, &ctx->vtx, &ctx->keys, &ctx->signs, &ctx->tri, &ctx->corners,
                       &ctx->tile_tab, &ctx->tile_lists, &ctx->tile_lens,
                       &ctx->tile_flops, &ctx->sp_slot_of_tile, &ctx->sp_tile_of_slot,
@@ -1619,6 +1665,7 @@ void s2m_destroy(s2m_ctx* ctx)
     for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
     for (hipEvent_t e : ctx->pass_done) (void)hipEventDestroy(e);
     for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.second);
+    (void)hipStreamDestroy(ctx->eval_stream);
     (void)hipStreamDestroy(ctx->copy_stream);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;

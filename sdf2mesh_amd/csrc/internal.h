@@ -13,8 +13,9 @@ namespace s2m {
 constexpr uint32_t kBlock = 256;             // threads per block of the cell kernels
 constexpr uint32_t kRounds = 4;              // cells per thread (strided by kBlock)
 constexpr uint32_t kCellsPerBlock = kBlock * kRounds;
-constexpr uint32_t kScanThreads = 1024;      // single-block scan of block counts
-constexpr uint32_t kMaxScanBlocks = kScanThreads * 64;
+constexpr uint32_t kMaxScanBlocks = 65536;   // most block counts one dense slab scans
+constexpr uint32_t kDenseScanChunk = 1024;        // block counts per workgroup of the dense scan
+constexpr uint32_t kDenseScanParts = kMaxScanBlocks / kDenseScanChunk;  // its chunk sums, after the counters
 constexpr uint32_t kNone = 0xffffffffu;      // "no vertex" (u32::MAX, mesh.rs:330)
 
 // Unsigned division by a runtime constant d >= 1 (round-up multiplier method):

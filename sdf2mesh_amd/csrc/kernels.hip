@@ -2,11 +2,14 @@
 //
 // Per z-slab of cells the host launches, in stream order:
 //   k_eval_lattice     SDF at every corner-lattice point of the slab       (VALU bound)
-//   k_cells<0>         active-cell count per block of 1024 cells           (L2/HBM bound)
-//   k_scan             exclusive scan of block counts + running vertex total
-//   k_cells<1>         vertex position/signs, ordered write, cell->vertex map
+//   k_count_cols + k_popc   activity bits and active-cell count per block of 1024 cells
+//                      (k_cells<0> for per-cell corner dumps)
+//   k_scan_sums + k_scan_offsets   exclusive scan of block counts + running vertex total
+//   k_cells_map + k_cell_vertices  cell->vertex map, vertex position/signs in order
+//                      (k_cells<1> for corner dumps)
 //   k_normals          4-tap normal of the slab's new vertices
-//   k_quads<0>/k_scan/k_quads<1>  the same count/scan/emit scheme for triangles
+//   k_quads<0> / scan / k_quads<1>  the same count/scan/emit scheme for triangles
+// (with several slabs the evaluation runs one slab ahead on a second stream)
 // Cells are numbered in (z, y, x) order inside the slab and every block owns 1024
 // consecutive cells, so block offsets from the scan give each vertex and triangle its
 // place in the reference's output order (VertexList order, mesh.rs:280-321).
@@ -466,85 +469,98 @@ __global__ __launch_bounds__(256) void k_popc(const uint64_t* __restrict__ bits,
 }
 
 // ---------------------------------------------------------------------------------
-// Single-block exclusive scan of the per-block counts.  which = 0: vertices, 1:
-// triangles.  Adds the running total of earlier slabs and advances it.
+// Exclusive scan of the per-block counts (which = 0: vertices, 1: triangles), in two
+// launches of one workgroup per kDenseScanChunk counts, so that no single workgroup's latency
+// is on the slab's critical path (the dense path evaluates the next slab on another stream
+// at the same time).  k_scan_sums: each chunk's sum into part[g] (the words after the
+// counters); workgroup 0 saves the running total of earlier slabs as the *_PREV counter.
+// k_scan_offsets: each chunk's offsets from that total, the chunk sums before it and a
+// scan within the workgroup; the last workgroup advances the running total.
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_scan(uint32_t* __restrict__ bcount, uint32_t nb,
-                                               uint32_t* __restrict__ boffset,
-                                               uint64_t* __restrict__ counters, uint32_t which)
+__device__ __forceinline__ uint4 load_counts4(const uint32_t* __restrict__ bcount, uint32_t nb, uint32_t i)
 {
-    // Thread t owns the run [t * per, t * per + per) (per a multiple of 4, at most
-    // kScanPer), read as independent 16-byte vectors twice: summed, scanned across the
-    // block, then read again (from L2) and written as offsets.
-    constexpr uint32_t kScanPer = kMaxScanBlocks / kScanThreads;
-    static_assert(kScanPer % 4 == 0, "runs are whole 16-byte vectors");
-    __shared__ uint64_t wsum[kScanThreads / 64];
-    const uint32_t per = ((nb + kScanThreads - 1) / kScanThreads + 3u) & ~3u;
-    const uint32_t beg = threadIdx.x * per;
-    auto load4 = [&](uint32_t i) {
-        uint4 q = make_uint4(0u, 0u, 0u, 0u);
-        if (i + 3 < nb) {
-            q = *reinterpret_cast<const uint4*>(bcount + i);
-        } else if (i < nb) {
-            q.x = bcount[i];
-            q.y = i + 1 < nb ? bcount[i + 1] : 0u;
-            q.z = i + 2 < nb ? bcount[i + 2] : 0u;
-        }
-        return q;
-    };
-    uint64_t s = 0;
-#pragma unroll
-    for (uint32_t k = 0; k < kScanPer; k += 4) {
-        if (k < per) {
-            const uint4 q = load4(beg + k);
-            s += (uint64_t)q.x + q.y + q.z + q.w;
-        }
+    uint4 q = make_uint4(0u, 0u, 0u, 0u);
+    if (i + 3 < nb) {
+        q = *reinterpret_cast<const uint4*>(bcount + i);
+    } else if (i < nb) {
+        q.x = bcount[i];
+        q.y = i + 1 < nb ? bcount[i + 1] : 0u;
+        q.z = i + 2 < nb ? bcount[i + 2] : 0u;
     }
+    return q;
+}
+
+// Inclusive prefix sum over a 256-thread workgroup (wsum: 4 words of LDS); *total gets
+// the workgroup's sum.  Contains barriers.
+__device__ __forceinline__ uint64_t block_inclusive_256(uint64_t v, uint64_t* wsum, uint64_t* total)
+{
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    uint64_t inc = s;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        const uint64_t t = __shfl_up(inc, d, 64);
-        if (lane >= (uint32_t)d) inc += t;
+        const uint64_t t = __shfl_up(v, d, 64);
+        if (lane >= (uint32_t)d) v += t;
     }
-    if (lane == 63) wsum[w] = inc;
-    uint64_t* tot_ptr = counters + (which ? kTriTotal : kVtxTotal);
-    const uint64_t base = *tot_ptr;
+    if (lane == 63) wsum[w] = v;
     __syncthreads();
-    uint64_t woff = 0, total = 0;
-    for (uint32_t k = 0; k < kScanThreads / 64; ++k) {
-        woff += (k < w) ? wsum[k] : 0ull;
-        total += wsum[k];
-    }
-    uint64_t run = base + woff + inc - s;
+    uint64_t before = 0, all = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < kScanPer; k += 4) {
-        const uint32_t i = beg + k;
-        if (k < per && i < nb) {
-            const uint4 q = load4(i);
-            uint4 o;
-            o.x = (uint32_t)run;
-            run += q.x;
-            o.y = (uint32_t)run;
-            run += q.y;
-            o.z = (uint32_t)run;
-            run += q.z;
-            o.w = (uint32_t)run;
-            run += q.w;
-            if (i + 3 < nb) {
-                *reinterpret_cast<uint4*>(boffset + i) = o;
-            } else {
-                boffset[i] = o.x;
-                if (i + 1 < nb) boffset[i + 1] = o.y;
-                if (i + 2 < nb) boffset[i + 2] = o.z;
-            }
+    for (uint32_t k = 0; k < kBlock / 64; ++k) {
+        before += k < w ? wsum[k] : 0ull;
+        all += wsum[k];
+    }
+    __syncthreads();  // wsum may be reused
+    *total = all;
+    return before + v;
+}
+
+static_assert(kDenseScanChunk == 4 * kBlock, "one 16-byte vector of counts per thread");
+
+__global__ __launch_bounds__(256) void k_scan_sums(const uint32_t* __restrict__ bcount, uint32_t nb,
+                                                   uint64_t* __restrict__ counters, uint32_t which)
+{
+    __shared__ uint64_t wsum[kBlock / 64];
+    const uint4 q = load_counts4(bcount, nb, blockIdx.x * kDenseScanChunk + 4u * threadIdx.x);
+    uint64_t total;
+    (void)block_inclusive_256((uint64_t)q.x + q.y + q.z + q.w, wsum, &total);
+    if (threadIdx.x == 0) {
+        counters[kNumCounters + blockIdx.x] = total;
+        if (blockIdx.x == 0) counters[which ? kTriPrev : kVtxPrev] = counters[which ? kTriTotal : kVtxTotal];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_scan_offsets(const uint32_t* __restrict__ bcount, uint32_t nb,
+                                                      uint32_t* __restrict__ boffset,
+                                                      uint64_t* __restrict__ counters, uint32_t which)
+{
+    __shared__ uint64_t wsum[kBlock / 64];
+    const uint32_t g = blockIdx.x;
+    uint64_t parts;  // the chunk sums before this chunk (at most kDenseScanParts of them)
+    (void)block_inclusive_256(threadIdx.x < g ? counters[kNumCounters + threadIdx.x] : 0ull, wsum, &parts);
+    const uint32_t i = g * kDenseScanChunk + 4u * threadIdx.x;
+    const uint4 q = load_counts4(bcount, nb, i);
+    const uint64_t s = (uint64_t)q.x + q.y + q.z + q.w;
+    uint64_t total;
+    const uint64_t inc = block_inclusive_256(s, wsum, &total);
+    const uint64_t base = counters[which ? kTriPrev : kVtxPrev] + parts;
+    uint64_t run = base + inc - s;
+    if (i < nb) {
+        uint4 o;
+        o.x = (uint32_t)run;
+        run += q.x;
+        o.y = (uint32_t)run;
+        run += q.y;
+        o.z = (uint32_t)run;
+        run += q.z;
+        o.w = (uint32_t)run;
+        if (i + 3 < nb) {
+            *reinterpret_cast<uint4*>(boffset + i) = o;
+        } else {
+            boffset[i] = o.x;
+            if (i + 1 < nb) boffset[i + 1] = o.y;
+            if (i + 2 < nb) boffset[i + 2] = o.z;
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        counters[which ? kTriPrev : kVtxPrev] = base;
-        *tot_ptr = base + total;
-    }
+    if (g + 1 == gridDim.x && threadIdx.x == 0) counters[which ? kTriTotal : kVtxTotal] = base + total;
 }
 
 // ---------------------------------------------------------------------------------
@@ -802,9 +818,11 @@ hipError_t launch_bailout_check(hipStream_t s, uint64_t* cnt)
 
 hipError_t launch_scan(hipStream_t s, const MeshBufs& b, uint32_t nblocks, uint32_t which)
 {
-    if (nblocks > kMaxScanBlocks) return hipErrorInvalidValue;  // k_scan holds at most this many
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanThreads), 0, s, b.bcount, nblocks, b.boffset,
-                       b.counters, which);
+    if (nblocks > kMaxScanBlocks) return hipErrorInvalidValue;  // kDenseScanParts chunk sums at most
+    const uint32_t chunks = nblocks ? (nblocks + kDenseScanChunk - 1) / kDenseScanChunk : 1u;
+    hipLaunchKernelGGL(k_scan_sums, dim3(chunks), dim3(kBlock), 0, s, b.bcount, nblocks, b.counters, which);
+    hipLaunchKernelGGL(k_scan_offsets, dim3(chunks), dim3(kBlock), 0, s, b.bcount, nblocks, b.boffset, b.counters,
+                       which);
     return hipGetLastError();
 }
 
