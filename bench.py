@@ -222,7 +222,9 @@ def single_job(name: str, args, ctx, headline: bool) -> dict:
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
             "frac_of_issue_ceiling": round(achieved / ISSUE_CEILING_TOPS, 4),
-            "traffic": measured_traffic(name + ("-dense" if args.dense else ""), kernel),
+            "traffic": measured_traffic(name + ("-dense" if args.dense else ""), kernel,
+                                        {} if tuned is None or args.slab else
+                                        {k: tuned[k] for k in ("stream_passes", "dense_slab")}),
             "launch_ms": round(eval_avg_ms, 4),
             "pruned_program_flops_per_launch" if tiled else "flops_per_launch": flops_per_launch,
             "points_per_launch": agg["lattice_points"] / launches,
@@ -421,14 +423,17 @@ def run_copy_calibration(args) -> dict:
             "bytes_written_per_launch": 4 * n}
 
 
-def measured_traffic(workload: str, kernel: str):
+def measured_traffic(workload: str, kernel: str, shape: dict):
     """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-    summarised in profiles/hbm_traffic.json (scaled by the copy calibration), or None."""
+    summarised in profiles/hbm_traffic.json (scaled by the copy calibration), or None when
+    the run's pass / slab shape differs from the one measured there."""
     f = ROOT / "profiles" / "hbm_traffic.json"
     if not f.exists():
         return None
     entry = json.loads(f.read_text()).get(workload, {})
     if entry.get("kernel") != kernel:
+        return None
+    if any(shape.get(k) != v for k, v in entry.get("shape", {}).items()):
         return None
     return entry.get("bytes_per_launch")
 
