@@ -127,7 +127,8 @@ struct s2m_ctx {
     int profiling = 0;  // 0 off, 1 total + evaluation kernels, 2 every stage
     s2m_timing timing{};
     DevBuf lattice, vidx, bcount, boffset, actbits, counters, tables, check;
-    DevBuf lattice2;  // the dense path's second slab lattice (evaluation one slab ahead)
+    DevBuf lattice2, actbits2, bcount2;  // the dense path's second slab buffers (evaluation and
+                                         // count pass one slab ahead)
     DevBuf vtx, keys, signs, tri, corners;
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_slot_rows, sp_boundary;
@@ -1318,18 +1319,26 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
     const uint32_t nslabs = (j.nz_cells + T - 1) / T;
     if (nslabs == 1) rb = nullptr;  // nothing to overlap: the caller copies the finished mesh
     if (rb) HIP_TRY(ctx->host_snap.ensure((size_t)nslabs * 4 * sizeof(uint64_t)));
-    // Several slabs: slab k + 1's lattice is evaluated (VALU-bound) on eval_stream into the
-    // other of two lattice buffers while slab k is contoured (memory- and latency-bound) on
-    // the context's stream.  Events order it: a slab's contouring waits for its evaluation,
-    // an evaluation waits until the previous user of its buffer (the slab two before) has
-    // read it for the last time (the vertex pass), and the first evaluation waits for the
-    // tables and clears queued before it.
+    // Several slabs: slab k + 1's lattice is evaluated (VALU-bound) and counted (the
+    // activity bits and block counts: latency-bound gathers) on eval_stream, into the other
+    // of two sets of slab buffers, while slab k's map, vertices, normals and triangles are
+    // made on the context's stream.  The count pass runs right after its evaluation rather
+    // than beside the next one, which slows both.  Events order it: a slab's map waits for
+    // its count, an evaluation waits until the slab two before has finished with its
+    // buffers (its last kernel), and the first evaluation waits for the tables and clears
+    // queued before it.
     const bool piped = !fx && !h_corners && nslabs > 1;
     hipStream_t es = piped ? ctx->eval_stream : ctx->stream;
     float* lat_buf[2] = {lat, lat};
+    uint64_t* act_buf[2] = {actbits, actbits};
+    uint32_t* cnt_buf[2] = {bcount, bcount};
     if (piped) {
         HIP_TRY(ctx->lattice2.ensure(L.lattice));
+        HIP_TRY(ctx->actbits2.ensure(L.actbits));
+        HIP_TRY(ctx->bcount2.ensure(L.blocks));
         lat_buf[1] = ctx->lattice2.as<float>();
+        act_buf[1] = ctx->actbits2.as<uint64_t>();
+        cnt_buf[1] = ctx->bcount2.as<uint32_t>();
     }
 
     for (int attempt = 0; attempt < (fx ? 1 : 3); ++attempt) {
@@ -1365,8 +1374,9 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
         HIP_TRY(hipMemsetAsync(b.counters, 0, kNumCounters * sizeof(uint64_t), ctx->stream));
         HIP_TRY(hipMemsetAsync(vidx, 0xff, (size_t)plane * sizeof(uint32_t), ctx->stream));
         // the count pass ORs into zeroed words; each slab's emit pass clears what it read
-        HIP_TRY(hipMemsetAsync(b.actbits, 0, (size_t)max_blocks * (kCellsPerBlock / 64) * sizeof(uint64_t),
-                               ctx->stream));
+        for (uint64_t* words : {act_buf[0], act_buf[1]})
+            HIP_TRY(hipMemsetAsync(words, 0, (size_t)max_blocks * (kCellsPerBlock / 64) * sizeof(uint64_t),
+                                   ctx->stream));
 
         ctx->events_used = 0;
         // per slab 6 marks: evaluation start and end (on the evaluation's stream), then on
@@ -1382,7 +1392,7 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             if (e && hipEventRecord(e, st) != hipSuccess) e = nullptr;
             return e;
         };
-        hipEvent_t lat_free[2] = {nullptr, nullptr};  // per lattice buffer: its last read done
+        hipEvent_t lat_free[2] = {nullptr, nullptr};  // per buffer set: its slab's last kernel done
         if (piped) {
             hipEvent_t ready = mark(ctx->stream);
             if (!ready) return fail(S2M_E_HIP, "event record failed");
@@ -1430,33 +1440,36 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
                 lattice_points += (uint64_t)nx * ny * nzl;
             }
             if (m[1]) HIP_TRY(hipEventRecord(m[1], es));
+            if (m[5]) HIP_TRY(hipEventRecord(m[5], es));
+            MeshBufs bs = b;  // this slab's activity bits and block counts
+            bs.actbits = act_buf[slabs & 1];
+            bs.bcount = cnt_buf[slabs & 1];
+            if (!h_corners) HIP_TRY(launch_cells(es, a, bs, 0, 0));  // the count pass
             if (piped) {
-                hipEvent_t evald = mark(es);
-                if (!evald) return fail(S2M_E_HIP, "event record failed");
-                HIP_TRY(hipStreamWaitEvent(ctx->stream, evald, 0));
+                hipEvent_t counted = mark(es);
+                if (!counted) return fail(S2M_E_HIP, "event record failed");
+                HIP_TRY(hipStreamWaitEvent(ctx->stream, counted, 0));
             }
-            if (m[5]) HIP_TRY(hipEventRecord(m[5], ctx->stream));
             if (h_corners) {
                 const float* src = ctx->corners.as<float>() + (size_t)z0 * plane * 8;
                 HIP_TRY(launch_cells_from_corners(ctx->stream, a, src, b, 0));
                 HIP_TRY(launch_scan(ctx->stream, b, nb, 0));
                 HIP_TRY(launch_cells_from_corners(ctx->stream, a, src, b, 1));
             } else {
-                HIP_TRY(launch_cells(ctx->stream, a, b, 0, 0));
-                HIP_TRY(launch_scan(ctx->stream, b, nb, 0));
-                HIP_TRY(launch_cells(ctx->stream, a, b, 1, normals ? 0 : 1));
-                if (piped) {  // the vertex pass was the slab lattice's last reader
-                    lat_free[slabs & 1] = mark(ctx->stream);
-                    if (!lat_free[slabs & 1]) return fail(S2M_E_HIP, "event record failed");
-                }
+                HIP_TRY(launch_scan(ctx->stream, bs, nb, 0));
+                HIP_TRY(launch_cells(ctx->stream, a, bs, 1, normals ? 0 : 1));
             }
             if (m[2]) HIP_TRY(hipEventRecord(m[2], ctx->stream));
-            if (normals) HIP_TRY(normals_prog(ctx->stream, prog, p->eps, b));
+            if (normals) HIP_TRY(normals_prog(ctx->stream, prog, p->eps, bs));
             if (m[3]) HIP_TRY(hipEventRecord(m[3], ctx->stream));
-            HIP_TRY(launch_quads(ctx->stream, a, b, 0));
-            HIP_TRY(launch_scan(ctx->stream, b, nb, 1));
-            HIP_TRY(launch_quads(ctx->stream, a, b, 1));
+            HIP_TRY(launch_quads(ctx->stream, a, bs, 0));
+            HIP_TRY(launch_scan(ctx->stream, bs, nb, 1));
+            HIP_TRY(launch_quads(ctx->stream, a, bs, 1));
             if (m[4]) HIP_TRY(hipEventRecord(m[4], ctx->stream));
+            if (piped) {  // the slab's lattice, bits and counts are free for the slab two later
+                lat_free[slabs & 1] = mark(ctx->stream);
+                if (!lat_free[slabs & 1]) return fail(S2M_E_HIP, "event record failed");
+            }
             if (rb) {  // the slab's vertices and triangles are final; the copies of the slab two
                        // before (waiting for it leaves this slab and the next evaluation queued)
                 int rrc = readback_mark(ctx, b.counters, (uint32_t)slabs);
@@ -1650,7 +1663,7 @@ void s2m_destroy(s2m_ctx* ctx)
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipStreamSynchronize(ctx->copy_stream);
     (void)hipStreamSynchronize(ctx->eval_stream);
-    for (DevBuf* b : {&ctx->lattice, &ctx->lattice2, &ctx->vidx, &ctx->bcount, &ctx->boffset, &ctx->actbits, &ctx->check, &ctx->counters,
+    for (DevBuf* b : {&ctx->lattice, &ctx->lattice2, &ctx->actbits2, &ctx->bcount2, &ctx->vidx, &ctx->bcount, &ctx->boffset, &ctx->actbits, &ctx->check, &ctx->counters,
                       &ctx->tables, &ctx->vtx, &ctx->keys, &ctx->signs, &ctx->tri, &ctx->corners,
                       &ctx->tile_tab, &ctx->tile_lists, &ctx->tile_lens,
                       &ctx->tile_flops, &ctx->sp_slot_of_tile, &ctx->sp_tile_of_slot,
