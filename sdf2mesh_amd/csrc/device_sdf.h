@@ -142,9 +142,10 @@ __device__ __forceinline__ float sd_mandelbulb(float px, float py, float pz, flo
         // ((zx, zy) / rho)^8 as complex numbers (rho = |(zx, zy)|).  phi = asin(zz / r) lies
         // in [-pi/2, pi/2], so (cos phi, sin phi) = (rho / r, zz / r) and (cos 8 phi,
         // sin 8 phi) is its 8th power.  Three complex squarings each; 1 / r and 1 / rho from
-        // reciprocal square roots; r^8 and r^7 by products of r^2.  The same function as the
-        // written form, with fewer roundings (its error against the fp64 estimator is
-        // bounded as for the written form).  The bailout test compares r^2 with the
+        // reciprocal square roots; r^8 and r^7 by products of r^2; fused multiply-adds where
+        // a product meets a sum.  The same function as the written form, with fewer
+        // roundings (its error against the fp64 estimator is bounded as for the written
+        // form).  The bailout test compares r^2 with the
         // threshold whose correctly rounded square root first exceeds the bailout, so every
         // point leaves the loop where the written form does; the r it leaves with is the
         // same correctly rounded length.  rho = 0 gives NaN, as atan(0 / 0) does.
@@ -161,9 +162,9 @@ __device__ __forceinline__ float sd_mandelbulb(float px, float py, float pz, flo
             const float rho = xy2 * irho;
             float ct = zx * irho, st = zy * irho, cp = rho * ir, sp = zz * ir;
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                const float c2 = ct * ct - st * st, s2 = 2.0f * (ct * st);
-                const float p2 = cp * cp - sp * sp, q2 = 2.0f * (cp * sp);
+            for (int k = 0; k < 3; ++k) {  // (c + i s)^2 = (c^2 - s^2) + i 2cs, one rounding for the real part
+                const float c2 = __builtin_fmaf(ct, ct, -(st * st)), s2 = (ct + ct) * st;
+                const float p2 = __builtin_fmaf(cp, cp, -(sp * sp)), q2 = (cp + cp) * sp;
                 ct = c2;
                 st = s2;
                 cp = p2;
@@ -171,10 +172,10 @@ __device__ __forceinline__ float sd_mandelbulb(float px, float py, float pz, flo
             }
             const float r4 = r2 * r2;
             r = r4 * r4;
-            dr = ((r * ir) * dr) * 8.0f + 1.0f;
-            zx = r * (ct * cp) + px;
-            zy = r * (st * cp) + py;
-            zz = r * sp + pz;
+            dr = __builtin_fmaf((r * ir) * dr, 8.0f, 1.0f);
+            zx = __builtin_fmaf(r, ct * cp, px);
+            zy = __builtin_fmaf(r, st * cp, py);
+            zz = __builtin_fmaf(r, sp, pz);
         }
         if (r2_out >= 0.0f) r = sqrtf(r2_out);
         return ((0.5f * logf(r)) * r) / dr - offset;
