@@ -310,9 +310,10 @@ int s2m_mesh_device_noalloc(s2m_ctx* ctx, const s2m_program* prog, const s2m_par
  * into consecutive passes.  Compiled programs (no tape) are meshed one at a time. */
 int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
                    const s2m_params* params, int want_keys, s2m_mesh_out* outs);
-/* One pass only, results left in the context's device buffers (valid until its next
- * meshing call): outs[i] points at job i's vertices and triangles.  Tape programs only
- * (S2M_E_ARG for a compiled one). */
+/* One pass only, results left in device buffers the context owns (valid until its next
+ * meshing call): outs[i] points at job i's vertices and triangles.  Jobs for the dense
+ * path (compiled programs, the Mandelbulb above 128^3 cells, S2M_DENSE_EVAL) are meshed
+ * one at a time before the pass, each into buffers of its own. */
 int s2m_mesh_batch_device(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
                           const s2m_params* params, s2m_device_mesh* outs);
 

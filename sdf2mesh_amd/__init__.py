@@ -566,7 +566,8 @@ class Context:
         pa = (_Params * n)(*[q.c() for q in params])
         outs = (_DeviceMesh * n)()
         _check(lib().s2m_mesh_batch_device(C.c_void_p(self.handle), n, hp, pa, outs))
-        return [{"nvtx": int(o.nvtx), "ntri": int(o.ntri), "invalid_quads": int(o.invalid_quads)}
+        return [{"nvtx": int(o.nvtx), "ntri": int(o.ntri), "invalid_quads": int(o.invalid_quads),
+                 "vtx_ptr": o.vtx, "tri_ptr": o.tri, "keys_ptr": o.cell_keys, "signs_ptr": o.signs}
                 for o in outs]
 
     def mesh_device(self, prog: Program, params: Params) -> dict:

@@ -129,6 +129,9 @@ struct s2m_ctx {
     DevBuf lattice, vidx, bcount, boffset, actbits, counters, tables, check;
     DevBuf lattice2, actbits2, bcount2;  // the dense path's second slab buffers (evaluation and
                                          // count pass one slab ahead)
+    // s2m_mesh_batch_device: the meshes of its dense-path jobs (vertices, keys, signs,
+    // triangles: four buffers per job), kept beside the shared pass's output buffers
+    std::vector<DevBuf> batch_dense;
     DevBuf vtx, keys, signs, tri, corners;
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_slot_rows, sp_boundary;
@@ -1672,6 +1675,7 @@ void s2m_destroy(s2m_ctx* ctx)
                       &ctx->sp_super_lists, &ctx->sp_super_lens, &ctx->sp_super_act, &ctx->sp_rows,
                       &ctx->sp_stage, &ctx->sp_slot_nb, &ctx->sp_slot_stage, &ctx->sp_slot_live, &ctx->sp_slot_hdr, &ctx->sp_slot_rec, &ctx->sp_block_stats})
         b->release();
+    for (DevBuf& b : ctx->batch_dense) b.release();
     ctx->host_img.release();
     ctx->host_back.release();
     ctx->host_snap.release();
@@ -2053,18 +2057,48 @@ int s2m_mesh_batch_device(s2m_ctx* ctx, uint32_t n, const s2m_program* const* pr
     std::vector<const s2m_params*> qq;
     std::vector<Job> jj;
     std::vector<uint32_t> idx;
-    for (uint32_t i = 0; i < n; ++i)
-        if (progs[i]->module)
-            return fail(S2M_E_ARG, "program %u is compiled; device batches take tape programs (use s2m_mesh_device)", i);
+    for (uint32_t i = 0; i < n; ++i) std::memset(&outs[i], 0, sizeof outs[i]);
+    // jobs for the dense path (compiled programs, the Mandelbulb above 128^3, S2M_DENSE_EVAL)
+    // are meshed one at a time first; each mesh moves to buffers of its own, since the
+    // shared pass below writes the context's output buffers
+    uint32_t ndense = 0;
     for (uint32_t i = 0; i < n; ++i) {
-        std::memset(&outs[i], 0, sizeof outs[i]);
-        if ((uint64_t)jobs[i].res[0] * jobs[i].res[1] * jobs[i].nz_cells == 0) continue;
+        const Job& j = jobs[i];
+        if ((uint64_t)j.res[0] * j.res[1] * j.nz_cells == 0 || !dense_job(progs[i], params[i], j)) continue;
+        rc = mesh_one(ctx, progs[i], &params[i], nullptr);
+        if (rc) return rc;
+        const uint64_t nv = ctx->last_nvtx, nt = ctx->last_ntri;
+        if (ctx->batch_dense.size() < 4 * (size_t)(ndense + 1)) ctx->batch_dense.resize(4 * (size_t)(ndense + 1));
+        DevBuf* d = &ctx->batch_dense[4 * (size_t)ndense++];
+        HIP_TRY(d[0].ensure(std::max<uint64_t>(nv, 1) * 6 * sizeof(float)));
+        HIP_TRY(d[1].ensure(std::max<uint64_t>(nv, 1) * sizeof(uint64_t)));
+        HIP_TRY(d[2].ensure(std::max<uint64_t>(nv, 1)));
+        HIP_TRY(d[3].ensure(std::max<uint64_t>(nt, 1) * 3 * sizeof(uint32_t)));
+        HIP_TRY(hipMemcpyAsync(d[0].p, ctx->vtx.p, nv * 6 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(d[1].p, ctx->keys.p, nv * sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(d[2].p, ctx->signs.p, nv, hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(d[3].p, ctx->tri.p, nt * 3 * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
+        s2m_device_mesh& o = outs[i];
+        o.vtx = d[0].as<float>();
+        o.cell_keys = d[1].as<uint64_t>();
+        o.signs = d[2].as<uint8_t>();
+        o.tri = d[3].as<uint32_t>();
+        o.nvtx = nv;
+        o.ntri = nt;
+        o.invalid_quads = ctx->last_invalid;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        const Job& j = jobs[i];
+        if ((uint64_t)j.res[0] * j.res[1] * j.nz_cells == 0 || dense_job(progs[i], params[i], j)) continue;
         pp.push_back(progs[i]);
         qq.push_back(&params[i]);
         jj.push_back(jobs[i]);
         idx.push_back(i);
     }
-    if (idx.empty()) return S2M_OK;
+    if (idx.empty()) {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));  // the dense meshes' copies
+        return S2M_OK;
+    }
     std::vector<SpRange> r;
     rc = run_sparse(ctx, (uint32_t)idx.size(), pp.data(), qq.data(), jj.data(), r);
     if (rc) return rc;

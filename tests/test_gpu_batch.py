@@ -59,3 +59,44 @@ def test_batch_repeatable(ctx):
     b = ctx.mesh_batch(progs[::-1], params[::-1])[::-1]  # order of jobs does not matter
     for x, y in zip(a, b):
         same(x, y)
+
+
+def device_copy(dm):
+    """A device-resident mesh (vtx/tri/keys/signs addresses) copied with plain hipMemcpy."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    nv, nt = dm["nvtx"], dm["ntri"]
+
+    def get(ptr, dtype, count):
+        a = np.empty(count, dtype)
+        if count:
+            assert hip.hipMemcpy(a.ctypes.data, ptr, a.nbytes, 2) == 0  # device to host
+        return a
+    return s2m.TriangleMesh(get(dm["vtx_ptr"], np.float32, 6 * nv).reshape(nv, 6),
+                            get(dm["tri_ptr"], np.uint32, 3 * nt).reshape(nt, 3), dm["invalid_quads"],
+                            get(dm["keys_ptr"], np.uint64, nv), get(dm["signs_ptr"], np.uint8, nv))
+
+
+def test_device_batch_with_dense_jobs(ctx):
+    # s2m_mesh_batch_device with jobs for the dense path among the tile-sparse ones: a
+    # compiled program, a Mandelbulb above 128^3, a CSG tape forced dense; each result,
+    # read back from the device, equals the job meshed alone
+    from pathlib import Path
+
+    golden = Path(__file__).resolve().parent / "golden"
+    eng = ctx.program_from_source(golden / "engraved_cube.sdf3d", path=True)
+    assert eng.kind == s2m.PROGRAM_COMPILED
+    mb = ctx.program(tape.mandelbulb(6))
+    csg = [ctx.program(tape.random_csg(s)) for s in (2, 9)]
+    progs = [csg[0], eng, mb, csg[1], csg[0], eng]
+    params = [s2m.Params.reference(96, 2.0), s2m.Params.reference(64, 2.0), s2m.Params.reference(160, 2.5),
+              s2m.Params.reference(48, 2.0, flags=s2m.REFERENCE_COMPAT | s2m.DENSE_EVAL),
+              s2m.Params.reference(0, 2.0), s2m.Params.reference(37, 2.0, flags=0)]
+    dev = ctx.mesh_batch_device(progs, params)
+    got = [device_copy(d) for d in dev]  # before any other meshing call on the context
+    for i, (p, q) in enumerate(zip(progs, params)):
+        solo = ctx.mesh(p, q)
+        print(f"device batch job {i}: {len(solo.vertices)} vertices", flush=True)
+        same(got[i], solo)
