@@ -420,6 +420,9 @@ int s2m_format_f32(float x, char* buf);
  * (hipHostMalloc calls), [1] page-locked bytes held, [2] idle blocks kept for reuse,
  * [3] their bytes.  A repeated meshing call of the same size adds no block. */
 void s2m_host_pool_stats(uint64_t stats[4]);
+/* No reference counterpart: returns every idle page-locked block of that pool to the
+ * system (results still held by callers are not touched). */
+void s2m_host_pool_trim(void);
 /* Per-axis coordinate tables used for corner evaluation.  coord receives the
  * distinct coordinates (at most 2*res+1), lo/hi (res entries each) the index of each
  * cell's min and max corner.  Returns the number of coordinates, or 0 on error. */

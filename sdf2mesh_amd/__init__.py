@@ -168,6 +168,7 @@ _SIGNATURES = {
     "s2m_check_sincos": (C.c_int, [C.c_void_p, C.c_void_p]),
     "s2m_check_bailout": (C.c_int, [C.c_void_p, C.c_void_p]),
     "s2m_host_pool_stats": (None, [C.c_void_p]),
+    "s2m_host_pool_trim": (None, []),
     "s2m_axis_table": (C.c_uint32, [C.c_float, C.c_float, C.c_uint32, C.c_int, C.c_void_p,
                                     C.c_void_p, C.c_void_p]),
     "s2m_eval_lattice": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(_Params), C.c_void_p,
@@ -349,6 +350,11 @@ def host_pool_stats() -> dict:
     lib().s2m_host_pool_stats(st)
     return {"pinned_allocs": int(st[0]), "pinned_bytes": int(st[1]), "idle_blocks": int(st[2]),
             "idle_bytes": int(st[3])}
+
+
+def host_pool_trim() -> None:
+    """s2m_host_pool_trim: return the pool's idle page-locked blocks to the system."""
+    lib().s2m_host_pool_trim()
 
 
 def tune_pick(ms) -> int:

@@ -132,6 +132,11 @@ struct s2m_ctx {
     // s2m_mesh_batch_device: the meshes of its dense-path jobs (vertices, keys, signs,
     // triangles: four buffers per job), kept beside the shared pass's output buffers
     std::vector<DevBuf> batch_dense;
+    // s2m_mesh_batch: two staging sets (vertices, keys, signs, triangles) that a pass's
+    // meshes are copied into on the device, so that their copy to host memory (on
+    // copy_stream) overlaps the next pass; per set, events for "staged" and "copied out"
+    DevBuf stage[2][4];
+    hipEvent_t staged[2] = {nullptr, nullptr}, stage_free[2] = {nullptr, nullptr};
     DevBuf vtx, keys, signs, tri, corners;
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_slot_rows, sp_boundary;
@@ -1676,6 +1681,10 @@ void s2m_destroy(s2m_ctx* ctx)
                       &ctx->sp_stage, &ctx->sp_slot_nb, &ctx->sp_slot_stage, &ctx->sp_slot_live, &ctx->sp_slot_hdr, &ctx->sp_slot_rec, &ctx->sp_block_stats})
         b->release();
     for (DevBuf& b : ctx->batch_dense) b.release();
+    for (auto& set : ctx->stage)
+        for (DevBuf& b : set) b.release();
+    for (hipEvent_t e : {ctx->staged[0], ctx->staged[1], ctx->stage_free[0], ctx->stage_free[1]})
+        if (e) (void)hipEventDestroy(e);
     ctx->host_img.release();
     ctx->host_back.release();
     ctx->host_snap.release();
@@ -1843,11 +1852,15 @@ int s2m_mesh(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, in
 namespace {
 
 // Copy job ranges of the context's output buffers to fresh host meshes.
-int copy_ranges(s2m_ctx* ctx, const SpRange* r, uint32_t n, int want_keys, s2m_mesh_out* outs)
+int copy_ranges(s2m_ctx* ctx, const SpRange* r, uint32_t n, int want_keys, s2m_mesh_out* outs, int set = -1)
 {
     // One page-locked block per array for the whole pass (one DMA copy each): every job's
     // array points inside it, and the block is recycled after the last job's mesh is freed.
-    // Jobs without vertices (or triangles) get arrays of their own.
+    // Jobs without vertices (or triangles) get arrays of their own.  set < 0: the copies run
+    // on the compute stream and are finished on return.  set 0 / 1: the pass's ranges are
+    // first copied on the device into staging set `set` (once its previous copy-out is
+    // done), then to host memory on copy_stream, unfinished on return: the caller
+    // synchronises copy_stream before handing the meshes out.
     uint64_t v_lo = ~0ull, v_hi = 0, t_lo = ~0ull, t_hi = 0;
     uint32_t v_refs = 0, t_refs = 0;
     for (uint32_t i = 0; i < n; ++i) {
@@ -1902,20 +1915,54 @@ int copy_ranges(s2m_ctx* ctx, const SpRange* r, uint32_t n, int want_keys, s2m_m
         for (; t_given < t_refs; ++t_given) host_release(Tr);
         return fail(S2M_E_OOM, "host allocation failed");
     }
-    if (v_refs) {
-        HIP_TRY(hipMemcpyAsync(V, ctx->vtx.as<float>() + 6 * v_lo, (v_hi - v_lo) * 6 * sizeof(float),
-                               hipMemcpyDeviceToHost, ctx->stream));
+    const float* src_v = ctx->vtx.as<float>() + 6 * v_lo;
+    const uint64_t* src_k = ctx->keys.as<uint64_t>() + v_lo;
+    const uint8_t* src_s = ctx->signs.as<uint8_t>() + v_lo;
+    const uint32_t* src_t = ctx->tri.as<uint32_t>() + 3 * t_lo;
+    const uint64_t nv = v_refs ? v_hi - v_lo : 0, nt = t_refs ? t_hi - t_lo : 0;
+    hipStream_t out_stream = ctx->stream;
+    if (set >= 0) {
+        for (hipEvent_t* e : {&ctx->staged[set], &ctx->stage_free[set]})
+            if (!*e) HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        DevBuf* st = ctx->stage[set];
+        // a larger staging buffer replaces the old one only after its copy-out is done
+        if (st[0].bytes < nv * 6 * sizeof(float) || st[3].bytes < nt * 3 * sizeof(uint32_t) ||
+            (want_keys && (st[1].bytes < nv * sizeof(uint64_t) || st[2].bytes < nv)))
+            HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
+        HIP_TRY(st[0].ensure(std::max<uint64_t>(nv, 1) * 6 * sizeof(float)));
+        HIP_TRY(st[3].ensure(std::max<uint64_t>(nt, 1) * 3 * sizeof(uint32_t)));
         if (want_keys) {
-            HIP_TRY(hipMemcpyAsync(K, ctx->keys.as<uint64_t>() + v_lo, (v_hi - v_lo) * sizeof(uint64_t),
-                                   hipMemcpyDeviceToHost, ctx->stream));
-            HIP_TRY(hipMemcpyAsync(S, ctx->signs.as<uint8_t>() + v_lo, v_hi - v_lo, hipMemcpyDeviceToHost,
-                                   ctx->stream));
+            HIP_TRY(st[1].ensure(std::max<uint64_t>(nv, 1) * sizeof(uint64_t)));
+            HIP_TRY(st[2].ensure(std::max<uint64_t>(nv, 1)));
+        }
+        HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->stage_free[set], 0));
+        HIP_TRY(hipMemcpyAsync(st[0].p, src_v, nv * 6 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(st[3].p, src_t, nt * 3 * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
+        if (want_keys) {
+            HIP_TRY(hipMemcpyAsync(st[1].p, src_k, nv * sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream));
+            HIP_TRY(hipMemcpyAsync(st[2].p, src_s, nv, hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        HIP_TRY(hipEventRecord(ctx->staged[set], ctx->stream));
+        HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, ctx->staged[set], 0));
+        src_v = st[0].as<float>();
+        src_k = st[1].as<uint64_t>();
+        src_s = st[2].as<uint8_t>();
+        src_t = st[3].as<uint32_t>();
+        out_stream = ctx->copy_stream;
+    }
+    if (v_refs) {
+        HIP_TRY(hipMemcpyAsync(V, src_v, nv * 6 * sizeof(float), hipMemcpyDeviceToHost, out_stream));
+        if (want_keys) {
+            HIP_TRY(hipMemcpyAsync(K, src_k, nv * sizeof(uint64_t), hipMemcpyDeviceToHost, out_stream));
+            HIP_TRY(hipMemcpyAsync(S, src_s, nv, hipMemcpyDeviceToHost, out_stream));
         }
     }
-    if (t_refs)
-        HIP_TRY(hipMemcpyAsync(Tr, ctx->tri.as<uint32_t>() + 3 * t_lo, (t_hi - t_lo) * 3 * sizeof(uint32_t),
-                               hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (t_refs) HIP_TRY(hipMemcpyAsync(Tr, src_t, nt * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, out_stream));
+    if (set >= 0) {
+        HIP_TRY(hipEventRecord(ctx->stage_free[set], ctx->copy_stream));
+    } else {
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+    }
     return S2M_OK;
 }
 
@@ -1996,6 +2043,13 @@ int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
             if (rc) return rc;
             add_timing(ctx->timing);
         }
+    // each pass's meshes go to host memory on copy_stream while the next pass computes
+    // (staging sets alternate); the copies are finished before the meshes are returned
+    uint32_t npass = 0;
+    auto drain = [ctx](int code) {
+        (void)hipStreamSynchronize(ctx->copy_stream);
+        return code;
+    };
     while (i < n) {
         uint32_t k = i;
         uint64_t cells = 0, tiles = 0;
@@ -2022,11 +2076,11 @@ int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
         if (!idx.empty()) {
             std::vector<SpRange> r;
             rc = run_sparse(ctx, (uint32_t)idx.size(), pp.data(), qq.data(), jj.data(), r);
-            if (rc) return rc;
+            if (rc) return drain(rc);
             std::vector<s2m_mesh_out> tmp(idx.size());
-            rc = copy_ranges(ctx, r.data(), (uint32_t)idx.size(), want_keys, tmp.data());
+            rc = copy_ranges(ctx, r.data(), (uint32_t)idx.size(), want_keys, tmp.data(), (int)(npass++ & 1u));
             for (size_t q = 0; q < idx.size(); ++q) outs[idx[q]] = tmp[q];
-            if (rc) return rc;
+            if (rc) return drain(rc);
             add_timing(ctx->timing);
         }
         for (uint32_t q = i; q < k; ++q)  // empty grids: empty meshes
@@ -2040,6 +2094,7 @@ int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
             }
         i = k;
     }
+    HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
     ctx->timing = sum;
     return S2M_OK;
 }

@@ -147,3 +147,19 @@ extern "C" void s2m_host_pool_stats(uint64_t stats[4])
     stats[2] = P.idle.size();
     stats[3] = P.idle_bytes;
 }
+
+extern "C" void s2m_host_pool_trim(void)
+{
+    Pool& P = pool();
+    std::vector<void*> drop;
+    {
+        std::lock_guard<std::mutex> g(P.mu);
+        for (auto& kv : P.idle) {
+            P.pinned_bytes -= kv.first;
+            drop.push_back(kv.second);
+        }
+        P.idle.clear();
+        P.idle_bytes = 0;
+    }
+    for (void* q : drop) (void)hipHostFree(q);
+}
