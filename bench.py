@@ -57,6 +57,69 @@ PEAK_HBM_GBS = 8000.0      # MI355X HBM3E
 SINGLE = ("csg64-512", "csg64-512-lattice", "torus-128", "mandelbulb-1024")
 
 
+class CopyProbe:
+    """The device-to-host copy rate right now: two 16 MB copies from device memory into
+    page-locked host memory (HIP runtime through ctypes), timed on the host.
+
+    Device memory that is freed is cleared by the kernel driver in the background on a
+    DMA engine, at about 10 GB/s; meanwhile copies to host memory run at half rate (1 GB
+    freed: ~60 ms, 8 GB: ~0.8 s).  Tuning regrows the library's pass buffers once per
+    candidate shape, so a workload's timed steps start only after ``settle`` has seen the
+    rate back near the link rate measured when the bench started (the library frees no
+    device memory in steady state)."""
+
+    def __init__(self):
+        import ctypes as C
+
+        self.C = C
+        hip = C.CDLL("libamdhip64.so")
+        for f, a in {"hipMalloc": [C.POINTER(C.c_void_p), C.c_size_t],
+                     "hipHostMalloc": [C.POINTER(C.c_void_p), C.c_size_t, C.c_uint],
+                     "hipMemcpyAsync": [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p],
+                     "hipStreamSynchronize": [C.c_void_p]}.items():
+            getattr(hip, f).restype = C.c_int
+            getattr(hip, f).argtypes = a
+        self.hip, self.n = hip, 16 << 20
+        self.dev, self.host = C.c_void_p(), C.c_void_p()
+        if hip.hipMalloc(C.byref(self.dev), self.n) or hip.hipHostMalloc(C.byref(self.host), self.n, 0):
+            raise RuntimeError("copy probe: allocation failed")
+        self.rate()
+        # the largest rate seen, once it has not grown by 5 % for 0.5 s (at most 3 s: memory
+        # freed by a process that ended just before may still be being cleared)
+        t0 = t_up = time.perf_counter()
+        self.link = 0.0
+        while time.perf_counter() - t_up < 0.5 and time.perf_counter() - t0 < 3.0:
+            r = self.rate()
+            if r > 1.05 * self.link:
+                t_up = time.perf_counter()
+            self.link = max(self.link, r)
+
+    def rate(self) -> float:
+        t0 = time.perf_counter()
+        for _ in range(2):
+            if self.hip.hipMemcpyAsync(self.host, self.dev, self.n, 2, None):  # device to host
+                raise RuntimeError("copy probe: hipMemcpyAsync failed")
+        if self.hip.hipStreamSynchronize(None):
+            raise RuntimeError("copy probe: hipStreamSynchronize failed")
+        return 2 * self.n / (time.perf_counter() - t0) / 1e9
+
+    def settle(self, max_s: float = 5.0) -> dict:
+        """Wait (at most max_s) until three rates in a row reach 85 % of the link rate."""
+        t0, ok, r = time.perf_counter(), 0, 0.0
+        while ok < 3 and time.perf_counter() - t0 < max_s:
+            r = self.rate()
+            ok = ok + 1 if r >= 0.85 * self.link else 0
+        return {"link_GB_per_s": round(self.link, 1), "waited_ms": round((time.perf_counter() - t0) * 1e3, 1),
+                "GB_per_s": round(r, 1)}
+
+
+_PROBE: CopyProbe | None = None
+
+
+def settle_copies() -> dict | None:
+    return _PROBE.settle() if _PROBE is not None else None
+
+
 def workload(name: str):
     import sdf2mesh_amd as s2m
     from sdf2mesh_amd import tape
@@ -150,6 +213,7 @@ def single_job(name: str, args, ctx, headline: bool) -> dict:
     for _ in range(args.warmup):
         ctx.mesh_to_host(prog, params).free()
         ctx.mesh_device(prog, params)
+    settle = settle_copies()
     ctx.set_profiling(1)  # events around the whole call and the evaluation launches only
     host_s, raw = [], []
     last = None
@@ -208,7 +272,8 @@ def single_job(name: str, args, ctx, headline: bool) -> dict:
         "config": cfg,
         "mesh_in": "host memory (vertices pos+normal f32x6, triangles u32x3)",
         "d2h": {"bytes_per_step": int(d2h_bytes), "ms_per_step": round(d2h_ms, 3),
-                "GB_per_s": round(d2h_bytes / (d2h_ms / 1e3) / 1e9, 1) if d2h_ms > 0 else None},
+                "GB_per_s": round(d2h_bytes / (d2h_ms / 1e3) / 1e9, 1) if d2h_ms > 0 else None,
+                "settle_before_timing": settle},
         "device_resident": {"value": round(voxels / 1e6 / (dev_ms / 1e3), 1), "unit": "Mvoxels/s",
                             "ms_per_step": round(dev_ms, 3),
                             "note": "same job, mesh left in HBM (s2m_mesh_device)"},
@@ -267,6 +332,7 @@ def run_batch(args) -> dict:
         for m in ctx.mesh_batch_to_host(progs, params):
             m.free()
         ctx.mesh_batch_device(progs, params)
+    settle = settle_copies()
     bt, nv, nt = [], 0, 0
     for _ in range(steps):
         t0 = time.perf_counter()
@@ -304,6 +370,7 @@ def run_batch(args) -> dict:
                    "mesh_in": "host memory",
                    "autotuned": None if tuned is None else {
                        k: tuned[k] for k in ("batch_cells", "ms", "measured")}},
+        "d2h": {"bytes_per_step": int(nv) * 24 + int(nt) * 12, "settle_before_timing": settle},
         "one_at_a_time_jobs_per_s": round(jobs_s, 1),
         "speedup_vs_one_at_a_time": round(jobs_b / jobs_s, 2),
         "one_at_a_time_latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
@@ -347,6 +414,7 @@ def run_serve(args) -> dict:
                     dense_workers=args.serve_dense_workers) as srv:
         for t, p in zip(tapes[:8], params[:8]):  # warm every worker's buffers
             srv.take(srv.submit(t, p), host=True)[0].free()
+        settle = settle_copies()
         for rate in rates + [float("inf")]:  # inf: every request at once (the capacity)
             n = n_max if rate == float("inf") else min(n_max, max(60, int(rate * 4)))  # ~4 s at low rates
             arr = (np.zeros(n) if rate == float("inf") else
@@ -396,6 +464,7 @@ def run_serve(args) -> dict:
                    "workers": args.serve_workers, "dense_workers": args.serve_dense_workers,
                    "max_batch": args.serve_batch,
                    "mesh_in": "host memory"},
+        "d2h": {"settle_before_timing": settle},
         "p50_ms": top["p50_ms"], "p99_ms": top["p99_ms"],
         "one_at_a_time_jobs_per_s": round(solo_jobs_s, 1),
         "speedup_vs_one_at_a_time": round(top["jobs_per_s"] / solo_jobs_s, 2),
@@ -472,6 +541,8 @@ def main() -> int:
     if args.workload == "copy-calibration":
         print(json.dumps(run_copy_calibration(args)), flush=True)
         return 0
+    global _PROBE
+    _PROBE = CopyProbe()  # the link rate, before the library frees any device memory
     import sdf2mesh_amd as s2m
 
     if args.only and args.workload == "batch256":

@@ -132,11 +132,18 @@ struct s2m_ctx {
     // s2m_mesh_batch_device: the meshes of its dense-path jobs (vertices, keys, signs,
     // triangles: four buffers per job), kept beside the shared pass's output buffers
     std::vector<DevBuf> batch_dense;
-    // s2m_mesh_batch: two staging sets (vertices, keys, signs, triangles) that a pass's
-    // meshes are copied into on the device, so that their copy to host memory (on
-    // copy_stream) overlaps the next pass; per set, events for "staged" and "copied out"
-    DevBuf stage[2][4];
-    hipEvent_t staged[2] = {nullptr, nullptr}, stage_free[2] = {nullptr, nullptr};
+    // s2m_mesh_batch: two sets of what a pass leaves for the host (its output buffers, its
+    // tables image and the mapped block its counters land in).  Pass p + 1 is queued into
+    // one set before pass p, in the other, is read back, and pass p's copy to host memory
+    // (on copy_stream) runs beside pass p + 1.  Per set: "pass done" and "copied out".
+    struct PassSet {
+        DevBuf vtx, keys, signs, tri, tile_tab;
+        PinnedBuf host_img, host_back;
+        std::vector<uint8_t> dev_img;
+        const void* dev_img_at = nullptr;
+        hipEvent_t done = nullptr, copied = nullptr;
+    };
+    PassSet pset[2];
     DevBuf vtx, keys, signs, tri, corners;
     DevBuf tile_tab, tile_lists, tile_lens, tile_flops;
     DevBuf sp_slot_of_tile, sp_tile_of_slot, sp_job_of_slot, sp_cellbits, sp_slot_rows, sp_boundary;
@@ -163,7 +170,7 @@ struct s2m_ctx {
     PinnedBuf host_snap;                 // per pass: k_snapshot's totals and capacity use
     // captured graphs of repeated one-pass jobs, most recent first, and the keys of jobs
     // seen once (run_graph)
-    static constexpr size_t kGraphs = 4, kSeen = 8;
+    static constexpr size_t kGraphs = 8, kSeen = 16;
     std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
     std::vector<std::string> graph_seen;
     std::vector<hipEvent_t> pass_done;   // per pass: recorded after its snapshot
@@ -749,12 +756,93 @@ std::string tune_batch_key(const s2m_ctx* ctx, uint32_t n, uint64_t cells)
     return buf;
 }
 
+// Capacities a sparse pass was queued with (slots, staged cells, vertices, triangles).
+struct SparseCaps {
+    uint64_t want_v = 0, want_t = 0, want_s = 0, stage_cap = 0;
+    uint64_t tiles_pass = 0, tile_blocks = 0;
+};
+
+// A batch pass queued by run_sparse without waiting for it (s2m_mesh_batch reads it back
+// with sparse_read once its event has passed).
+struct SparsePending {
+    bool active = false;
+    uint32_t n = 0, npass = 0;
+    uint64_t tiles_total = 0, graph_launches = 0;
+    SparseCaps caps;
+};
+
+// Exchange the context's pass outputs, tables image and read-back block with batch set k
+// (twice restores them): run_sparse then writes set k.
+void swap_pass_set(s2m_ctx* ctx, int k)
+{
+    s2m_ctx::PassSet& S = ctx->pset[k];
+    std::swap(ctx->vtx, S.vtx);
+    std::swap(ctx->keys, S.keys);
+    std::swap(ctx->signs, S.signs);
+    std::swap(ctx->tri, S.tri);
+    std::swap(ctx->tile_tab, S.tile_tab);
+    std::swap(ctx->host_img, S.host_img);
+    std::swap(ctx->host_back, S.host_back);
+    std::swap(ctx->dev_img, S.dev_img);
+    std::swap(ctx->dev_img_at, S.dev_img_at);
+}
+
+// A finished sparse pass: the counters and per-job rows its last kernel wrote to `back`
+// (mapped host memory).  Sets the context's timing and last-mesh fields; *fits is true
+// with every job's range in `out`, false when a capacity overflowed, with `c` grown for
+// the next attempt.
+int sparse_read(s2m_ctx* ctx, const uint64_t* back, uint32_t n, uint32_t npass, uint64_t tiles_total,
+                SparseCaps& c, std::vector<SpRange>& out, bool* fits)
+{
+    const uint64_t* cnt = back;
+    const uint64_t* jo = cnt + kNumCounters;
+    *fits = false;
+    ctx->timing.eval_launches = npass;
+    ctx->timing.slabs = npass;
+    ctx->timing.lattice_points = cnt[kEvalPoints];
+    ctx->timing.tiles_total = tiles_total;
+    ctx->timing.tiles_active = cnt[kTilesActive];
+    ctx->timing.eval_flops = cnt[kEvalFlops];
+    if (cnt[kVtxTotal] >= 0xffffffffull || cnt[kTriTotal] >= 0xffffffffull)
+        return fail(S2M_E_ARG, "output of %llu vertices / %llu triangles exceeds u32 indices",
+                    (unsigned long long)cnt[kVtxTotal], (unsigned long long)cnt[kTriTotal]);
+    const bool ok_v = cnt[kVtxTotal] <= c.want_v, ok_t = cnt[kTriTotal] <= c.want_t;
+    const bool ok_s = cnt[kSlotsMax] <= c.want_s;
+    const bool ok_stage = cnt[kStageMax] <= c.stage_cap;
+    if (ok_v && ok_t && ok_s && ok_stage) {
+        out.assign(n, SpRange{});
+        for (uint32_t i = 0; i < n; ++i) {
+            SpRange& r = out[i];
+            r.v0 = jo[(size_t)i * kJobOut + 0];
+            r.t0 = jo[(size_t)i * kJobOut + 1];
+            r.invalid = jo[(size_t)i * kJobOut + 2];
+            const uint64_t v1 = (i + 1 < n) ? jo[(size_t)(i + 1) * kJobOut + 0] : cnt[kVtxTotal];
+            const uint64_t t1 = (i + 1 < n) ? jo[(size_t)(i + 1) * kJobOut + 1] : cnt[kTriTotal];
+            r.nv = v1 - r.v0;
+            r.nt = t1 - r.t0;
+        }
+        ctx->last_nvtx = cnt[kVtxTotal];
+        ctx->last_ntri = cnt[kTriTotal];
+        ctx->last_invalid = cnt[kInvalid];
+        *fits = true;
+        return S2M_OK;
+    }
+    if (!ok_s) c.want_s = std::min<uint64_t>(c.tiles_pass, cnt[kSlotsMax] + cnt[kSlotsMax] / 8 + 64);
+    if (!ok_stage) c.stage_cap = cnt[kStageMax] + cnt[kStageMax] / 8 + c.tile_blocks * kStageChunkHost;
+    if (!ok_v) c.want_v = cnt[kVtxTotal] + cnt[kVtxTotal] / 8 + 1024;
+    if (!ok_t) c.want_t = cnt[kTriTotal] + cnt[kTriTotal] / 8 + 2048;
+    return S2M_OK;
+}
+
 // Mesh `n` jobs with the sparse pipeline.  n == 1 may be cut into z-slabs; a batch runs
 // as one pass with every job whole.  Capacities (slots, vertices, triangles) start from
-// estimates and grow if the device counters report more.
+// estimates and grow if the device counters report more.  With `pend` (a batch, no
+// profiling) the pass is only queued: the caller reads it back with sparse_read once it
+// has finished, and queues it again (without `pend`) if a capacity overflowed.
 int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const s2m_params* const* ps,
-               const Job* jobs, std::vector<SpRange>& out, Readback* rb = nullptr)
+               const Job* jobs, std::vector<SpRange>& out, Readback* rb = nullptr, SparsePending* pend = nullptr)
 {
+    if (pend) pend->active = false;
     out.assign(n, SpRange{});
     std::vector<SpTables> tabs(n);
     std::vector<size_t> tab_off(n);
@@ -1102,12 +1190,26 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
             const int erc = enqueue();
             if (erc) return erc;
         }
+        SparseCaps caps;
+        caps.want_v = want_v;
+        caps.want_t = want_t;
+        caps.want_s = want_s;
+        caps.stage_cap = stage_cap;
+        caps.tiles_pass = tiles_pass;
+        caps.tile_blocks = tile_blocks;
+        if (pend && !rb && !ctx->profiling && attempt == 0) {  // queued only: read back by the caller
+            pend->active = true;
+            pend->n = n;
+            pend->npass = npass;
+            pend->tiles_total = tiles_total;
+            pend->graph_launches = ctx->timing.graph_launches;
+            pend->caps = caps;
+            return S2M_OK;
+        }
         if (rb) {  // copies of each pass's mesh while the later passes run
             const int rrc = readback_passes(ctx, *rb, npass, want_v, want_t, want_s, stage_cap);
             if (rrc) return rrc;
         }
-        const uint64_t* cnt = ctx->host_back.as<uint64_t>();
-        const uint64_t* jo = cnt + kNumCounters;
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         if (ctx->profiling) {
             float ms = 0;
@@ -1124,38 +1226,13 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
                 ctx->timing.normal_ms += t[4];   // vertex, normal and triangle emission
             }
         }
-        ctx->timing.eval_launches = npass;
-        ctx->timing.slabs = npass;
-        ctx->timing.lattice_points = cnt[kEvalPoints];
-        ctx->timing.tiles_total = tiles_total;
-        ctx->timing.tiles_active = cnt[kTilesActive];
-        ctx->timing.eval_flops = cnt[kEvalFlops];
-        if (cnt[kVtxTotal] >= 0xffffffffull || cnt[kTriTotal] >= 0xffffffffull)
-            return fail(S2M_E_ARG, "output of %llu vertices / %llu triangles exceeds u32 indices",
-                        (unsigned long long)cnt[kVtxTotal], (unsigned long long)cnt[kTriTotal]);
-        const bool ok_v = cnt[kVtxTotal] <= want_v, ok_t = cnt[kTriTotal] <= want_t;
-        const bool ok_s = cnt[kSlotsMax] <= want_s;
-        const bool ok_stage = cnt[kStageMax] <= stage_cap;
-        if (ok_v && ok_t && ok_s && ok_stage) {
-            for (uint32_t i = 0; i < n; ++i) {
-                SpRange& r = out[i];
-                r.v0 = jo[(size_t)i * kJobOut + 0];
-                r.t0 = jo[(size_t)i * kJobOut + 1];
-                r.invalid = jo[(size_t)i * kJobOut + 2];
-                const uint64_t v1 = (i + 1 < n) ? jo[(size_t)(i + 1) * kJobOut + 0] : cnt[kVtxTotal];
-                const uint64_t t1 = (i + 1 < n) ? jo[(size_t)(i + 1) * kJobOut + 1] : cnt[kTriTotal];
-                r.nv = v1 - r.v0;
-                r.nt = t1 - r.t0;
-            }
-            ctx->last_nvtx = cnt[kVtxTotal];
-            ctx->last_ntri = cnt[kTriTotal];
-            ctx->last_invalid = cnt[kInvalid];
-            return S2M_OK;
-        }
-        if (!ok_s) want_s = std::min<uint64_t>(tiles_pass, cnt[kSlotsMax] + cnt[kSlotsMax] / 8 + 64);
-        if (!ok_stage) stage_cap = cnt[kStageMax] + cnt[kStageMax] / 8 + tile_blocks * kStageChunkHost;
-        if (!ok_v) want_v = cnt[kVtxTotal] + cnt[kVtxTotal] / 8 + 1024;
-        if (!ok_t) want_t = cnt[kTriTotal] + cnt[kTriTotal] / 8 + 2048;
+        bool fits = false;
+        const int src = sparse_read(ctx, ctx->host_back.as<uint64_t>(), n, npass, tiles_total, caps, out, &fits);
+        if (src || fits) return src;
+        want_v = caps.want_v;
+        want_t = caps.want_t;
+        want_s = caps.want_s;
+        stage_cap = caps.stage_cap;
     }
     return fail(S2M_E_OOM, "output buffers kept overflowing");
 }
@@ -1681,10 +1758,13 @@ void s2m_destroy(s2m_ctx* ctx)
                       &ctx->sp_stage, &ctx->sp_slot_nb, &ctx->sp_slot_stage, &ctx->sp_slot_live, &ctx->sp_slot_hdr, &ctx->sp_slot_rec, &ctx->sp_block_stats})
         b->release();
     for (DevBuf& b : ctx->batch_dense) b.release();
-    for (auto& set : ctx->stage)
-        for (DevBuf& b : set) b.release();
-    for (hipEvent_t e : {ctx->staged[0], ctx->staged[1], ctx->stage_free[0], ctx->stage_free[1]})
-        if (e) (void)hipEventDestroy(e);
+    for (s2m_ctx::PassSet& S : ctx->pset) {
+        for (DevBuf* b : {&S.vtx, &S.keys, &S.signs, &S.tri, &S.tile_tab}) b->release();
+        S.host_img.release();
+        S.host_back.release();
+        for (hipEvent_t e : {S.done, S.copied})
+            if (e) (void)hipEventDestroy(e);
+    }
     ctx->host_img.release();
     ctx->host_back.release();
     ctx->host_snap.release();
@@ -1851,16 +1931,14 @@ int s2m_mesh(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, in
 
 namespace {
 
-// Copy job ranges of the context's output buffers to fresh host meshes.
-int copy_ranges(s2m_ctx* ctx, const SpRange* r, uint32_t n, int want_keys, s2m_mesh_out* outs, int set = -1)
+// Copy the job ranges of a batch pass's output buffers to fresh host meshes.
+int copy_ranges(const s2m_ctx::PassSet& src, const SpRange* r, uint32_t n, int want_keys, s2m_mesh_out* outs,
+                hipStream_t s)
 {
-    // One page-locked block per array for the whole pass (one DMA copy each): every job's
+    // One page-locked block per array for the whole pass (one DMA copy each, from the
+    // pass's output buffers in `src`, queued on `s` and unfinished on return): every job's
     // array points inside it, and the block is recycled after the last job's mesh is freed.
-    // Jobs without vertices (or triangles) get arrays of their own.  set < 0: the copies run
-    // on the compute stream and are finished on return.  set 0 / 1: the pass's ranges are
-    // first copied on the device into staging set `set` (once its previous copy-out is
-    // done), then to host memory on copy_stream, unfinished on return: the caller
-    // synchronises copy_stream before handing the meshes out.
+    // Jobs without vertices (or triangles) get arrays of their own.
     uint64_t v_lo = ~0ull, v_hi = 0, t_lo = ~0ull, t_hi = 0;
     uint32_t v_refs = 0, t_refs = 0;
     for (uint32_t i = 0; i < n; ++i) {
@@ -1915,54 +1993,18 @@ int copy_ranges(s2m_ctx* ctx, const SpRange* r, uint32_t n, int want_keys, s2m_m
         for (; t_given < t_refs; ++t_given) host_release(Tr);
         return fail(S2M_E_OOM, "host allocation failed");
     }
-    const float* src_v = ctx->vtx.as<float>() + 6 * v_lo;
-    const uint64_t* src_k = ctx->keys.as<uint64_t>() + v_lo;
-    const uint8_t* src_s = ctx->signs.as<uint8_t>() + v_lo;
-    const uint32_t* src_t = ctx->tri.as<uint32_t>() + 3 * t_lo;
     const uint64_t nv = v_refs ? v_hi - v_lo : 0, nt = t_refs ? t_hi - t_lo : 0;
-    hipStream_t out_stream = ctx->stream;
-    if (set >= 0) {
-        for (hipEvent_t* e : {&ctx->staged[set], &ctx->stage_free[set]})
-            if (!*e) HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
-        DevBuf* st = ctx->stage[set];
-        // a larger staging buffer replaces the old one only after its copy-out is done
-        if (st[0].bytes < nv * 6 * sizeof(float) || st[3].bytes < nt * 3 * sizeof(uint32_t) ||
-            (want_keys && (st[1].bytes < nv * sizeof(uint64_t) || st[2].bytes < nv)))
-            HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
-        HIP_TRY(st[0].ensure(std::max<uint64_t>(nv, 1) * 6 * sizeof(float)));
-        HIP_TRY(st[3].ensure(std::max<uint64_t>(nt, 1) * 3 * sizeof(uint32_t)));
-        if (want_keys) {
-            HIP_TRY(st[1].ensure(std::max<uint64_t>(nv, 1) * sizeof(uint64_t)));
-            HIP_TRY(st[2].ensure(std::max<uint64_t>(nv, 1)));
-        }
-        HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->stage_free[set], 0));
-        HIP_TRY(hipMemcpyAsync(st[0].p, src_v, nv * 6 * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream));
-        HIP_TRY(hipMemcpyAsync(st[3].p, src_t, nt * 3 * sizeof(uint32_t), hipMemcpyDeviceToDevice, ctx->stream));
-        if (want_keys) {
-            HIP_TRY(hipMemcpyAsync(st[1].p, src_k, nv * sizeof(uint64_t), hipMemcpyDeviceToDevice, ctx->stream));
-            HIP_TRY(hipMemcpyAsync(st[2].p, src_s, nv, hipMemcpyDeviceToDevice, ctx->stream));
-        }
-        HIP_TRY(hipEventRecord(ctx->staged[set], ctx->stream));
-        HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, ctx->staged[set], 0));
-        src_v = st[0].as<float>();
-        src_k = st[1].as<uint64_t>();
-        src_s = st[2].as<uint8_t>();
-        src_t = st[3].as<uint32_t>();
-        out_stream = ctx->copy_stream;
-    }
     if (v_refs) {
-        HIP_TRY(hipMemcpyAsync(V, src_v, nv * 6 * sizeof(float), hipMemcpyDeviceToHost, out_stream));
+        HIP_TRY(hipMemcpyAsync(V, src.vtx.as<float>() + 6 * v_lo, nv * 6 * sizeof(float), hipMemcpyDeviceToHost, s));
         if (want_keys) {
-            HIP_TRY(hipMemcpyAsync(K, src_k, nv * sizeof(uint64_t), hipMemcpyDeviceToHost, out_stream));
-            HIP_TRY(hipMemcpyAsync(S, src_s, nv, hipMemcpyDeviceToHost, out_stream));
+            HIP_TRY(hipMemcpyAsync(K, src.keys.as<uint64_t>() + v_lo, nv * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                   s));
+            HIP_TRY(hipMemcpyAsync(S, src.signs.as<uint8_t>() + v_lo, nv, hipMemcpyDeviceToHost, s));
         }
     }
-    if (t_refs) HIP_TRY(hipMemcpyAsync(Tr, src_t, nt * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost, out_stream));
-    if (set >= 0) {
-        HIP_TRY(hipEventRecord(ctx->stage_free[set], ctx->copy_stream));
-    } else {
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-    }
+    if (t_refs)
+        HIP_TRY(hipMemcpyAsync(Tr, src.tri.as<uint32_t>() + 3 * t_lo, nt * 3 * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               s));
     return S2M_OK;
 }
 
@@ -2043,56 +2085,122 @@ int s2m_mesh_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs,
             if (rc) return rc;
             add_timing(ctx->timing);
         }
-    // each pass's meshes go to host memory on copy_stream while the next pass computes
-    // (staging sets alternate); the copies are finished before the meshes are returned
-    uint32_t npass = 0;
-    auto drain = [ctx](int code) {
-        (void)hipStreamSynchronize(ctx->copy_stream);
-        return code;
-    };
-    while (i < n) {
-        uint32_t k = i;
-        uint64_t cells = 0, tiles = 0;
+    // The passes: consecutive jobs up to pass_cells cells and 2M tiles; empty grids and
+    // the dense jobs meshed above belong to no pass.
+    struct PassJobs {
         std::vector<const s2m_program*> pp;
         std::vector<const s2m_params*> qq;
         std::vector<Job> jj;
         std::vector<uint32_t> idx;
+    };
+    std::vector<PassJobs> passes;
+    while (i < n) {
+        uint32_t k = i;
+        uint64_t cells = 0, tiles = 0;
+        PassJobs P;
         while (k < n) {
             const Job& j = jobs[k];
             const uint64_t c = (uint64_t)j.res[0] * j.res[1] * j.nz_cells;
             const uint64_t t = ((j.res[0] + 7) / 8ull) * ((j.res[1] + 7) / 8ull) * ((j.nz_cells + 7) / 8ull);
             const bool batched = c && !dense_job(progs[k], params[k], j);  // else: meshed above
-            if (batched && !idx.empty() && (cells + c > pass_cells || tiles + t > (2ull << 20))) break;
+            if (batched && !P.idx.empty() && (cells + c > pass_cells || tiles + t > (2ull << 20))) break;
             if (batched) {
-                pp.push_back(progs[k]);
-                qq.push_back(&params[k]);
-                jj.push_back(j);
-                idx.push_back(k);
+                P.pp.push_back(progs[k]);
+                P.qq.push_back(&params[k]);
+                P.jj.push_back(j);
+                P.idx.push_back(k);
                 cells += c;
                 tiles += t;
             }
             ++k;
         }
-        if (!idx.empty()) {
-            std::vector<SpRange> r;
-            rc = run_sparse(ctx, (uint32_t)idx.size(), pp.data(), qq.data(), jj.data(), r);
-            if (rc) return drain(rc);
-            std::vector<s2m_mesh_out> tmp(idx.size());
-            rc = copy_ranges(ctx, r.data(), (uint32_t)idx.size(), want_keys, tmp.data(), (int)(npass++ & 1u));
-            for (size_t q = 0; q < idx.size(); ++q) outs[idx[q]] = tmp[q];
-            if (rc) return drain(rc);
-            add_timing(ctx->timing);
-        }
-        for (uint32_t q = i; q < k; ++q)  // empty grids: empty meshes
-            if (!outs[q].vtx) {
-                outs[q].vtx = (float*)std::malloc(6 * sizeof(float));
-                outs[q].tri = (uint32_t*)std::malloc(3 * sizeof(uint32_t));
-                if (want_keys) {
-                    outs[q].cell_keys = (uint64_t*)std::malloc(sizeof(uint64_t));
-                    outs[q].signs = (uint8_t*)std::malloc(1);
-                }
-            }
+        if (!P.idx.empty()) passes.push_back(std::move(P));
         i = k;
+    }
+    for (uint32_t q = 0; q < n; ++q)  // empty grids (not meshed on the dense path above): empty meshes
+        if (!outs[q].vtx && (uint64_t)jobs[q].res[0] * jobs[q].res[1] * jobs[q].nz_cells == 0) {
+            outs[q].vtx = (float*)std::malloc(6 * sizeof(float));
+            outs[q].tri = (uint32_t*)std::malloc(3 * sizeof(uint32_t));
+            if (want_keys) {
+                outs[q].cell_keys = (uint64_t*)std::malloc(sizeof(uint64_t));
+                outs[q].signs = (uint8_t*)std::malloc(1);
+            }
+        }
+    // Pass p runs in set p % 2.  Pass p + 1 is queued before pass p is read back, so the
+    // device goes from one pass to the next while the host reads the finished pass's
+    // counters and queues its copies to host memory on copy_stream, which run beside the
+    // next pass; pass p + 2 waits for those copies before writing the set again.  With
+    // profiling on, every pass runs to completion in turn.
+    for (s2m_ctx::PassSet& S : ctx->pset)
+        for (hipEvent_t* e : {&S.done, &S.copied})
+            if (!*e) HIP_TRY(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    SparsePending pend[2];
+    std::vector<SpRange> ranges[2];
+    auto drain = [ctx](int code) {
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(ctx->copy_stream);
+        return code;
+    };
+    auto enqueue_pass = [&](size_t p) -> int {
+        const int k = (int)(p & 1u);
+        const PassJobs& P = passes[p];
+        if (p >= 2) HIP_TRY(hipStreamWaitEvent(ctx->stream, ctx->pset[k].copied, 0));
+        swap_pass_set(ctx, k);
+        ctx->timing = s2m_timing{};
+        const int erc = run_sparse(ctx, (uint32_t)P.idx.size(), P.pp.data(), P.qq.data(), P.jj.data(), ranges[k],
+                                   nullptr, &pend[k]);
+        swap_pass_set(ctx, k);
+        if (erc) return erc;
+        if (!pend[k].active) add_timing(ctx->timing);  // ran to completion
+        HIP_TRY(hipEventRecord(ctx->pset[k].done, ctx->stream));
+        return S2M_OK;
+    };
+    auto collect_pass = [&](size_t p) -> int {
+        const int k = (int)(p & 1u);
+        const PassJobs& P = passes[p];
+        s2m_ctx::PassSet& S = ctx->pset[k];
+        if (pend[k].active) {
+            HIP_TRY(hipEventSynchronize(S.done));
+            bool fits = false;
+            ctx->timing = s2m_timing{};
+            const int crc = sparse_read(ctx, S.host_back.as<uint64_t>(), pend[k].n, pend[k].npass,
+                                        pend[k].tiles_total, pend[k].caps, ranges[k], &fits);
+            if (crc) return crc;
+            ctx->timing.graph_launches = pend[k].graph_launches;
+            add_timing(ctx->timing);
+            if (!fits) {  // a capacity overflowed: the pass again, to completion, with the grown capacities
+                ctx->vcap = std::max(ctx->vcap, pend[k].caps.want_v);
+                ctx->tcap = std::max(ctx->tcap, pend[k].caps.want_t);
+                ctx->slot_cap = std::max(ctx->slot_cap, pend[k].caps.want_s);
+                ctx->stage_cap = std::max(ctx->stage_cap, pend[k].caps.stage_cap);
+                swap_pass_set(ctx, k);
+                ctx->timing = s2m_timing{};
+                const int rrc = run_sparse(ctx, (uint32_t)P.idx.size(), P.pp.data(), P.qq.data(), P.jj.data(),
+                                           ranges[k]);
+                swap_pass_set(ctx, k);
+                if (rrc) return rrc;
+                add_timing(ctx->timing);
+                HIP_TRY(hipEventRecord(S.done, ctx->stream));
+            }
+            pend[k].active = false;
+        }
+        std::vector<s2m_mesh_out> tmp(P.idx.size());
+        HIP_TRY(hipStreamWaitEvent(ctx->copy_stream, S.done, 0));
+        const int crc = copy_ranges(S, ranges[k].data(), (uint32_t)P.idx.size(), want_keys, tmp.data(),
+                                    ctx->copy_stream);
+        for (size_t q = 0; q < P.idx.size(); ++q) outs[P.idx[q]] = tmp[q];
+        if (crc) return crc;
+        HIP_TRY(hipEventRecord(S.copied, ctx->copy_stream));
+        return S2M_OK;
+    };
+    for (size_t p = 0; p < passes.size(); ++p) {
+        rc = enqueue_pass(p);
+        if (!rc && p > 0) rc = collect_pass(p - 1);
+        if (rc) return drain(rc);
+    }
+    if (!passes.empty()) {
+        rc = collect_pass(passes.size() - 1);
+        if (rc) return drain(rc);
     }
     HIP_TRY(hipStreamSynchronize(ctx->copy_stream));
     ctx->timing = sum;

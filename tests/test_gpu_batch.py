@@ -100,3 +100,38 @@ def test_device_batch_with_dense_jobs(ctx):
         solo = ctx.mesh(p, q)
         print(f"device batch job {i}: {len(solo.vertices)} vertices", flush=True)
         same(got[i], solo)
+
+
+def stacked_boxes(n: int, z0: float) -> tape.Node:
+    """n thin boxes stacked in z over most of the xy extent: 2n surface crossings per
+    lattice column, far above the first vertex estimate of a pass (6 per column)."""
+    return tape.fold([tape.box((0.0, 0.0, z0 + 0.15 * i), (1.5, 1.5, 0.05)) for i in range(n)],
+                     op=tape.union)
+
+
+def test_batch_passes_overflow_and_repeat():
+    # A fresh context, three batch passes (four 256^3 jobs fill one at the default cut of
+    # 64M cells): the first passes overflow their first vertex, triangle and slot
+    # estimates and are queued again after the next pass has been queued.  Every job
+    # equals the job alone; a second batch (capacities grown) and a profiled batch (every
+    # pass run to completion in turn) equal the first.
+    print("batch overflow", flush=True)
+    with s2m.Context(0) as fresh:
+        nodes = [stacked_boxes(12, -0.85 + 0.01 * s) for s in range(9)]
+        nodes.insert(3, tape.torus_example())
+        progs = [fresh.program(x) for x in nodes]
+        params = [s2m.Params.reference(256, 2.0) for _ in progs]
+        params[3] = s2m.Params.reference(64, 2.0)
+        first = fresh.mesh_batch(progs, params)
+        second = fresh.mesh_batch(progs, params)
+        fresh.set_profiling(2)
+        profiled = fresh.mesh_batch(progs, params)
+        fresh.set_profiling(0)
+        for i, (p, q) in enumerate(zip(progs, params)):
+            solo = fresh.mesh(p, q)
+            print(f"overflow batch job {i}: {len(solo.vertices)} vertices", flush=True)
+            if q.res[0] == 256:  # above its share of the pass's first estimate
+                assert len(solo.vertices) > 6 * 256 * 256
+            same(first[i], solo)
+            same(second[i], solo)
+            same(profiled[i], solo)
