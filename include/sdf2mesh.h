@@ -326,8 +326,10 @@ int s2m_mesh_batch_device(s2m_ctx* ctx, uint32_t n, const s2m_program* const* pr
  *   dense_slab     cell planes per slab of the dense path (Mandelbulb, compiled programs)
  *   batch_cells    most cells per tile-sparse pass of s2m_mesh_batch
  * The tile edge is fixed at 8 cells: the tile kernels' LDS layouts are built for it.
- * s2m_tune times each candidate of the knob that applies to (prog, params) with s2m_mesh
- * (one untimed run, then the median of `reps` runs, at least 3), keeps the first candidate
+ * s2m_tune times each candidate of the knob that applies to (prog, params) with s2m_mesh,
+ * largest device buffers first (one untimed run, then the median of `reps` runs, at least
+ * 3; regrowing a buffer frees device memory, which the driver clears on a DMA engine while
+ * copies to host memory slow down, so buffers grow once), keeps the first candidate
  * unless a later one is more than 2 % faster (s2m_tune_pick), stores the choice in a
  * process-wide table that every later meshing call of that key uses, and reports it.
  * A key already in the table is returned without timing (measured = 0).  s2m_params.slab

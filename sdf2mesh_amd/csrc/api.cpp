@@ -2391,14 +2391,18 @@ int s2m_tune(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* params, ui
         out->ms = ms;
         return S2M_OK;
     }
+    // Candidates in decreasing order of the device buffers they need, so that buffers grow
+    // once, in the first candidate's untimed run: device memory freed when a buffer is
+    // regrown is cleared by the driver on a DMA engine, and copies to host memory run at
+    // half rate until that is done, which would bias the candidates measured meanwhile.
     std::vector<TuneChoice> cand;
     const bool tiled = !prog->module && !(p.flags & S2M_DENSE_EVAL) && tape_prunable(prog->ops);
-    if (tiled) {  // z-passes of the streamed readback
+    if (tiled) {  // z-passes of the streamed readback (one pass holds the whole job)
         const uint32_t layers = std::max<uint32_t>(1, (j.nz_cells + kTile - 1) / kTile);
         for (uint32_t c : {1u, 2u, 3u, 4u, 6u, 8u})
             if (c <= layers) cand.push_back(TuneChoice{c, 0, 0});
     } else {  // cell planes per dense slab
-        for (uint32_t c : {4u, 8u, 16u, 32u, 64u, 128u})
+        for (uint32_t c : {128u, 64u, 32u, 16u, 8u, 4u})
             if (c <= std::max<uint32_t>(j.nz_cells, 4)) cand.push_back(TuneChoice{0, c, 0});
     }
     auto run = [&]() {
@@ -2429,8 +2433,8 @@ int s2m_tune_batch(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, co
         out->ms = ms;
         return S2M_OK;
     }
-    std::vector<TuneChoice> cand;
-    for (uint64_t c : {16ull << 20, 32ull << 20, 64ull << 20, 128ull << 20, 256ull << 20})
+    std::vector<TuneChoice> cand;  // largest passes first (buffers grow once, as in s2m_tune)
+    for (uint64_t c : {512ull << 20, 384ull << 20, 256ull << 20, 128ull << 20, 64ull << 20, 32ull << 20, 16ull << 20})
         cand.push_back(TuneChoice{0, 0, c});
     std::vector<s2m_mesh_out> outs(n);
     auto run = [&]() {

@@ -55,7 +55,7 @@ def test_tune_batch(ctx):
     ref = ctx.mesh_batch(progs, params)
     t = ctx.tune_batch(progs, params, reps=3)
     print(f"tuned batch: {t}", flush=True)
-    assert t["measured"] == 5 and t["batch_cells"] >= 16 << 20
+    assert t["measured"] == 7 and 16 << 20 <= t["batch_cells"] <= 512 << 20
     for a, b in zip(ctx.mesh_batch(progs, params), ref):
         same(a, b)
     s2m.tuning_clear()
