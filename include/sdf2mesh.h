@@ -70,7 +70,8 @@ enum s2m_status {
  *   CAPSULE   f[0..2]=a, f[3..5]=b, f[6]=r sdf3d_capsule(p, a, b, r)   primitives.wgsl:19-25
  *   MANDELBULB f[0]=power, f[1]=bailout, f[2]=offset, iarg = iterations | (swap_yz<<16)
  *             | (trig<<17)
- *             the distance estimator of examples/mandelmesh.frag:3-28
+ *             the distance estimator of examples/mandelmesh.frag:3-28 (f[3] is not read
+ *             from the caller's tape: the device copy holds the bailout's r^2 threshold there)
  *   UNION min(d1,d2)  INTERSECTION max(d1,d2)  SUBTRACTION max(-d1,d2)
  *   SMOOTH_UNION / SMOOTH_INTERSECTION / SMOOTH_SUBTRACTION  f[0]=k,  sdf_op.wgsl:7-23
  * `p - c` is a per-component fp32 subtraction, so c = 0 leaves p unchanged.

@@ -295,13 +295,24 @@ def _take_mesh(out: _MeshOut, want_keys: bool) -> TriangleMesh:
         lib().s2m_mesh_free(C.byref(out))
 
 
+# s2m_mesh_out as a numpy record (pointers as integers): the counts of a whole batch's
+# results read at once
+_MESH_OUT_DTYPE = np.dtype([("vtx", "<u8"), ("nvtx", "<u8"), ("tri", "<u8"), ("ntri", "<u8"),
+                            ("invalid_quads", "<u8"), ("cell_keys", "<u8"), ("signs", "<u8")])
+assert _MESH_OUT_DTYPE.itemsize == C.sizeof(_MeshOut)
+
+
 class HostMesh:
     """A mesh in library-owned host memory (s2m_mesh_out); views, no copies."""
 
-    def __init__(self, out: _MeshOut):
+    __slots__ = ("_out", "nvtx", "ntri", "invalid_quads")
+
+    def __init__(self, out: _MeshOut, nvtx: int | None = None, ntri: int | None = None,
+                 invalid_quads: int | None = None):
         self._out = out
-        self.nvtx, self.ntri = int(out.nvtx), int(out.ntri)
-        self.invalid_quads = int(out.invalid_quads)
+        self.nvtx = int(out.nvtx) if nvtx is None else nvtx
+        self.ntri = int(out.ntri) if ntri is None else ntri
+        self.invalid_quads = int(out.invalid_quads) if invalid_quads is None else invalid_quads
 
     def vertices(self) -> np.ndarray:
         return np.ctypeslib.as_array(self._out.vtx, (max(self.nvtx, 1) * 6,))[: self.nvtx * 6] \
@@ -563,7 +574,10 @@ class Context:
             for o in outs:
                 lib().s2m_mesh_free(C.byref(o))
             _check(rc)
-        return [HostMesh(outs[i]) for i in range(n)]
+        rec = np.frombuffer(outs, _MESH_OUT_DTYPE) if n else None
+        nv, nt, iq = ((rec["nvtx"].tolist(), rec["ntri"].tolist(), rec["invalid_quads"].tolist())
+                      if n else ([], [], []))
+        return [HostMesh(outs[i], nv[i], nt[i], iq[i]) for i in range(n)]
 
     def mesh_batch_device(self, progs, params) -> list:
         """One shared pass; results stay in device memory owned by the context."""

@@ -1820,6 +1820,8 @@ int s2m_program_from_tape(s2m_ctx* ctx, const s2m_op* ops, uint32_t n_ops, s2m_p
     hipError_t e = hipMalloc(&p->d_ops, n_ops * sizeof(s2m_op));
     if (e == hipSuccess)
         e = hipMemcpy(p->d_ops, ops, n_ops * sizeof(s2m_op), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_tape_prepare(ctx->stream, p->d_ops, n_ops);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         if (p->d_ops) (void)hipFree(p->d_ops);
         delete p;

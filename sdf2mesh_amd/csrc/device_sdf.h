@@ -123,9 +123,11 @@ __device__ __forceinline__ float bailout_threshold(float b)
 
 // examples/mandelmesh.frag:3-28 with power, bailout, offset and the iteration count as
 // operands.  A bailed-out point keeps r = length(z): the loop `continue`s, so r is
-// recomputed from the unchanged z in every remaining iteration.
+// recomputed from the unchanged z in every remaining iteration.  t2 =
+// bailout_threshold(bailout), computed once per tape upload (k_tape_prepare) rather than
+// at every point.
 __device__ __forceinline__ float sd_mandelbulb(float px, float py, float pz, float power,
-                                               float bailout, float offset, uint32_t iarg)
+                                               float bailout, float offset, uint32_t iarg, float t2)
 {
     if (iarg & S2M_MANDEL_SWAP_YZ) {  // p.xyz = p.xzy
         const float t = py;
@@ -139,18 +141,22 @@ __device__ __forceinline__ float sd_mandelbulb(float px, float py, float pz, flo
         // The power-8 step without transcendentals or divisions.  theta = atan(zy / zx)
         // differs from the direction angle of (zx, zy) by 0 or pi, which 8 * theta turns
         // into a multiple of 8 pi: cos / sin of 8 theta are the components of
-        // ((zx, zy) / rho)^8 as complex numbers (rho = |(zx, zy)|).  phi = asin(zz / r) lies
-        // in [-pi/2, pi/2], so (cos phi, sin phi) = (rho / r, zz / r) and (cos 8 phi,
-        // sin 8 phi) is its 8th power.  Three complex squarings each; 1 / r and 1 / rho from
-        // reciprocal square roots; r^8 and r^7 by products of r^2; fused multiply-adds where
-        // a product meets a sum.  The same function as the written form, with fewer
-        // roundings (its error against the fp64 estimator is bounded as for the written
-        // form).  The bailout test compares r^2 with the
-        // threshold whose correctly rounded square root first exceeds the bailout, so every
-        // point leaves the loop where the written form does; the r it leaves with is the
-        // same correctly rounded length.  rho = 0 gives NaN, as atan(0 / 0) does.
-        const float t2 = bailout_threshold(bailout);
+        // ((zx, zy) / rho)^8 as complex numbers (rho = |(zx, zy)|), three complex squarings
+        // of the unit vector (1 / rho from a reciprocal square root).  phi = asin(zz / r)
+        // lies in [-pi/2, pi/2], so r (cos phi, sin phi) = (rho, zz), and
+        // r^8 (cos 8 phi, sin 8 phi) = (rho + i zz)^8: its first square is
+        // (rho^2 - zz^2, 2 rho zz) with rho^2 taken as zx^2 + zy^2, then two more squarings.
+        // So z' = (r^8 cos 8 phi (cos 8 theta, sin 8 theta), r^8 sin 8 phi) + p without 1 / r;
+        // r^7 for dr is r^6 sqrt(r^2).  Fused multiply-adds where a product meets a sum.
+        // The same function as the written form, with fewer roundings (its error against
+        // the fp64 estimator is bounded as for the written form).  The bailout test
+        // compares r^2 with the threshold whose correctly rounded square root first exceeds
+        // the bailout, so every point leaves the loop where the written form does; the r it
+        // leaves with is the same correctly rounded length, and a point that never bails
+        // out leaves with r = |z|^8 of the last step, as the written form's r = pow(r, 8).
+        // rho = 0 gives NaN, as atan(0 / 0) does.
         float r2_out = -1.0f;  // r^2 where the point bailed out (the root is taken once, below)
+        float r2_last = 0.0f;  // r^2 of the last step taken
         for (uint32_t i = 0; i < iters; ++i) {
             const float xy2 = zx * zx + zy * zy;
             const float r2 = xy2 + zz * zz;
@@ -158,26 +164,35 @@ __device__ __forceinline__ float sd_mandelbulb(float px, float py, float pz, flo
                 r2_out = r2;
                 break;
             }
-            const float ir = rsqrt_full(r2), irho = rsqrt_full(xy2);
+            r2_last = r2;
+            const float irho = rsqrt_full(xy2);
             const float rho = xy2 * irho;
-            float ct = zx * irho, st = zy * irho, cp = rho * ir, sp = zz * ir;
+            float ct = zx * irho, st = zy * irho;
+            float cp = __builtin_fmaf(-zz, zz, xy2), sp = (rho + rho) * zz;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {  // (c + i s)^2 = (c^2 - s^2) + i 2cs, one rounding for the real part
                 const float c2 = __builtin_fmaf(ct, ct, -(st * st)), s2 = (ct + ct) * st;
-                const float p2 = __builtin_fmaf(cp, cp, -(sp * sp)), q2 = (cp + cp) * sp;
                 ct = c2;
                 st = s2;
+            }
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const float p2 = __builtin_fmaf(cp, cp, -(sp * sp)), q2 = (cp + cp) * sp;
                 cp = p2;
                 sp = q2;
             }
-            const float r4 = r2 * r2;
-            r = r4 * r4;
-            dr = __builtin_fmaf((r * ir) * dr, 8.0f, 1.0f);
-            zx = __builtin_fmaf(r, ct * cp, px);
-            zy = __builtin_fmaf(r, st * cp, py);
-            zz = __builtin_fmaf(r, sp, pz);
+            const float r7 = ((r2 * r2) * r2) * __builtin_amdgcn_sqrtf(r2);
+            dr = __builtin_fmaf(r7 * dr, 8.0f, 1.0f);
+            zx = __builtin_fmaf(cp, ct, px);
+            zy = __builtin_fmaf(cp, st, py);
+            zz = sp + pz;
         }
-        if (r2_out >= 0.0f) r = sqrtf(r2_out);
+        if (r2_out >= 0.0f) {
+            r = sqrtf(r2_out);
+        } else {
+            const float r4 = r2_last * r2_last;
+            r = r4 * r4;
+        }
         return ((0.5f * logf(r)) * r) / dr - offset;
     }
     for (uint32_t i = 0; i < iters; ++i) {
@@ -213,7 +228,7 @@ __device__ __forceinline__ float eval_primitive(uint32_t code, uint32_t iarg, co
     case S2M_OP_TORUS: return sd_torus(px - f[0], py - f[1], pz - f[2], f[3], f[4]);
     case S2M_OP_CYLINDER: return sd_cylinder(px - f[0], py - f[1], pz - f[2], f[3], f[4]);
     case S2M_OP_CAPSULE: return sd_capsule(px, py, pz, f[0], f[1], f[2], f[3], f[4], f[5], f[6]);
-    default: return sd_mandelbulb(px, py, pz, f[0], f[1], f[2], iarg);
+    default: return sd_mandelbulb(px, py, pz, f[0], f[1], f[2], iarg, f[3]);  // f[3]: k_tape_prepare
     }
 }
 

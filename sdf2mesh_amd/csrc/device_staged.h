@@ -148,10 +148,10 @@ __device__ __forceinline__ void staged_prim(const TT& T, uint32_t k, uint32_t co
     }
     default:
         if (MANDEL) {
-            const float pw = f[0], bo = f[1], off = f[2];
+            const float pw = f[0], bo = f[1], off = f[2], t2 = f[3];  // f[3]: k_tape_prepare
             const uint32_t ia = T.iarg_at(k);
 #pragma unroll
-            for (int p = 0; p < NP; ++p) d[p] = sd_mandelbulb(x[p], y[p], z[p], pw, bo, off, ia);
+            for (int p = 0; p < NP; ++p) d[p] = sd_mandelbulb(x[p], y[p], z[p], pw, bo, off, ia, t2);
         }
         break;
     }

@@ -189,6 +189,9 @@ hipError_t launch_scan(hipStream_t s, const MeshBufs& b, uint32_t nblocks, uint3
 hipError_t launch_sincos_check(hipStream_t s, uint64_t* cnt);
 // cnt[0..2) (zeroed by the caller): k_bailout_check's comparisons and disagreements.
 hipError_t launch_bailout_check(hipStream_t s, uint64_t* cnt);
+// A tape just copied to the device: every Mandelbulb op's f[3] set to the bailout
+// threshold of its f[1] (sd_mandelbulb reads it there instead of computing it per point).
+hipError_t launch_tape_prepare(hipStream_t s, s2m_op* ops, uint32_t n);
 hipError_t launch_normals(hipStream_t s, const s2m_op* ops, uint32_t nops, float eps,
                           const MeshBufs& b);
 hipError_t launch_quads(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int emit);

@@ -809,6 +809,21 @@ hipError_t launch_sincos_check(hipStream_t s, uint64_t* cnt)
     return hipGetLastError();
 }
 
+// One thread per op of a tape in device memory: a Mandelbulb op's f[3] (unused by the
+// estimator's operands) gets bailout_threshold(f[1]), the r^2 bound of its power-8 step.
+__global__ __launch_bounds__(256) void k_tape_prepare(s2m_op* __restrict__ ops, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && ops[i].code == S2M_OP_MANDELBULB) ops[i].f[3] = bailout_threshold(ops[i].f[1]);
+}
+
+hipError_t launch_tape_prepare(hipStream_t s, s2m_op* ops, uint32_t n)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tape_prepare, dim3((n + 255) / 256), dim3(256), 0, s, ops, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_bailout_check(hipStream_t s, uint64_t* cnt)
 {
     hipLaunchKernelGGL(k_bailout_check, dim3(((1u << 20) + 256u) / 256u), dim3(256), 0, s,

@@ -150,8 +150,9 @@ void s2m_server::worker_main(uint32_t wid, int own)
                 }
                 params.push_back(r->params);
             }
-            if (nops && hipMemcpyAsync(arena.d, arena.staging.data(), nops * sizeof(s2m_op), hipMemcpyHostToDevice,
-                                       stream) != hipSuccess)
+            if (nops && (hipMemcpyAsync(arena.d, arena.staging.data(), nops * sizeof(s2m_op), hipMemcpyHostToDevice,
+                                        stream) != hipSuccess ||
+                         s2m::launch_tape_prepare(stream, arena.d, (uint32_t)nops) != hipSuccess))
                 rc = S2M_E_HIP;
             if (rc == S2M_OK) {
                 rc = batch.size() == 1 ? s2m_mesh(ctx, progs[0], &params[0], cfg.want_keys, &outs[0])
