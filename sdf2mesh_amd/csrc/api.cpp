@@ -1243,6 +1243,8 @@ hipError_t eval_lattice_prog(hipStream_t s, const s2m_program* prog, const float
                              const float* cz, uint32_t nx, uint32_t ny, uint32_t lz0, uint32_t nzl,
                              uint32_t pitch, float* lat)
 {
+    if (!prog->module && prog->n == 1 && prog->ops[0].code == S2M_OP_MANDELBULB)
+        return launch_eval_mandelbulb(s, prog->d_ops, cx, cy, cz, nx, ny, lz0, nzl, pitch, lat);
     if (!prog->module)
         return launch_eval_lattice(s, prog->d_ops, prog->n, cx, cy, cz, nx, ny, lz0, nzl, pitch, lat);
     if (nx == 0 || ny == 0 || nzl == 0) return hipSuccess;

@@ -148,13 +148,13 @@ __device__ __forceinline__ float sd_mandelbulb(float px, float py, float pz, flo
         // (rho^2 - zz^2, 2 rho zz) with rho^2 taken as zx^2 + zy^2, then two more squarings.
         // So z' = (r^8 cos 8 phi (cos 8 theta, sin 8 theta), r^8 sin 8 phi) + p without 1 / r;
         // r^7 for dr is r^6 sqrt(r^2).  Fused multiply-adds where a product meets a sum.
-        // The same function as the written form, with fewer roundings (its error against
+        // The same function as the written form, with its own roundings (its error against
         // the fp64 estimator is bounded as for the written form).  The bailout test
         // compares r^2 with the threshold whose correctly rounded square root first exceeds
-        // the bailout, so every point leaves the loop where the written form does; the r it
-        // leaves with is the same correctly rounded length, and a point that never bails
-        // out leaves with r = |z|^8 of the last step, as the written form's r = pow(r, 8).
-        // rho = 0 gives NaN, as atan(0 / 0) does.
+        // the bailout, so every point leaves the loop where the written form does, with
+        // r = |z| (hardware square root); a point that never bails out leaves with
+        // r = |z|^8 of the last step, as the written form's r = pow(r, 8).  rho = 0 gives
+        // NaN, as atan(0 / 0) does.
         float r2_out = -1.0f;  // r^2 where the point bailed out (the root is taken once, below)
         float r2_last = 0.0f;  // r^2 of the last step taken
         for (uint32_t i = 0; i < iters; ++i) {
@@ -188,12 +188,17 @@ __device__ __forceinline__ float sd_mandelbulb(float px, float py, float pz, flo
             zz = sp + pz;
         }
         if (r2_out >= 0.0f) {
-            r = sqrtf(r2_out);
+            r = __builtin_amdgcn_sqrtf(r2_out);
         } else {
             const float r4 = r2_last * r2_last;
             r = r4 * r4;
         }
-        return ((0.5f * logf(r)) * r) / dr - offset;
+        // 0.5 ln(r) r / dr from the hardware log2 (times ln 2; a denormal r scaled by 2^24
+        // first) and reciprocal, each within about an ulp: dr >= 1, and r = 0 gives NaN as
+        // log(0) * 0 does
+        const bool tiny = r < 0x1p-126f;
+        const float lr = (__builtin_amdgcn_logf(tiny ? r * 0x1p24f : r) - (tiny ? 24.0f : 0.0f)) * 0.693147182f;
+        return ((0.5f * lr) * r) * __builtin_amdgcn_rcpf(dr) - offset;
     }
     for (uint32_t i = 0; i < iters; ++i) {
         r = len3(zx, zy, zz);

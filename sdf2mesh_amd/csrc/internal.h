@@ -182,6 +182,11 @@ struct MeshBufs {
 hipError_t launch_eval_lattice(hipStream_t s, const s2m_op* ops, uint32_t nops, const float* cx,
                                const float* cy, const float* cz, uint32_t nx, uint32_t ny,
                                uint32_t lz0, uint32_t nzl, uint32_t pitch, float* lat);
+// The lattice of a tape that is one Mandelbulb op (k_eval_mandelbulb: equal to
+// launch_eval_lattice's values).
+hipError_t launch_eval_mandelbulb(hipStream_t s, const s2m_op* op, const float* cx, const float* cy,
+                                  const float* cz, uint32_t nx, uint32_t ny, uint32_t lz0, uint32_t nzl,
+                                  uint32_t pitch, float* lat);
 hipError_t launch_cells(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int emit,
                         int write_zero_normals);
 hipError_t launch_scan(hipStream_t s, const MeshBufs& b, uint32_t nblocks, uint32_t which);

@@ -78,6 +78,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) voi
     lat[((size_t)iz * ny + iy) * pitch + ix] = tape_eval(ops, nops, cx[ix], cy[iy], cz[lz0 + iz]);
 }
 
+// The same lattice for a tape that is one Mandelbulb op: the estimator called directly
+// with its operands (wave-uniform loads), as tape_eval calls it, so the values are the
+// interpreter's bit for bit without its per-point dispatch.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_eval_mandelbulb(
+    const s2m_op* __restrict__ op, const float* __restrict__ cx, const float* __restrict__ cy,
+    const float* __restrict__ cz, uint32_t nx, uint32_t ny, uint32_t lz0, uint32_t nzl, uint32_t pitch,
+    float* __restrict__ lat)
+{
+    const uint32_t l = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t ix = blockIdx.x * 16u + w * 4u + (l & 3u);
+    const uint32_t iy = blockIdx.y * 4u + ((l >> 2) & 3u);
+    const uint32_t iz = blockIdx.z * 4u + (l >> 4);
+    if (ix >= nx || iy >= ny || iz >= nzl) return;
+    const float* f = op->f;
+    lat[((size_t)iz * ny + iy) * pitch + ix] =
+        sd_mandelbulb(cx[ix], cy[iy], cz[lz0 + iz], f[0], f[1], f[2], op->iarg, f[3]);
+}
+
 // ---------------------------------------------------------------------------------
 // One cell: dualcontour.wgsl:72-131 (crossings, centroid) and :57-69 (sign bits).
 // c[] in corner order 000,100,010,110,001,101,011,111.
@@ -771,6 +789,16 @@ hipError_t launch_eval_lattice(hipStream_t s, const s2m_op* ops, uint32_t nops, 
     dim3 grid((nx + 15) / 16, (ny + 3) / 4, (nzl + 3) / 4);
     hipLaunchKernelGGL(k_eval_lattice, grid, dim3(256), 0, s, ops, nops, cx, cy, cz, nx, ny, lz0, nzl,
                        pitch, lat);
+    return hipGetLastError();
+}
+
+hipError_t launch_eval_mandelbulb(hipStream_t s, const s2m_op* op, const float* cx, const float* cy,
+                                  const float* cz, uint32_t nx, uint32_t ny, uint32_t lz0, uint32_t nzl,
+                                  uint32_t pitch, float* lat)
+{
+    if (nx == 0 || ny == 0 || nzl == 0) return hipSuccess;
+    dim3 grid((nx + 15) / 16, (ny + 3) / 4, (nzl + 3) / 4);
+    hipLaunchKernelGGL(k_eval_mandelbulb, grid, dim3(256), 0, s, op, cx, cy, cz, nx, ny, lz0, nzl, pitch, lat);
     return hipGetLastError();
 }
 

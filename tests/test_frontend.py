@@ -52,6 +52,8 @@ _HOST_PREFIX = f"""
 #define __forceinline__ inline
 #define __builtin_amdgcn_rsqf(x) (1.0f / sqrtf(x))
 #define __builtin_amdgcn_sqrtf(x) sqrtf(x)
+#define __builtin_amdgcn_logf(x) log2f(x)
+#define __builtin_amdgcn_rcpf(x) (1.0f / (x))
 static inline uint32_t __float_as_uint(float f) {{ uint32_t u; memcpy(&u, &f, 4); return u; }}
 static inline float __uint_as_float(uint32_t u) {{ float f; memcpy(&f, &u, 4); return f; }}
 #include "{CSRC / 'device_sdf.h'}"
