@@ -1255,7 +1255,8 @@ hipError_t eval_lattice_prog(hipStream_t s, const s2m_program* prog, const float
 
 hipError_t normals_prog(hipStream_t s, const s2m_program* prog, float eps, const MeshBufs& b)
 {
-    if (!prog->module) return launch_normals(s, prog->d_ops, prog->n, eps, b);
+    if (!prog->module)
+        return launch_normals(s, prog->d_ops, prog->n, eps, b, prog->n == 1 && prog->ops[0].code == S2M_OP_MANDELBULB);
     float* vtx = b.vtx;
     const uint64_t* counters = b.counters;
     uint64_t vcap = b.vcap;
@@ -1406,14 +1407,13 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
     const uint32_t nslabs = (j.nz_cells + T - 1) / T;
     if (nslabs == 1) rb = nullptr;  // nothing to overlap: the caller copies the finished mesh
     if (rb) HIP_TRY(ctx->host_snap.ensure((size_t)nslabs * 4 * sizeof(uint64_t)));
-    // Several slabs: slab k + 1's lattice is evaluated (VALU-bound) and counted (the
-    // activity bits and block counts: latency-bound gathers) on eval_stream, into the other
-    // of two sets of slab buffers, while slab k's map, vertices, normals and triangles are
-    // made on the context's stream.  The count pass runs right after its evaluation rather
-    // than beside the next one, which slows both.  Events order it: a slab's map waits for
-    // its count, an evaluation waits until the slab two before has finished with its
-    // buffers (its last kernel), and the first evaluation waits for the tables and clears
-    // queued before it.
+    // Several slabs: slab k + 1's lattice is evaluated (VALU-bound) on eval_stream, into
+    // the other of two sets of slab buffers, while slab k is counted (the activity bits and
+    // block counts: latency-bound gathers) and gets its map, vertices, normals and
+    // triangles on the context's stream; the two streams then carry about equal work per
+    // slab.  Events order it: a slab's count waits for its evaluation, an evaluation waits
+    // until the slab two before has finished with its buffers (its last kernel), and the
+    // first evaluation waits for the tables and clears queued before it.
     const bool piped = !fx && !h_corners && nslabs > 1;
     hipStream_t es = piped ? ctx->eval_stream : ctx->stream;
     float* lat_buf[2] = {lat, lat};
@@ -1531,12 +1531,12 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             MeshBufs bs = b;  // this slab's activity bits and block counts
             bs.actbits = act_buf[slabs & 1];
             bs.bcount = cnt_buf[slabs & 1];
-            if (!h_corners) HIP_TRY(launch_cells(es, a, bs, 0, 0));  // the count pass
             if (piped) {
-                hipEvent_t counted = mark(es);
-                if (!counted) return fail(S2M_E_HIP, "event record failed");
-                HIP_TRY(hipStreamWaitEvent(ctx->stream, counted, 0));
+                hipEvent_t evaluated = mark(es);
+                if (!evaluated) return fail(S2M_E_HIP, "event record failed");
+                HIP_TRY(hipStreamWaitEvent(ctx->stream, evaluated, 0));
             }
+            if (!h_corners) HIP_TRY(launch_cells(ctx->stream, a, bs, 0, 0));  // the count pass
             if (h_corners) {
                 const float* src = ctx->corners.as<float>() + (size_t)z0 * plane * 8;
                 HIP_TRY(launch_cells_from_corners(ctx->stream, a, src, b, 0));

@@ -198,7 +198,7 @@ hipError_t launch_bailout_check(hipStream_t s, uint64_t* cnt);
 // threshold of its f[1] (sd_mandelbulb reads it there instead of computing it per point).
 hipError_t launch_tape_prepare(hipStream_t s, s2m_op* ops, uint32_t n);
 hipError_t launch_normals(hipStream_t s, const s2m_op* ops, uint32_t nops, float eps,
-                          const MeshBufs& b);
+                          const MeshBufs& b, bool mandel = false);  // mandel: ops is one Mandelbulb op
 hipError_t launch_quads(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int emit);
 // Between passes of a job: slot and staging counters folded into their maxima, the
 // previous pass's totals made the next pass's first indices, work queues and the

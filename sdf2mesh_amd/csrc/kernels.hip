@@ -591,6 +591,9 @@ static_assert(kDenseVtxTotal == kVtxTotal && kDenseVtxPrev == kVtxPrev, "counter
 // library_normal over the vertices [counters[prev], min(counters[total], vcap)) with one
 // lane per tap: lanes 4j .. 4j + 3 of a wave evaluate taps d1 .. d4 of one vertex (the
 // same points as library_normal), lane 4j gathers them and writes the unit gradient.
+// MANDEL = 1: the tape is one Mandelbulb op, called directly as tape_eval calls it (the
+// same values without the interpreter's per-point dispatch).
+template <int MANDEL>
 __global__ __launch_bounds__(256) void k_normals(const s2m_op* __restrict__ ops, uint32_t nops,
                                                  float eps, float* __restrict__ vtx,
                                                  const uint64_t* __restrict__ counters, uint64_t vcap)
@@ -606,7 +609,9 @@ __global__ __launch_bounds__(256) void k_normals(const s2m_op* __restrict__ ops,
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; beg + (i >> 2) < end;
          i += (uint64_t)gridDim.x * blockDim.x) {
         float* o = vtx + 6 * (beg + (i >> 2));
-        const float d = tape_eval(ops, nops, o[0] + ex, o[1] + ey, o[2] + ez);
+        const float px = o[0] + ex, py = o[1] + ey, pz = o[2] + ez;
+        const float d = MANDEL ? sd_mandelbulb(px, py, pz, ops->f[0], ops->f[1], ops->f[2], ops->iarg, ops->f[3])
+                               : tape_eval(ops, nops, px, py, pz);
         const float d1 = __shfl(d, (int)l0, 64), d2 = __shfl(d, (int)l0 + 1, 64);
         const float d3 = __shfl(d, (int)l0 + 2, 64), d4 = __shfl(d, (int)l0 + 3, 64);
         if (tap == 0u) {
@@ -809,7 +814,9 @@ hipError_t launch_cells(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int
     if (nb == 0) return hipSuccess;
     if (emit) {
         hipLaunchKernelGGL(k_cells_map, dim3(nb), dim3(kBlock), 0, s, a, b);
-        hipLaunchKernelGGL(k_cell_vertices, dim3(1024), dim3(kBlock), 0, s, a, b, write_zero_normals);
+        // about one vertex per thread (a slab's vertex count is only known on the device):
+        // each vertex is a chain of dependent gathers, so many waves keep the loads in flight
+        hipLaunchKernelGGL(k_cell_vertices, dim3(4096), dim3(kBlock), 0, s, a, b, write_zero_normals);
     } else {
         const uint32_t nzl = a.ncells / (a.rx * a.ry);
         const uint32_t nblk = (a.rx + kBlock - 1) / kBlock * a.ry;  // one block per 256 cells of a row
@@ -870,10 +877,12 @@ hipError_t launch_scan(hipStream_t s, const MeshBufs& b, uint32_t nblocks, uint3
 }
 
 hipError_t launch_normals(hipStream_t s, const s2m_op* ops, uint32_t nops, float eps,
-                          const MeshBufs& b)
+                          const MeshBufs& b, bool mandel)
 {
-    hipLaunchKernelGGL(k_normals, dim3(4096), dim3(256), 0, s, ops, nops, eps, b.vtx, b.counters,
-                       b.vcap);
+    if (mandel)
+        hipLaunchKernelGGL(k_normals<1>, dim3(4096), dim3(256), 0, s, ops, nops, eps, b.vtx, b.counters, b.vcap);
+    else
+        hipLaunchKernelGGL(k_normals<0>, dim3(4096), dim3(256), 0, s, ops, nops, eps, b.vtx, b.counters, b.vcap);
     return hipGetLastError();
 }
 
