@@ -20,7 +20,8 @@ runs one of them alone (profiling runs).
 
   roofline      the SDF evaluation kernel (k_sp_tiles on the tile-pruned path: the pruned
                 program at each live tile's lattice points fused with contouring;
-                k_eval_lattice on the dense path): executed pruned-program fp32 flops per
+                k_eval_lattice, or k_eval_mandelbulb for a one-op Mandelbulb tape, on the
+                dense path): executed pruned-program fp32 flops per
                 launch (static op count of the program each tile runs x the lattice points
                 it evaluates, counted on the device) / mean launch time from HIP events
                 recorded around the launches on the library's stream during the timed
@@ -200,6 +201,7 @@ def single_job(name: str, args, ctx, headline: bool) -> dict:
     """Host-landing and device-resident step times of one single-job workload, with the
     evaluation kernel's roofline from events recorded during the host-landing steps."""
     import sdf2mesh_amd as s2m
+    from sdf2mesh_amd import tape
 
     node, params, cfg = workload(name)
     if args.dense:
@@ -249,7 +251,9 @@ def single_job(name: str, args, ctx, headline: bool) -> dict:
     flops_per_launch = agg["eval_flops"] / launches
     achieved = flops_per_launch / (eval_avg_ms / 1e3) / 1e12 if eval_avg_ms > 0 else 0.0
     tiled = agg["tiles_total"] > 0
-    kernel = "k_sp_tiles" if tiled else "k_eval_lattice"
+    ops = node.tape()
+    one_mandelbulb = len(ops) == 1 and int(ops[0]["code"]) == tape.MANDELBULB
+    kernel = "k_sp_tiles" if tiled else ("k_eval_mandelbulb" if one_mandelbulb else "k_eval_lattice")
     d2h_bytes = nvtx * 24 + ntri * 12
     d2h_ms = agg["d2h_ms"] / args.steps
     cfg = {**cfg, "eval": "tile-pruned (exact)" if tiled else
