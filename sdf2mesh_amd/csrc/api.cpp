@@ -1460,7 +1460,7 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
 
         HIP_TRY(hipMemsetAsync(b.counters, 0, kNumCounters * sizeof(uint64_t), ctx->stream));
         HIP_TRY(hipMemsetAsync(vidx, 0xff, (size_t)plane * sizeof(uint32_t), ctx->stream));
-        // the count pass ORs into zeroed words; each slab's emit pass clears what it read
+        // the count pass ORs into zeroed words; each slab's words are cleared after its quad pass
         for (uint64_t* words : {act_buf[0], act_buf[1]})
             HIP_TRY(hipMemsetAsync(words, 0, (size_t)max_blocks * (kCellsPerBlock / 64) * sizeof(uint64_t),
                                    ctx->stream));
@@ -1552,6 +1552,7 @@ int run_job(s2m_ctx* ctx, const s2m_program* prog, const s2m_params* p, const Jo
             HIP_TRY(launch_quads(ctx->stream, a, bs, 0));
             HIP_TRY(launch_scan(ctx->stream, bs, nb, 1));
             HIP_TRY(launch_quads(ctx->stream, a, bs, 1));
+            HIP_TRY(launch_clear_bits(ctx->stream, bs.actbits, a.ncells));  // for the count two slabs on
             if (m[4]) HIP_TRY(hipEventRecord(m[4], ctx->stream));
             if (piped) {  // the slab's lattice, bits and counts are free for the slab two later
                 lat_free[slabs & 1] = mark(ctx->stream);

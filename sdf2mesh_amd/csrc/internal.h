@@ -199,6 +199,8 @@ hipError_t launch_bailout_check(hipStream_t s, uint64_t* cnt);
 hipError_t launch_tape_prepare(hipStream_t s, s2m_op* ops, uint32_t n);
 hipError_t launch_normals(hipStream_t s, const s2m_op* ops, uint32_t nops, float eps,
                           const MeshBufs& b, bool mandel = false);  // mandel: ops is one Mandelbulb op
+// After a slab's quad pass: its activity bits zeroed (the words of `ncells` cells).
+hipError_t launch_clear_bits(hipStream_t s, uint64_t* bits, uint32_t ncells);
 hipError_t launch_quads(hipStream_t s, const SlabArgs& a, const MeshBufs& b, int emit);
 // Between passes of a job: slot and staging counters folded into their maxima, the
 // previous pass's totals made the next pass's first indices, work queues and the

@@ -212,7 +212,7 @@ __device__ __forceinline__ void cell_coords(const SlabArgs& a, uint32_t i, uint3
 // Cell passes over per-cell corner dumps (FROM_CORNERS; the slab lattice has k_count_cols,
 // k_cells_map and k_cell_vertices).  EMIT = 0: every cell's vertex rule; block active
 // counts and each cell's activity bit (b.actbits, one ballot word per 64 cells).  EMIT = 1: the cell -> vertex
-// index map for every cell, its ranks from the activity bits, then the block's active
+// index map for every cell (kNone where no vertex), its ranks from the activity bits, then the block's active
 // cells, listed in LDS in vertex order, are gathered and contoured again on full waves
 // (the same rule on the same corners gives the same vertex) and their vertices written.
 // FROM_CORNERS selects the per-cell corner dump source.
@@ -300,10 +300,7 @@ __global__ __launch_bounds__(256) void k_cells(SlabArgs a, MeshBufs b, const flo
             }
             before += total;
         }
-        __syncthreads();
-        // every read of the block's words is done: clear them for the next slab's count
-        if (threadIdx.x < kCellsPerBlock / 64 && base + 64u * threadIdx.x < a.ncells)
-            b.actbits[(base >> 6) + threadIdx.x] = 0ull;
+        __syncthreads();  // the block's list is complete
         for (uint32_t q = threadIdx.x; q < before; q += kBlock) {
             const uint32_t i = list[q];
             const uint64_t v = (uint64_t)first + q;
@@ -330,8 +327,9 @@ __global__ __launch_bounds__(256) void k_cells(SlabArgs a, MeshBufs b, const flo
 
 // ---------------------------------------------------------------------------------
 // Emit pass over the slab lattice, in two kernels.  k_cells_map: the cell -> vertex map
-// for every cell, the ranks from the activity bits (as k_cells<1>), and each vertex's
-// cell key; it clears the bit words it read.  k_cell_vertices: a grid-stride walk over
+// for the cells with a vertex (and every cell of the slab's last plane), the ranks from
+// the activity bits (as k_cells<1>), and each vertex's cell key; the bits stay for the
+// quad pass (cleared after it).  k_cell_vertices: a grid-stride walk over
 // the slab's vertices [kVtxPrev, min(kVtxTotal, vcap)) that gathers each vertex's cell
 // from its key and writes the vertex (contour_cell, as the count pass decided) and its
 // sign bits.
@@ -369,16 +367,18 @@ __global__ __launch_bounds__(256) void k_cells_map(SlabArgs a, MeshBufs b)
         }
         rank += act ? 1u : 0u;
     }
+    // entries for the cells with a vertex only (k_quads tests a cell's activity bit before
+    // reading its entry), except in the slab's last cell plane, which the next slab reads
+    // as its previous plane without the bits: written in full, kNone where no vertex
     uint32_t* m = a.vidx + plane + i0;
-    if (i0 + 3u < a.ncells && (plane & 3u) == 0u) {
+    const uint32_t last = a.ncells - (uint32_t)plane;  // first cell of the last plane
+    if (i0 >= last && i0 + 3u < a.ncells && (plane & 3u) == 0u) {
         *reinterpret_cast<uint4*>(m) = make_uint4(v[0], v[1], v[2], v[3]);
     } else {
 #pragma unroll
         for (uint32_t u = 0; u < 4u; ++u)
-            if (i0 + u < a.ncells) m[u] = v[u];
+            if (i0 + u < a.ncells && (v[u] != kNone || i0 + u >= last)) m[u] = v[u];
     }
-    __syncthreads();  // every read of the block's words is done
-    if (t < kWords && base + 64u * t < a.ncells) b.actbits[(base >> 6) + t] = 0ull;
 }
 
 __global__ __launch_bounds__(256) void k_cell_vertices(SlabArgs a, MeshBufs b, int zero_normals)
@@ -413,7 +413,7 @@ __global__ __launch_bounds__(256) void k_cell_vertices(SlabArgs a, MeshBufs b, i
 // faces shared with its z-neighbour are loaded twice; the second load hits the cache.
 // The vertex rule's verdict (cell_active) sets bit i % 64 of word i / 64 of `bits` (i = the cell's slab index); a wave's 64
 // cells of a row span at most two words, ORed in with atomics only where a bit is set.
-// `bits` is zero on entry (k_cells<1> clears the words it reads).
+// `bits` is zero on entry (cleared after the slab's quad pass, launch_clear_bits).
 // ---------------------------------------------------------------------------------
 constexpr uint32_t kCountDepth = 8;
 
@@ -656,8 +656,16 @@ __global__ __launch_bounds__(256) void k_quads(SlabArgs a, MeshBufs b)
             const uint64_t key = b.keys[v];
             const uint32_t x = (uint32_t)(key & 0xffffu), y = (uint32_t)((key >> 16) & 0xffffu);
             const uint32_t zl = (uint32_t)(key >> 32) - a.z0;
-            const uint32_t* cur = a.vidx + (size_t)(zl + 1) * plane;
-            const uint32_t* prv = a.vidx + (size_t)zl * plane;
+            // the vertex of the cell at offset `off` of map plane P (P = zl: the plane below,
+            // P = zl + 1: this plane): plane 0 (the previous slab's last plane) is written in
+            // full; in the slab's own planes only cells with a vertex have entries, so the
+            // cell's activity bit is tested first
+            auto at = [&](uint32_t P, size_t off) -> uint32_t {
+                if (P == 0) return a.vidx[off];
+                const size_t i = (size_t)(P - 1) * plane + off;  // the cell's slab index
+                return ((b.actbits[i >> 6] >> (i & 63u)) & 1ull) ? a.vidx[(size_t)P * plane + off] : kNone;
+            };
+            const uint32_t pc = zl + 1, pp = zl;  // map planes of this and the previous cell plane
             const uint32_t s = b.signs[v];
             const bool c0 = s & 1, c1 = (s >> 1) & 1, c2 = (s >> 2) & 1, c3 = (s >> 3) & 1;
             const uint32_t zg = a.z0 + zl;
@@ -665,7 +673,7 @@ __global__ __launch_bounds__(256) void k_quads(SlabArgs a, MeshBufs b)
             // x-edge: Quad(v(x,y-1,z-1), v(x,y,z-1), v(x,y,z), v(x,y-1,z)).swap(c.0)
             if (c0 != c3 && y > 0) {
                 if (zg > 0) {
-                    uint32_t t[4] = {prv[rm + x], prv[r0 + x], v, cur[rm + x]};
+                    uint32_t t[4] = {at(pp, rm + x), at(pp, r0 + x), v, at(pc, rm + x)};
                     if (c0) { q[0][0] = t[3]; q[0][1] = t[2]; q[0][2] = t[1]; q[0][3] = t[0]; }
                     else { q[0][0] = t[0]; q[0][1] = t[1]; q[0][2] = t[2]; q[0][3] = t[3]; }
                     if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 1;
@@ -677,7 +685,7 @@ __global__ __launch_bounds__(256) void k_quads(SlabArgs a, MeshBufs b)
             // y-edge: Quad(v(x-1,y,z-1), v(x,y,z-1), v(x,y,z), v(x-1,y,z)).swap(!c.1)
             if (c1 != c3 && x > 0) {
                 if (zg > 0) {
-                    uint32_t t[4] = {prv[r0 + x - 1], prv[r0 + x], v, cur[r0 + x - 1]};
+                    uint32_t t[4] = {at(pp, r0 + x - 1), at(pp, r0 + x), v, at(pc, r0 + x - 1)};
                     if (!c1) { q[1][0] = t[3]; q[1][1] = t[2]; q[1][2] = t[1]; q[1][3] = t[0]; }
                     else { q[1][0] = t[0]; q[1][1] = t[1]; q[1][2] = t[2]; q[1][3] = t[3]; }
                     if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 2;
@@ -688,7 +696,7 @@ __global__ __launch_bounds__(256) void k_quads(SlabArgs a, MeshBufs b)
             }
             // z-edge: Quad(v(x-1,y-1,z), v(x,y-1,z), v(x,y,z), v(x-1,y,z)).swap(c.2)
             if (c2 != c3 && x > 0 && y > 0) {
-                uint32_t t[4] = {cur[rm + x - 1], cur[rm + x], v, cur[r0 + x - 1]};
+                uint32_t t[4] = {at(pc, rm + x - 1), at(pc, rm + x), v, at(pc, r0 + x - 1)};
                 if (c2) { q[2][0] = t[3]; q[2][1] = t[2]; q[2][2] = t[1]; q[2][3] = t[0]; }
                 else { q[2][0] = t[0]; q[2][1] = t[1]; q[2][2] = t[2]; q[2][3] = t[3]; }
                 if (t[0] != kNone && t[1] != kNone && t[3] != kNone) nq |= 4;
@@ -883,6 +891,21 @@ hipError_t launch_normals(hipStream_t s, const s2m_op* ops, uint32_t nops, float
         hipLaunchKernelGGL(k_normals<1>, dim3(4096), dim3(256), 0, s, ops, nops, eps, b.vtx, b.counters, b.vcap);
     else
         hipLaunchKernelGGL(k_normals<0>, dim3(4096), dim3(256), 0, s, ops, nops, eps, b.vtx, b.counters, b.vcap);
+    return hipGetLastError();
+}
+
+// Zero `n` words (a slab's activity bits once its quad pass has read them, for the count
+// pass that ORs into them two slabs later).
+__global__ __launch_bounds__(256) void k_clear_words(uint64_t* __restrict__ w, uint32_t n)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) w[i] = 0ull;
+}
+
+hipError_t launch_clear_bits(hipStream_t s, uint64_t* bits, uint32_t ncells)
+{
+    const uint32_t n = (ncells + 63u) / 64u;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clear_words, dim3(std::min<uint32_t>((n + 255u) / 256u, 2048u)), dim3(256), 0, s, bits, n);
     return hipGetLastError();
 }
 

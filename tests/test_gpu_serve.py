@@ -43,9 +43,10 @@ def test_served_equals_alone(ctx):
 def test_served_config5_mix(ctx):
     # the serving workload's request mix at full size: resolution log-uniform in [32, 512]
     # (rounded up to a power of two), 80 % random CSG trees, 20 % Mandelbulbs with 4-8
-    # iterations; 120 requests submitted on a seeded Poisson schedule at 400 requests/s
-    # so that arrivals overlap running passes.  Every served mesh must equal the request
-    # meshed alone, and the smallest CSG requests the oracle's meshes.
+    # iterations; 120 requests submitted on a seeded Poisson schedule at 2000 requests/s
+    # (the workload's highest rate) so that arrivals overlap running passes.  Every served
+    # mesh must equal the request meshed alone, and the smallest CSG requests the oracle's
+    # meshes.
     import oracle
     from sdf2mesh_amd.workloads import poisson_arrivals
 
@@ -57,7 +58,7 @@ def test_served_config5_mix(ctx):
     mandel = [nd.tape()[0]["code"] == tape.MANDELBULB for nd in nodes]
     print(f"config-5 mix: {sum(mandel)} Mandelbulbs, resolutions {sorted(set(res))}", flush=True)
     assert max(res) == 512 and min(res) <= 64 and sum(mandel) >= 12
-    arrivals = poisson_arrivals(n, 400.0, seed=5)
+    arrivals = poisson_arrivals(n, 2000.0, seed=5)
     with s2m.Server(0, workers=2, max_batch=16) as srv:
         tickets = []
         t0 = time.perf_counter()
