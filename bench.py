@@ -67,7 +67,8 @@ class CopyProbe:
     freed: ~60 ms, 8 GB: ~0.8 s).  Tuning regrows the library's pass buffers once per
     candidate shape, so a workload's timed steps start only after ``settle`` has seen the
     rate back near the link rate measured when the bench started (the library frees no
-    device memory in steady state)."""
+    device memory in steady state).  Tuning waits for it too: the contexts of earlier
+    workloads are closed before the next one tunes."""
 
     def __init__(self):
         import ctypes as C
@@ -211,6 +212,7 @@ def single_job(name: str, args, ctx, headline: bool) -> dict:
         cfg = {**cfg, "slab": args.slab}
     voxels = int(params.res[0]) * int(params.res[1]) * int(params.res[2])
     prog = ctx.program(node)
+    settle_copies()  # memory freed by the workload before this one: cleared before tuning
     tuned = ctx.tune(prog, params) if not args.no_tune else None  # before the timed steps
     for _ in range(args.warmup):
         ctx.mesh_to_host(prog, params).free()
@@ -331,6 +333,7 @@ def run_batch(args) -> dict:
     ctx = s2m.Context(0)
     progs = [ctx.program(x) for x in nodes]
     steps = max(3, min(args.steps, 10))
+    settle_copies()  # memory freed by the workload before this one: cleared before tuning
     tuned = ctx.tune_batch(progs, params) if not args.no_tune else None
     for _ in range(max(1, args.warmup)):
         for m in ctx.mesh_batch_to_host(progs, params):
