@@ -902,7 +902,7 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
     if (npass == 1) rb = nullptr;  // nothing to overlap: the caller copies the finished mesh
     const uint64_t tiles_pass = n == 1 ? (uint64_t)tabs[0].nt[0] * tabs[0].nt[1] * layers : ntiles_all;
     const uint64_t rows_pass = n == 1 ? (uint64_t)layers * kTile * jobs[0].res[1] : nrows_all;
-    if (tiles_pass > 0x7fffffffull || rows_pass > 0xffffffffull ||
+    if (tiles_pass > 0x7fffffffull || rows_pass > 0xffffffffull || plane_max > 0xffffffffull ||
         (rows_pass + 4095) / 4096 > kMaxScanBlocks)
         return fail(S2M_E_ARG, "grid too large for one pass; mesh the jobs separately");
 
@@ -1077,9 +1077,9 @@ int run_sparse(s2m_ctx* ctx, uint32_t n, const s2m_program* const* progs, const 
         auto enqueue = [&]() -> int {  // the job's kernels, in stream order
         HIP_TRY(launch_sparse_job_begin(ctx->stream, b.counters, kNumCounters + n * kJobOut,
                                         ctx->sp_queues.as<uint32_t>(), b.bcount, 2 * nsb_max,
-                                        ctx->sp_block_stats.as<uint64_t>(), (uint32_t)tile_blocks));
-        if (n == 1 && npass > 1)
-            HIP_TRY(hipMemsetAsync(ctx->sp_boundary.p, 0xff, plane_max * sizeof(uint32_t), ctx->stream));
+                                        ctx->sp_block_stats.as<uint64_t>(), (uint32_t)tile_blocks,
+                                        ctx->sp_boundary.as<uint32_t>(),
+                                        n == 1 && npass > 1 ? (uint32_t)plane_max : 0u));
         if (ctx->profiling) {
             ev_start = ctx->next_event();
             HIP_TRY(hipEventRecord(ev_start, ctx->stream));

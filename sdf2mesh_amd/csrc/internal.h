@@ -214,9 +214,11 @@ hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t*
 hipError_t launch_sparse_job_end(hipStream_t s, uint64_t* counters, uint32_t n, uint64_t* out,
                                  const uint64_t* stats, uint32_t nstats);
 // A job's first pass: counters[0 .. n) (the counters and the job output rows), the work
-// queues and the nbsum chunk sums (bsum, kChunkSumStride words apart) cleared.
+// queues and the nbsum chunk sums (bsum, kChunkSumStride words apart) cleared, and the
+// nbnd entries of a multi-pass job's boundary plane set to "no vertex".
 hipError_t launch_sparse_job_begin(hipStream_t s, uint64_t* counters, uint32_t n, uint32_t* queues,
-                                   uint32_t* bsum, uint32_t nbsum, uint64_t* stats, uint32_t nstats);
+                                   uint32_t* bsum, uint32_t nbsum, uint64_t* stats, uint32_t nstats,
+                                   uint32_t* boundary, uint32_t nbnd);
 // cus: compute units of the device; tile_blocks: most blocks of a k_sp_tiles launch (each
 // holds a staging chunk); general_tiles: some tile's pruned program may not be a left fold
 // (false when every job's whole program is one: its pruned programs are then left folds).

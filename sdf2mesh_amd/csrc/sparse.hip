@@ -1692,12 +1692,17 @@ __global__ void k_snapshot(const uint64_t* counters, uint64_t* out)
 
 __global__ __launch_bounds__(256) void k_sp_job_begin(uint64_t* counters, uint32_t n, uint32_t* queues,
                                                       uint32_t* bsum, uint32_t nbsum, uint64_t* stats,
-                                                      uint32_t nstats)
+                                                      uint32_t nstats, uint32_t* boundary, uint32_t nbnd)
 {
-    for (uint32_t i = thread_id(); i < n; i += blockDim.x) counters[i] = 0;
-    for (uint32_t i = thread_id(); i < 3u * nstats; i += blockDim.x) stats[i] = 0;
-    for (uint32_t i = thread_id(); i < kQueues * kQueueGroups; i += blockDim.x) queues[i * 32u] = 0;
-    for (uint32_t i = thread_id(); i < nbsum; i += blockDim.x) bsum[i * kChunkSumStride] = 0;
+    if (blockIdx.x == 0) {
+        for (uint32_t i = thread_id(); i < n; i += blockDim.x) counters[i] = 0;
+        for (uint32_t i = thread_id(); i < 3u * nstats; i += blockDim.x) stats[i] = 0;
+        for (uint32_t i = thread_id(); i < kQueues * kQueueGroups; i += blockDim.x) queues[i * 32u] = 0;
+        for (uint32_t i = thread_id(); i < nbsum; i += blockDim.x) bsum[i * kChunkSumStride] = 0;
+    }
+    // a multi-pass job's boundary plane: no vertex carried yet
+    for (uint32_t i = blockIdx.x * blockDim.x + thread_id(); i < nbnd; i += gridDim.x * blockDim.x)
+        boundary[i] = 0xffffffffu;
 }
 
 // After a job's last pass: the pass folded into the maxima as k_sp_pass_begin does, then
@@ -1748,10 +1753,12 @@ hipError_t launch_sparse_pass_begin(hipStream_t s, uint64_t* counters, uint32_t*
 }
 
 hipError_t launch_sparse_job_begin(hipStream_t s, uint64_t* counters, uint32_t n, uint32_t* queues,
-                                   uint32_t* bsum, uint32_t nbsum, uint64_t* stats, uint32_t nstats)
+                                   uint32_t* bsum, uint32_t nbsum, uint64_t* stats, uint32_t nstats,
+                                   uint32_t* boundary, uint32_t nbnd)
 {
-    hipLaunchKernelGGL(k_sp_job_begin, dim3(1), dim3(256), 0, s, counters, n, queues, bsum, nbsum, stats,
-                       nstats);
+    const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>((nbnd + 255u) / 256u, 256u));
+    hipLaunchKernelGGL(k_sp_job_begin, dim3(blocks), dim3(256), 0, s, counters, n, queues, bsum, nbsum, stats,
+                       nstats, boundary, nbnd);
     return hipGetLastError();
 }
 
