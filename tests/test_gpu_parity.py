@@ -233,6 +233,29 @@ def test_mandelbulb_contour_parity(ctx, iters, res, flags):
     assert_same_mesh(m, r, normal_atol=None)
 
 
+@pytest.mark.parametrize("iters,res,flags", [(8, 160, s2m.REFERENCE_COMPAT), (5, 97, s2m.CORNERS_EXACT)])
+def test_mandelbulb_kernel_equals_interpreter(ctx, iters, res, flags):
+    # A tape that is one Mandelbulb op is evaluated by k_eval_mandelbulb (lattice) and
+    # k_normals<1> (normals), which call the estimator without the tape interpreter.
+    # union(M, M) = min(d, d) = d takes the interpreter: on the dense path (S2M_DENSE_EVAL)
+    # both must give the same mesh bit for bit, normals included, over several slabs.
+    print(f"mandelbulb kernel vs interpreter iters={iters} res={res}", flush=True)
+    one = tape.mandelbulb(iters)
+    two = tape.union(tape.mandelbulb(iters), tape.mandelbulb(iters))
+    assert len(one.tape()) == 1 and len(two.tape()) == 3
+    p = s2m.Params.reference(res, 2.5, flags=flags | s2m.DENSE_EVAL, slab=24)
+    a = ctx.mesh(ctx.program(one), p)
+    b = ctx.mesh(ctx.program(two), p)
+    assert len(a.vertices) > 1000
+    assert a.invalid_quads == b.invalid_quads
+    assert np.array_equal(a.cell_keys, b.cell_keys) and np.array_equal(a.signs, b.signs)
+    assert np.array_equal(a.vertices.view(np.uint32), b.vertices.view(np.uint32))  # positions and normals
+    assert np.array_equal(a.triangles, b.triangles)
+    la = ctx.eval_lattice(ctx.program(one), p)
+    lb = ctx.eval_lattice(ctx.program(two), p)
+    assert np.array_equal(la.view(np.uint32), lb.view(np.uint32))
+
+
 MANDEL_ULP_FACTOR = 32    # C in |f32 - f64| <= C * (ulp(f) + s)
 MANDEL_BAIL_MARGIN = 1e-5  # points whose bailout tests all clear the threshold by this
 
