@@ -163,7 +163,7 @@ def cpu_info() -> dict:
 def cpu_baseline(node, params, seconds: float) -> dict:
     """The oracle on z-slices through the middle of the job: ~`seconds` of CPU work cut
     into 10 equal chunks, median chunk rate.  Threads: OMP_NUM_THREADS when set (the
-    CPU share of a one-GPU box), else every CPU this process may run on."""
+    CPU share given to this process), else every CPU this process may run on."""
     import oracle
 
     info = cpu_info()
