@@ -542,7 +542,7 @@ def main() -> int:
                     help="evaluate every lattice point with the full program (no tile pruning)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU (a box holds one); jobs would "
+        print(json.dumps({"error": "bench.py measures one GPU; jobs would "
                                    "shard by z-slab or by job across GPUs by design only, so "
                                    "--gpus must be 1"}))
         return 2

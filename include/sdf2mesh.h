@@ -450,7 +450,7 @@ int s2m_eval_lattice_range(s2m_ctx* ctx, const s2m_program* prog, const s2m_para
 int s2m_contour_cells(s2m_ctx* ctx, const float* corners, const uint32_t res[3],
                       const float* cell_lo, const float* cell_hi, uint32_t flags,
                       s2m_mesh_out* out);
-/* No reference counterpart: checks a device math identity this build relies on.  The
+/* No reference counterpart: checks a device math identity the Mandelbulb kernels rely on.  The
  * Mandelbulb's escape step takes the sine and cosine of an angle with one sincosf,
  * where the reference's shader calls sin and cos.  Over every float bit pattern,
  * counts[0] / counts[2] receive the sine / cosine results of sincosf that differ from
@@ -458,7 +458,7 @@ int s2m_contour_cells(s2m_ctx* ctx, const float* corners, const uint32_t res[3],
  * second on one GPU. */
 int s2m_check_sincos(s2m_ctx* ctx, uint64_t counts[4]);
 
-/* No reference counterpart: checks a device identity this build relies on.  The power-8
+/* No reference counterpart: checks a device identity the Mandelbulb kernels rely on.  The power-8
  * Mandelbulb step tests the bailout (`if(r > 2.0) continue;`, examples/mandelmesh.frag:13-14)
  * as r^2 > t, t the largest float whose correctly rounded square root does not exceed
  * the bailout.  Over 2^20 bailouts spread across every bit pattern and the squared

@@ -247,7 +247,7 @@ int main(int argc, char** argv)
     const char* sdf_name = nullptr;
     if (!shadertoy_id.empty()) {
         log_line("INFO", "Reading SDF from ShaderToy (shader ID %s)", shadertoy_id.c_str());
-        log_line("ERROR", "ShaderToy requests need network access, which this build does not have "
+        log_line("ERROR", "ShaderToy requests need network access, which this tool does not use "
                           "(RequestError)");
         return 1;
     } else if (!sdf.empty()) {
