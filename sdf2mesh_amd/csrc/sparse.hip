@@ -918,7 +918,26 @@ __device__ __forceinline__ void tile_corners(const float* box, const TileAxes& A
     for (int k = 0; k < 8; ++k) cv[k] = box[(zi[(k >> 2) & 1] * by + yi[(k >> 1) & 1]) * bx + xi[k & 1]];
 }
 
-__device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, uint32_t slot, uint32_t tx,
+// The job's extents as wave-uniform values, read once per tile: the job table sits in
+// global memory that the kernel also writes, so the compiler re-reads fields of a
+// reference to it inside every loop.
+struct TileGrid {
+    uint32_t rx, ry, nz;  // cells per axis of the job's grid (nz: of this pass)
+    uint32_t bx, by;      // box strides
+};
+
+__device__ __forceinline__ TileGrid tile_grid(const SpJob& J)
+{
+    TileGrid G;
+    G.rx = __builtin_amdgcn_readfirstlane(J.rx);
+    G.ry = __builtin_amdgcn_readfirstlane(J.ry);
+    G.nz = __builtin_amdgcn_readfirstlane(J.nz);
+    G.bx = __builtin_amdgcn_readfirstlane(J.bx);
+    G.by = __builtin_amdgcn_readfirstlane(J.by);
+    return G;
+}
+
+__device__ __forceinline__ void contour_tile(const SpPass& P, const TileGrid& G, uint32_t slot, uint32_t tx,
                                              uint32_t ty, uint32_t tzl, const float* box, const TileAxes& A,
                                              uint32_t nlive, const uint8_t* lb, uint16_t* cand, ContourLds& L)
 {
@@ -933,9 +952,9 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
             const uint32_t cxl = (b & 3u) * 2u + (j & 1u), cyl = ((b >> 2) & 3u) * 2u + ((j >> 1) & 1u);
             const uint32_t czl = (b >> 4) * 2u + (j >> 2);
             c = (czl * 8u + cyl) * 8u + cxl;
-            if (tx * 8u + cxl < J.rx && ty * 8u + cyl < J.ry && tzl * 8u + czl < J.nz) {
+            if (tx * 8u + cxl < G.rx && ty * 8u + cyl < G.ry && tzl * 8u + czl < G.nz) {
                 float cv[8];
-                tile_corners(box, A, J.bx, J.by, c, cv);
+                tile_corners(box, A, G.bx, G.by, c, cv);
                 uint32_t pm = 0;
 #pragma unroll
                 for (int i = 0; i < 8; ++i) pm |= (uint32_t)(cv[i] > 0.0f) << i;
@@ -946,7 +965,7 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
         const uint32_t lead = (uint32_t)__builtin_ctzll(w | (1ull << 63));
         uint32_t wbase = 0;
         if (w && lane() == lead) wbase = atomicAdd(&L.ncand, (uint32_t)__builtin_popcountll(w));
-        wbase = __shfl(wbase, lead, 64);
+        wbase = (uint32_t)__builtin_amdgcn_readlane((int)wbase, (int)lead);
         if (cand_c) cand[wbase + (uint32_t)__builtin_popcountll(w & ((1ull << lane()) - 1ull))] = (uint16_t)c;
     }
     __syncthreads();
@@ -962,7 +981,7 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
             const uint32_t c = cand[k];
             const uint32_t cxl = c & 7u, cyl = (c >> 3) & 7u, czl = c >> 6;
             float cv[8];
-            tile_corners(box, A, J.bx, J.by, c, cv);
+            tile_corners(box, A, G.bx, G.by, c, cv);
             const float lo[3] = {A.clo[0][cxl], A.clo[1][cyl], A.clo[2][czl]};
             const float hi[3] = {A.chi[0][cxl], A.chi[1][cyl], A.chi[2][czl]};
             act[r] = contour_cell_sp(cv, lo, hi, pos[r]);
@@ -975,7 +994,7 @@ __device__ __forceinline__ void contour_tile(const SpPass& P, const SpJob& J, ui
         const uint32_t lead = (uint32_t)__builtin_ctzll(w | (1ull << 63));
         uint32_t wbase = 0;
         if (w && lane() == lead) wbase = atomicAdd(&L.nact, (uint32_t)__builtin_popcountll(w));
-        wbase = __shfl(wbase, lead, 64);
+        wbase = (uint32_t)__builtin_amdgcn_readlane((int)wbase, (int)lead);
         if (act[r]) {
             k_at[r] = wbase + (uint32_t)__builtin_popcountll(w & ((1ull << lane()) - 1ull));
             atomicOr(&L.rows[cell[r] >> 3], 1u << (cell[r] & 7u));
@@ -1084,7 +1103,8 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
         const uint32_t bxh = __builtin_amdgcn_readfirstlane(H.box[0]), byh = __builtin_amdgcn_readfirstlane(H.box[1]);
         const uint32_t bzh = __builtin_amdgcn_readfirstlane(H.box[2]);
         const uint32_t nx = bxh >> 24, ny = byh >> 24, nz = bzh >> 24;
-        const uint32_t bx = J.bx, by = J.by;
+        const TileGrid G = tile_grid(J);
+        const uint32_t bx = G.bx, by = G.by;
         const uint16_t* Lst = P.lists + (size_t)tile * P.list_stride;
         uint64_t lv = ~0ull;
         uint32_t evaluated = 0;
@@ -1126,16 +1146,19 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
                     ((rm & 8u) ? R.xb[3] : 0u);
                 code = (iz << 10) | (iy << 5);
             }
+            // the exclusive prefix sum of cnt (<= 16, 5 bits) over the wave's lanes, bit by
+            // bit: lanes below with the bit set (mbcnt over its ballot) times its weight
             const uint32_t cnt = (uint32_t)__builtin_popcount(M);
-            uint32_t inc = cnt;  // inclusive prefix sum over the wave's lanes
+            uint32_t below = 0, wtotal = 0;
 #pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const uint32_t v = __shfl_up(inc, (unsigned)d, 64);
-                if (lane() >= (uint32_t)d) inc += v;
+            for (uint32_t b = 0; b < 5; ++b) {
+                const uint64_t m = __ballot((int)((cnt >> b) & 1u));
+                below += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << b;
+                wtotal += (uint32_t)__builtin_popcountll(m) << b;
             }
             uint32_t wbase = 0;
-            if (lane() == 63 && inc) wbase = atomicAdd(&nlist, inc);
-            const uint32_t off = __shfl(wbase, 63, 64) + inc - cnt;
+            if (lane() == 0 && wtotal) wbase = atomicAdd(&nlist, wtotal);
+            const uint32_t off = (uint32_t)__builtin_amdgcn_readlane((int)wbase, 0) + below;
             if (kr < need) reinterpret_cast<uint4*>(&R)[kr] = rest;
             // window 0 is filled at once (every wave has its offsets); the total is read
             // after the barrier that completes it
@@ -1202,7 +1225,7 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
             if (thread_id() == 0) nlist = 0;  // every thread has read the total
             prefetch();  // the next slot's record, while this one is contoured
         }
-        contour_tile(P, J, slot, tx, ty, tzl, box, R.A, (uint32_t)__builtin_popcountll(lv), lb, list, L);
+        contour_tile(P, G, slot, tx, ty, tzl, box, R.A, (uint32_t)__builtin_popcountll(lv), lb, list, L);
         if (thread_id() == 0) {
             tiles_done += 1;
             points_done += evaluated;
