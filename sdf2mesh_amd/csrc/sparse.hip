@@ -1138,10 +1138,17 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
             uint32_t M = 0, code = 0;
             if (q < ny * nz) {
                 const uint32_t iz = div16(q, ny, magic16(ny)), iy = q - iz * ny;
-                uint32_t rm = 0;
-                for (uint32_t mz = R.bm[1][iz]; mz; mz &= mz - 1u)
-                    for (uint32_t my = R.bm[0][iy]; my; my &= my - 1u)
-                        rm |= (uint32_t)(lv >> ((__builtin_ctz(mz) * 4 + __builtin_ctz(my)) * 4)) & 0xfu;
+                // rm = OR over the z-bricks bz and y-bricks by that use the row of the live
+                // bits of x-bricks (bz, by, 0..3): the used z layers of lv ORed (16 bits,
+                // by * 4 + bx), then the used y rows of that
+                const uint32_t mz = R.bm[1][iz], my = R.bm[0][iy];
+                uint32_t layers = 0, rm = 0;
+#pragma unroll
+                for (uint32_t bz = 0; bz < 4; ++bz)
+                    if ((mz >> bz) & 1u) layers |= (uint32_t)(lv >> (16u * bz)) & 0xffffu;
+#pragma unroll
+                for (uint32_t by = 0; by < 4; ++by)
+                    if ((my >> by) & 1u) rm |= (layers >> (4u * by)) & 0xfu;
                 M = ((rm & 1u) ? R.xb[0] : 0u) | ((rm & 2u) ? R.xb[1] : 0u) | ((rm & 4u) ? R.xb[2] : 0u) |
                     ((rm & 8u) ? R.xb[3] : 0u);
                 code = (iz << 10) | (iy << 5);
@@ -1166,9 +1173,13 @@ __device__ __forceinline__ void tiles_body(const SpPass& P)
             for (uint32_t w0 = 0;; w0 += kList) {
                 if (w0) __syncthreads();  // the previous window's readers of the list are done
                 if (off < w0 + kList && off + cnt > w0) {
-                    uint32_t at = off - w0;  // wraps below w0: skipped by the bound test
-                    for (uint32_t mm = M; mm; mm &= mm - 1u, ++at)
-                        if (at < kList) list[at] = (uint16_t)(code | (uint32_t)__builtin_ctz(mm));
+                    // the row's k-th needed x position at off + k, over the box's x positions
+                    // (a loop that is the same for every lane)
+                    const uint32_t at0 = off - w0;  // wraps below w0: skipped by the bound test
+                    for (uint32_t b = 0; b < nx; ++b) {
+                        const uint32_t at = at0 + (uint32_t)__builtin_popcount(M & ((1u << b) - 1u));
+                        if (((M >> b) & 1u) && at < kList) list[at] = (uint16_t)(code | b);
+                    }
                 }
                 __syncthreads();
                 if (w0 == 0) total = nlist;  // reset for the next slot after the box is complete
