@@ -571,13 +571,18 @@ def main() -> int:
         out = {**common(args, metric, "Mvoxels/s", head.pop("value"), head.pop("ms_per_step"),
                         "synthetic (seeded SDF program)"), **head}
         if not args.only:
+            # The Mandelbulb job runs last: its context's buffers are the largest, and
+            # the workloads measured right after they were freed ran slower and varied
+            # from run to run (batch256 at 51,000-66,000 jobs/s, 65,000-98,000 with the
+            # meshes left in HBM; alone or before it, 66,500-67,100 and ~99,800).
             others = {}
-            for w in ("torus-128", "mandelbulb-1024"):
-                if w != args.workload:
-                    others[w] = single_job(w, args, ctx, headline=False)
-            ctx.close()
+            if args.workload != "torus-128":
+                others["torus-128"] = single_job("torus-128", args, ctx, headline=False)
             others["batch256"] = run_batch(args)
             others["serve"] = run_serve(args)
+            if args.workload != "mandelbulb-1024":
+                others["mandelbulb-1024"] = single_job("mandelbulb-1024", args, ctx, headline=False)
+            ctx.close()
             out["workloads"] = others
         else:
             ctx.close()
