@@ -198,12 +198,14 @@ def test_contour_cells_special_values(ctx):
 
 def lattice_corners(lat, params, res):
     """Per-cell corners (rz, ry, rx, 8) and per-axis cell bounds read from a corner
-    lattice evaluated on the library's coordinate tables."""
+    lattice evaluated on the library's coordinate tables (res: cells per axis, one
+    number for a cube or (rx, ry, rz))."""
+    res = (res,) * 3 if np.isscalar(res) else tuple(res)
     exact = bool(params.flags & s2m.CORNERS_EXACT)
-    tabs = [s2m.axis_table(params.bb_min[a], params.bb_max[a], res, exact) for a in range(3)]
+    tabs = [s2m.axis_table(params.bb_min[a], params.bb_max[a], res[a], exact) for a in range(3)]
     (cx, lx, hx), (cy, ly, hy), (cz, lz, hz) = tabs
     idx = [(lx, hx), (ly, hy), (lz, hz)]
-    corners = np.empty((res, res, res, 8), np.float32)
+    corners = np.empty((res[2], res[1], res[0], 8), np.float32)
     for k in range(8):
         xi = idx[0][k & 1]
         yi = idx[1][(k >> 1) & 1]
@@ -332,3 +334,46 @@ def test_csg64_linear_vs_dense_small(ctx, res):
         assert np.array_equal(a.triangles, b.triangles)
         assert a.invalid_quads == b.invalid_quads
         np.testing.assert_allclose(a.vertices[:, 3:], b.vertices[:, 3:], atol=1e-6, rtol=0)
+
+
+# grids and boxes that differ per axis (the uniform's dims and bounds are per-axis vectors,
+# dualcontour.wgsl:22-27; the CLI only makes cubes)
+ANISO = [((40, 24, 56), (-1.0, -0.6, -1.2), (1.0, 0.6, 1.2)),
+         ((17, 64, 9), (-0.9, -1.0, -0.3), (0.9, 1.0, 0.3)),
+         ((64, 8, 33), (-1.0, -0.2, -1.0), (1.0, 0.2, 1.0))]
+
+
+@pytest.mark.parametrize("grid", range(len(ANISO)))
+@pytest.mark.parametrize("flags", [0, s2m.CORNERS_EXACT, s2m.REFERENCE_COMPAT])
+def test_anisotropic_grids(ctx, grid, flags):
+    # per-axis coordinate tables, tile counts and partial tiles on each axis separately:
+    # the tile-sparse path (a left fold and a general program), its multi-pass form and
+    # the dense path, each against the oracle
+    res, bmin, bmax = ANISO[grid]
+    for name, node in (("csg64", tape.csg64()), ("random_csg(5)", tape.random_csg(5)),
+                       ("torus", tape.torus_example())):
+        prog = ctx.program(node)
+        r = None
+        for extra, slab in ((0, 0), (0, 16), (s2m.DENSE_EVAL, 0), (s2m.DENSE_EVAL, 8)):
+            p = s2m.Params(bmin, bmax, res, 1e-4, flags | extra, slab)
+            print(f"anisotropic {name} res={res} flags={flags | extra} slab={slab}", flush=True)
+            if r is None:
+                r = reference_mesh(node, p)
+                assert len(r.vertices) > 100
+            assert_same_mesh(ctx.mesh(prog, p), r)
+
+
+@pytest.mark.parametrize("res,flags", [((33, 20, 27), s2m.REFERENCE_COMPAT), ((24, 40, 31), 0)])
+def test_anisotropic_mandelbulb(ctx, res, flags):
+    # the Mandelbulb kernel on a grid and box that differ per axis, over several slabs:
+    # the oracle's contouring of the GPU's own corners must give the same mesh
+    print(f"anisotropic mandelbulb res={res} flags={flags}", flush=True)
+    p = s2m.Params((-1.25, -0.9, -1.1), (1.25, 0.9, 1.1), res, 1e-4, flags, 8)
+    cells = [s2m.round_resolution(r) for r in res] if flags & s2m.COMPAT_POW2 else list(res)
+    prog = ctx.program(tape.mandelbulb(6))
+    lat = ctx.eval_lattice(prog, p)
+    corners, lo, hi = lattice_corners(lat, p, cells)
+    m = ctx.mesh(prog, p)
+    r = oracle.contour_corners(corners, lo, hi, lag=bool(flags & s2m.COMPAT_DROP_LAST_SLICE))
+    assert len(m.vertices) > 100
+    assert_same_mesh(m, r, normal_atol=None)

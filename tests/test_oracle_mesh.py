@@ -134,3 +134,28 @@ def test_normals_sphere():
         p = (0.5 * d).astype(np.float32)
         n = oracle.normal(node, p, 1e-4)
         np.testing.assert_allclose(n, d, atol=2e-3)
+
+
+def test_anisotropic_grid_and_box():
+    # Grids and boxes that differ per axis (the uniform's dims and bounds are per-axis
+    # vectors, dualcontour.wgsl:22-27): each axis has its own cell size, so a vertex placed
+    # through the wrong axis's table lands outside its cell.  A sphere inside the box
+    # still meshes closed, oriented and of genus 0, and its volume converges.
+    bmin, bmax, res = (-1.0, -0.6, -1.2), (1.0, 0.6, 1.2), (40, 24, 56)
+    r = 0.45
+    m = oracle.mesh(T.sphere((0.03, -0.02, 0.05), r), bmin, bmax, list(res))
+    ucnt, dcnt, chi = topology(m)
+    assert len(m.vertices) > 500
+    assert np.all(ucnt == 2) and np.all(dcnt == 1) and chi == 2
+    assert m.invalid_quads == 0
+    want = 4 / 3 * math.pi * r ** 3
+    assert abs(abs(signed_volume(m)) - want) < 0.05 * want
+    keys = m.keys.astype(np.uint64)
+    assert np.all(np.diff(keys.astype(np.int64)) > 0)
+    idx = [(keys & 0xFFFF).astype(np.int64), ((keys >> np.uint64(16)) & 0xFFFF).astype(np.int64),
+           (keys >> np.uint64(32)).astype(np.int64)]
+    p = m.vertices[:, :3]
+    for a in range(3):
+        lo, hi = oracle.axis_bounds(np.float32(bmin[a]), np.float32(bmax[a]), res[a])
+        assert idx[a].max() < res[a]
+        assert np.all(p[:, a] >= lo[idx[a]]) and np.all(p[:, a] <= hi[idx[a]])
