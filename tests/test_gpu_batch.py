@@ -52,6 +52,28 @@ def test_batch_config4_equals_solo(ctx):
         sum(len(m.vertices) for m in batch), sum(len(m.triangles) for m in batch)), flush=True)
 
 
+def test_batch_anisotropic_jobs(ctx):
+    # jobs whose grids and boxes differ per axis, and from each other, share the passes:
+    # per-job tile, super-tile and row numbering with different tile counts per axis;
+    # each batched mesh equals the job alone, and the first jobs the oracle's meshes
+    from test_gpu_parity import ANISO, assert_same_mesh, reference_mesh
+
+    nodes, params = [], []
+    for k in range(24):
+        res, bmin, bmax = ANISO[k % len(ANISO)]
+        node = tape.csg64() if k % 3 == 0 else tape.random_csg(50 + k)
+        flags = (0, s2m.CORNERS_EXACT, s2m.REFERENCE_COMPAT)[k % 3]
+        nodes.append(node)
+        params.append(s2m.Params(bmin, bmax, res, 1e-4, flags))
+    progs = [ctx.program(n) for n in nodes]
+    batch = ctx.mesh_batch(progs, params)
+    for i in range(len(nodes)):
+        same(batch[i], ctx.mesh(progs[i], params[i]))
+    for i in range(4):
+        assert_same_mesh(batch[i], reference_mesh(nodes[i], params[i]))
+    assert sum(len(m.vertices) for m in batch) > 10000
+
+
 def test_batch_repeatable(ctx):
     nodes, params = batch_jobs(32, seed=7)
     progs = [ctx.program(n) for n in nodes]

@@ -340,7 +340,8 @@ def test_csg64_linear_vs_dense_small(ctx, res):
 # dualcontour.wgsl:22-27; the CLI only makes cubes)
 ANISO = [((40, 24, 56), (-1.0, -0.6, -1.2), (1.0, 0.6, 1.2)),
          ((17, 64, 9), (-0.9, -1.0, -0.3), (0.9, 1.0, 0.3)),
-         ((64, 8, 33), (-1.0, -0.2, -1.0), (1.0, 0.2, 1.0))]
+         ((64, 8, 33), (-1.0, -0.2, -1.0), (1.0, 0.2, 1.0)),
+         ((48, 36, 20), (-0.3, -1.1, -0.2), (1.1, 0.4, 0.9))]  # not centred on the origin
 
 
 @pytest.mark.parametrize("grid", range(len(ANISO)))
