@@ -65,6 +65,28 @@ def test_compiled_mesh_matches_oracle(ctx, engraved, res, flags):
                      reference_mesh(oracle.example_tape(oracle.ORA_OP_ENGRAVED_CUBE), p))
 
 
+@pytest.mark.parametrize("flags", [0, s2m.REFERENCE_COMPAT])
+def test_compiled_anisotropic_matches_oracle(ctx, engraved, flags):
+    # the compiled lattice kernel on grids and boxes that differ per axis, over several
+    # dense slabs: mesh and lattice against the oracle's restatement
+    from test_gpu_parity import ANISO
+
+    ora = oracle.example_tape(oracle.ORA_OP_ENGRAVED_CUBE)
+    for res, bmin, bmax in ANISO:
+        print(f"engraved anisotropic res={res} flags={flags}", flush=True)
+        p = s2m.Params(bmin, bmax, res, 1e-4, flags, 8)
+        assert_same_mesh(ctx.mesh(engraved, p), reference_mesh(ora, p))
+        got = ctx.eval_lattice(engraved, p)
+        cells = [s2m.round_resolution(r) for r in res] if flags & s2m.COMPAT_POW2 else list(res)
+        exact = bool(flags & s2m.CORNERS_EXACT)
+        ax = [s2m.axis_table(bmin[a], bmax[a], cells[a], exact)[0] for a in range(3)]
+        g = np.stack(np.meshgrid(ax[0], ax[1], ax[2], indexing="ij"), -1)  # (x, y, z, 3)
+        pts = g.transpose(2, 1, 0, 3).reshape(-1, 3)                        # z slowest
+        want = oracle.eval_f32(ora, pts)
+        assert got.shape == (len(ax[2]), len(ax[1]), len(ax[0]))
+        assert np.array_equal(got.reshape(-1).view(np.uint32), want.view(np.uint32))
+
+
 @pytest.mark.parametrize("name", ["csg64.sdf3d", "random_csg2.sdf3d"])
 def test_forced_compiled_equals_tape(ctx, name):
     print(f"forced compiled {name}", flush=True)
