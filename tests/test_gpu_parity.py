@@ -378,3 +378,29 @@ def test_anisotropic_mandelbulb(ctx, res, flags):
     r = oracle.contour_corners(corners, lo, hi, lag=bool(flags & s2m.COMPAT_DROP_LAST_SLICE))
     assert len(m.vertices) > 100
     assert_same_mesh(m, r, normal_atol=None)
+
+
+# boxes far smaller or larger than the programs' features: cells a few dozen float
+# spacings wide around a point of the surface (the interval pruning's margins and the
+# crossing parameters at the limit of fp32), and whole primitives within a few cells
+SCALES = [("torus", (0.69, -0.01, -0.01), (0.71, 0.01, 0.01), (64, 64, 64)),
+          ("torus", (0.6999, -1e-4, -1e-4), (0.7001, 1e-4, 1e-4), (48, 48, 48)),
+          ("csg64", (-0.05, -0.05, -0.05), (0.05, 0.05, 0.05), (64, 64, 64)),
+          ("csg64", (-6.0, -6.0, -6.0), (6.0, 6.0, 6.0), (128, 128, 128)),
+          ("torus", (-20.0, -20.0, -20.0), (20.0, 20.0, 20.0), (256, 256, 256))]
+
+
+@pytest.mark.parametrize("case", range(len(SCALES)))
+@pytest.mark.parametrize("flags", [0, s2m.REFERENCE_COMPAT])
+def test_tiny_and_large_boxes(ctx, case, flags):
+    name, bmin, bmax, res = SCALES[case]
+    node = tape.csg64() if name == "csg64" else tape.torus_example()
+    prog = ctx.program(node)
+    r = None
+    for extra in (0, s2m.DENSE_EVAL):
+        p = s2m.Params(bmin, bmax, res, 1e-4, flags | extra)
+        print(f"scale {name} box {bmin}..{bmax} res={res} flags={flags | extra}", flush=True)
+        if r is None:
+            r = reference_mesh(node, p)
+            assert len(r.vertices) > 100
+        assert_same_mesh(ctx.mesh(prog, p), r)
