@@ -404,3 +404,36 @@ def test_tiny_and_large_boxes(ctx, case, flags):
             r = reference_mesh(node, p)
             assert len(r.vertices) > 100
         assert_same_mesh(ctx.mesh(prog, p), r)
+
+
+def op_zoo(k: float):
+    """Every operator (the smooth ones with radius k) over every primitive, nested so
+    that each operator's operands are themselves composite somewhere."""
+    sph = tape.sphere((0.1, -0.05, 0.0), 0.45)
+    box = tape.box((-0.2, 0.1, 0.05), (0.6, 0.35, 0.5))
+    tor = tape.torus((0.05, 0.0, -0.1), (0.4, 0.12))
+    cyl = tape.cylinder((0.0, 0.2, 0.15), 0.7, 0.18)
+    cap = tape.capsule((-0.5, -0.3, -0.2), (0.45, 0.35, 0.3), 0.12)
+    a = tape.smooth_intersection(tape.smooth_union(sph, cyl, k), box, k)
+    b = tape.smooth_subtraction(tape.intersection(tor, tape.sphere((0.3, 0.0, 0.0), 0.5)), cap, k)
+    c = tape.subtraction(tape.sphere((0.0, 0.0, 0.0), 0.2), tape.union(a, b))
+    return tape.smooth_union(c, tape.smooth_subtraction(cyl, tor, k), k)
+
+
+@pytest.mark.parametrize("k", [0.2, 0.05, 1e-3, 0.0])
+@pytest.mark.parametrize("res,flags", [(48, s2m.REFERENCE_COMPAT), (61, 0), (80, s2m.CORNERS_EXACT)])
+def test_operator_zoo(ctx, k, res, flags):
+    # all six operators over all five library primitives in one general (not left-fold)
+    # program: the interval rules of every operator in the pruning (k = 0 disables the
+    # smooth ones' pruning; their division by k then gives the same infinities and NaNs
+    # on both sides), the tape interpreter's dispatch, sparse and dense paths
+    node = op_zoo(k)
+    prog = ctx.program(node)
+    r = None
+    for extra in (0, s2m.DENSE_EVAL):
+        p = s2m.Params.reference(res, 2.0, flags=flags | extra)
+        print(f"operator zoo k={k} res={res} flags={flags | extra}", flush=True)
+        if r is None:
+            r = reference_mesh(node, p)
+            assert len(r.vertices) > 200
+        assert_same_mesh(ctx.mesh(prog, p), r)
