@@ -437,3 +437,69 @@ def test_operator_zoo(ctx, k, res, flags):
             r = reference_mesh(node, p)
             assert len(r.vertices) > 200
         assert_same_mesh(ctx.mesh(prog, p), r)
+
+
+def clustered_fold(n: int, seed: int, k: float):
+    """A smooth-union fold of n overlapping spheres (2n - 1 ops); with a large radius k
+    no primitive can be pruned, so every tile keeps the whole program."""
+    rng = tape.Rng(seed)
+    prims = [tape.sphere((rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3), rng.uniform(-0.3, 0.3)),
+                         rng.uniform(0.12, 0.3)) for _ in range(n)]
+    return tape.fold(prims, tape.smooth_union, k)
+
+
+def balanced_tree(n: int, seed: int):
+    """Pairwise reduction of n mixed primitives under cycling operators (2n - 1 ops,
+    operand stack depth log2(n) + 1): a general program, not a left fold."""
+    rng = tape.Rng(seed)
+    level = []
+    for i in range(n):
+        c = (rng.uniform(-0.6, 0.6), rng.uniform(-0.6, 0.6), rng.uniform(-0.6, 0.6))
+        s = rng.uniform(0.05, 0.15)
+        level.append([tape.sphere(c, s), tape.box(c, (2 * s, s, 1.5 * s)), tape.torus(c, (s, 0.4 * s))][i % 3])
+    ops = [tape.union, lambda a, b: tape.smooth_union(a, b, 0.03), tape.union,
+           lambda a, b: tape.smooth_subtraction(a, b, 0.02), tape.subtraction]
+    i = 0
+    while len(level) > 1:
+        nxt = []
+        for j in range(0, len(level) - 1, 2):
+            nxt.append(ops[i % len(ops)](level[j], level[j + 1]))
+            i += 1
+        if len(level) % 2:
+            nxt.append(level[-1])
+        level = nxt
+    return level[0]
+
+
+def right_chain(n: int, seed: int):
+    """P0 op (P1 op (... op P(n-1))): operand stack depth n (16 is the accepted maximum)."""
+    rng = tape.Rng(seed)
+    prims = [tape.sphere((rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5)),
+                         rng.uniform(0.1, 0.35)) for _ in range(n)]
+    d = prims[-1]
+    for p in reversed(prims[:-1]):
+        d = tape.smooth_union(p, d, 0.04)
+    return d
+
+
+LONG = {"fold300_k0.05": lambda: clustered_fold(300, 3, 0.05),
+        "fold300_k0.5": lambda: clustered_fold(300, 3, 0.5),
+        "tree256": lambda: balanced_tree(256, 5),
+        "chain16": lambda: right_chain(16, 7)}
+
+
+@pytest.mark.parametrize("name", list(LONG))
+@pytest.mark.parametrize("res,flags", [(64, 0), (57, s2m.CORNERS_EXACT), (48, s2m.REFERENCE_COMPAT)])
+def test_long_and_deep_programs(ctx, name, res, flags):
+    # programs past the 128 ops staged in LDS (the pruning's unstaged walk, the tile and
+    # emission kernels' interpreter over global memory) and the deepest operand stack
+    node = LONG[name]()
+    prog = ctx.program(node)
+    r = None
+    for extra in (0, s2m.DENSE_EVAL):
+        p = s2m.Params.reference(res, 2.0, flags=flags | extra)
+        print(f"program {name} ({len(node.tape())} ops) res={res} flags={flags | extra}", flush=True)
+        if r is None:
+            r = reference_mesh(node, p)
+            assert len(r.vertices) > 500
+        assert_same_mesh(ctx.mesh(prog, p), r)
