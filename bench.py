@@ -314,6 +314,18 @@ def single_job(name: str, args, ctx, headline: bool) -> dict:
                 stages[k] += t[k]
         ctx.set_profiling(0)
         out["stages_ms_per_step"] = {k: round(v / n_stage, 3) for k, v in stages.items()}
+        if tiled and stages["eval_ms"] > 0:
+            # the same kernel over the whole job in one launch (the device-resident job is
+            # one pass): the streamed headline's half-size launches each carry a ramp and
+            # a tail of their own
+            one_ms = stages["eval_ms"] / n_stage
+            job_flops = agg["eval_flops"] / args.steps
+            one = job_flops / (one_ms / 1e3) / 1e12
+            out["roofline"]["single_pass"] = {
+                "launch_ms": round(one_ms, 4), "achieved": round(one, 2),
+                "frac": round(one / PEAK_FP32_TFLOPS, 4),
+                "frac_of_issue_ceiling": round(one / ISSUE_CEILING_TOPS, 4),
+                "pruned_program_flops_per_launch": job_flops}
         out["roofline"]["reference_equivalent_tflops"] = round(
             node.flops() * (int(params.res[0]) + 1) ** 3 / (ms / 1e3) / 1e12, 1)
     prog.close()
