@@ -503,3 +503,29 @@ def test_long_and_deep_programs(ctx, name, res, flags):
             r = reference_mesh(node, p)
             assert len(r.vertices) > 500
         assert_same_mesh(ctx.mesh(prog, p), r)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_largest_resolution_per_axis(ctx, axis):
+    # 65535 cells on one axis, the most a u16 cell-key component holds (mesh.rs:214): a
+    # capsule through the whole length of a 2000-long box, so that the last cell index
+    # carries vertices; 8192 tiles in a row along x, 65535 rows along y, 65535 cell planes
+    # (passes and slabs) along z.  Rounding 65535 up to a power of two is refused.
+    res = [4, 4, 4]
+    res[axis] = 65535
+    a, b, lo, hi = [0.0] * 3, [0.0] * 3, [-0.5] * 3, [0.5] * 3
+    a[axis], b[axis], lo[axis], hi[axis] = -1100.0, 1100.0, -1000.0, 1000.0
+    node = tape.capsule(tuple(a), tuple(b), 0.3)
+    prog = ctx.program(node)
+    for flags in (0, s2m.CORNERS_EXACT | s2m.COMPAT_DROP_LAST_SLICE):
+        want = None
+        for extra in (0, s2m.DENSE_EVAL):
+            p = s2m.Params(tuple(lo), tuple(hi), tuple(res), 1e-4, flags | extra)
+            print(f"largest resolution axis={axis} res={res} flags={flags | extra}", flush=True)
+            if want is None:
+                want = reference_mesh(node, p)
+                top = (want.keys.astype(np.uint64) >> np.uint64(16 * axis)) & np.uint64(0xFFFF)
+                assert int(top.max()) >= 65533  # 65534, or 65533 under the readback lag on z
+            assert_same_mesh(ctx.mesh(prog, p), want)
+    with pytest.raises(s2m.S2MError):
+        ctx.mesh(prog, s2m.Params(tuple(lo), tuple(hi), tuple(res), 1e-4, s2m.COMPAT_POW2))
