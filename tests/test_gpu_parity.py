@@ -529,3 +529,33 @@ def test_largest_resolution_per_axis(ctx, axis):
             assert_same_mesh(ctx.mesh(prog, p), want)
     with pytest.raises(s2m.S2MError):
         ctx.mesh(prog, s2m.Params(tuple(lo), tuple(hi), tuple(res), 1e-4, s2m.COMPAT_POW2))
+
+
+def degenerate_capsule_programs():
+    """A capsule whose segment has length 0 (its clamp takes 0 / 0 = NaN, which the IEEE
+    max of the clamp turns into 0: a sphere) under every operator, on both sides."""
+    deg = tape.capsule((0.1, 0.0, 0.0), (0.1, 0.0, 0.0), 0.3)
+    sph = tape.sphere((0.0, 0.0, 0.0), 0.5)
+    tor = tape.torus_example()
+    return {"union": tape.union(deg, sph), "union_rev": tape.union(sph, deg),
+            "intersection": tape.intersection(tor, deg), "subtraction": tape.subtraction(deg, sph),
+            "smooth_union": tape.smooth_union(deg, sph, 0.1),
+            "smooth_intersection": tape.smooth_intersection(sph, deg, 0.1),
+            "mixed": tape.union(tape.smooth_union(tor, deg, 0.05), tape.intersection(sph, tape.union(deg, tor)))}
+
+
+@pytest.mark.parametrize("name", list(degenerate_capsule_programs()))
+@pytest.mark.parametrize("res,flags", [(48, 0), (61, s2m.REFERENCE_COMPAT)])
+def test_degenerate_capsule(ctx, name, res, flags):
+    # the NaN inside the capsule's clamp: the pruning's interval of the primitive, the
+    # interpreters and the oracle must agree on its value everywhere
+    node = degenerate_capsule_programs()[name]
+    prog = ctx.program(node)
+    r = None
+    for extra in (0, s2m.DENSE_EVAL):
+        p = s2m.Params.reference(res, 2.0, flags=flags | extra)
+        print(f"degenerate capsule {name} res={res} flags={flags | extra}", flush=True)
+        if r is None:
+            r = reference_mesh(node, p)
+            assert len(r.vertices) > 100
+        assert_same_mesh(ctx.mesh(prog, p), r)
