@@ -614,6 +614,7 @@ struct Checker {
         switch (s.kind) {
         case Stmt::Decl: {
             Sym sym;
+            if (!s.has_type && !s.init) return fail(s.line, "'" + s.name + "' needs a type or an initializer");
             if (s.init && !expr(*s.init)) return false;
             if (s.has_type) {
                 if (s.init && !convertible(s.init->type, s.decl_type, lang))

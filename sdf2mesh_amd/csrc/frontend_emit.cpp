@@ -725,7 +725,10 @@ struct Emitter {
             o << in2 << "for (; " << g << "++ < " << kLoopBound;
             if (s.init) o << " && (" << ex(*s.init) << ")";
             o << ";";
-            if (s.for_update) o << " " << assign_expr(*s.for_update);
+            if (s.for_update) {  // an assignment, or any expression (GLSL's loop-expression)
+                if (s.for_update->kind == Stmt::Assign) o << " " << assign_expr(*s.for_update);
+                else o << " (void)(" << ex(*s.for_update->init) << ")";
+            }
             o << ") {\n";
             body(s.body, in2 + "    ");
             o << in2 << "}\n" << ind << "}\n";

@@ -539,6 +539,8 @@ struct Parser {
                 if (!s->init) return false;
             } else if (!s->is_var) {
                 return fail("'" + kw + "' needs an initializer");
+            } else if (!s->has_type) {
+                return fail("'var " + s->name + "' needs a type or an initializer");
             }
             out.push_back(s);
             return true;

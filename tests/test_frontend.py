@@ -281,6 +281,17 @@ void main() {}
     np.testing.assert_array_equal(bits(f(pts)), bits(want))
 
 
+def test_glsl_loop_expression_is_any_expression():
+    # GLSL's for-loop expression may be any expression, not only an assignment: a bare
+    # `i` is a no-op step (this loop never ends by its condition; the emitted trip-count
+    # guard ends it), and the generated code evaluates it as an expression statement
+    glsl = ("#version 450\nfloat sdf(vec3 p) { float d = length(p) - 0.5; "
+            "for (int i = 0; i < 8; i) { if (d > 10.0) break; } return d; }\nvoid main() {}\n")
+    hip = s2m.translate(glsl, s2m.STAGE_HIP, lang="glsl").decode()
+    assert "(void)(u_i)" in hip
+    assert len(s2m.translate(glsl, s2m.STAGE_CODE, lang="glsl")) > 0  # hiprtc accepts it
+
+
 def test_unbounded_loop_terminates():
     src = "use sdf3d::*;\nfn sdf3d(p: vec3f) -> f32 { var i = 0; while (true) { i = i + 1; } return sdf3d_sphere(p, 0.5); }"
     f = host_program(hip_of(src))
@@ -314,6 +325,7 @@ def test_use_and_include_expansion(tmp_path, monkeypatch):
     ("use sdf3d::*;\nfn sdf3d_sphere(p: vec3f, s: f32) -> f32 { return 0.0; }\nfn sdf3d(p: vec3f) -> f32 { return 0.0; }", "sdf3d", s2m.E_VALIDATION),
     ("use sdf3d::*;\nfn sdf3d(p: vec3f) -> f32 { let a = 1.0; a = 2.0; return a; }", "sdf3d", s2m.E_VALIDATION),
     ("use sdf3d::*;\nfn sdf3d(p: vec3f) -> vec3f { return p; }", "sdf3d", s2m.E_VALIDATION),
+    ("use sdf3d::*;\nfn sdf3d(p: vec3f) -> f32 { var d; d = 1.0; return d; }", "sdf3d", s2m.E_PARSE),
     ("#version 450\nfloat sdf(vec3 p) { return length(p) - 1.0; }\n", "glsl", s2m.E_SHADER),
     ("#version 450\nfloat other(vec3 p) { return length(p) - 1.0; }\nvoid main() {}\n", "glsl", s2m.E_MISSING_SDF),
     ("#version 450\nfloat sdf(vec3 p) { return length(p) - 1.0 }\nvoid main() {}\n", "glsl", s2m.E_PARSE),
