@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import ctypes as C
 import subprocess
+import sys
 import tempfile
 from pathlib import Path
 
@@ -381,3 +382,42 @@ def test_hiprtc_builds_code_objects(path, lang):
 def test_hiprtc_builds_varied_programs(src, lang):
     code = s2m.translate(src, s2m.STAGE_CODE, lang=lang)
     assert code[:4] == b"\x7fELF"
+
+
+_MUTATE = r'''
+import random, sys
+from pathlib import Path
+import sdf2mesh_amd as s2m
+G = Path(sys.argv[1])
+seeds = [(p.read_text(), "sdf3d") for p in sorted(G.glob("*.sdf3d"))] + [((G / "mandelbulb8.frag").read_text(), "glsl")]
+alph = list("(){}[];:,.+-*/<>=!&|?^%~ \n0123456789exyz_") + ["fn ", "let ", "var ", "return ", "if ", "else ",
+       "for ", "while ", "vec3f(", "sdf3d", "1e38", "1e-45", "i32", "u32", "f32", "break;", "float ", "vec3 "]
+rng = random.Random(int(sys.argv[2]))
+for it in range(int(sys.argv[3])):
+    src, lang = rng.choice(seeds)
+    for _ in range(rng.randint(1, 6)):
+        if not src:
+            break
+        i = rng.randrange(len(src)); j = min(len(src), i + rng.randint(1, 40)); k = rng.randrange(6)
+        if k == 0: src = src[:i] + src[j:]
+        elif k == 1: src = src[:i] + src[i:j] + src[i:]
+        elif k == 2: src = src[:i] + rng.choice(alph) + src[i:]
+        elif k == 3: src = src[:i] + chr(rng.randrange(1, 128)) + src[i + 1:]
+        elif k == 4: src = src[:i]
+        else: src = src[:i] + src[j:] + src[i:j]
+    for stage in (s2m.STAGE_SOURCE, s2m.STAGE_TAPE, s2m.STAGE_HIP):
+        try:
+            s2m.translate(src, stage, lang)
+        except s2m.S2MError:
+            pass
+print("ok")
+'''
+
+
+@pytest.mark.parametrize("seed", [1, 6, 7])
+def test_mutated_programs_fail_cleanly(seed):
+    # random edits of the test programs through every host stage: each is translated or
+    # rejected with an error code, never a crash (a child process, so a crash fails here)
+    r = subprocess.run([sys.executable, "-c", _MUTATE, str(GOLDEN), str(seed), "1500"], capture_output=True,
+                       text=True, timeout=600, cwd=str(REPO))
+    assert r.returncode == 0 and r.stdout.strip().endswith("ok"), (r.returncode, r.stderr[-2000:])
