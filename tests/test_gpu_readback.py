@@ -120,3 +120,30 @@ def test_graph_replays_equal_eager(ctx, batch):
             assert np.array_equal(gt, rt)
     print("graph launches per call:", launches, flush=True)
     assert launches[1:] == [1] * 5  # the second call captures, every later call replays
+
+
+def test_graph_after_buffers_regrow():
+    # A job captured into a graph, then a larger job that regrows (frees and allocates
+    # again) the context's buffers, then the first job again: its graph holds the old
+    # addresses, so the call must capture anew and still give the oracle's mesh.
+    from test_gpu_parity import assert_same_mesh, reference_mesh
+
+    s2m.tuning_clear()
+    c = s2m.Context(0)  # a fresh context: buffers sized for the first job
+    try:
+        node = tape.csg64()
+        prog = c.program(node)
+        p = s2m.Params.reference(64, 2.0)
+        ref = reference_mesh(node, p)
+        launches = []
+        for _ in range(3):
+            assert_same_mesh(c.mesh(prog, p), ref)
+            launches.append(c.last_timing()["graph_launches"])
+        c.mesh(c.program(tape.random_csg(9)), s2m.Params.reference(512, 2.0))  # regrows the buffers
+        for _ in range(3):
+            assert_same_mesh(c.mesh(prog, p), ref)
+            launches.append(c.last_timing()["graph_launches"])
+        print("graph launches per call:", launches, flush=True)
+        assert launches[:3] == [0, 1, 1] and launches[4:] == [1, 1]
+    finally:
+        c.close()
