@@ -147,3 +147,55 @@ def test_graph_after_buffers_regrow():
         assert launches[:3] == [0, 1, 1] and launches[4:] == [1, 1]
     finally:
         c.close()
+
+
+def test_threads_with_own_contexts(ctx):
+    # One context per host thread (the C ABI's threading contract): four threads mesh
+    # tile-sparse, dense, Mandelbulb and batched jobs at once, each result equal bit for
+    # bit to the same job meshed alone, and each thread's failing call reports its own
+    # error (the last-error message is per thread).
+    import threading
+
+    jobs = [(tape.csg64(), s2m.Params.reference(96, 2.0)),
+            (tape.random_csg(21), s2m.Params.reference(80, 2.0, flags=0)),
+            (tape.torus_example(), s2m.Params.reference(64, 2.0, flags=s2m.REFERENCE_COMPAT | s2m.DENSE_EVAL)),
+            (tape.mandelbulb(5), s2m.Params.reference(48, 2.5))]
+    solo = [ctx.mesh(ctx.program(n), p) for n, p in jobs]
+    problems, results = {}, {}
+
+    def work(t):
+        try:
+            c = s2m.Context(0)
+            try:
+                out = []
+                for _ in range(3):
+                    for k, (n, p) in enumerate(jobs):
+                        out.append((k, c.mesh(c.program(n), p)))
+                    bad = s2m.Params((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), (70000 + t, 4, 4), 1e-4, 0)
+                    try:
+                        c.mesh(c.program(jobs[0][0]), bad)
+                        problems[t] = "no error for an oversized grid"
+                    except s2m.S2MError as e:
+                        if str(70000 + t) not in str(e):
+                            problems[t] = f"another thread's message: {e}"
+                    b = c.mesh_batch([c.program(n) for n, _ in jobs[:2]], [p for _, p in jobs[:2]])
+                    out += [(0, b[0]), (1, b[1])]
+                results[t] = out
+            finally:
+                c.close()
+        except Exception as e:  # reported below, with the thread
+            problems[t] = repr(e)
+
+    threads = [threading.Thread(target=work, args=(t,)) for t in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=240)
+    assert not problems, problems
+    for t in range(4):
+        assert len(results[t]) == 18
+        for k, m in results[t]:
+            r = solo[k]
+            assert np.array_equal(m.vertices.view(np.uint32), r.vertices.view(np.uint32))
+            assert np.array_equal(m.triangles, r.triangles)
+            assert np.array_equal(m.cell_keys, r.cell_keys) and m.invalid_quads == r.invalid_quads
