@@ -128,3 +128,31 @@ def test_take_busy_then_done():
         assert len(r[0].vertices) > 0
         with pytest.raises(s2m.S2MError):
             srv.take(t)  # a ticket can be taken once
+
+
+def test_failing_requests_do_not_stop_the_server(ctx):
+    # requests the library refuses (a grid past the 65535-cell key limit, an invalid
+    # tape) fail on their own, at submission or when taken; the requests around them are
+    # served and equal to meshing them alone
+    good = [(tape.csg64(), s2m.Params.reference(64, 2.0)), (tape.random_csg(3), s2m.Params.reference(96, 2.0)),
+            (tape.mandelbulb(5), s2m.Params.reference(64, 2.5))]
+    bad_grid = s2m.Params((-1.0, -1.0, -1.0), (1.0, 1.0, 1.0), (70000, 8, 8), 1e-4, 0)
+    bad_tape = np.zeros(3, s2m.TAPE_DTYPE)  # three zero ops: not a valid program
+    with s2m.Server(0, workers=2, max_batch=8) as srv:
+        tickets, failed = [], 0
+        for r in range(3):
+            for n, p in good:
+                tickets.append((srv.submit(n, p), n, p))
+            for prog, p in ((tape.torus_example(), bad_grid), (bad_tape, s2m.Params.reference(32, 2.0))):
+                try:
+                    t = srv.submit(prog, p)
+                except s2m.S2MError:
+                    failed += 1
+                    continue
+                with pytest.raises(s2m.S2MError):
+                    srv.take(t)
+                failed += 1
+        assert failed == 6
+        for t, n, p in tickets:
+            m, _ = srv.take(t)
+            same(m, ctx.mesh(ctx.program(n), p))
