@@ -559,3 +559,47 @@ def test_degenerate_capsule(ctx, name, res, flags):
             r = reference_mesh(node, p)
             assert len(r.vertices) > 100
         assert_same_mesh(ctx.mesh(prog, p), r)
+
+
+def test_randomized_sweep(ctx):
+    # Seeded random cases: random CSG trees, CSG-64, the operator zoo with random radii,
+    # clustered folds of random length and radius and balanced trees, on random grids
+    # (often anisotropic), boxes, flag sets and pass sizes, sparse and dense, each vs the
+    # oracle.
+    import random
+
+    rng = random.Random(20261203)
+    for it in range(200):
+        seed = rng.randrange(1, 10 ** 6)
+        u = rng.random()
+        if u < 0.7:
+            node, name = tape.random_csg(seed), f"random_csg({seed})"
+        elif u < 0.75:
+            node, name = tape.csg64(), "csg64"
+        elif u < 0.85:
+            k = rng.choice([0.0, 1e-3, 0.02, 0.1, 0.3])
+            node, name = op_zoo(k), f"op_zoo({k})"
+        elif u < 0.95:
+            m, k = rng.randint(20, 150), rng.choice([0.01, 0.05, 0.2, 0.5])
+            node, name = clustered_fold(m, seed, k), f"clustered_fold({m}, {seed}, {k})"
+        else:
+            m = rng.randint(8, 200)
+            node, name = balanced_tree(m, seed), f"balanced_tree({m}, {seed})"
+        res = tuple(rng.choice([9, 16, 17, 24, 31, 33, 40, 48, 57, 64, 80, 96]) for _ in range(3))
+        if rng.random() < 0.4:
+            res = (res[0],) * 3
+        half = [rng.uniform(0.3, 1.3) for _ in range(3)]
+        ctr = [rng.uniform(-0.3, 0.3) for _ in range(3)]
+        bmin = tuple(c - h for c, h in zip(ctr, half))
+        bmax = tuple(c + h for c, h in zip(ctr, half))
+        flags = rng.choice([0, s2m.CORNERS_EXACT, s2m.REFERENCE_COMPAT, s2m.COMPAT_DROP_LAST_SLICE,
+                            s2m.COMPAT_POW2])
+        slab = rng.choice([0, 0, 0, 8, 16])
+        print(f"sweep case {it}: {name} res={res} box={bmin}..{bmax} flags={flags} slab={slab}", flush=True)
+        prog = ctx.program(node)
+        want = None
+        for extra in (0, s2m.DENSE_EVAL):
+            p = s2m.Params(bmin, bmax, res, 1e-4, flags | extra, slab)
+            if want is None:
+                want = reference_mesh(node, p)
+            assert_same_mesh(ctx.mesh(prog, p), want)
