@@ -241,6 +241,10 @@ ORA_API uint32_t ora_cell_signs(const float c[8])
            ((uint32_t)(c[4] > 0.0f) << 2) | ((uint32_t)(c[0] > 0.0f) << 3);
 }
 
+/* A NaN result is written as the canonical quiet NaN (0x7fc00000), as the library writes
+ * it: the sign and payload of a NaN depend on which operand each operation propagates. */
+static float canon_nan(float v) { return v != v ? NAN : v; }
+
 /* dualcontour.wgsl:72-131.  c[] in corner order 000,100,010,110,001,101,011,111.
  * Returns 1 and writes pos when at least two crossings are counted. */
 ORA_API int ora_cell_vertex(const float c[8], const float cmin[3], const float cmax[3],
@@ -289,9 +293,9 @@ ORA_API int ora_cell_vertex(const float c[8], const float cmin[3], const float c
     /* :126-130 */
     if (cnt <= 1.0f) return 0;
     float sx = cmax[0] - cmin[0], sy = cmax[1] - cmin[1], sz = cmax[2] - cmin[2];
-    pos[0] = cmin[0] + (sx * ax) / cnt;
-    pos[1] = cmin[1] + (sy * ay) / cnt;
-    pos[2] = cmin[2] + (sz * az) / cnt;
+    pos[0] = canon_nan(cmin[0] + (sx * ax) / cnt);
+    pos[1] = canon_nan(cmin[1] + (sy * ay) / cnt);
+    pos[2] = canon_nan(cmin[2] + (sz * az) / cnt);
     return 1;
 }
 
@@ -310,9 +314,9 @@ ORA_API void ora_normal(const s2m_op* ops, uint32_t n, const float p[3], float e
     for (int a = 0; a < 3; ++a)
         g[a] = ((V[0][a] * d[0] + V[1][a] * d[1]) + V[2][a] * d[2]) + V[3][a] * d[3];
     float len = sqrtf((g[0] * g[0] + g[1] * g[1]) + g[2] * g[2]);
-    nrm[0] = g[0] / len;
-    nrm[1] = g[1] / len;
-    nrm[2] = g[2] / len;
+    nrm[0] = canon_nan(g[0] / len);
+    nrm[1] = canon_nan(g[1] / len);
+    nrm[2] = canon_nan(g[2] / len);
 }
 
 /* ---- results -------------------------------------------------------------------- */

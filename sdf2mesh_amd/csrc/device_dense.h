@@ -32,13 +32,19 @@ __device__ __forceinline__ void eval_lattice_point(F sdf, const float* __restric
     lat[((size_t)iz * ny + iy) * pitch + ix] = sdf(cx[ix], cy[iy], cz[lz0 + iz]);
 }
 
+// A NaN result is stored as the canonical quiet NaN (0x7fc00000): which NaN operand an
+// instruction propagates, and so the sign and payload of a NaN result, differ between
+// kernels that compute the same value (the dense and the tile-sparse path), so vertex
+// positions and normals that are NaN are written with one bit pattern on every path.
+__device__ __forceinline__ float canonical_nan(float v) { return v != v ? __builtin_nanf("") : v; }
+
 // normalize(g) written to o[3..5] (dualcontour.wgsl:171): g / length(g).
 __device__ __forceinline__ void write_unit(float* o, float gx, float gy, float gz)
 {
     const float len = sqrtf((gx * gx + gy * gy) + gz * gz);
-    o[3] = gx / len;
-    o[4] = gy / len;
-    o[5] = gz / len;
+    o[3] = canonical_nan(gx / len);
+    o[4] = canonical_nan(gy / len);
+    o[5] = canonical_nan(gz / len);
 }
 
 // sdf3d_normal (sdf3d_normal.wgsl:4-9): taps v1=(1,-1,-1), v2=(-1,-1,1), v3=(-1,1,-1),

@@ -138,9 +138,9 @@ __device__ __forceinline__ bool contour_cell(const float c[8], const float lo[3]
         }
     }
     if (cnt <= 1.0f) return false;
-    pos[0] = lo[0] + ((hi[0] - lo[0]) * ax) / cnt;
-    pos[1] = lo[1] + ((hi[1] - lo[1]) * ay) / cnt;
-    pos[2] = lo[2] + ((hi[2] - lo[2]) * az) / cnt;
+    pos[0] = canonical_nan(lo[0] + ((hi[0] - lo[0]) * ax) / cnt);
+    pos[1] = canonical_nan(lo[1] + ((hi[1] - lo[1]) * ay) / cnt);
+    pos[2] = canonical_nan(lo[2] + ((hi[2] - lo[2]) * az) / cnt);
     return true;
 }
 

@@ -40,6 +40,7 @@
 #include <mutex>
 #include <vector>
 
+#include "device_dense.h"
 #include "device_interval.h"
 #include "device_sdf.h"
 #include "device_staged.h"
@@ -207,9 +208,9 @@ __device__ __forceinline__ bool contour_cell_sp(const float c[8], const float lo
         }
     }
     if (cnt <= 1.0f) return false;
-    pos[0] = lo[0] + ((hi[0] - lo[0]) * ax) / cnt;
-    pos[1] = lo[1] + ((hi[1] - lo[1]) * ay) / cnt;
-    pos[2] = lo[2] + ((hi[2] - lo[2]) * az) / cnt;
+    pos[0] = canonical_nan(lo[0] + ((hi[0] - lo[0]) * ax) / cnt);
+    pos[1] = canonical_nan(lo[1] + ((hi[1] - lo[1]) * ay) / cnt);
+    pos[2] = canonical_nan(lo[2] + ((hi[2] - lo[2]) * az) / cnt);
     return true;
 }
 
@@ -1456,9 +1457,9 @@ __global__ __launch_bounds__(64) void k_sp_emit(SpPass P, MeshBufs b)
             const float gy = (((-d1) + (-d2)) + d3) + d4;
             const float gz = (((-d1) + d2) + (-d3)) + d4;
             const float ln = sqrtf((gx * gx + gy * gy) + gz * gz);
-            o[3] = gx / ln;
-            o[4] = gy / ln;
-            o[5] = gz / ln;
+            o[3] = canonical_nan(gx / ln);
+            o[4] = canonical_nan(gy / ln);
+            o[5] = canonical_nan(gz / ln);
         }
         __syncthreads();  // every cell's triangle count is in tc
         // triangles, in vertex order: each quad (q2, q1, q0), (q0, q3, q2)

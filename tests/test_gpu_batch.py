@@ -157,3 +157,21 @@ def test_batch_passes_overflow_and_repeat():
             same(first[i], solo)
             same(second[i], solo)
             same(profiled[i], solo)
+
+
+def test_batch_nan_results_equal_solo(ctx):
+    # A 3-iteration Mandelbulb whose estimator gives NaN at some lattice points, so some
+    # vertex positions and normals are NaN: meshed alone it takes the dense path, in a
+    # batch the shared tile pass; both store NaN results as the canonical quiet NaN, so
+    # the meshes are equal bit for bit, NaNs included.
+    half = (0.8948788543249019, 0.9494887433921423, 0.6620873067782631)
+    p = s2m.Params(tuple(-h for h in half), half, (9, 33, 33), 1e-4, 0)
+    prog = ctx.program(tape.mandelbulb(3))
+    others = [ctx.program(tape.random_csg(5)), ctx.program(tape.csg64())]
+    q = s2m.Params.reference(48, 2.0)
+    solo = ctx.mesh(prog, p)
+    assert np.isnan(solo.vertices).any()
+    canonical = np.float32(np.nan).view(np.uint32)
+    assert np.all(solo.vertices.view(np.uint32)[np.isnan(solo.vertices)] == canonical)
+    batch = ctx.mesh_batch([others[0], prog, others[1]], [q, p, q])
+    same(batch[1], solo)
