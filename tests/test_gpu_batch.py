@@ -175,3 +175,37 @@ def test_batch_nan_results_equal_solo(ctx):
     assert np.all(solo.vertices.view(np.uint32)[np.isnan(solo.vertices)] == canonical)
     batch = ctx.mesh_batch([others[0], prog, others[1]], [q, p, q])
     same(batch[1], solo)
+
+
+def test_randomized_batches_equal_solo(ctx):
+    # Seeded random batches of 2-40 jobs: random CSG trees, CSG-64, the operator zoo,
+    # clustered folds and Mandelbulbs on random grids (often anisotropic), boxes and flag
+    # sets; each batched mesh equals the job meshed alone, bit for bit.
+    import random
+
+    from test_gpu_parity import clustered_fold, op_zoo
+
+    rng = random.Random(3)
+    for bi in range(24):
+        progs, params = [], []
+        for _ in range(rng.randint(2, 40)):
+            seed = rng.randrange(1, 10 ** 6)
+            u = rng.random()
+            if u < 0.75:
+                node = tape.random_csg(seed)
+            elif u < 0.8:
+                node = tape.csg64()
+            elif u < 0.88:
+                node = op_zoo(rng.choice([0.0, 0.02, 0.3]))
+            elif u < 0.96:
+                node = clustered_fold(rng.randint(20, 120), seed, rng.choice([0.05, 0.5]))
+            else:
+                node = tape.mandelbulb(rng.randint(3, 8))
+            res = tuple(rng.choice([9, 16, 24, 33, 48, 64, 96]) for _ in range(3))
+            half = tuple(rng.uniform(0.4, 1.3) for _ in range(3))
+            flags = rng.choice([0, s2m.CORNERS_EXACT, s2m.REFERENCE_COMPAT])
+            progs.append(ctx.program(node))
+            params.append(s2m.Params(tuple(-h for h in half), half, res, 1e-4, flags))
+        print(f"random batch {bi}: {len(progs)} jobs", flush=True)
+        for pr, p, m in zip(progs, params, ctx.mesh_batch(progs, params)):
+            same(m, ctx.mesh(pr, p))
